@@ -1,0 +1,90 @@
+/*
+ * specrel.h — C ABI of the MI355X (gfx950) renderer: the drop-in for `GpuRenderer`
+ * (src/gpu.rs:432-901). Plain pointers and sizes only; callable from C, C++, Python (ctypes)
+ * or a Rust `extern "C"` block.
+ *
+ * Semantics follow the reference:
+ *   - every call is synchronous unless its name says otherwise (render waits like
+ *     gpu.rs:871-876);
+ *   - a context is owned by one host thread at a time (GpuRenderer methods take &mut self);
+ *     separate contexts may be used concurrently;
+ *   - errors are return codes, never aborts: where the reference panics (framebuffer size
+ *     mismatch, gpu.rs:808) the library returns SPECREL_EINVAL and sets the message;
+ *   - scene arrays are copied to device memory at upload; the caller keeps ownership.
+ * There is no CPU path inside this library: without a usable gfx950 device, specrel_create
+ * fails (SPECREL_ENODEV) and the caller selects its own CPU renderer, as main.rs:275-296 does
+ * when GpuRenderer::new returns Err.
+ */
+#ifndef SPECREL_H
+#define SPECREL_H
+
+#include "specrel_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPECREL_OK 0
+#define SPECREL_ENODEV (-1)  /* no HIP device / not gfx950 */
+#define SPECREL_EINVAL (-2)  /* bad argument, size mismatch, malformed scene */
+#define SPECREL_ENOMEM (-3)  /* device allocation failed */
+#define SPECREL_EDEVICE (-4) /* a HIP runtime call failed */
+
+typedef struct specrel_ctx specrel_ctx;
+
+/* Where the last render's time went (GpuTimings, gpu.rs:408-420), in milliseconds. Kernel
+ * times come from HIP events on the context's stream. */
+typedef struct specrel_timings {
+    float submit_ms;      /* host time to enqueue everything */
+    float wait_ms;        /* host wait for the device */
+    float kernel_ms;      /* first kernel start to last kernel end, device clock */
+    float readback_ms;    /* device->host copies */
+    float star_prep_ms;   /* per-view star pass */
+    float trace_ms;       /* the fused ray-generation/trace/shade/star kernel */
+    float bloom_ms;       /* pyramid down + up passes */
+    float composite_ms;   /* composite + tone map + pack */
+} specrel_timings;
+
+/* Options for specrel_set_option. */
+enum {
+    SPECREL_OPT_STATS = 1,      /* 1: run the instrumented trace variant and fill specrel_stats */
+    SPECREL_OPT_STAR_CULL = 2   /* 1 (default): per-tile star culling; 0: every star per pixel */
+};
+
+/* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
+int specrel_create(int device_ordinal, specrel_ctx **out);
+void specrel_destroy(specrel_ctx *ctx);
+/* Message for the last failing call on ctx (or of specrel_create when ctx is NULL). */
+const char *specrel_last_error(const specrel_ctx *ctx);
+/* GpuRenderer::adapter_name (gpu.rs:734-736). */
+const char *specrel_device_name(const specrel_ctx *ctx);
+int specrel_set_option(specrel_ctx *ctx, int option, int value);
+
+/* GpuRenderer::upload_scene (gpu.rs:622-699): copy (and re-pack) one scene into slot
+ * scene_idx, replacing what was there. The arrays are PackedSceneGpuData's plus `order`. */
+int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene);
+/* Number of scene slots in use (highest uploaded index + 1). */
+uint32_t specrel_num_scenes(const specrel_ctx *ctx);
+
+/* GpuRenderer::render (gpu.rs:798-900): render one view of scenes[scene_idx] and wait.
+ * out_rgb receives width*height 0x00RRGGBB pixels (required). The optional per-pixel outputs
+ * (NULL to skip) are the hit tetrahedron's input-order index (SPECREL_NO_HIT on a miss), the
+ * hit's ray parameter lambda (+inf on a miss) and its interpolated proper time (0 on a miss). */
+int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *out_rgb,
+                   uint32_t *out_hit, float *out_lambda, float *out_tau);
+
+/* The same render, leaving every output in device memory (no readback) and returning once
+ * the work is enqueued; with want_hits the hit/lambda/tau buffers are written too.
+ * specrel_sync waits; specrel_download copies the resident outputs of the last render. */
+int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
+int specrel_sync(specrel_ctx *ctx);
+int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau);
+
+int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
+/* Counters of the last render made with SPECREL_OPT_STATS on. */
+int specrel_last_stats(const specrel_ctx *ctx, specrel_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,155 @@
+/*
+ * specrel_types.h — byte layouts shared across the drop-in boundary.
+ *
+ * The scene records are the reference's packed GPU records, byte for byte, so a
+ * Rust caller can hand over what `PackedSceneGpuData::from_scene` already builds:
+ *   specrel_node     <- GpuNode     (src/gpu.rs:42-64)     48 B
+ *   specrel_geom     <- GpuGeom     (src/gpu.rs:66-74)     80 B, leaf order
+ *   specrel_attr     <- GpuAttr     (src/gpu.rs:76-92, packed at 358-381) 192 B, leaf order
+ *   specrel_material <- GpuMaterial (src/gpu.rs:94-100, packed at 255-275) 48 B
+ *   star words       <- pack_stars  (src/gpu.rs:102-119)   float4 headers + 3 float4 per segment
+ * plus the leaf-slot -> input-order map `order` (src/bvh4d.rs:72-76), which the reference
+ * keeps on the host; the library uses it to report the hit tetrahedron's input index.
+ *
+ * specrel_frame replaces `GpuUniforms` (src/gpu.rs:121-212). It carries the same fields, but
+ * keeps each flash's colour and intensity apart: the CPU renderer multiplies them in a
+ * different order (src/renderer.rs:685) from the WGSL port's premultiplied colour
+ * (gpu.rs:164), and the CPU order is the one reproduced bit for bit.
+ *
+ * All vectors are (t, x, y, z) for 4-vectors and (x, y, z, _) for colours.
+ */
+#ifndef SPECREL_TYPES_H
+#define SPECREL_TYPES_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPECREL_MAX_FLASHES 16 /* renderer.rs:179 */
+#define SPECREL_MAX_HAZE 5     /* renderer.rs:224 */
+#define SPECREL_BLOOM_LEVELS 6 /* bloom.rs:23 */
+#define SPECREL_LUT_SIZE 4096  /* renderer.rs:1102 */
+#define SPECREL_NUM_GLYPHS 14  /* gpu.rs:33-35: '0'-'9', '+', '-', '.', 'S' */
+#define SPECREL_NO_HIT 0xFFFFFFFFu
+
+/* RenderMode (renderer.rs:11), ShadowMode (128), ViewMode/SliceMode (69), SurfacePattern
+ * (mesh4d.rs:9); numbering as GpuUniforms::build (gpu.rs:181-197). */
+enum { SPECREL_MODE_SHADED = 0, SPECREL_MODE_ISOCHRONES = 1, SPECREL_MODE_INSPECTOR = 2,
+       SPECREL_MODE_DECISION_AGE = 3 };
+enum { SPECREL_SHADOW_OFF = 0, SPECREL_SHADOW_NULL = 1, SPECREL_SHADOW_CAMERA_SLICE = 2 };
+enum { SPECREL_VIEW_PAST_CONE = 0, SPECREL_VIEW_SIMULTANEITY = 1, SPECREL_VIEW_FRONTIER = 2,
+       SPECREL_VIEW_PROPHECY = 3 };
+enum { SPECREL_PATTERN_CHECKER = 0, SPECREL_PATTERN_CLOCK = 1, SPECREL_PATTERN_LIGHT_CLOCK = 2 };
+
+typedef struct specrel_node {      /* GpuNode, gpu.rs:44-51 */
+    float min[4];
+    float max[4];
+    uint32_t right_or_first;       /* leaf: first leaf slot; interior: right child */
+    uint32_t count;                /* > 0 for leaves */
+    uint32_t left;                 /* interior: left child */
+    uint32_t pad;
+} specrel_node;
+
+typedef struct specrel_geom {      /* GpuGeom, gpu.rs:68-74 (SimplexGeom4D, mesh4d.rs:153-159) */
+    float v0[4];
+    float n[4];
+    float b1[4];
+    float b2[4];
+    float b3[4];
+} specrel_geom;
+
+typedef struct specrel_attr {      /* GpuAttr, gpu.rs:79-92: one row per component, 4 vertices */
+    float uv_u[4];
+    float uv_v[4];
+    float tb_x[4];
+    float tb_y[4];
+    float tb_z[4];
+    float tau[4];
+    float grad_u[4];
+    float grad_v[4];
+    float normal[4];
+    float four_vel[4];
+    float misc[4];                 /* uv_world_scale.0, .1, bits(material_id), 0 */
+    float decided_at[4];
+} specrel_attr;
+
+typedef struct specrel_material {  /* GpuMaterial, gpu.rs:96-100 */
+    float base[4];                 /* base rgb, emission */
+    float checker[4];              /* checker rgb, grid_scale */
+    float params[4];               /* wireframe_width, bits(pattern), cavity_length, 0 */
+} specrel_material;
+
+/* The arrays `specrel_upload_scene` copies to the device (PackedSceneGpuData, gpu.rs:232-244). */
+typedef struct specrel_scene_arrays {
+    const specrel_node *nodes;
+    uint32_t num_nodes;
+    uint32_t num_prims;
+    const specrel_geom *geoms;     /* num_prims, leaf order */
+    const specrel_attr *attrs;     /* num_prims, leaf order */
+    const uint32_t *order;         /* num_prims: leaf slot -> input simplex index */
+    const specrel_material *materials;
+    uint32_t num_materials;
+    uint32_t num_stars;
+    const float *star_words;       /* pack_stars: 4 * num_star_words floats */
+    uint32_t num_star_words;       /* float4 count: num_stars headers + 3 per segment */
+    uint32_t pad;
+    float light_pos[4];            /* (0, x, y, z) as gpu.rs:286 */
+} specrel_scene_arrays;
+
+typedef struct specrel_frame {     /* per-view inputs; replaces GpuUniforms (gpu.rs:123-144) */
+    float cam_pos[4];              /* Camera4D::position */
+    float e[4][4];                 /* Camera4D::tetrad.e[0..4] */
+    float half_wh[2];              /* aspect*tan(fov/2), tan(fov/2) (renderer.rs:336-338) */
+    uint32_t size[2];              /* width, height */
+    uint32_t mode, shadow_mode, slice_mode, num_flashes;
+    float frontier_origin[4];      /* frontier_or_camera (renderer.rs:266) */
+    float frontier_tau;            /* its proper time */
+    float frontier_view_offset;    /* FRONTIER_RAY_OFFSET + frontier_lag (renderer.rs:235) */
+    float prophecy_delta;
+    uint32_t num_haze;
+    float haze_origin[SPECREL_MAX_HAZE][4];      /* haze_origins (renderer.rs:253) */
+    float flash_at[SPECREL_MAX_FLASHES][4];
+    float flash_color_width[SPECREL_MAX_FLASHES][4]; /* colour rgb (unscaled), width */
+    float flash_intensity[SPECREL_MAX_FLASHES];
+} specrel_frame;
+
+/* Traversal counters of one render (the instrumented variant), summed over all rays. */
+typedef struct specrel_stats {
+    uint64_t rays;               /* primary rays */
+    uint64_t node_pops;          /* closest-hit stack pops that reached a node */
+    uint64_t box_tests;          /* child boxes tested, closest-hit */
+    uint64_t prim_tests;         /* intersect_fast calls, closest-hit */
+    uint64_t hits;               /* primary rays that hit */
+    uint64_t shadow_rays;        /* any-hit queries (light + flashes) */
+    uint64_t shadow_node_pops;
+    uint64_t shadow_box_tests;
+    uint64_t shadow_prim_tests;
+    uint64_t star_tests;         /* per-pixel star kernel tests after tile culling */
+    uint64_t reserved[6];
+} specrel_stats;
+
+#ifdef __cplusplus
+}
+#endif
+
+#if defined(__cplusplus)
+static_assert(sizeof(specrel_node) == 48, "GpuNode is 48 B");
+static_assert(sizeof(specrel_geom) == 80, "GpuGeom is 80 B");
+static_assert(sizeof(specrel_attr) == 192, "GpuAttr is 192 B");
+static_assert(sizeof(specrel_material) == 48, "GpuMaterial is 48 B");
+static_assert(offsetof(specrel_attr, misc) == 160, "GpuAttr.misc offset");
+static_assert(offsetof(specrel_attr, decided_at) == 176, "GpuAttr.decided_at offset");
+static_assert(offsetof(specrel_frame, haze_origin) == 144, "frame layout");
+static_assert(sizeof(specrel_frame) == 800, "frame layout");
+#else
+_Static_assert(sizeof(specrel_node) == 48, "GpuNode is 48 B");
+_Static_assert(sizeof(specrel_geom) == 80, "GpuGeom is 80 B");
+_Static_assert(sizeof(specrel_attr) == 192, "GpuAttr is 192 B");
+_Static_assert(sizeof(specrel_material) == 48, "GpuMaterial is 48 B");
+_Static_assert(sizeof(specrel_frame) == 800, "frame layout");
+#endif
+
+#endif

@@ -1,0 +1,522 @@
+// C ABI of the gfx950 renderer (include/specrel.h): context, scene upload with re-packing
+// into the device layout, render, timings. The counterpart of GpuRenderer (src/gpu.rs:432-901).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../core/tables.h"
+#include "kernels.hpp"
+#include "specrel.h"
+
+using namespace dev;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct SceneSlot {
+    bool used = false;
+    DevBuf nbox, ninfo, big, geoms, attrs, order, mats, stars;
+    Scene S{};
+    uint32_t num_stars = 0, num_star_words = 0, num_materials = 0;
+};
+
+struct Fail {
+    int code;
+    std::string msg;
+};
+
+}  // namespace
+
+struct specrel_ctx {
+    int device = 0;
+    std::string name, err;
+    hipStream_t stream = nullptr;
+    std::vector<SceneSlot> scenes;
+    uint32_t fw = 0, fh = 0;
+    DevBuf hdr, rgb, hit, lam, tau, prep, frame, tables, stats;
+    hipEvent_t ev[6] = {};
+    specrel_timings timings{};
+    specrel_stats stats_host{};
+    int opt_stats = 0, opt_cull = 1;
+    bool have_hits = false;
+};
+
+namespace {
+
+std::string g_create_err;
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Fail{SPECREL_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void dfree(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+void dalloc(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return;
+    dfree(b);
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        throw Fail{SPECREL_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    b.bytes = bytes;
+}
+
+void upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s) {
+    dalloc(b, bytes);
+    if (bytes) check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+template <class F>
+int guarded(specrel_ctx *ctx, F &&f) {
+    try {
+        return f();
+    } catch (const Fail &e) {
+        if (ctx) ctx->err = e.msg;
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return SPECREL_EINVAL;
+    }
+}
+
+size_t pyramid_len(uint32_t w, uint32_t h) {  // bloom.rs:47-49
+    size_t n = 0;
+    for (int k = 0; k <= SPECREL_BLOOM_LEVELS; ++k) {
+        n += (size_t)w * h;
+        w = (w + 1) / 2;
+        h = (h + 1) / 2;
+    }
+    return n;
+}
+
+void level_size(uint32_t w, uint32_t h, int k, uint32_t &ow, uint32_t &oh) {
+    for (int i = 0; i < k; ++i) {
+        w = (w + 1) / 2;
+        h = (h + 1) / 2;
+    }
+    ow = w;
+    oh = h;
+}
+
+size_t level_offset(uint32_t w, uint32_t h, int k) {
+    size_t off = 0;
+    for (int i = 0; i < k; ++i) {
+        uint32_t lw, lh;
+        level_size(w, h, i, lw, lh);
+        off += (size_t)lw * lh;
+    }
+    return off;
+}
+
+// Re-pack the reference arrays into the device layout, validating the tree on the way.
+void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_arrays *a) {
+    if (!a) throw Fail{SPECREL_EINVAL, "scene is NULL"};
+    const uint32_t nn = a->num_nodes, np = a->num_prims;
+    if (nn && !a->nodes) throw Fail{SPECREL_EINVAL, "nodes is NULL"};
+    if (np && (!a->geoms || !a->attrs || !a->order)) throw Fail{SPECREL_EINVAL, "prim arrays are NULL"};
+    if (a->num_materials == 0 && np) throw Fail{SPECREL_EINVAL, "no materials"};
+    if (np >= 0x7FFFFFFFu) throw Fail{SPECREL_EINVAL, "too many primitives"};
+    // Materials referenced by every primitive must exist (Scene4D::materials index, renderer.rs:422).
+    for (uint32_t i = 0; i < np; ++i) {
+        uint32_t m;
+        std::memcpy(&m, &a->attrs[i].misc[2], 4);
+        if (m >= a->num_materials) throw Fail{SPECREL_EINVAL, "primitive " + std::to_string(i) + " has material " +
+                                                                  std::to_string(m) + " out of range"};
+    }
+    // Stars (pack_stars layout).
+    if (a->num_stars) {
+        if (!a->star_words || a->num_star_words < a->num_stars) throw Fail{SPECREL_EINVAL, "star words too short"};
+        uint64_t nseg = (a->num_star_words - a->num_stars) / 3;
+        for (uint32_t i = 0; i < a->num_stars; ++i) {
+            uint32_t first, count;
+            std::memcpy(&first, &a->star_words[4 * i + 2], 4);
+            std::memcpy(&count, &a->star_words[4 * i + 3], 4);
+            if ((uint64_t)first + count > nseg) throw Fail{SPECREL_EINVAL, "star segments out of range"};
+        }
+    }
+
+    std::vector<float4> nbox;
+    std::vector<uint2> ninfo;
+    std::vector<uint2> big;
+    auto encode = [&](uint32_t j) -> uint32_t {
+        const specrel_node &n = a->nodes[j];
+        if (n.count > 0) {
+            if (n.count <= COUNT_MAX && n.right_or_first <= FIRST_MASK)
+                return LEAF | (n.count << COUNT_SHIFT) | n.right_or_first;
+            if (big.size() > FIRST_MASK) throw Fail{SPECREL_EINVAL, "too many large leaves"};
+            big.push_back(make_uint2(n.right_or_first, n.count));
+            return LEAF | (uint32_t)(big.size() - 1);
+        }
+        return j;
+    };
+    Scene &S = slot.S;
+    S.empty = nn == 0;
+    if (nn) {
+        if (nn > 0x7FFFFFFFu) throw Fail{SPECREL_EINVAL, "too many nodes"};
+        nbox.assign((size_t)nn * 4, make_float4(0, 0, 0, 0));
+        ninfo.assign(nn, make_uint2(0, 0));
+        // Walk the tree from the root: each node reached once, children in range, leaves in range.
+        std::vector<uint8_t> seen(nn, 0);
+        std::vector<uint32_t> todo{0};
+        seen[0] = 1;
+        while (!todo.empty()) {
+            uint32_t i = todo.back();
+            todo.pop_back();
+            const specrel_node &n = a->nodes[i];
+            if (n.count > 0) {
+                if ((uint64_t)n.right_or_first + n.count > np) throw Fail{SPECREL_EINVAL, "leaf range out of bounds"};
+                continue;
+            }
+            uint32_t l = n.left, r = n.right_or_first;
+            if (l >= nn || r >= nn) throw Fail{SPECREL_EINVAL, "child index out of range"};
+            if (seen[l] || seen[r] || l == r) throw Fail{SPECREL_EINVAL, "node graph is not a tree"};
+            seen[l] = seen[r] = 1;
+            todo.push_back(r);
+            todo.push_back(l);
+            const specrel_node &L = a->nodes[l], &R = a->nodes[r];
+            nbox[4 * (size_t)i + 0] = make_float4(L.min[0], L.min[1], L.min[2], L.min[3]);
+            nbox[4 * (size_t)i + 1] = make_float4(L.max[0], L.max[1], L.max[2], L.max[3]);
+            nbox[4 * (size_t)i + 2] = make_float4(R.min[0], R.min[1], R.min[2], R.min[3]);
+            nbox[4 * (size_t)i + 3] = make_float4(R.max[0], R.max[1], R.max[2], R.max[3]);
+            ninfo[i] = make_uint2(encode(l), encode(r));
+        }
+        const specrel_node &root = a->nodes[0];
+        S.root_entry = encode(0);
+        S.root_min = make_float4(root.min[0], root.min[1], root.min[2], root.min[3]);
+        S.root_max = make_float4(root.max[0], root.max[1], root.max[2], root.max[3]);
+    }
+    hipStream_t s = ctx->stream;
+    upload(slot.nbox, nbox.data(), nbox.size() * sizeof(float4), s);
+    upload(slot.ninfo, ninfo.data(), ninfo.size() * sizeof(uint2), s);
+    upload(slot.big, big.data(), big.size() * sizeof(uint2), s);
+    upload(slot.geoms, a->geoms, (size_t)np * sizeof(specrel_geom), s);
+    upload(slot.attrs, a->attrs, (size_t)np * sizeof(specrel_attr), s);
+    upload(slot.order, a->order, (size_t)np * 4, s);
+    upload(slot.mats, a->materials, (size_t)a->num_materials * sizeof(specrel_material), s);
+    upload(slot.stars, a->star_words, (size_t)a->num_star_words * 16, s);
+    check(hipStreamSynchronize(s), "upload");
+    S.nbox = (const float4 *)slot.nbox.p;
+    S.ninfo = (const uint2 *)slot.ninfo.p;
+    S.big_leaves = (const uint2 *)slot.big.p;
+    S.geoms = (const float4 *)slot.geoms.p;
+    S.attrs = (const float4 *)slot.attrs.p;
+    S.order = (const uint32_t *)slot.order.p;
+    S.materials = (const float4 *)slot.mats.p;
+    S.num_prims = np;
+    S.light = make_float4(a->light_pos[0], a->light_pos[1], a->light_pos[2], a->light_pos[3]);
+    slot.num_stars = a->num_stars;
+    slot.num_star_words = a->num_star_words;
+    slot.num_materials = a->num_materials;
+    slot.used = true;
+}
+
+void ensure_frame(specrel_ctx *ctx, uint32_t w, uint32_t h, uint32_t num_stars) {
+    if (ctx->fw != w || ctx->fh != h) {
+        dalloc(ctx->hdr, pyramid_len(w, h) * sizeof(float4));
+        dalloc(ctx->rgb, (size_t)w * h * 4);
+        dalloc(ctx->hit, (size_t)w * h * 4);
+        dalloc(ctx->lam, (size_t)w * h * 4);
+        dalloc(ctx->tau, (size_t)w * h * 4);
+        ctx->fw = w;
+        ctx->fh = h;
+    }
+    dalloc(ctx->prep, (size_t)(num_stars ? num_stars : 1) * sizeof(StarPrep));
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+    return ms;
+}
+
+int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f, bool want_hits) {
+    if (!f) throw Fail{SPECREL_EINVAL, "frame is NULL"};
+    if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
+        throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
+    const uint32_t w = f->size[0], h = f->size[1];
+    if (w == 0 || h == 0 || w > 32768 || h > 32768) throw Fail{SPECREL_EINVAL, "bad frame size"};
+    if (f->mode > 3 || f->shadow_mode > 2 || f->slice_mode > 3) throw Fail{SPECREL_EINVAL, "bad render mode"};
+    if (f->num_flashes > SPECREL_MAX_FLASHES || f->num_haze > SPECREL_MAX_HAZE)
+        throw Fail{SPECREL_EINVAL, "too many flashes or haze origins"};
+    SceneSlot &sc = ctx->scenes[scene_idx];
+    ensure_frame(ctx, w, h, sc.num_stars);
+    hipStream_t s = ctx->stream;
+    auto t0 = std::chrono::steady_clock::now();
+    check(hipMemcpyAsync(ctx->frame.p, f, sizeof(specrel_frame), hipMemcpyHostToDevice, s), "frame upload");
+    const bool stats = ctx->opt_stats != 0;
+    want_hits = want_hits || stats;
+    if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
+
+    check(hipEventRecord(ctx->ev[0], s), "event");
+    const bool draw_stars = f->mode == SPECREL_MODE_SHADED && sc.num_stars > 0;
+    if (draw_stars) {
+        k_star_prep<<<(sc.num_stars + 255) / 256, 256, 0, s>>>((const float4 *)sc.stars.p, sc.num_stars,
+                                                             (const specrel_frame *)ctx->frame.p,
+                                                             (StarPrep *)ctx->prep.p);
+        check(hipGetLastError(), "k_star_prep launch");
+    }
+    check(hipEventRecord(ctx->ev[1], s), "event");
+    RenderArgs ra;
+    ra.S = sc.S;
+    ra.T.lut = (const uint8_t *)ctx->tables.p;
+    ra.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
+    ra.frame = (const specrel_frame *)ctx->frame.p;
+    ra.stars = (const StarPrep *)ctx->prep.p;
+    ra.num_stars = sc.num_stars;
+    ra.cull = ctx->opt_cull;
+    ra.want_hits = want_hits ? 1 : 0;
+    ra.hdr = (float4 *)ctx->hdr.p;
+    ra.hit = (uint32_t *)ctx->hit.p;
+    ra.lambda = (float *)ctx->lam.p;
+    ra.tau = (float *)ctx->tau.p;
+    ra.stats = (unsigned long long *)ctx->stats.p;
+    dim3 grid((w + 15) / 16, (h + 15) / 16);
+    if (stats) k_render<true><<<grid, 256, 0, s>>>(ra);
+    else k_render<false><<<grid, 256, 0, s>>>(ra);
+    check(hipGetLastError(), "k_render launch");
+    check(hipEventRecord(ctx->ev[2], s), "event");
+    float4 *pyr = (float4 *)ctx->hdr.p;
+    for (int k = 1; k <= SPECREL_BLOOM_LEVELS; ++k) {
+        uint32_t sw, sh, dw, dh;
+        level_size(w, h, k - 1, sw, sh);
+        level_size(w, h, k, dw, dh);
+        dim3 g((dw + 15) / 16, (dh + 15) / 16);
+        k_bloom_down<<<g, 256, 0, s>>>(pyr + level_offset(w, h, k - 1), sw, sh, pyr + level_offset(w, h, k), dw, dh);
+    }
+    for (int k = SPECREL_BLOOM_LEVELS - 1; k >= 1; --k) {
+        uint32_t lw, lh, cw, ch;
+        level_size(w, h, k, lw, lh);
+        level_size(w, h, k + 1, cw, ch);
+        dim3 g((lw + 15) / 16, (lh + 15) / 16);
+        k_bloom_up<<<g, 256, 0, s>>>(pyr + level_offset(w, h, k), lw, lh, pyr + level_offset(w, h, k + 1), cw, ch);
+    }
+    check(hipGetLastError(), "bloom launch");
+    check(hipEventRecord(ctx->ev[3], s), "event");
+    {
+        uint32_t w1, h1;
+        level_size(w, h, 1, w1, h1);
+        dim3 g((w + 15) / 16, (h + 15) / 16);
+        k_composite<<<g, 256, 0, s>>>(pyr, w, h, pyr + level_offset(w, h, 1), w1, h1, (const uint8_t *)ctx->tables.p,
+                                      (uint32_t *)ctx->rgb.p);
+        check(hipGetLastError(), "k_composite launch");
+    }
+    check(hipEventRecord(ctx->ev[4], s), "event");
+    auto t1 = std::chrono::steady_clock::now();
+    ctx->timings = specrel_timings{};
+    ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    ctx->have_hits = want_hits;
+    return SPECREL_OK;
+}
+
+void collect(specrel_ctx *ctx) {
+    specrel_timings &t = ctx->timings;
+    t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
+    t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
+    t.bloom_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
+    t.composite_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
+    t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[4]);
+    if (ctx->opt_stats) {
+        unsigned long long c[16];
+        check(hipMemcpy(c, ctx->stats.p, sizeof c, hipMemcpyDeviceToHost), "stats readback");
+        specrel_stats &st = ctx->stats_host;
+        std::memset(&st, 0, sizeof st);
+        st.rays = c[0];
+        st.node_pops = c[1];
+        st.box_tests = c[2];
+        st.prim_tests = c[3];
+        st.hits = c[4];
+        st.shadow_rays = c[5];
+        st.shadow_node_pops = c[6];
+        st.shadow_box_tests = c[7];
+        st.shadow_prim_tests = c[8];
+        st.star_tests = c[9];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int specrel_create(int device_ordinal, specrel_ctx **out) {
+    if (!out) return SPECREL_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        g_create_err = "no HIP device";
+        return SPECREL_ENODEV;
+    }
+    if (device_ordinal < 0 || device_ordinal >= n) {
+        g_create_err = "device ordinal out of range";
+        return SPECREL_ENODEV;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_ordinal) != hipSuccess) {
+        g_create_err = "hipGetDeviceProperties failed";
+        return SPECREL_ENODEV;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        g_create_err = std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950";
+        return SPECREL_ENODEV;
+    }
+    specrel_ctx *ctx = new specrel_ctx();
+    int rc = guarded(ctx, [&] {
+        ctx->device = device_ordinal;
+        ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ", HIP)";
+        check(hipSetDevice(device_ordinal), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto &e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
+        std::vector<uint32_t> words(SPECREL_LUT_SIZE + 14 * 7);
+        sr::build_tables(words.data());
+        std::vector<uint8_t> bytes(words.size());
+        for (size_t i = 0; i < words.size(); ++i) bytes[i] = (uint8_t)words[i];
+        upload(ctx->tables, bytes.data(), bytes.size(), ctx->stream);
+        dalloc(ctx->frame, sizeof(specrel_frame));
+        dalloc(ctx->stats, 16 * 8);
+        check(hipStreamSynchronize(ctx->stream), "create");
+        return SPECREL_OK;
+    });
+    if (rc != SPECREL_OK) {
+        g_create_err = ctx->err;
+        specrel_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return SPECREL_OK;
+}
+
+void specrel_destroy(specrel_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &sl : ctx->scenes) {
+        dfree(sl.nbox), dfree(sl.ninfo), dfree(sl.big), dfree(sl.geoms), dfree(sl.attrs), dfree(sl.order);
+        dfree(sl.mats), dfree(sl.stars);
+    }
+    dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
+    dfree(ctx->frame), dfree(ctx->tables), dfree(ctx->stats);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *specrel_last_error(const specrel_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+const char *specrel_device_name(const specrel_ctx *ctx) { return ctx ? ctx->name.c_str() : ""; }
+
+int specrel_set_option(specrel_ctx *ctx, int option, int value) {
+    if (!ctx) return SPECREL_EINVAL;
+    if (option == SPECREL_OPT_STATS) ctx->opt_stats = value != 0;
+    else if (option == SPECREL_OPT_STAR_CULL) ctx->opt_cull = value != 0;
+    else {
+        ctx->err = "unknown option";
+        return SPECREL_EINVAL;
+    }
+    return SPECREL_OK;
+}
+
+int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (scene_idx > 4096) throw Fail{SPECREL_EINVAL, "scene index too large"};
+        if (ctx->scenes.size() <= scene_idx) ctx->scenes.resize(scene_idx + 1);
+        SceneSlot fresh;
+        // Keep the old device buffers for reuse; rebuild the slot's contents.
+        SceneSlot &slot = ctx->scenes[scene_idx];
+        slot.used = false;
+        build_device_scene(ctx, slot, scene);
+        (void)fresh;
+        return SPECREL_OK;
+    });
+}
+
+uint32_t specrel_num_scenes(const specrel_ctx *ctx) { return ctx ? (uint32_t)ctx->scenes.size() : 0; }
+
+int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *out_rgb,
+                   uint32_t *out_hit, float *out_lambda, float *out_tau) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        if (!out_rgb) throw Fail{SPECREL_EINVAL, "out_rgb is NULL"};
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        bool hits = out_hit || out_lambda || out_tau;
+        enqueue_render(ctx, scene_idx, frame, hits);
+        auto t0 = std::chrono::steady_clock::now();
+        check(hipStreamSynchronize(ctx->stream), "render");
+        auto t1 = std::chrono::steady_clock::now();
+        size_t n = (size_t)frame->size[0] * frame->size[1];
+        check(hipMemcpy(out_rgb, ctx->rgb.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_hit) check(hipMemcpy(out_hit, ctx->hit.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_lambda) check(hipMemcpy(out_lambda, ctx->lam.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_tau) check(hipMemcpy(out_tau, ctx->tau.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        auto t2 = std::chrono::steady_clock::now();
+        collect(ctx);
+        ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+        ctx->timings.readback_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
+        return SPECREL_OK;
+    });
+}
+
+int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        return enqueue_render(ctx, scene_idx, frame, want_hits != 0);
+    });
+}
+
+int specrel_sync(specrel_ctx *ctx) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        auto t0 = std::chrono::steady_clock::now();
+        check(hipStreamSynchronize(ctx->stream), "sync");
+        auto t1 = std::chrono::steady_clock::now();
+        collect(ctx);
+        ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+        return SPECREL_OK;
+    });
+}
+
+int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        if (ctx->fw == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+        if ((out_hit || out_lambda || out_tau) && !ctx->have_hits)
+            throw Fail{SPECREL_EINVAL, "the last render did not write hit buffers"};
+        check(hipStreamSynchronize(ctx->stream), "download");
+        size_t n = (size_t)ctx->fw * ctx->fh;
+        if (out_rgb) check(hipMemcpy(out_rgb, ctx->rgb.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_hit) check(hipMemcpy(out_hit, ctx->hit.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_lambda) check(hipMemcpy(out_lambda, ctx->lam.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        if (out_tau) check(hipMemcpy(out_tau, ctx->tau.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        return SPECREL_OK;
+    });
+}
+
+int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out) {
+    if (!ctx || !out) return SPECREL_EINVAL;
+    *out = ctx->timings;
+    return SPECREL_OK;
+}
+
+int specrel_last_stats(const specrel_ctx *ctx, specrel_stats *out) {
+    if (!ctx || !out) return SPECREL_EINVAL;
+    *out = ctx->stats_host;
+    return SPECREL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,712 @@
+// Device-side renderer for gfx950: ray generation, 4D BVH traversal, affine ray-tetrahedron
+// intersection, proper-time shading, null / camera-slice shadow rays and star light.
+//
+// Semantics follow the reference CPU renderer exactly (src/renderer.rs, src/bvh4d.rs,
+// src/mesh4d.rs, src/star.rs) so that RGB8, hit index, lambda and tau are bit-identical;
+// the code is compiled with -ffp-contract=off and without fast-math. The WGSL port's
+// deviations (render.wgsl: dividing camera beta, truncated `%`, round_away) are not followed.
+//
+// Memory layout on the device (built by the upload):
+//   nbox[i]  : 4 x float4 = left child min, left max, right min, right max, for each interior
+//              node i (reference node numbering); 64 B, 64-B aligned.
+//   ninfo[i] : uint2 = left entry, right entry (see encode below).
+//   geoms    : 5 x float4 per leaf slot (GpuGeom, 80 B), attrs 12 x float4 (GpuAttr, 192 B).
+// A stack entry is (entry, t_enter); an entry names either an interior node or a leaf with
+// its first slot and count, so popping a leaf needs no node load.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../core/minkowski.h"
+#include "specrel_types.h"
+
+namespace dev {
+using namespace sr;
+
+constexpr uint32_t LEAF = 0x80000000u;
+constexpr uint32_t COUNT_SHIFT = 26;
+constexpr uint32_t FIRST_MASK = (1u << COUNT_SHIFT) - 1u;  // small leaf: first < 2^26
+constexpr uint32_t COUNT_MAX = 31u;                         // small leaf: count in 5 bits
+// Big leaf: LEAF | index into big_leaves (count field 0).
+
+struct Scene {
+    const float4 *nbox;
+    const uint2 *ninfo;
+    const uint2 *big_leaves;
+    const float4 *geoms;
+    const float4 *attrs;
+    const uint32_t *order;
+    const float4 *materials;
+    uint32_t root_entry;
+    uint32_t num_prims;
+    uint32_t empty;  // no nodes: every ray misses
+    float4 root_min, root_max;
+    float4 light;  // (0, x, y, z)
+};
+
+struct Tables {
+    const uint8_t *lut;     // 4096
+    const uint8_t *glyphs;  // 14 x 7
+};
+
+// Per-star values that depend only on the view (star.rs:251-283): r = d - tau e0, the
+// blackbody colour at the Doppler-shifted temperature, the colour norm, and the direction
+// of r in the camera's rest frame (for tile culling).
+struct StarPrep {
+    float4 r;
+    float4 bb_cn;      // blackbody rgb, colour_norm
+    double4 dir;       // unit local direction (x,y,z), w unused
+    uint32_t flags;    // bit0: exists on the cone; bit1: non-finite, never culled
+    uint32_t pad[3];
+};
+
+SR_HD V4 f4v(float4 a) { return V4{a.x, a.y, a.z, a.w}; }
+SR_HD V4 lv(const float *p) { return V4{p[0], p[1], p[2], p[3]}; }
+
+struct Counters {
+    uint64_t node_pops, box_tests, prim_tests, shadow_rays, shadow_node_pops, shadow_box_tests,
+        shadow_prim_tests, star_tests;
+};
+
+// Aabb4D::intersect_ray (mesh4d.rs:113-146)
+__device__ __forceinline__ bool box_hit(float4 mn, float4 mx, V4 o, V4 inv, float tmin, float tmax, float &enter) {
+    float t0 = (mn.x - o.t) * inv.t, t1 = (mx.x - o.t) * inv.t;
+    float x0 = (mn.y - o.x) * inv.x, x1 = (mx.y - o.x) * inv.x;
+    float y0 = (mn.z - o.y) * inv.y, y1 = (mx.z - o.y) * inv.y;
+    float z0 = (mn.w - o.z) * inv.z, z1 = (mx.w - o.z) * inv.z;
+    float en = fmaxf(fmaxf(fmaxf(fmaxf(tmin, fminf(t0, t1)), fminf(x0, x1)), fminf(y0, y1)), fminf(z0, z1));
+    float ex = fminf(fminf(fminf(fminf(tmax, fmaxf(t0, t1)), fmaxf(x0, x1)), fmaxf(y0, y1)), fmaxf(z0, z1));
+    enter = en;
+    return ex >= en;
+}
+
+// Bvh4D::safe_inv_dir (bvh4d.rs:430-441)
+__device__ __forceinline__ float safe_inv(float v) {
+    if (fabsf(v) > 1e-12f) return 1.0f / v;
+    return v >= 0.0f ? 1e12f : -1e12f;
+}
+
+// SimplexGeom4D::intersect_fast (mesh4d.rs:175-211)
+__device__ __forceinline__ bool intersect(const float4 *g, V4 o, V4 d, float tmin, float tmax, float &lam,
+                                          V4 &ev, float bary[4]) {
+    V4 n = f4v(g[1]);
+    float denom = edot(n, d);
+    if (fabsf(denom) < 1e-9f) return false;
+    V4 v0 = f4v(g[0]);
+    float lambda = edot(n, v0 - o) / denom;
+    if (lambda <= tmin || lambda >= tmax) return false;
+    V4 event = o + d * lambda;
+    V4 q = event - v0;
+    const float eps = -1e-5f;
+    float u = edot(f4v(g[2]), q);
+    if (u < eps || u > 1.0f - eps) return false;
+    float v = edot(f4v(g[3]), q);
+    if (v < eps || u + v > 1.0f - eps) return false;
+    float w = edot(f4v(g[4]), q);
+    if (w < eps || u + v + w > 1.0f - eps) return false;
+    lam = lambda;
+    ev = event;
+    bary[0] = 1.0f - u - v - w;
+    bary[1] = u;
+    bary[2] = v;
+    bary[3] = w;
+    return true;
+}
+
+__device__ __forceinline__ void leaf_range(const Scene &S, uint32_t e, uint32_t &first, uint32_t &count) {
+    uint32_t c = (e >> COUNT_SHIFT) & COUNT_MAX;
+    if (c != 0) {
+        first = e & FIRST_MASK;
+        count = c;
+    } else {
+        uint2 b = S.big_leaves[e & FIRST_MASK];
+        first = b.x;
+        count = b.y;
+    }
+}
+
+struct Hit {
+    int32_t slot;
+    float lambda;
+    V4 event;
+    float bary[4];
+};
+
+constexpr int STACK = 64;  // the reference's stack size (bvh4d.rs:454, 537), overflow semantics kept
+
+// Bvh4D::trace (bvh4d.rs:445-520)
+template <bool STATS>
+__device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C) {
+    hit.slot = -1;
+    V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    float root_enter;
+    if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, root_enter)) return false;
+    uint2 stk[STACK];
+    int sp = 1;
+    stk[0] = make_uint2(S.root_entry, __float_as_uint(root_enter));
+    while (sp > 0) {
+        sp -= 1;
+        uint32_t e = stk[sp].x;
+        float t_enter = __uint_as_float(stk[sp].y);
+        if (t_enter >= tmax) continue;
+        if (STATS) C.node_pops++;
+        if (e & LEAF) {
+            uint32_t first, count;
+            leaf_range(S, e, first, count);
+            for (uint32_t i = first; i < first + count; ++i) {
+                float lam, b[4];
+                V4 ev;
+                if (STATS) C.prim_tests++;
+                if (intersect(S.geoms + 5 * (size_t)i, o, d, tmin, tmax, lam, ev, b)) {
+                    tmax = lam;
+                    hit.slot = (int32_t)i;
+                    hit.lambda = lam;
+                    hit.event = ev;
+                    hit.bary[0] = b[0];
+                    hit.bary[1] = b[1];
+                    hit.bary[2] = b[2];
+                    hit.bary[3] = b[3];
+                }
+            }
+        } else {
+            const float4 *bx = S.nbox + 4 * (size_t)e;
+            float4 lmn = bx[0], lmx = bx[1], rmn = bx[2], rmx = bx[3];
+            uint2 info = S.ninfo[e];
+            float tl, tr;
+            if (STATS) C.box_tests += 2;
+            bool hl = box_hit(lmn, lmx, o, inv, tmin, tmax, tl);
+            bool hr = box_hit(rmn, rmx, o, inv, tmin, tmax, tr);
+            if (hl && hr) {
+                if (sp + 2 <= STACK) {
+                    if (tl <= tr) {
+                        stk[sp] = make_uint2(info.y, __float_as_uint(tr));
+                        stk[sp + 1] = make_uint2(info.x, __float_as_uint(tl));
+                    } else {
+                        stk[sp] = make_uint2(info.x, __float_as_uint(tl));
+                        stk[sp + 1] = make_uint2(info.y, __float_as_uint(tr));
+                    }
+                    sp += 2;
+                }
+            } else if (hl) {
+                if (sp < STACK) stk[sp++] = make_uint2(info.x, __float_as_uint(tl));
+            } else if (hr) {
+                if (sp < STACK) stk[sp++] = make_uint2(info.y, __float_as_uint(tr));
+            }
+        }
+    }
+    return hit.slot >= 0;
+}
+
+// Bvh4D::is_occluded (bvh4d.rs:523-578)
+template <bool STATS>
+__device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Counters &C) {
+    V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
+    float en;
+    if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, en)) return false;
+    uint32_t stk[STACK];
+    int sp = 1;
+    stk[0] = S.root_entry;
+    while (sp > 0) {
+        sp -= 1;
+        uint32_t e = stk[sp];
+        if (STATS) C.shadow_node_pops++;
+        if (e & LEAF) {
+            uint32_t first, count;
+            leaf_range(S, e, first, count);
+            for (uint32_t i = first; i < first + count; ++i) {
+                float lam, b[4];
+                V4 ev;
+                if (STATS) C.shadow_prim_tests++;
+                if (intersect(S.geoms + 5 * (size_t)i, o, d, tmin, tmax, lam, ev, b)) return true;
+            }
+        } else {
+            const float4 *bx = S.nbox + 4 * (size_t)e;
+            uint2 info = S.ninfo[e];
+            float t;
+            if (STATS) C.shadow_box_tests += 2;
+            bool hl = box_hit(bx[0], bx[1], o, inv, tmin, tmax, t);
+            bool hr = box_hit(bx[2], bx[3], o, inv, tmin, tmax, t);
+            if (hr && sp < STACK) stk[sp++] = info.y;
+            if (hl && sp < STACK) stk[sp++] = info.x;
+        }
+    }
+    return false;
+}
+
+// HitRecord4D via Simplex4D::interpolate_hit (mesh4d.rs:382-425)
+struct Rec {
+    float lambda;
+    V4 event;
+    float tet_bary[4];
+    V3 tri_bary;
+    float uv_u, uv_v, tau, grad_u, grad_v, uvws_u, uvws_v, decided_at;
+    V4 normal, four_vel;
+    uint32_t material;
+};
+
+__device__ __forceinline__ float blend(float4 a, const float b[4]) {
+    return a.x * b[0] + a.y * b[1] + a.z * b[2] + a.w * b[3];
+}
+
+__device__ __forceinline__ void interpolate(const float4 *A, const Hit &h, Rec &r) {
+    r.lambda = h.lambda;
+    r.event = h.event;
+    for (int i = 0; i < 4; ++i) r.tet_bary[i] = h.bary[i];
+    r.uv_u = blend(A[0], h.bary);
+    r.uv_v = blend(A[1], h.bary);
+    r.tri_bary = V3{blend(A[2], h.bary), blend(A[3], h.bary), blend(A[4], h.bary)};
+    r.tau = blend(A[5], h.bary);
+    r.grad_u = blend(A[6], h.bary);
+    r.grad_v = blend(A[7], h.bary);
+    r.normal = f4v(A[8]);
+    r.four_vel = f4v(A[9]);
+    float4 misc = A[10];
+    r.uvws_u = misc.x;
+    r.uvws_v = misc.y;
+    r.material = __float_as_uint(misc.z);
+    r.decided_at = blend(A[11], h.bary);
+}
+
+struct Mat {
+    V3 base, checker;
+    float emission, grid, wire, cavity;
+    uint32_t pattern;
+};
+__device__ __forceinline__ Mat load_mat(const Scene &S, uint32_t id) {
+    const float4 *m = S.materials + 3 * (size_t)id;
+    float4 a = m[0], b = m[1], c = m[2];
+    return Mat{V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, a.w, b.w, c.x, c.z, __float_as_uint(c.y)};
+}
+
+constexpr float TAU_F = 6.28318530717958647692f;
+constexpr float PI_F = 3.14159265358979323846f;
+
+__device__ V3 hsv_to_rgb(float h_deg, float s, float v) {  // renderer.rs:1081-1100
+    float c = v * s;
+    float h = rem_euclid_(h_deg / 60.0f, 6.0f);
+    float x = c * (1.0f - fabsf(rem_euclid_(h, 2.0f) - 1.0f));
+    float m = v - c;
+    float r, g, b;
+    if (h < 1.0f) { r = c; g = x; b = 0.0f; }
+    else if (h < 2.0f) { r = x; g = c; b = 0.0f; }
+    else if (h < 3.0f) { r = 0.0f; g = c; b = x; }
+    else if (h < 4.0f) { r = 0.0f; g = x; b = c; }
+    else if (h < 5.0f) { r = x; g = 0.0f; b = c; }
+    else { r = c; g = 0.0f; b = x; }
+    return V3{r + m, g + m, b + m};
+}
+
+__device__ V3 sky_gradient(V4 dir) {  // renderer.rs:1056-1062
+    V3 s = normalized(spatial(dir));
+    float horizon = clamp_(s.y * 0.5f + 0.5f, 0.0f, 1.0f);
+    return V3{0.12f, 0.16f, 0.25f} * (1.0f - horizon) + V3{0.04f, 0.06f, 0.13f} * horizon;
+}
+
+__device__ V3 shade_sky(V4 dir) {  // renderer.rs:1064-1079
+    V3 s = normalized(spatial(dir));
+    V3 sky = sky_gradient(dir);
+    float theta = acosf_(clamp_(s.y, -1.0f, 1.0f));
+    float phi = atan2f_(s.z, s.x);
+    float gu = rem_euclid_(phi / PI_F * 12.0f, 1.0f);
+    float gv = rem_euclid_(theta / PI_F * 6.0f, 1.0f);
+    if (gu < 0.025f || gv < 0.025f) sky += V3{0.06f, 0.09f, 0.14f};
+    return sky;
+}
+
+__device__ V3 distance_fog(V3 c, V4 dir, float lambda) {  // renderer.rs:691-695
+    float fog = expf_(-0.008f * lambda);
+    V3 sky = sky_gradient(dir);
+    return c * fog + sky * (1.0f - fog);
+}
+
+__device__ float decision_age(V4 event, V4 origin, V4 u) {  // renderer.rs:319-325
+    V4 d = event - origin;
+    float time_ago = mdot(d, u);
+    float dist = sqrtf(fmax_(time_ago * time_ago + mdot(d, d), 0.0f));
+    return time_ago + dist;
+}
+
+__device__ V3 decision_age_color(float age) {  // renderer.rs:494-507
+    if (age < 0.0f) return V3{0.14f, 0.14f, 0.18f};
+    V3 col = hsv_to_rgb(fmin_(age * 4.0f, 240.0f), 0.82f, 0.88f);
+    if (age < 0.45f) {
+        float f = 1.0f - age / 0.45f;
+        col = col * (1.0f - f * 0.7f) + V3{1.0f, 0.96f, 0.65f} * (f * 0.9f);
+    }
+    if (fabsf(rem_euclid_(age * 0.5f, 1.0f) - 0.5f) > 0.465f) col = V3{0.95f, 0.98f, 1.0f};
+    return col;
+}
+
+__device__ V3 haze(V3 base, V4 event, const specrel_frame &f) {  // renderer.rs:514-541
+    const float SHELL = 0.18f;
+    float inside = -__builtin_inff();
+    for (uint32_t i = 0; i < f.num_haze; ++i) {
+        V4 d = event - lv(f.haze_origin[i]);
+        inside = fmax_(inside, d.t - length(spatial(d)));
+    }
+    if (inside <= -SHELL) return base;
+    V3 col = base;
+    if (fabsf(inside) < SHELL) {
+        float k = 1.0f - fabsf(inside) / SHELL;
+        col += V3{0.45f, 0.92f, 1.0f} * (0.85f * k * k);
+    }
+    if (inside > 0.0f) {
+        float ripple = 0.5f + 0.5f * sinf_(event.x * 3.5f + event.z * 3.5f - event.t * 4.0f) * cosf_(inside * 5.0f);
+        V3 hz = V3{0.32f, 0.18f, 0.62f} * (1.0f - 0.35f * ripple) + V3{0.18f, 0.72f, 0.92f} * (0.35f * ripple);
+        float alpha = 0.55f + 0.25f * fmin_(inside * 0.5f, 1.0f);
+        col = col * (1.0f - alpha) + hz * alpha;
+    }
+    return col;
+}
+
+__device__ bool point_on_hand(float dx, float dy, float angle_cw, float len, float half_w, float back) {
+    float hx = sinf_(angle_cw), hy = cosf_(angle_cw);  // renderer.rs:930-943
+    float along = dx * hx + dy * hy;
+    float perp = fabsf(dx * hy - dy * hx);
+    return along >= back && along <= len && perp <= half_w;
+}
+
+__device__ V3 mini_clock(float mdx, float mdy, float mr, float mini_tau, const Mat &m) {  // renderer.rs:880-927
+    if (mr > 0.080f) return m.base * 1.15f;
+    float angle = rem_euclid_(atan2f_(mdx, mdy), TAU_F);
+    float fast = rem_euclid_((mini_tau / 1.0f) * TAU_F, TAU_F);
+    float main_a = rem_euclid_((mini_tau / 10.0f) * TAU_F, TAU_F);
+    V3 col = V3{0.04f, 0.06f, 0.10f};
+    if (angle <= fast && mr < 0.066f) col += m.base * 0.25f;
+    if (0.062f <= mr && mr <= 0.078f) {
+        float sector = angle / (TAU_F / 10.0f);
+        float dist = fabsf(sector - round_(sector));
+        int32_t tick = f2i_sat(round_(sector)) % 10;
+        bool card = tick == 0 || tick == 5;
+        float hw = card ? 0.13f : 0.07f;
+        if (dist < hw) return tick == 0 ? V3{1.0f, 0.88f, 0.15f} : V3{0.80f, 0.85f, 0.92f};
+    }
+    if (point_on_hand(mdx, mdy, fast, 0.068f, 0.0065f, -0.008f)) return V3{0.25f, 0.98f, 1.0f};
+    if (point_on_hand(mdx, mdy, main_a, 0.052f, 0.0085f, -0.008f)) return V3{1.0f, 0.95f, 0.30f};
+    if (mr < 0.012f) return V3{1.0f, 1.0f, 1.0f};
+    return col;
+}
+
+__device__ bool digital_tau(const Tables &T, float u, float v, float tau) {  // renderer.rs:946-981
+    float rel_u = (u - 0.28f) / (0.72f - 0.28f);
+    float rel_v = (0.18f - v) / (0.18f - 0.06f);
+    uint64_t col = f2u_sat(floorf(rel_u * 35.0f));
+    uint64_t row = f2u_sat(floorf(rel_v * 7.0f));
+    if (col >= 35 || row >= 7) return false;
+    uint32_t ci = (uint32_t)(col / 6), cc = (uint32_t)(col % 6);
+    if (ci >= 6 || cc >= 5) return false;
+    int32_t t_tenths = f2i_sat(round_(tau * 10.0f));
+    uint32_t abs_t = t_tenths < 0 ? 0u - (uint32_t)t_tenths : (uint32_t)t_tenths;
+    uint32_t tenths = abs_t % 10, ones = (abs_t / 10) % 10, tens = (abs_t / 100) % 10;
+    uint32_t g;
+    switch (ci) {
+        case 0: g = t_tenths < 0 ? 11 : 10; break;
+        case 1: g = tens; break;
+        case 2: g = ones; break;
+        case 3: g = 12; break;
+        case 4: g = tenths; break;
+        default: g = 13; break;
+    }
+    return ((T.glyphs[g * 7 + row] >> (4 - cc)) & 1) != 0;
+}
+
+__device__ V3 clock_face(const Tables &T, const Rec &h, const Mat &m) {  // renderer.rs:737-874
+    float grid_scale = fmax_(m.grid, 1e-4f);
+    float inv_grid = 1.0f / grid_scale;
+    float u = rem_euclid_(h.uv_u * grid_scale, 1.0f);
+    float v = rem_euclid_(h.uv_v * grid_scale, 1.0f);
+    // clock_tile_central_proper_time (renderer.rs:704-715)
+    float ig2 = 1.0f / fmax_(grid_scale, 1e-4f);
+    float ctau = h.tau - (((u - 0.5f) * ig2) * h.grad_u + ((v - 0.5f) * ig2) * h.grad_v);
+    const float cx[4] = {0.135f, 0.865f, 0.135f, 0.865f};
+    const float cy[4] = {0.865f, 0.865f, 0.135f, 0.135f};
+    for (int k = 0; k < 4; ++k) {
+        float mdx = u - cx[k], mdy = v - cy[k];
+        float mr = sqrtf(mdx * mdx + mdy * mdy);
+        if (mr <= 0.095f) {
+            float osc = ctau + ((cx[k] - 0.5f) * inv_grid) * h.grad_u + ((cy[k] - 0.5f) * inv_grid) * h.grad_v;
+            float px = (mdx * inv_grid) * h.uvws_u;
+            float py = (mdy * inv_grid) * h.uvws_v;
+            float mr_phys = sqrtf(px * px + py * py);
+            return mini_clock(mdx, mdy, mr, osc - mr_phys, m);
+        }
+    }
+    if ((0.035f <= v && v <= 0.205f) && (0.25f <= u && u <= 0.75f)) {
+        if (!(0.050f <= v && v <= 0.190f) || !(0.265f <= u && u <= 0.735f)) return m.base * 0.9f;
+        if ((0.06f <= v && v <= 0.18f) && (0.28f <= u && u <= 0.72f) && digital_tau(T, u, v, ctau))
+            return V3{1.0f, 0.96f, 0.30f};
+        return V3{0.03f, 0.04f, 0.07f};
+    }
+    float dx = u - 0.50f, dy = v - 0.57f;
+    float r = sqrtf(dx * dx + dy * dy);
+    if (r > 0.330f) {
+        int32_t gu = f2i_sat(floorf(u * 4.0f)), gv = f2i_sat(floorf(v * 4.0f));
+        int32_t sum = (int32_t)((uint32_t)gu + (uint32_t)gv);
+        return (((sum % 2) + 2) % 2) == 0 ? m.base * 0.78f : m.checker * 0.78f;
+    }
+    if (r > 0.292f) return m.base * 1.15f;
+    float dxp = (dx * inv_grid) * h.uvws_u, dyp = (dy * inv_grid) * h.uvws_v;
+    float r_phys = sqrtf(dxp * dxp + dyp * dyp);
+    float dial_tau = ctau - r_phys;
+    float angle = rem_euclid_(atan2f_(dx, dy), TAU_F);
+    float main_a = rem_euclid_((dial_tau / 10.0f) * TAU_F, TAU_F);
+    V3 dial = V3{0.04f, 0.06f, 0.10f};
+    if (angle <= main_a && r < 0.236f) dial += m.base * 0.22f;
+    if (0.220f <= r && r <= 0.286f) {
+        float sector = angle / (TAU_F / 10.0f);
+        float dist = fabsf(sector - round_(sector));
+        int32_t tick = f2i_sat(round_(sector)) % 10;
+        bool card = tick == 0 || tick == 5;
+        float hw = card ? 0.105f : 0.055f;
+        float inner = card ? 0.218f : 0.242f;
+        if (dist < hw && r >= inner) {
+            if (tick == 0) return V3{1.0f, 0.88f, 0.15f};
+            if (card) return V3{1.0f, 1.0f, 1.0f};
+            return V3{0.75f, 0.80f, 0.88f};
+        }
+    }
+    float fast = rem_euclid_((dial_tau / 1.0f) * TAU_F, TAU_F);
+    if (point_on_hand(dx, dy, fast, 0.254f, 0.012f, -0.025f)) return V3{0.25f, 0.98f, 1.0f};
+    if (point_on_hand(dx, dy, main_a, 0.228f, 0.022f, -0.025f)) return V3{1.0f, 0.95f, 0.30f};
+    if (point_on_hand(dx, dy, main_a, 0.236f, 0.030f, -0.025f)) return V3{0.02f, 0.02f, 0.04f};
+    float slow = rem_euclid_((dial_tau / 100.0f) * TAU_F, TAU_F);
+    if (point_on_hand(dx, dy, slow, 0.162f, 0.026f, -0.025f)) return V3{1.0f, 0.35f, 0.25f};
+    if (r < 0.034f) return V3{1.0f, 1.0f, 1.0f};
+    return dial;
+}
+
+__device__ V3 light_clock(float uvu, float uvv, float tau, const Mat &m) {  // renderer.rs:988-1052
+    float u = rem_euclid_(uvu * m.grid, 1.0f);
+    float v = rem_euclid_(uvv * m.grid, 1.0f);
+    if (!(0.12f <= u && u <= 0.88f)) return m.base * 0.65f;
+    if ((0.04f <= v && v <= 0.10f) || (0.90f <= v && v <= 0.96f)) return V3{0.75f, 0.90f, 0.98f};
+    if (!(0.10f <= v && v <= 0.90f)) return V3{0.08f, 0.12f, 0.16f};
+    float period = fmax_(2.0f * m.cavity, 1e-4f);
+    float cycle = rem_euclid_(tau / period, 1.0f);
+    float bounce = cycle < 0.5f ? cycle * 2.0f : (1.0f - cycle) * 2.0f;
+    float photon_v = 0.10f + 0.80f * bounce;
+    float dv = fabsf(v - photon_v), du = fabsf(u - 0.5f);
+    float dist = sqrtf(du * du * 0.7f + dv * dv);
+    V3 col = V3{0.05f, 0.09f, 0.13f};
+    float ruler = rem_euclid_((v - 0.10f) / 0.80f * 8.0f, 1.0f);
+    if ((u < 0.22f || u > 0.78f) && ruler < 0.08f) col += V3{0.20f, 0.35f, 0.35f};
+    if (du < 0.015f) col += V3{0.06f, 0.16f, 0.16f};
+    if (dv < 0.032f) {
+        float a = 1.0f - dv / 0.032f;
+        col += m.base * (0.9f * (a * a));
+    }
+    if (dist < 0.11f) {
+        float a = 1.0f - dist / 0.11f;
+        col += V3{1.0f, 1.0f, 0.85f} * (1.4f * (a * a));
+    }
+    return col;
+}
+
+__device__ V3 albedo(const Tables &T, const Rec &h, const Mat &m) {  // renderer.rs:544-580
+    V3 a;
+    if (m.pattern == SPECREL_PATTERN_CLOCK) {
+        a = clock_face(T, h, m);
+    } else if (m.pattern == SPECREL_PATTERN_LIGHT_CLOCK) {
+        a = light_clock(h.uv_u, h.uv_v, h.tau, m);
+    } else {
+        int32_t gu = f2i_sat(floorf(h.uv_u * m.grid)), gv = f2i_sat(floorf(h.uv_v * m.grid));
+        int32_t sum = (int32_t)((uint32_t)gu + (uint32_t)gv);
+        V3 col = (((sum % 2) + 2) % 2) == 0 ? m.base : m.checker;
+        float mtb = fmin_(fmin_(h.tri_bary.x, h.tri_bary.y), h.tri_bary.z);
+        if (mtb < m.wire) {
+            float et = clamp_(mtb / m.wire, 0.0f, 1.0f);
+            col = V3{0.07f, 0.07f, 0.07f} * (1.0f - et) + col * et;
+        }
+        a = col;
+    }
+    if (m.pattern != SPECREL_PATTERN_CHECKER) {
+        float ut = rem_euclid_(h.uv_u * m.grid, 1.0f), vt = rem_euclid_(h.uv_v * m.grid, 1.0f);
+        float bd = fmin_(fmin_(ut, 1.0f - ut), fmin_(vt, 1.0f - vt));
+        if (bd < 0.025f) a = a * 0.35f;
+    }
+    return a;
+}
+
+// compute_illumination + light_emission_event (renderer.rs:585-653)
+template <bool STATS>
+__device__ float illumination(const Scene &S, const specrel_frame &f, V4 dir, const Rec &h, const Mat &m,
+                              Counters &C) {
+    V3 light = V3{S.light.y, S.light.z, S.light.w};
+    V3 s2l = light - spatial(h.event);
+    float dist = fmax_(length(s2l), 1e-4f);
+    float t_emit;
+    if (f.shadow_mode == SPECREL_SHADOW_CAMERA_SLICE) {
+        V3 beta = beta_from_four_velocity(lv(f.e[0]));
+        t_emit = h.event.t + dot(beta, s2l);
+    } else {
+        t_emit = h.event.t - dist;
+    }
+    V4 to_light = (from_ts(t_emit, light) - h.event) * (1.0f / dist);
+    float cl = fmax_(rest_frame_cos(to_light, h.normal, h.four_vel), 0.0f);
+    float cv = fmax_(-rest_frame_cos(dir, h.normal, h.four_vel), 0.0f);
+    if (m.pattern == SPECREL_PATTERN_CHECKER) {
+        bool in_shadow = false;
+        if (cl > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF) {
+            if (STATS) C.shadow_rays++;
+            in_shadow = occluded<STATS>(S, h.event, to_light, 2e-3f, dist - 2e-3f, C);
+        }
+        float sf = in_shadow ? 0.32f : 1.0f;
+        float atten = 1.0f / (1.0f + 0.006f * dist * dist);
+        return m.emission + 0.30f * cv + 0.80f * sf * cl * atten;
+    }
+    return 0.72f + 0.38f * cv + 0.22f * cl;
+}
+
+// compute_flash_illumination (renderer.rs:658-688)
+template <bool STATS>
+__device__ V3 flashes(const Scene &S, const specrel_frame &f, const Rec &h, Counters &C) {
+    V3 total = V3{0.0f, 0.0f, 0.0f};
+    for (uint32_t i = 0; i < f.num_flashes; ++i) {
+        V4 at = lv(f.flash_at[i]);
+        V3 color = V3{f.flash_color_width[i][0], f.flash_color_width[i][1], f.flash_color_width[i][2]};
+        float width = f.flash_color_width[i][3], intensity = f.flash_intensity[i];
+        float dt = h.event.t - at.t;
+        if (dt <= 0.0f) continue;
+        float dist = fmax_(length(spatial(at) - spatial(h.event)), 1e-4f);
+        float shell_err = fabsf(dt - dist);
+        float half_w = fmax_(0.5f * width, 1e-3f);
+        if (shell_err >= half_w) continue;
+        float k = 1.0f - shell_err / half_w;
+        float pulse = k * k;
+        V4 to_flash = (at - h.event) * (1.0f / dist);
+        float cf = fmax_(rest_frame_cos(to_flash, h.normal, h.four_vel), 0.0f);
+        bool in_shadow = false;
+        if (cf > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF) {
+            if (STATS) C.shadow_rays++;
+            in_shadow = occluded<STATS>(S, h.event, to_flash, 2e-3f, dist - 2e-3f, C);
+        }
+        float sf = in_shadow ? 0.25f : 1.0f;
+        float atten = 1.0f / (1.0f + 0.015f * dist * dist);
+        total += color * (intensity * pulse * atten * (0.25f + 0.75f * sf * cf));
+    }
+    return total;
+}
+
+// shade_hit (renderer.rs:414-489)
+template <bool STATS>
+__device__ V3 shade_hit(const Scene &S, const Tables &T, const specrel_frame &f, V4 dir, const Rec &h,
+                        Counters &C) {
+    Mat m = load_mat(S, h.material);
+    if (f.mode == SPECREL_MODE_SHADED) {
+        V3 a = albedo(T, h, m);
+        float lighting = illumination<STATS>(S, f, dir, h, m, C);
+        V3 lit = a * lighting;
+        if (f.num_flashes > 0) {
+            V3 fl = flashes<STATS>(S, f, h, C);
+            lit += (a * 0.65f + V3{0.35f, 0.35f, 0.35f}) * fl;
+        }
+        return distance_fog(lit, dir, h.lambda);
+    }
+    if (f.mode == SPECREL_MODE_ISOCHRONES) {
+        float phase = rem_euclid_(h.lambda * 0.12f, 1.0f);
+        V3 base = hsv_to_rgb(phase * 300.0f, 0.8f, 0.85f);
+        float tc = fabsf(rem_euclid_(h.event.t, 1.0f) - 0.5f);
+        float uc = fabsf(rem_euclid_(h.tau, 1.0f) - 0.5f);
+        if (tc > 0.45f) return V3{1.0f, 1.0f, 1.0f};
+        if (uc > 0.46f) return V3{0.1f, 0.1f, 0.1f};
+        return base;
+    }
+    if (f.mode == SPECREL_MODE_INSPECTOR) {
+        const float *b = h.tet_bary;
+        float min_b = fmin_(fmin_(fmin_(b[0], b[1]), b[2]), b[3]);
+        float edge = min_b < 0.035f ? 0.12f : 1.0f;
+        float sb = clamp_((h.four_vel.t - 1.0f) * 0.8f, 0.0f, 1.0f);
+        V3 tint = V3{0.25f + 0.65f * b[1] + 0.3f * sb, 0.25f + 0.65f * b[2], 0.35f + 0.55f * b[3] * (1.0f - 0.5f * sb)};
+        return tint * edge;
+    }
+    float age;
+    if (h.decided_at > -1.0e9f * 0.5f) age = f.frontier_tau - h.decided_at;
+    else age = decision_age(h.event, lv(f.frontier_origin), lv(f.e[0]));
+    float cv = fmax_(-rest_frame_cos(dir, h.normal, h.four_vel), 0.0f);
+    return distance_fog(decision_age_color(age) * (0.55f + 0.45f * cv), dir, h.lambda);
+}
+
+// ---------------------------------------------------------------------------------------
+// Stars (star.rs:114-286)
+// ---------------------------------------------------------------------------------------
+__device__ float planck(float lambda_um, float temperature) {  // star.rs:162-166
+    float x = fmin_(14388.0f / (lambda_um * fmax_(temperature, 1.0f)), 80.0f);
+    float l5 = (float)pow((double)lambda_um, 5.0);
+    return 1.0f / (l5 * (expf_(x) - 1.0f));
+}
+__device__ V3 blackbody(float T) {  // star.rs:171-180
+    return V3{planck(0.610f, T) / planck(0.610f, 6504.0f), planck(0.550f, T) / planck(0.550f, 6504.0f),
+              planck(0.465f, T) / planck(0.465f, 6504.0f)};
+}
+
+__device__ float past_the_cone(V4 event, V4 origin, V4 e0, float cone) {  // star.rs:215-223
+    V4 d = event - origin;
+    float tau = -mdot(d, e0);
+    if (cone == 0.0f) return tau;
+    float rho = sqrtf(fmax_(mdot(d, d) + tau * tau, 0.0f));
+    return tau - cone * rho;
+}
+
+__device__ float cone_crossing(V4 ev, V4 u, V4 origin, V4 e0, float cone) {  // star.rs:185-210
+    V4 d0 = ev - origin;
+    if (cone == 0.0f) {
+        float tau0 = -mdot(d0, e0);
+        float gamma = -mdot(u, e0);
+        return -tau0 / gamma;
+    }
+    float a = mdot(d0, d0);
+    float b = mdot(d0, u);
+    float sq = sqrtf(fmax_(b * b + a, 0.0f));
+    if (cone < 0.0f) return b > 0.0f ? -a / (b + sq) : b - sq;
+    return b < 0.0f ? -a / (b - sq) : b + sq;
+}
+
+// worldline_cone_event (star.rs:228-235) with slice::partition_point's binary search.
+__device__ bool cone_event(const float4 *words, uint32_t num_stars, uint32_t star, V4 origin, V4 e0, float cone,
+                           V4 &d_out, V4 &u_out) {
+    float4 hdr = words[star];
+    uint32_t first = __float_as_uint(hdr.z), count = __float_as_uint(hdr.w);
+    const float4 *segs = words + num_stars;
+    if (count == 0) return false;
+    uint32_t size = count, base = 0;
+    while (size > 1) {
+        uint32_t half = size / 2, mid = base + half;
+        bool less = past_the_cone(f4v(segs[3 * (first + mid)]), origin, e0, cone) <= 0.0f;
+        base = less ? mid : base;
+        size -= half;
+    }
+    bool less = past_the_cone(f4v(segs[3 * (first + base)]), origin, e0, cone) <= 0.0f;
+    uint32_t before = base + (less ? 1u : 0u);
+    uint32_t idx = (before > 1 ? before : 1) - 1;
+    if (idx >= count) return false;
+    const float4 *sg = segs + 3 * (first + idx);
+    V4 ev = f4v(sg[0]), u = f4v(sg[1]);
+    float4 lim = sg[2];
+    float s = cone_crossing(ev, u, origin, e0, cone);
+    if (!(s >= lim.x && s <= lim.y)) return false;
+    d_out = (ev - origin) + u * s;
+    u_out = u;
+    return true;
+}
+
+// Per-pixel part of star_light for one star (star.rs:255-283 with its per-view values
+// precomputed): adds to total exactly as the reference's loop body does.
+template <bool STATS>
+__device__ __forceinline__ void star_pixel(const StarPrep &p, V4 n, float spread, float lambda_max, V3 &total,
+                                           Counters &C) {
+    if (!(p.flags & 1u)) return;
+    if (STATS) C.star_tests++;
+    V4 r = f4v(p.r);
+    float along = mdot(r, n);
+    if (along <= 0.0f || along >= lambda_max) return;
+    V4 perp = r - n * along;
+    float d2 = fmax_(mdot(perp, perp), 0.0f);
+    float sigma = 0.6f * (0.0f + spread * along);
+    float cut = 4.0f * sigma;
+    if (d2 >= cut * cut) return;
+    float inv_2s2 = 1.0f / (2.0f * sigma * sigma);
+    float kernel = expf_(-d2 * inv_2s2) * inv_2s2 * 0.318309886183790671538f;
+    total += V3{p.bb_cn.x, p.bb_cn.y, p.bb_cn.z} * (p.bb_cn.w * kernel);
+}
+
+}  // namespace dev

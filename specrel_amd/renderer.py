@@ -1,0 +1,203 @@
+"""`GpuRenderer` for the gfx950 backend: the host-side mirror of src/gpu.rs:432-901.
+
+    gpu = GpuRenderer([scene_a, scene_b])          # GpuRenderer::new (gpu.rs:472)
+    gpu.render(0, camera, cfg, dynamics, fb)        # GpuRenderer::render (gpu.rs:798)
+    gpu.update_scene(0, scene_a)                    # GpuRenderer::update_scene (gpu.rs:704)
+    gpu.adapter_name(); gpu.last_timings            # gpu.rs:734, 451
+
+Like the reference, construction raises (the Rust `Err`) when no usable device exists; the
+caller then chooses a CPU renderer itself. There is no CPU path here: every render runs the
+HIP kernels of libspecrel_hip.so.
+"""
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _ffi as F
+from .scene import build_frame
+
+OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
+OPT_STATS, OPT_STAR_CULL = 1, 2
+
+
+class Timings(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
+                                         "trace_ms", "bloom_ms", "composite_ms")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SpecrelError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"specrel error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+_FP = C.POINTER(C.c_float)
+
+# Every symbol of include/specrel.h with its ctypes signature.
+SIGNATURES = {
+    "specrel_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "specrel_destroy": (None, [_P]),
+    "specrel_last_error": (C.c_char_p, [_P]),
+    "specrel_device_name": (C.c_char_p, [_P]),
+    "specrel_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
+    "specrel_upload_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays)]),
+    "specrel_num_scenes": (C.c_uint32, [_P]),
+    "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP]),
+    "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
+    "specrel_sync": (C.c_int, [_P]),
+    "specrel_download": (C.c_int, [_P, _U32P, _U32P, _FP, _FP]),
+    "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
+    "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
+}
+
+
+def hip_lib_path():
+    return os.path.join(F.LIB_DIR, "libspecrel_hip.so")
+
+
+def hip_lib():
+    """Load libspecrel_hip.so; raises if it was not built (no silent fallback)."""
+    global _lib
+    if _lib is None:
+        path = hip_lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: build it with `python -m specrel_amd.build`")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, t):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+class Context:
+    """Thin owner of a specrel_ctx (one device)."""
+
+    def __init__(self, device=0):
+        lib = hip_lib()
+        h = C.c_void_p()
+        rc = lib.specrel_create(device, C.byref(h))
+        if rc != OK:
+            raise SpecrelError(rc, lib.specrel_last_error(None).decode())
+        self._h = h
+        self.lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.lib.specrel_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != OK:
+            raise SpecrelError(rc, self.lib.specrel_last_error(self._h).decode())
+        return rc
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_name(self):
+        return self.lib.specrel_device_name(self._h).decode()
+
+    def set_option(self, opt, value):
+        self._check(self.lib.specrel_set_option(self._h, opt, int(value)))
+
+    def upload(self, idx, scene):
+        self._check(self.lib.specrel_upload_scene(self._h, idx, C.byref(scene.arrays)))
+
+    def render_frame(self, idx, frame, want_hits=True):
+        w, h = frame.size[0], frame.size[1]
+        n = w * h
+        rgb = np.zeros(n, np.uint32)
+        hit = np.zeros(n, np.uint32) if want_hits else None
+        lam = np.zeros(n, np.float32) if want_hits else None
+        tau = np.zeros(n, np.float32) if want_hits else None
+        self._check(self.lib.specrel_render(self._h, idx, C.byref(frame), _ptr(rgb, _U32P), _ptr(hit, _U32P),
+                                            _ptr(lam, _FP), _ptr(tau, _FP)))
+        out = {"rgb": rgb.reshape(h, w)}
+        if want_hits:
+            out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
+        return out
+
+    def render_resident(self, idx, frame, want_hits=False):
+        self._check(self.lib.specrel_render_resident(self._h, idx, C.byref(frame), int(want_hits)))
+
+    def sync(self):
+        self._check(self.lib.specrel_sync(self._h))
+
+    def download(self, w, h, want_hits=False):
+        n = w * h
+        rgb = np.zeros(n, np.uint32)
+        hit = np.zeros(n, np.uint32) if want_hits else None
+        lam = np.zeros(n, np.float32) if want_hits else None
+        tau = np.zeros(n, np.float32) if want_hits else None
+        self._check(self.lib.specrel_download(self._h, _ptr(rgb, _U32P), _ptr(hit, _U32P), _ptr(lam, _FP),
+                                              _ptr(tau, _FP)))
+        out = {"rgb": rgb.reshape(h, w)}
+        if want_hits:
+            out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
+        return out
+
+    def timings(self):
+        t = Timings()
+        self._check(self.lib.specrel_last_timings(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def stats(self):
+        s = F.Stats()
+        self._check(self.lib.specrel_last_stats(self._h, C.byref(s)))
+        return {n: getattr(s, n) for n, _ in F.Stats._fields_ if n != "reserved"}
+
+
+@dataclass
+class GpuTimings:  # gpu.rs:411-420
+    submit_ms: float = 0.0
+    wait_ms: float = 0.0
+    kernel_ms: float = 0.0
+    readback_ms: float = 0.0
+
+
+class GpuRenderer:
+    """Drop-in for the reference's GpuRenderer (gpu.rs:433-452)."""
+
+    def __init__(self, scenes, device=0):
+        try:
+            self.ctx = Context(device)
+        except SpecrelError as e:
+            raise RuntimeError(f"no GPU adapter: {e}") from e
+        for i, s in enumerate(scenes):
+            self.ctx.upload(i, s)
+        self.last_timings = GpuTimings()
+
+    def adapter_name(self):
+        return self.ctx.device_name()
+
+    def update_scene(self, scene_idx, scene):
+        # The reference patches dirty ranges when it can (gpu.rs:704-731); a full re-upload
+        # is its fallback and gives the same device contents.
+        self.ctx.upload(scene_idx, scene)
+
+    def render(self, scene_idx, camera, cfg, dynamics, framebuffer):
+        if framebuffer.size != cfg.width * cfg.height:
+            raise ValueError("framebuffer size mismatch")
+        frame = build_frame(camera, cfg, dynamics)
+        out = self.ctx.render_frame(scene_idx, frame, want_hits=False)
+        framebuffer.reshape(-1)[:] = out["rgb"].reshape(-1)
+        t = self.ctx.timings()
+        self.last_timings = GpuTimings(t["submit_ms"], t["wait_ms"], t["kernel_ms"], t["readback_ms"])

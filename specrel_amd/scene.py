@@ -1,0 +1,303 @@
+"""Host-side scene, camera and frame objects (the caller side of the renderer boundary).
+
+Mirrors the reference's Rust types that feed `GpuRenderer`: `Scene4D` (src/scene.rs:150-161),
+`Camera4D` (src/renderer.rs:45-65), `RenderConfig` (renderer.rs:160-168), `FrameDynamics`
+(renderer.rs:205-218) and `Tetrad` (src/minkowski.rs:399-609). The heavy lifting (meshes,
+worldlines, extrusion, SAH BVH, packing) runs in libspecrel_scene.so.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _ffi as F
+
+RelativisticShaded, LookbackIsochrones, Simplex4DInspector, DecisionAge = range(4)
+ShadowOff, ShadowLorentzInvariant, ShadowCameraSlice = range(3)
+PastLightCone, CameraSimultaneity, Frontier, ProphecySlice = range(4)
+PROPHECY_DELTA_DEFAULT = 3.5
+
+
+def _check(rc):
+    if rc < 0:
+        raise RuntimeError(F.scene_lib().specrel_scene_last_error().decode())
+    return rc
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Tetrad:
+    """Orthonormal Lorentz frame e[0..4] (minkowski.rs:399)."""
+
+    def __init__(self, e=None):
+        self.e = np.zeros((4, 4), np.float32)
+        if e is None:
+            F.scene_lib().specrel_tetrad_rest(_fptr(self.e))
+        else:
+            self.e[:] = np.asarray(e, np.float32).reshape(4, 4)
+
+    @classmethod
+    def from_beta(cls, beta):
+        t = cls()
+        b = np.asarray(beta, np.float32)
+        F.scene_lib().specrel_tetrad_from_beta(_fptr(t.e), _fptr(b))
+        return t
+
+    def boost_local(self, d_rapidity):
+        r = np.asarray(d_rapidity, np.float32)
+        F.scene_lib().specrel_tetrad_boost_local(_fptr(self.e), _fptr(r))
+        return self
+
+    def rotate_local(self, yaw, pitch, roll):
+        F.scene_lib().specrel_tetrad_rotate_local(_fptr(self.e), yaw, pitch, roll)
+        return self
+
+    def boost_to(self, u):
+        uu = np.asarray(u, np.float32)
+        F.scene_lib().specrel_tetrad_boost_to(_fptr(self.e), _fptr(uu))
+        return self
+
+    def copy(self):
+        return Tetrad(self.e.copy())
+
+
+@dataclass
+class Camera4D:
+    position: Sequence[float] = (0.0, 0.0, 0.0, 0.0)
+    tetrad: Tetrad = field(default_factory=Tetrad)
+    proper_time: float = 0.0
+    fov_y_rad: float = float(np.float32(65.0) * np.float32(np.float32(np.pi) / np.float32(180.0)))
+
+    @classmethod
+    def new(cls, position, fov_deg=65.0):
+        # f32::to_radians multiplies by PI/180 rounded to f32.
+        rad = np.float32(fov_deg) * (np.float32(np.pi) / np.float32(180.0))
+        return cls(tuple(position), Tetrad(), 0.0, float(np.float32(rad)))
+
+    def to_c(self):
+        c = F.Camera()
+        for i in range(4):
+            c.position[i] = self.position[i]
+        for k in range(4):
+            for i in range(4):
+                c.e[k][i] = float(self.tetrad.e[k, i])
+        c.proper_time = self.proper_time
+        c.fov_y_rad = self.fov_y_rad
+        return c
+
+
+@dataclass
+class RenderConfig:
+    width: int
+    height: int
+    mode: int = RelativisticShaded
+    shadow_mode: int = ShadowLorentzInvariant
+    slice_mode: int = PastLightCone
+    prophecy_delta: float = PROPHECY_DELTA_DEFAULT
+
+
+@dataclass
+class FlashEvent:
+    at: Sequence[float]
+    width: float
+    intensity: float
+    color: Sequence[float]
+
+
+@dataclass
+class FrameDynamics:
+    frontier: Optional[tuple] = None  # (origin[4], proper_time)
+    flashes: List[FlashEvent] = field(default_factory=list)
+    frontier_lag: float = 0.0
+    remote_origins: List[Sequence[float]] = field(default_factory=list)
+
+
+def build_frame(camera: Camera4D, cfg: RenderConfig, dyn: Optional[FrameDynamics] = None) -> F.Frame:
+    """The per-view block (GpuUniforms::build, gpu.rs:147-212)."""
+    cc = camera.to_c()
+    rc = F.RenderConfigC(cfg.width, cfg.height, cfg.mode, cfg.shadow_mode, cfg.slice_mode, cfg.prophecy_delta)
+    d = F.DynamicsC()
+    keep = []
+    if dyn is not None:
+        if dyn.frontier is not None:
+            d.has_frontier = 1
+            for i in range(4):
+                d.frontier_origin[i] = dyn.frontier[0][i]
+            d.frontier_proper_time = dyn.frontier[1]
+        d.frontier_lag = dyn.frontier_lag
+        if dyn.flashes:
+            arr = (F.FlashC * len(dyn.flashes))()
+            for i, fl in enumerate(dyn.flashes):
+                for k in range(4):
+                    arr[i].at[k] = fl.at[k]
+                arr[i].width = fl.width
+                arr[i].intensity = fl.intensity
+                for k in range(3):
+                    arr[i].color[k] = fl.color[k]
+            keep.append(arr)
+            d.num_flashes = len(dyn.flashes)
+            d.flashes = C.cast(arr, C.POINTER(F.FlashC))
+        if dyn.remote_origins:
+            ro = (F.F4 * len(dyn.remote_origins))()
+            for i, o in enumerate(dyn.remote_origins):
+                for k in range(4):
+                    ro[i][k] = o[k]
+            keep.append(ro)
+            d.num_remote = len(dyn.remote_origins)
+            d.remote_origins = C.cast(ro, C.POINTER(F.F4))
+    out = F.Frame()
+    _check(F.scene_lib().specrel_frame_build(C.byref(cc), C.byref(rc), C.byref(d), C.byref(out)))
+    return out
+
+
+class Scene:
+    """A packed scene (PackedSceneGpuData, gpu.rs:232-244) owned by the scene library."""
+
+    def __init__(self, handle, name=""):
+        self._h = C.c_void_p(handle)
+        self.name = name
+        self.arrays = F.SceneArrays()
+        F.scene_lib().specrel_scene_get_arrays(self._h, C.byref(self.arrays))
+
+    @classmethod
+    def generate(cls, name, params: Optional[F.GenParams] = None, **overrides):
+        lib = F.scene_lib()
+        if params is None and overrides:
+            params = gen_params(name, **overrides)
+        h = C.c_void_p()
+        _check(lib.specrel_scene_generate(name.encode(), C.byref(params) if params is not None else None,
+                                          C.byref(h)))
+        s = cls(h.value, name)
+        s.params = params
+        return s
+
+    def default_camera(self):
+        return default_camera(self.name, getattr(self, "params", None))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            F.scene_lib().specrel_scene_free(self._h)
+            self._h = C.c_void_p()
+
+    @property
+    def num_prims(self):
+        return self.arrays.num_prims
+
+    @property
+    def num_nodes(self):
+        return self.arrays.num_nodes
+
+    @property
+    def num_stars(self):
+        return self.arrays.num_stars
+
+    def _view(self, ptr, n, dtype, width):
+        if n == 0:
+            return np.zeros((0, width), dtype)
+        buf = (C.c_char * (n * width * np.dtype(dtype).itemsize)).from_address(ptr)
+        return np.frombuffer(buf, dtype).reshape(n, width)
+
+    def nodes(self):
+        return self._view(self.arrays.nodes, self.num_nodes, np.float32, 12)
+
+    def geoms(self):
+        return self._view(self.arrays.geoms, self.num_prims, np.float32, 20)
+
+    def attrs(self):
+        return self._view(self.arrays.attrs, self.num_prims, np.float32, 48)
+
+    def order(self):
+        return self._view(self.arrays.order, self.num_prims, np.uint32, 1)[:, 0]
+
+    def star_words(self):
+        return self._view(self.arrays.star_words, self.arrays.num_star_words, np.float32, 4)
+
+    def star_params(self):
+        out = np.zeros((self.num_stars, 2), np.float32)
+        if self.num_stars:
+            F.scene_lib().specrel_scene_star_params(self._h, _fptr(out), self.num_stars)
+        return out
+
+
+def gen_params(name, **overrides) -> F.GenParams:
+    p = F.GenParams()
+    _check(F.scene_lib().specrel_gen_defaults(name.encode(), C.byref(p)))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def default_camera(name, params=None) -> Camera4D:
+    c = F.Camera()
+    _check(F.scene_lib().specrel_scene_default_camera(name.encode(), C.byref(params) if params is not None else None,
+                                                      C.byref(c)))
+    t = Tetrad(np.array([[c.e[k][i] for i in range(4)] for k in range(4)], np.float32))
+    return Camera4D(tuple(c.position[i] for i in range(4)), t, c.proper_time, c.fov_y_rad)
+
+
+class Builder:
+    """Incremental scene construction (worldtubes, raw simplices, stars)."""
+
+    def __init__(self):
+        self._b = C.c_void_p(F.scene_lib().specrel_builder_new())
+
+    def __del__(self):
+        if getattr(self, "_b", None) and self._b.value:
+            F.scene_lib().specrel_builder_free(self._b)
+
+    def add_tube(self, mesh_kind, mesh_params, wl_kind, wl_params, material, sync_t=None, decided_at=None):
+        mp = np.asarray(mesh_params, np.float32)
+        wp = np.asarray(wl_params, np.float64)
+        dec = None if decided_at is None else np.asarray(decided_at, np.float32)
+        _check(F.scene_lib().specrel_builder_add_tube(
+            self._b, mesh_kind, _fptr(mp), wl_kind, wp.ctypes.data_as(C.POINTER(C.c_double)),
+            0.0 if sync_t is None else sync_t, 0 if sync_t is None else 1, material,
+            None if dec is None else _fptr(dec), 0 if dec is None else len(dec)))
+        return self
+
+    def add_simplex(self, verts, uv=None, tri_bary=None, tau=None, grad=None, uvws=(1.0, 1.0), normal=(0, 0, 0, 1),
+                    four_vel=(1, 0, 0, 0), decided_at=None, material=0):
+        z = lambda x, n: np.zeros(n, np.float32) if x is None else np.asarray(x, np.float32).ravel()
+        args = [z(verts, 16), z(uv, 8), z(tri_bary, 12), z(tau, 4), z(grad, 8), z(uvws, 2), z(normal, 4),
+                z(four_vel, 4)]
+        dec = None if decided_at is None else np.asarray(decided_at, np.float32)
+        rc = F.scene_lib().specrel_builder_add_simplex(self._b, *[_fptr(a) for a in args],
+                                                       None if dec is None else _fptr(dec), material)
+        return _check(rc) == 1
+
+    def add_star(self, pos_or_events, temperature, intensity, beta=(0, 0, 0)):
+        ev = np.asarray(pos_or_events, np.float32)
+        n = 1 if ev.ndim == 1 else ev.shape[0]
+        b = np.asarray(beta, np.float32)
+        _check(F.scene_lib().specrel_builder_add_star(self._b, _fptr(ev), n, _fptr(b), temperature, intensity))
+        return self
+
+    def set_light(self, x, y, z):
+        F.scene_lib().specrel_builder_set_light(self._b, x, y, z)
+        return self
+
+    def __len__(self):
+        return F.scene_lib().specrel_builder_num_simplices(self._b)
+
+    def finish(self, name="custom") -> Scene:
+        h = C.c_void_p()
+        _check(F.scene_lib().specrel_builder_finish(self._b, C.byref(h)))
+        return Scene(h.value, name)
+
+
+def worldline(kind, params, cap=4096):
+    out = (F.WlSample * cap)()
+    n = C.c_uint32()
+    wp = np.asarray(params, np.float64)
+    _check(F.scene_lib().specrel_worldline_sample(kind, wp.ctypes.data_as(C.POINTER(C.c_double)), out, cap,
+                                                  C.byref(n)))
+    return out, n.value
+
+
+def build_tables():
+    out = (C.c_uint32 * (4096 + 98))()
+    F.scene_lib().specrel_build_tables(out)
+    return np.frombuffer(out, np.uint32).copy()

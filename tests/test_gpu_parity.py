@@ -1,0 +1,115 @@
+"""GPU parity: the HIP renderer against the oracle (CPU restatement of render_frame) on the same
+packed scene and view. RGB8 and hit index must match bit for bit; lambda and tau within 1e-6
+relative (they come out identical). Mirrors the reference's GPU-vs-CPU image tests
+(src/gpu.rs:1015-1311) with zero tolerance instead of "<0.5% of pixels off by >4 levels"."""
+import numpy as np
+import pytest
+
+from specrel_amd import scene as S
+from tests.helpers import assert_same, frame_for, render_both
+
+pytestmark = pytest.mark.gpu
+
+# gpu_matches_cpu_reference's 9 (mode, shadow, view) configurations (gpu.rs:1268-1278).
+CONFIGS = [
+    (S.RelativisticShaded, S.ShadowLorentzInvariant, S.PastLightCone),
+    (S.RelativisticShaded, S.ShadowCameraSlice, S.CameraSimultaneity),
+    (S.LookbackIsochrones, S.ShadowOff, S.PastLightCone),
+    (S.Simplex4DInspector, S.ShadowOff, S.PastLightCone),
+    (S.DecisionAge, S.ShadowOff, S.PastLightCone),
+    (S.DecisionAge, S.ShadowOff, S.ProphecySlice),
+    (S.RelativisticShaded, S.ShadowLorentzInvariant, S.Frontier),
+    (S.RelativisticShaded, S.ShadowCameraSlice, S.ProphecySlice),
+    (S.LookbackIsochrones, S.ShadowOff, S.Frontier),
+]
+
+
+@pytest.fixture(scope="module")
+def twin():
+    return S.Scene.generate("twin")
+
+
+def test_cube_config_bit_exact(gpu_ctx):
+    sc = S.Scene.generate("cube")
+    fr = frame_for("cube", 320, 200)
+    g, o = render_both(gpu_ctx, sc, fr)
+    assert (o["hit"] != 0xFFFFFFFF).sum() > 500
+    assert_same(g, o, "cube 320x200")
+
+
+@pytest.mark.parametrize("mode,shadow,view", CONFIGS)
+def test_twin_scene_all_modes(gpu_ctx, twin, mode, shadow, view):
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.2, 0.0, 0.6))
+    cam.tetrad.rotate_local(0.15, -0.05, 0.0)
+    fr = S.build_frame(cam, S.RenderConfig(192, 120, mode, shadow, view))
+    g, o = render_both(gpu_ctx, twin, fr)
+    assert len(np.unique(o["rgb"])) > 50
+    assert_same(g, o, f"twin {mode}/{shadow}/{view}")
+
+
+def test_flashes(gpu_ctx, twin):
+    # gpu_matches_cpu_with_flashes (gpu.rs:1015-1082)
+    cam = twin.default_camera()
+    p = cam.position
+    at = lambda r: (p[0] - 2.0 * r, p[1], p[2], p[3])
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent(at(3.0), 1.2, 2.0, (1.0, 0.3, 0.2)),
+                                   S.FlashEvent(at(6.0), 1.5, 3.0, (0.2, 0.6, 1.0))])
+    for shadow in (S.ShadowLorentzInvariant, S.ShadowOff):
+        fr = S.build_frame(cam, S.RenderConfig(192, 120, S.RelativisticShaded, shadow, S.PastLightCone), dyn)
+        g, o = render_both(gpu_ctx, twin, fr)
+        assert_same(g, o, f"flashes shadow={shadow}")
+
+
+def test_decision_age_with_frontier(gpu_ctx):
+    # gpu_matches_cpu_decision_age_with_frontier (gpu.rs:1147-1230)
+    sc = S.Scene.generate("decision_age")
+    cam = S.Camera4D.new((0.0, 0.0, 0.0, -4.0), 65.0)
+    dyn = S.FrameDynamics(frontier=((-0.5, 0.2, 0.0, -4.0), 24.0),
+                          remote_origins=[(-3.0, 2.0, 0.0, 2.0)])
+    for view in (S.PastLightCone, S.CameraSimultaneity, S.ProphecySlice):
+        fr = S.build_frame(cam, S.RenderConfig(192, 120, S.DecisionAge, S.ShadowOff, view), dyn)
+        g, o = render_both(gpu_ctx, sc, fr)
+        assert len(np.unique(o["rgb"])) > 50
+        assert_same(g, o, f"decision age view={view}")
+
+
+def test_large_scene_reduced(gpu_ctx):
+    p = S.gen_params("large", n_objects=10, slices=24, n_stars=800)
+    sc = S.Scene.generate("large", p)
+    cam = S.default_camera("large", p)
+    fr = S.build_frame(cam, S.RenderConfig(320, 200))
+    g, o = render_both(gpu_ctx, sc, fr)
+    assert_same(g, o, "large reduced")
+
+
+def test_star_culling_is_exact(gpu_ctx, twin):
+    from specrel_amd.renderer import OPT_STAR_CULL
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.0, 0.3, 0.9))
+    fr = S.build_frame(cam, S.RenderConfig(256, 160))
+    gpu_ctx.upload(0, twin)
+    a = gpu_ctx.render_frame(0, fr)
+    gpu_ctx.set_option(OPT_STAR_CULL, 0)
+    try:
+        b = gpu_ctx.render_frame(0, fr)
+    finally:
+        gpu_ctx.set_option(OPT_STAR_CULL, 1)
+    assert np.array_equal(a["rgb"], b["rgb"])
+
+
+def test_traversal_counters_match_oracle(gpu_ctx, twin):
+    from oracle import pyoracle as O
+    from specrel_amd.renderer import OPT_STATS
+    fr = S.build_frame(twin.default_camera(), S.RenderConfig(160, 100))
+    gpu_ctx.upload(0, twin)
+    gpu_ctx.set_option(OPT_STATS, 1)
+    try:
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+    oc = O.render(twin, fr, counts=True)["counts"]
+    for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops",
+              "shadow_box_tests", "shadow_prim_tests"):
+        assert st[k] == oc[k], (k, st[k], oc[k])
