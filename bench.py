@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Benchmark of the SpecRel per-pixel hot path on one MI355X.
+
+Workload (the headline single-GPU configuration): the large scene -- 128 seeded bodies
+(320-triangle icospheres, radius U[0.5,3], centres U[-50,50]^3, |v| U[0,0.9c), a quarter with
+constant proper acceleration), 128 time slices each = 15,728,640 tetrahedra, plus 10,000 stars
+on a sphere of radius 500 (T log-uniform 3000-30000 K) -- seen by an observer at the origin
+event boosted to rapidity 1.0, 1280x800 past-light-cone rays, null shadows, f32.
+
+A step is one full frame: star pass, fused trace/shade/star kernel, bloom pyramid, tone map to
+RGB8, with every input already resident in HBM and the image left there (no readback).
+value = primary rays per second of the whole job = W*H*steps / wall time, in Mrays/s.
+
+Prints ONE JSON line. Extra legs: `roofline` for the dominant kernel (k_render) from the
+instrumented traversal counters and HIP-event durations; `cpu_baseline` from the CPU oracle
+(a restatement of the reference renderer) timed on this host's cores on a bounded row sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s at 1280x800 (15.7M tets) vs CPU ref; batched views/s; fwd+adjoint ms"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(st, w, h, n_stars):
+    """Algorithmic bytes of one k_render launch, priced with the reference's record sizes:
+    48 B per node pop (BvhNode4D/GpuNode), 32 B per child box tested, 80 B per hot record
+    tested (SimplexGeom4D), 192 + 48 B per hit (cold attributes + material), the same for the
+    shadow queries, 64 B per star per view, and 16 B per pixel of HDR written."""
+    return (48 * st["node_pops"] + 32 * st["box_tests"] + 80 * st["prim_tests"] + 240 * st["hits"]
+            + 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"] + 80 * st["shadow_prim_tests"]
+            + 64 * n_stars + 16 * w * h)
+
+
+def cpu_baseline(scene, frame, w, h, rows):
+    from oracle import pyoracle as O  # the CPU restatement of the reference renderer
+    import numpy as np
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    step = max(1, h // rows)
+    O.render_rows(scene, frame, 0, step, 2, nthreads=cores)  # warm-up
+    t0 = time.perf_counter()
+    O.render_rows(scene, frame, step // 2, step, rows, nthreads=cores)
+    t_rows = time.perf_counter() - t0
+    hdr = np.full((h, w, 3), 0.25, np.float32)
+    t0 = time.perf_counter()
+    O.pack(O.bloom(hdr))
+    t_post = time.perf_counter() - t0
+    frame_s = t_rows * (h / rows) + t_post
+    return {
+        "value": round(w * h / frame_s / 1e6, 5),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{rows} of {h} rows (every {step}th, {rows * w} rays incl. the full star loop and shadow rays) "
+                  f"in {t_rows:.2f} s, plus bloom+tone map of a full {w}x{h} frame in {t_post:.3f} s; "
+                  f"extrapolated frame time {frame_s:.2f} s",
+    }
+
+
+def copy_probe():
+    """A plain copy kernel of known size: 2^28 floats (1 GiB read + 1 GiB written per launch).
+    Its FETCH_SIZE / WRITE_SIZE readings scale the counters of k_render to bytes."""
+    import torch
+
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(4):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy probe: 4 launches of a 1 GiB device-to-device copy kernel")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=800)
+    ap.add_argument("--objects", type=int, default=128)
+    ap.add_argument("--slices", type=int, default=128)
+    ap.add_argument("--stars", type=int, default=10000)
+    ap.add_argument("--cpu-rows", type=int, default=32)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by image tiles or "
+              f"views, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from specrel_amd import scene as S
+    from specrel_amd.renderer import OPT_STATS, Context
+
+    w, h = args.width, args.height
+    p = S.gen_params("large", n_objects=args.objects, slices=args.slices, n_stars=args.stars)
+    t0 = time.perf_counter()
+    log("building the large scene (meshes, worldlines, extrusion, SAH BVH)")
+    scene = S.Scene.generate("large", p)
+    log(f"scene: {scene.num_prims} tets, {scene.num_nodes} nodes, {scene.num_stars} stars "
+        f"in {time.perf_counter() - t0:.1f} s")
+    cam = S.default_camera("large", p)
+    frame = S.build_frame(cam, S.RenderConfig(w, h, S.RelativisticShaded, S.ShadowLorentzInvariant,
+                                               S.PastLightCone))
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ctx.upload(0, scene)
+    log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")
+
+    # Instrumented pass (untimed): traversal counters for the algorithmic bytes.
+    ctx.set_option(OPT_STATS, 1)
+    ctx.render_resident(0, frame, True)
+    ctx.sync()
+    st = ctx.stats()
+    ctx.set_option(OPT_STATS, 0)
+    log(f"counters: {st}")
+
+    for _ in range(args.warmup):
+        ctx.render_resident(0, frame)
+        ctx.sync()
+    parts = {k: 0.0 for k in ("star_prep_ms", "trace_ms", "bloom_ms", "composite_ms", "kernel_ms")}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ctx.render_resident(0, frame)
+        ctx.sync()
+        t = ctx.timings()
+        for k in parts:
+            parts[k] += t[k]
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    parts = {k: v / args.steps for k, v in parts.items()}
+    value = w * h * args.steps / wall / 1e6
+    log(f"{args.steps} steps in {wall:.3f} s: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s; kernels {parts}")
+
+    # End to end through the host boundary (the reference's render() includes the readback).
+    t0 = time.perf_counter()
+    ctx.render_frame(0, frame, want_hits=False)
+    e2e_ms = (time.perf_counter() - t0) * 1e3
+
+    bytes_per_launch = algorithmic_bytes(st, w, h, scene.num_stars)
+    achieved = bytes_per_launch / (parts["trace_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_render_large_1280x800_traffic.txt")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_render", "kernel_ms": round(parts["trace_ms"], 4),
+                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline: oracle on a row sample")
+        cpu = cpu_baseline(scene, frame, w, h, args.cpu_rows)
+        log(f"cpu baseline {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"large single-view forward, {w}x{h}, {scene.num_prims} tets "
+                               f"({args.objects} icospheres x {args.slices} slices), {scene.num_stars} stars, "
+                               f"rapidity-1 observer, null shadows",
+                   "width": w, "height": h, "tets": scene.num_prims, "nodes": scene.num_nodes,
+                   "stars": scene.num_stars},
+        "kernel_ms_per_step": {k: round(v, 4) for k, v in parts.items()},
+        "e2e_ms_with_readback": round(e2e_ms, 3),
+        "counters": st,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
