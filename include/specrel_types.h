@@ -144,6 +144,8 @@ static_assert(offsetof(specrel_attr, misc) == 160, "GpuAttr.misc offset");
 static_assert(offsetof(specrel_attr, decided_at) == 176, "GpuAttr.decided_at offset");
 static_assert(offsetof(specrel_frame, haze_origin) == 144, "frame layout");
 static_assert(sizeof(specrel_frame) == 800, "frame layout");
+static_assert(sizeof(specrel_scene_arrays) == 88 && offsetof(specrel_scene_arrays, light_pos) == 72, "arrays");
+static_assert(sizeof(specrel_stats) == 128, "stats layout");
 #else
 _Static_assert(sizeof(specrel_node) == 48, "GpuNode is 48 B");
 _Static_assert(sizeof(specrel_geom) == 80, "GpuGeom is 80 B");

@@ -1,0 +1,61 @@
+"""The C-ABI libraries load and export every symbol their headers declare; layouts match the
+ctypes mirrors; argument errors come back as codes. No compute without a GPU. CPU only."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from specrel_amd import _ffi as F
+from specrel_amd import renderer as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    with open(os.path.join(ROOT, "include", header)) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(specrel_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,loader", [("specrel.h", R.hip_lib), ("specrel_scene.h", F.scene_lib)])
+def test_library_exports_every_declared_symbol(header, loader):
+    lib = loader()
+    names = declared(header)
+    assert len(names) > 8
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_the_header():
+    assert set(declared("specrel.h")) == set(R.SIGNATURES)
+
+
+def test_layouts_match_the_header():
+    assert C.sizeof(F.Frame) == 800 and F.Frame.haze_origin.offset == 144
+    assert F.Attr.misc.offset == 160 and F.Attr.decided_at.offset == 176
+    assert C.sizeof(F.SceneArrays) == 88 and F.SceneArrays.light_pos.offset == 72
+    assert C.sizeof(F.Stats) == 128
+    assert C.sizeof(R.Timings) == 32
+
+
+def test_argument_errors_are_codes_not_aborts():
+    lib = R.hip_lib()
+    assert lib.specrel_render(None, 0, None, None, None, None, None) == R.EINVAL
+    assert lib.specrel_upload_scene(None, 0, None) == R.EINVAL
+    assert lib.specrel_sync(None) == R.EINVAL
+    assert lib.specrel_set_option(None, 1, 1) == R.EINVAL
+
+
+def test_create_reports_missing_device():
+    lib = R.hip_lib()
+    h = C.c_void_p()
+    rc = lib.specrel_create(0, C.byref(h))
+    if rc == R.OK:  # a gfx950 device is present: nothing more to check here
+        lib.specrel_destroy(h)
+        return
+    assert rc == R.ENODEV
+    assert lib.specrel_last_error(None)
+    with pytest.raises(RuntimeError):
+        R.GpuRenderer([])
