@@ -33,13 +33,17 @@ def log(msg):
 
 
 def algorithmic_bytes(st, w, h, n_stars):
-    """Algorithmic bytes of one k_render launch, priced with the reference's record sizes:
-    48 B per node pop (BvhNode4D/GpuNode), 32 B per child box tested, 80 B per hot record
-    tested (SimplexGeom4D), 192 + 48 B per hit (cold attributes + material), the same for the
-    shadow queries, 64 B per star per view, and 16 B per pixel of HDR written."""
-    return (48 * st["node_pops"] + 32 * st["box_tests"] + 80 * st["prim_tests"] + 240 * st["hits"]
-            + 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"] + 80 * st["shadow_prim_tests"]
-            + 64 * n_stars + 16 * w * h)
+    """Algorithmic bytes per launch of the two per-pixel kernels, priced with the reference's
+    record sizes: 48 B per node pop (BvhNode4D / GpuNode), 32 B per child box tested, 80 B per
+    hot record tested (SimplexGeom4D), 192 + 48 B per hit (cold attributes + material).
+      k_trace: the primary traversal, plus 4 B per pixel for the hit slot written.
+      k_shade: 4 B per pixel of hit slot read, per hit the 80 B record intersected again plus
+               192 + 48 B, the shadow queries priced like the primary ones, 64 B per star of the
+               view, and 16 B per pixel of HDR written."""
+    trace = 48 * st["node_pops"] + 32 * st["box_tests"] + 80 * st["prim_tests"] + 4 * w * h
+    shade = (4 * w * h + (80 + 240) * st["hits"] + 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"]
+             + 80 * st["shadow_prim_tests"] + 64 * n_stars + 16 * w * h)
+    return {"k_trace": trace, "k_shade": shade}
 
 
 def cpu_baseline(scene, frame, w, h, rows):
@@ -96,6 +100,10 @@ def main():
     ap.add_argument("--stars", type=int, default=10000)
     ap.add_argument("--cpu-rows", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stack", type=int, default=0,
+                    help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
+    ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent primary-ray kernel (-1: default)")
+    ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -109,7 +117,7 @@ def main():
 
     import numpy as np
     from specrel_amd import scene as S
-    from specrel_amd.renderer import OPT_STATS, Context
+    from specrel_amd.renderer import OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, Context
 
     w, h = args.width, args.height
     p = S.gen_params("large", n_objects=args.objects, slices=args.slices, n_stars=args.stars)
@@ -122,6 +130,12 @@ def main():
     frame = S.build_frame(cam, S.RenderConfig(w, h, S.RelativisticShaded, S.ShadowLorentzInvariant,
                                                S.PastLightCone))
     ctx = Context(0)
+    if args.stack:
+        ctx.set_option(OPT_STACK, args.stack)
+    if args.persistent >= 0:
+        ctx.set_option(OPT_PERSISTENT, args.persistent)
+    if args.shadow_queue >= 0:
+        ctx.set_option(OPT_SHADOW_QUEUE, args.shadow_queue)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")
@@ -137,7 +151,7 @@ def main():
     for _ in range(args.warmup):
         ctx.render_resident(0, frame)
         ctx.sync()
-    parts = {k: 0.0 for k in ("star_prep_ms", "trace_ms", "bloom_ms", "composite_ms", "kernel_ms")}
+    parts = {k: 0.0 for k in ("star_prep_ms", "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "kernel_ms")}
     t0 = time.perf_counter()
     for i in range(args.steps):
         ctx.render_resident(0, frame)
@@ -156,17 +170,22 @@ def main():
     ctx.render_frame(0, frame, want_hits=False)
     e2e_ms = (time.perf_counter() - t0) * 1e3
 
-    bytes_per_launch = algorithmic_bytes(st, w, h, scene.num_stars)
-    achieved = bytes_per_launch / (parts["trace_ms"] * 1e-3) / 1e9
+    alg = algorithmic_bytes(st, w, h, scene.num_stars)
+    times = {"k_trace": parts["trace_ms"], "k_shade": parts["shade_ms"]}
+    dom = max(times, key=times.get)
+    achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "k_render_large_1280x800_traffic.txt")
-    if os.path.exists(tf):
+    tf = os.path.join(ROOT, "profiles", f"{dom}_large_1280x800_traffic.txt")
+    if os.path.exists(tf) and (w, h) == (1280, 800):
         with open(tf) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_render", "kernel_ms": round(parts["trace_ms"], 4),
-                "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                "kernel": dom, "kernel_ms": round(times[dom], 4),
+                "algorithmic_bytes_per_launch": int(alg[dom]),
+                "other_kernels": {k: {"ms": round(times[k], 4), "algorithmic_bytes": int(alg[k]),
+                                      "achieved_GBs": round(alg[k] / (times[k] * 1e-3) / 1e9, 2)}
+                                  for k in times if k != dom}}
 
     cpu = None
     if not args.no_cpu:

@@ -39,16 +39,22 @@ typedef struct specrel_timings {
     float wait_ms;        /* host wait for the device */
     float kernel_ms;      /* first kernel start to last kernel end, device clock */
     float readback_ms;    /* device->host copies */
-    float star_prep_ms;   /* per-view star pass */
-    float trace_ms;       /* the fused ray-generation/trace/shade/star kernel */
+    float star_prep_ms;   /* per-view star pass + star binning */
+    float trace_ms;       /* primary rays: closest hit per pixel */
+    float shade_ms;       /* shading, shadow rays, haze, stars */
     float bloom_ms;       /* pyramid down + up passes */
     float composite_ms;   /* composite + tone map + pack */
+    float reserved[3];
 } specrel_timings;
 
 /* Options for specrel_set_option. */
 enum {
     SPECREL_OPT_STATS = 1,      /* 1: run the instrumented trace variant and fill specrel_stats */
-    SPECREL_OPT_STAR_CULL = 2   /* 1 (default): per-tile star culling; 0: every star per pixel */
+    SPECREL_OPT_STAR_CULL = 2,  /* 1 (default): per-tile star culling; 0: every star per pixel */
+    SPECREL_OPT_STACK = 3,      /* traversal stack: 8 or 16 (default) positions per lane in LDS, or
+                                   -4, -8, -12, -16: that many top entries in registers */
+    SPECREL_OPT_PERSISTENT = 4, /* 1: persistent primary-ray kernel with per-lane refill; 0 (default): one tile per block */
+    SPECREL_OPT_SHADOW_QUEUE = 5 /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
 };
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */

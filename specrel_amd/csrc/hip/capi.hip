@@ -40,11 +40,13 @@ struct specrel_ctx {
     hipStream_t stream = nullptr;
     std::vector<SceneSlot> scenes;
     uint32_t fw = 0, fh = 0;
-    DevBuf hdr, rgb, hit, lam, tau, prep, frame, tables, stats;
-    hipEvent_t ev[6] = {};
+    DevBuf hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, frame, tables, stats;
+    DevBuf counters, mask, squeue;
+    int cus = 0;
+    hipEvent_t ev[7] = {};
     specrel_timings timings{};
     specrel_stats stats_host{};
-    int opt_stats = 0, opt_cull = 1;
+    int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
     bool have_hits = false;
 };
 
@@ -224,6 +226,8 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
     slot.used = true;
 }
 
+uint32_t num_super(uint32_t w, uint32_t h) { return ((w + SUPER - 1) / SUPER) * ((h + SUPER - 1) / SUPER); }
+
 void ensure_frame(specrel_ctx *ctx, uint32_t w, uint32_t h, uint32_t num_stars) {
     if (ctx->fw != w || ctx->fh != h) {
         dalloc(ctx->hdr, pyramid_len(w, h) * sizeof(float4));
@@ -231,10 +235,69 @@ void ensure_frame(specrel_ctx *ctx, uint32_t w, uint32_t h, uint32_t num_stars) 
         dalloc(ctx->hit, (size_t)w * h * 4);
         dalloc(ctx->lam, (size_t)w * h * 4);
         dalloc(ctx->tau, (size_t)w * h * 4);
+        dalloc(ctx->slot, (size_t)w * h * 4);
+        dalloc(ctx->mask, (size_t)w * h * 4);
+        dalloc(ctx->super_counts, (size_t)num_super(w, h) * 4);
+        dalloc(ctx->super_lists, (size_t)num_super(w, h) * SUPER_CAP * 4);
         ctx->fw = w;
         ctx->fh = h;
     }
     dalloc(ctx->prep, (size_t)(num_stars ? num_stars : 1) * sizeof(StarPrep));
+    dalloc(ctx->dirs, (size_t)(num_stars ? num_stars : 1) * sizeof(StarDir));
+}
+
+// The traversal stack is a launch choice (SPECREL_OPT_STACK): K positions per lane in LDS, or
+// the top N entries in registers; deeper entries use private memory either way.
+// Persistent kernels are launched with exactly as many blocks as fit on the device at once.
+template <class Kern>
+dim3 resident_grid(specrel_ctx *ctx, Kern kern) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return dim3((unsigned)(per_cu * ctx->cus));
+}
+
+template <int K>
+void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
+    if (ctx->opt_persistent) {
+        if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
+        else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
+    } else {
+        if (stats) k_trace<true, K><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K><<<tiles, 256, 0, s>>>(ta);
+    }
+}
+template <int K>
+void launch_k(specrel_ctx *ctx, bool stats, dim3, hipStream_t s, const OccludeArgs &oa) {
+    if (stats) k_occlude_p<true, K><<<resident_grid(ctx, k_occlude_p<true, K>), 256, 0, s>>>(oa);
+    else k_occlude_p<false, K><<<resident_grid(ctx, k_occlude_p<false, K>), 256, 0, s>>>(oa);
+}
+template <int PASS>
+struct ShadePass {
+    const ShadeArgs &ra;
+};
+template <int K, int PASS>
+void launch_k(specrel_ctx *, bool stats, dim3 tiles, hipStream_t s, const ShadePass<PASS> &p) {
+    if (stats) k_shade<true, K, PASS><<<tiles, 256, 0, s>>>(p.ra);
+    else k_shade<false, K, PASS><<<tiles, 256, 0, s>>>(p.ra);
+}
+template <int K, class A>
+void launch_kk(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const A &args) {
+    launch_k<K>(ctx, stats, tiles, s, args);
+}
+template <int K, int PASS>
+void launch_kk(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const ShadePass<PASS> &args) {
+    launch_k<K, PASS>(ctx, stats, tiles, s, args);
+}
+template <class A>
+void launch_any(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const A &args) {
+    switch (ctx->opt_stack) {
+        case 8: launch_kk<8>(ctx, stats, tiles, s, args); break;
+        case -4: launch_kk<-4>(ctx, stats, tiles, s, args); break;
+        case -8: launch_kk<-8>(ctx, stats, tiles, s, args); break;
+        case -12: launch_kk<-12>(ctx, stats, tiles, s, args); break;
+        case -16: launch_kk<-16>(ctx, stats, tiles, s, args); break;
+        default: launch_kk<16>(ctx, stats, tiles, s, args); break;
+    }
 }
 
 float ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -261,34 +324,81 @@ int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f,
     want_hits = want_hits || stats;
     if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
 
+    k_reset<<<1, 64, 0, s>>>((uint32_t *)ctx->counters.p, 16);
+    check(hipGetLastError(), "k_reset launch");
     check(hipEventRecord(ctx->ev[0], s), "event");
     const bool draw_stars = f->mode == SPECREL_MODE_SHADED && sc.num_stars > 0;
+    const specrel_frame *dframe = (const specrel_frame *)ctx->frame.p;
     if (draw_stars) {
-        k_star_prep<<<(sc.num_stars + 255) / 256, 256, 0, s>>>((const float4 *)sc.stars.p, sc.num_stars,
-                                                             (const specrel_frame *)ctx->frame.p,
-                                                             (StarPrep *)ctx->prep.p);
+        k_star_prep<<<(sc.num_stars + 255) / 256, 256, 0, s>>>((const float4 *)sc.stars.p, sc.num_stars, dframe,
+                                                             (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
         check(hipGetLastError(), "k_star_prep launch");
+        if (ctx->opt_cull) {
+            k_star_bin<<<num_super(w, h), 256, 0, s>>>(dframe, (const StarDir *)ctx->dirs.p, sc.num_stars,
+                                                       (uint32_t *)ctx->super_lists.p, (uint32_t *)ctx->super_counts.p);
+            check(hipGetLastError(), "k_star_bin launch");
+        }
     }
     check(hipEventRecord(ctx->ev[1], s), "event");
-    RenderArgs ra;
+    uint32_t *counters = (uint32_t *)ctx->counters.p;
+    dim3 tiles((w + TILE - 1) / TILE, (h + TILE - 1) / TILE);
+    TraceArgs ta;
+    ta.S = sc.S;
+    ta.frame = dframe;
+    ta.slot = (uint32_t *)ctx->slot.p;
+    ta.queue = counters + 0;
+    ta.stats = (unsigned long long *)ctx->stats.p;
+    launch_any(ctx, stats, tiles, s, ta);
+    check(hipGetLastError(), "trace launch");
+    check(hipEventRecord(ctx->ev[2], s), "event");
+    ShadeArgs ra;
     ra.S = sc.S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;
     ra.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
-    ra.frame = (const specrel_frame *)ctx->frame.p;
+    ra.frame = dframe;
     ra.stars = (const StarPrep *)ctx->prep.p;
+    ra.dirs = (const StarDir *)ctx->dirs.p;
+    ra.super_lists = (const uint32_t *)ctx->super_lists.p;
+    ra.super_counts = (const uint32_t *)ctx->super_counts.p;
     ra.num_stars = sc.num_stars;
     ra.cull = ctx->opt_cull;
     ra.want_hits = want_hits ? 1 : 0;
+    ra.slot = (const uint32_t *)ctx->slot.p;
     ra.hdr = (float4 *)ctx->hdr.p;
     ra.hit = (uint32_t *)ctx->hit.p;
     ra.lambda = (float *)ctx->lam.p;
     ra.tau = (float *)ctx->tau.p;
+    ra.mask = (uint32_t *)ctx->mask.p;
+    ra.queue = nullptr;
+    ra.qcount = counters + 1;
+    ra.qcap = 0;
     ra.stats = (unsigned long long *)ctx->stats.p;
-    dim3 grid((w + 15) / 16, (h + 15) / 16);
-    if (stats) k_render<true><<<grid, 256, 0, s>>>(ra);
-    else k_render<false><<<grid, 256, 0, s>>>(ra);
-    check(hipGetLastError(), "k_render launch");
-    check(hipEventRecord(ctx->ev[2], s), "event");
+    // Shadow queries exist only for shaded views with shadows on (renderer.rs:631-641, 680-682).
+    const bool queries = f->mode == SPECREL_MODE_SHADED && f->shadow_mode != SPECREL_SHADOW_OFF;
+    if (queries && ctx->opt_shadow_queue) {
+        uint64_t cap = (uint64_t)w * h * (1u + f->num_flashes);
+        if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "frame too large for the shadow queue"};
+        dalloc(ctx->squeue, (size_t)cap * sizeof(ShadowQuery));
+        ra.queue = (ShadowQuery *)ctx->squeue.p;
+        ra.qcap = (uint32_t)cap;
+        launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
+        check(hipGetLastError(), "k_shade setup launch");
+        OccludeArgs oa;
+        oa.S = sc.S;
+        oa.queue = ra.queue;
+        oa.count = counters + 1;
+        oa.next = counters + 2;
+        oa.cap = ra.qcap;
+        oa.mask = ra.mask;
+        oa.stats = ra.stats;
+        launch_any(ctx, stats, tiles, s, oa);
+        check(hipGetLastError(), "k_occlude launch");
+        launch_any(ctx, stats, tiles, s, ShadePass<2>{ra});
+    } else {
+        launch_any(ctx, stats, tiles, s, ShadePass<0>{ra});
+    }
+    check(hipGetLastError(), "k_shade launch");
+    check(hipEventRecord(ctx->ev[3], s), "event");
     float4 *pyr = (float4 *)ctx->hdr.p;
     for (int k = 1; k <= SPECREL_BLOOM_LEVELS; ++k) {
         uint32_t sw, sh, dw, dh;
@@ -305,7 +415,7 @@ int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f,
         k_bloom_up<<<g, 256, 0, s>>>(pyr + level_offset(w, h, k), lw, lh, pyr + level_offset(w, h, k + 1), cw, ch);
     }
     check(hipGetLastError(), "bloom launch");
-    check(hipEventRecord(ctx->ev[3], s), "event");
+    check(hipEventRecord(ctx->ev[4], s), "event");
     {
         uint32_t w1, h1;
         level_size(w, h, 1, w1, h1);
@@ -314,7 +424,7 @@ int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f,
                                       (uint32_t *)ctx->rgb.p);
         check(hipGetLastError(), "k_composite launch");
     }
-    check(hipEventRecord(ctx->ev[4], s), "event");
+    check(hipEventRecord(ctx->ev[5], s), "event");
     auto t1 = std::chrono::steady_clock::now();
     ctx->timings = specrel_timings{};
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
@@ -326,9 +436,10 @@ void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
     t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
     t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
-    t.bloom_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
-    t.composite_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
-    t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[4]);
+    t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
+    t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
+    t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);
+    t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[5]);
     if (ctx->opt_stats) {
         unsigned long long c[16];
         check(hipMemcpy(c, ctx->stats.p, sizeof c, hipMemcpyDeviceToHost), "stats readback");
@@ -387,6 +498,8 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         upload(ctx->tables, bytes.data(), bytes.size(), ctx->stream);
         dalloc(ctx->frame, sizeof(specrel_frame));
         dalloc(ctx->stats, 16 * 8);
+        dalloc(ctx->counters, 16 * 4);
+        ctx->cus = prop.multiProcessorCount;
         check(hipStreamSynchronize(ctx->stream), "create");
         return SPECREL_OK;
     });
@@ -408,6 +521,8 @@ void specrel_destroy(specrel_ctx *ctx) {
         dfree(sl.mats), dfree(sl.stars);
     }
     dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
+    dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
+    dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
     dfree(ctx->frame), dfree(ctx->tables), dfree(ctx->stats);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -422,6 +537,14 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     if (!ctx) return SPECREL_EINVAL;
     if (option == SPECREL_OPT_STATS) ctx->opt_stats = value != 0;
     else if (option == SPECREL_OPT_STAR_CULL) ctx->opt_cull = value != 0;
+    else if (option == SPECREL_OPT_STACK) {
+        if (value != 8 && value != 16 && value != -4 && value != -8 && value != -12 && value != -16) {
+            ctx->err = "stack must be 8 or 16 (LDS positions) or -4, -8, -12, -16 (register entries)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_stack = value;
+    } else if (option == SPECREL_OPT_PERSISTENT) ctx->opt_persistent = value != 0;
+    else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
     else {
         ctx->err = "unknown option";
         return SPECREL_EINVAL;

@@ -55,16 +55,20 @@ struct Tables {
 struct StarPrep {
     float4 r;
     float4 bb_cn;      // blackbody rgb, colour_norm
-    double4 dir;       // unit local direction (x,y,z), w unused
     uint32_t flags;    // bit0: exists on the cone; bit1: non-finite, never culled
     uint32_t pad[3];
+};
+// Culling record of a star: unit direction of r in the camera frame, flags in w.
+struct StarDir {
+    float x, y, z;
+    uint32_t flags;
 };
 
 SR_HD V4 f4v(float4 a) { return V4{a.x, a.y, a.z, a.w}; }
 SR_HD V4 lv(const float *p) { return V4{p[0], p[1], p[2], p[3]}; }
 
 struct Counters {
-    uint64_t node_pops, box_tests, prim_tests, shadow_rays, shadow_node_pops, shadow_box_tests,
+    uint64_t node_pops, box_tests, prim_tests, hits, shadow_rays, shadow_node_pops, shadow_box_tests,
         shadow_prim_tests, star_tests;
 };
 
@@ -134,20 +138,89 @@ struct Hit {
 
 constexpr int STACK = 64;  // the reference's stack size (bvh4d.rs:454, 537), overflow semantics kept
 
-// Bvh4D::trace (bvh4d.rs:445-520)
-template <bool STATS>
-__device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C) {
+// The logical 64-entry traversal stack of one lane, two implementations with one interface:
+//   LdsStack<K>: positions below K in LDS (slot-major, so the 64 lanes of a wave touch
+//                consecutive words whatever their depth), deeper ones in private memory.
+//   RegStack<N>: the top N entries in registers (static-index shifts, no memory on push/pop),
+//                older ones spilled to private memory; a spilled entry is restored when it
+//                re-enters the window, N pops before it is needed, so its load latency hides.
+template <int K>
+struct LdsStack {
+    uint2 *lds;       // this lane's column: lds[pos * stride]
+    uint32_t stride;  // lanes per block
+    int n;
+    uint2 deep[STACK - K];
+    __device__ __forceinline__ void reset() { n = 0; }
+    __device__ __forceinline__ int size() const { return n; }
+    __device__ __forceinline__ void push(uint32_t e, float t) {
+        uint2 v = make_uint2(e, __float_as_uint(t));
+        if (n < K) lds[(uint32_t)n * stride] = v;
+        else deep[n - K] = v;
+        n += 1;
+    }
+    __device__ __forceinline__ void pop(uint32_t &e, float &t) {
+        n -= 1;
+        uint2 v = n < K ? lds[(uint32_t)n * stride] : deep[n - K];
+        e = v.x;
+        t = __uint_as_float(v.y);
+    }
+};
+
+template <int N>
+struct RegStack {
+    uint32_t e_[N];
+    float t_[N];
+    int n;
+    uint2 deep[STACK - N];
+    __device__ __forceinline__ void reset() { n = 0; }
+    __device__ __forceinline__ int size() const { return n; }
+    __device__ __forceinline__ void push(uint32_t e, float t) {
+        if (n >= N) deep[n - N] = make_uint2(e_[N - 1], __float_as_uint(t_[N - 1]));
+#pragma unroll
+        for (int i = N - 1; i > 0; --i) {
+            e_[i] = e_[i - 1];
+            t_[i] = t_[i - 1];
+        }
+        e_[0] = e;
+        t_[0] = t;
+        n += 1;
+    }
+    __device__ __forceinline__ void pop(uint32_t &e, float &t) {
+        e = e_[0];
+        t = t_[0];
+#pragma unroll
+        for (int i = 0; i < N - 1; ++i) {
+            e_[i] = e_[i + 1];
+            t_[i] = t_[i + 1];
+        }
+        n -= 1;
+        if (n >= N) {
+            uint2 v = deep[n - N];
+            e_[N - 1] = v.x;
+            t_[N - 1] = __uint_as_float(v.y);
+        }
+    }
+};
+
+// Bvh4D::trace (bvh4d.rs:445-520). The near child of a node whose children both hit is not
+// pushed and popped again but traversed at once: the reference pops it right after pushing it,
+// with nothing in between, so the visit order, the t_enter >= t_max skips and the stack-size
+// limit (both children only when sp + 2 <= 64) are the same.
+template <bool STATS, class ST>
+__device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk) {
     hit.slot = -1;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
-    float root_enter;
-    if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, root_enter)) return false;
-    uint2 stk[STACK];
-    int sp = 1;
-    stk[0] = make_uint2(S.root_entry, __float_as_uint(root_enter));
-    while (sp > 0) {
-        sp -= 1;
-        uint32_t e = stk[sp].x;
-        float t_enter = __uint_as_float(stk[sp].y);
+    float t_enter;
+    if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
+    uint32_t e = S.root_entry;
+    stk.reset();
+    bool have = true;
+    while (true) {
+        if (!have) {
+            if (stk.size() == 0) break;
+            stk.pop(e, t_enter);
+        }
+        have = false;
         if (t_enter >= tmax) continue;
         if (STATS) C.node_pops++;
         if (e & LEAF) {
@@ -177,38 +250,52 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
             bool hl = box_hit(lmn, lmx, o, inv, tmin, tmax, tl);
             bool hr = box_hit(rmn, rmx, o, inv, tmin, tmax, tr);
             if (hl && hr) {
-                if (sp + 2 <= STACK) {
+                if (stk.size() + 2 <= STACK) {
                     if (tl <= tr) {
-                        stk[sp] = make_uint2(info.y, __float_as_uint(tr));
-                        stk[sp + 1] = make_uint2(info.x, __float_as_uint(tl));
+                        stk.push(info.y, tr);
+                        e = info.x;
+                        t_enter = tl;
                     } else {
-                        stk[sp] = make_uint2(info.x, __float_as_uint(tl));
-                        stk[sp + 1] = make_uint2(info.y, __float_as_uint(tr));
+                        stk.push(info.x, tl);
+                        e = info.y;
+                        t_enter = tr;
                     }
-                    sp += 2;
+                    have = true;
                 }
             } else if (hl) {
-                if (sp < STACK) stk[sp++] = make_uint2(info.x, __float_as_uint(tl));
+                if (stk.size() < STACK) {
+                    e = info.x;
+                    t_enter = tl;
+                    have = true;
+                }
             } else if (hr) {
-                if (sp < STACK) stk[sp++] = make_uint2(info.y, __float_as_uint(tr));
+                if (stk.size() < STACK) {
+                    e = info.y;
+                    t_enter = tr;
+                    have = true;
+                }
             }
         }
     }
     return hit.slot >= 0;
 }
 
-// Bvh4D::is_occluded (bvh4d.rs:523-578)
-template <bool STATS>
-__device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Counters &C) {
+// Bvh4D::is_occluded (bvh4d.rs:523-578): right pushed, then left pushed and popped at once.
+template <bool STATS, class ST>
+__device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Counters &C, ST &stk) {
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float en;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, en)) return false;
-    uint32_t stk[STACK];
-    int sp = 1;
-    stk[0] = S.root_entry;
-    while (sp > 0) {
-        sp -= 1;
-        uint32_t e = stk[sp];
+    uint32_t e = S.root_entry;
+    stk.reset();
+    bool have = true;
+    while (true) {
+        if (!have) {
+            if (stk.size() == 0) break;
+            float unused;
+            stk.pop(e, unused);
+        }
+        have = false;
         if (STATS) C.shadow_node_pops++;
         if (e & LEAF) {
             uint32_t first, count;
@@ -226,8 +313,11 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
             if (STATS) C.shadow_box_tests += 2;
             bool hl = box_hit(bx[0], bx[1], o, inv, tmin, tmax, t);
             bool hr = box_hit(bx[2], bx[3], o, inv, tmin, tmax, t);
-            if (hr && sp < STACK) stk[sp++] = info.y;
-            if (hl && sp < STACK) stk[sp++] = info.x;
+            if (hr && stk.size() < STACK) stk.push(info.y, 0.0f);
+            if (hl && stk.size() < STACK) {
+                e = info.x;
+                have = true;
+            }
         }
     }
     return false;
@@ -528,9 +618,11 @@ __device__ V3 albedo(const Tables &T, const Rec &h, const Mat &m) {  // renderer
 }
 
 // compute_illumination + light_emission_event (renderer.rs:585-653)
-template <bool STATS>
-__device__ float illumination(const Scene &S, const specrel_frame &f, V4 dir, const Rec &h, const Mat &m,
-                              Counters &C) {
+// `sh.query(id, origin, dir, tmin, tmax)` answers Bvh4D::is_occluded for shadow query `id`
+// of this pixel (0: the light, 1 + i: flash i): traced at once, queued for a later pass, or
+// looked up from that pass's result.
+template <class SH>
+__device__ float illumination(const Scene &S, const specrel_frame &f, V4 dir, const Rec &h, const Mat &m, SH &sh) {
     V3 light = V3{S.light.y, S.light.z, S.light.w};
     V3 s2l = light - spatial(h.event);
     float dist = fmax_(length(s2l), 1e-4f);
@@ -546,10 +638,8 @@ __device__ float illumination(const Scene &S, const specrel_frame &f, V4 dir, co
     float cv = fmax_(-rest_frame_cos(dir, h.normal, h.four_vel), 0.0f);
     if (m.pattern == SPECREL_PATTERN_CHECKER) {
         bool in_shadow = false;
-        if (cl > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF) {
-            if (STATS) C.shadow_rays++;
-            in_shadow = occluded<STATS>(S, h.event, to_light, 2e-3f, dist - 2e-3f, C);
-        }
+        if (cl > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF)
+            in_shadow = sh.query(0u, h.event, to_light, 2e-3f, dist - 2e-3f);
         float sf = in_shadow ? 0.32f : 1.0f;
         float atten = 1.0f / (1.0f + 0.006f * dist * dist);
         return m.emission + 0.30f * cv + 0.80f * sf * cl * atten;
@@ -558,8 +648,8 @@ __device__ float illumination(const Scene &S, const specrel_frame &f, V4 dir, co
 }
 
 // compute_flash_illumination (renderer.rs:658-688)
-template <bool STATS>
-__device__ V3 flashes(const Scene &S, const specrel_frame &f, const Rec &h, Counters &C) {
+template <class SH>
+__device__ V3 flashes(const Scene &S, const specrel_frame &f, const Rec &h, SH &sh) {
     V3 total = V3{0.0f, 0.0f, 0.0f};
     for (uint32_t i = 0; i < f.num_flashes; ++i) {
         V4 at = lv(f.flash_at[i]);
@@ -576,10 +666,8 @@ __device__ V3 flashes(const Scene &S, const specrel_frame &f, const Rec &h, Coun
         V4 to_flash = (at - h.event) * (1.0f / dist);
         float cf = fmax_(rest_frame_cos(to_flash, h.normal, h.four_vel), 0.0f);
         bool in_shadow = false;
-        if (cf > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF) {
-            if (STATS) C.shadow_rays++;
-            in_shadow = occluded<STATS>(S, h.event, to_flash, 2e-3f, dist - 2e-3f, C);
-        }
+        if (cf > 0.0f && f.shadow_mode != SPECREL_SHADOW_OFF)
+            in_shadow = sh.query(1u + i, h.event, to_flash, 2e-3f, dist - 2e-3f);
         float sf = in_shadow ? 0.25f : 1.0f;
         float atten = 1.0f / (1.0f + 0.015f * dist * dist);
         total += color * (intensity * pulse * atten * (0.25f + 0.75f * sf * cf));
@@ -588,16 +676,15 @@ __device__ V3 flashes(const Scene &S, const specrel_frame &f, const Rec &h, Coun
 }
 
 // shade_hit (renderer.rs:414-489)
-template <bool STATS>
-__device__ V3 shade_hit(const Scene &S, const Tables &T, const specrel_frame &f, V4 dir, const Rec &h,
-                        Counters &C) {
+template <class SH>
+__device__ V3 shade_hit(const Scene &S, const Tables &T, const specrel_frame &f, V4 dir, const Rec &h, SH &sh) {
     Mat m = load_mat(S, h.material);
     if (f.mode == SPECREL_MODE_SHADED) {
         V3 a = albedo(T, h, m);
-        float lighting = illumination<STATS>(S, f, dir, h, m, C);
+        float lighting = illumination(S, f, dir, h, m, sh);
         V3 lit = a * lighting;
         if (f.num_flashes > 0) {
-            V3 fl = flashes<STATS>(S, f, h, C);
+            V3 fl = flashes(S, f, h, sh);
             lit += (a * 0.65f + V3{0.35f, 0.35f, 0.35f}) * fl;
         }
         return distance_fog(lit, dir, h.lambda);
@@ -624,6 +711,165 @@ __device__ V3 shade_hit(const Scene &S, const Tables &T, const specrel_frame &f,
     else age = decision_age(h.event, lv(f.frontier_origin), lv(f.e[0]));
     float cv = fmax_(-rest_frame_cos(dir, h.normal, h.four_vel), 0.0f);
     return distance_fog(decision_age_color(age) * (0.55f + 0.45f * cv), dir, h.lambda);
+}
+
+// Shadow queries traced where they are asked (the direct path).
+template <bool STATS, class ST>
+struct TraceShadow {
+    const Scene &S;
+    Counters &C;
+    ST &stk;
+    __device__ bool query(uint32_t, V4 o, V4 d, float tmin, float tmax) {
+        if (STATS) C.shadow_rays++;
+        return occluded<STATS>(S, o, d, tmin, tmax, C, stk);
+    }
+};
+
+// One queued shadow query: the occlusion test of Bvh4D::is_occluded for (pixel, id).
+struct ShadowQuery {
+    float4 o, d;
+    float tmin, tmax;
+    uint32_t pixel, id;
+};
+
+// Shadow queries appended to a queue (the value returned is not used by the caller's pass).
+struct EnqueueShadow {
+    ShadowQuery *queue;
+    uint32_t *count;
+    uint32_t cap;
+    uint32_t pixel;
+    __device__ bool query(uint32_t id, V4 o, V4 d, float tmin, float tmax) {
+        uint32_t k = atomicAdd(count, 1u);
+        if (k < cap)
+            queue[k] = ShadowQuery{make_float4(o.t, o.x, o.y, o.z), make_float4(d.t, d.x, d.y, d.z), tmin, tmax, pixel, id};
+        return false;
+    }
+};
+
+// Shadow queries answered from the occlusion pass: bit `id` of the pixel's mask.
+struct LookupShadow {
+    uint32_t mask;
+    __device__ bool query(uint32_t id, V4, V4, float, float) { return (mask >> id) & 1u; }
+};
+
+// Closest-hit traversal of one ray as a resumable state machine: step() processes one stack
+// entry (the loop body of trace() above) so that a lane can take a new ray when its ray ends.
+struct RayState {
+    V4 o, d, inv;
+    float tmax, t_enter;
+    int32_t best;
+    uint32_t e, pix;
+    bool have;
+};
+
+template <class ST>
+__device__ __forceinline__ bool trace_begin(const Scene &S, RayState &r, float tmin, ST &stk) {
+    r.inv = V4{safe_inv(r.d.t), safe_inv(r.d.x), safe_inv(r.d.y), safe_inv(r.d.z)};
+    r.best = -1;
+    stk.reset();
+    r.have = false;
+    if (S.empty) return false;
+    float te;
+    if (!box_hit(S.root_min, S.root_max, r.o, r.inv, tmin, r.tmax, te)) return false;
+    r.e = S.root_entry;
+    r.t_enter = te;
+    r.have = true;
+    return true;
+}
+
+// Returns false when the ray is finished.
+template <bool STATS, class ST>
+__device__ __forceinline__ bool trace_step(const Scene &S, RayState &r, float tmin, Counters &C, ST &stk) {
+    if (!r.have) {
+        if (stk.size() == 0) return false;
+        stk.pop(r.e, r.t_enter);
+    }
+    r.have = false;
+    if (r.t_enter >= r.tmax) return true;
+    if (STATS) C.node_pops++;
+    if (r.e & LEAF) {
+        uint32_t first, count;
+        leaf_range(S, r.e, first, count);
+        for (uint32_t i = first; i < first + count; ++i) {
+            float lam, b[4];
+            V4 ev;
+            if (STATS) C.prim_tests++;
+            if (intersect(S.geoms + 5 * (size_t)i, r.o, r.d, tmin, r.tmax, lam, ev, b)) {
+                r.tmax = lam;
+                r.best = (int32_t)i;
+            }
+        }
+    } else {
+        const float4 *bx = S.nbox + 4 * (size_t)r.e;
+        float4 lmn = bx[0], lmx = bx[1], rmn = bx[2], rmx = bx[3];
+        uint2 info = S.ninfo[r.e];
+        float tl, tr;
+        if (STATS) C.box_tests += 2;
+        bool hl = box_hit(lmn, lmx, r.o, r.inv, tmin, r.tmax, tl);
+        bool hr = box_hit(rmn, rmx, r.o, r.inv, tmin, r.tmax, tr);
+        if (hl && hr) {
+            if (stk.size() + 2 <= STACK) {
+                if (tl <= tr) {
+                    stk.push(info.y, tr);
+                    r.e = info.x;
+                    r.t_enter = tl;
+                } else {
+                    stk.push(info.x, tl);
+                    r.e = info.y;
+                    r.t_enter = tr;
+                }
+                r.have = true;
+            }
+        } else if (hl) {
+            if (stk.size() < STACK) {
+                r.e = info.x;
+                r.t_enter = tl;
+                r.have = true;
+            }
+        } else if (hr) {
+            if (stk.size() < STACK) {
+                r.e = info.y;
+                r.t_enter = tr;
+                r.have = true;
+            }
+        }
+    }
+    return true;
+}
+
+// Any-hit (is_occluded) as a state machine. step(): 0 = running, 1 = occluded, 2 = clear.
+template <bool STATS, class ST>
+__device__ __forceinline__ int occlude_step(const Scene &S, RayState &r, float tmin, Counters &C, ST &stk) {
+    if (!r.have) {
+        if (stk.size() == 0) return 2;
+        float unused;
+        stk.pop(r.e, unused);
+    }
+    r.have = false;
+    if (STATS) C.shadow_node_pops++;
+    if (r.e & LEAF) {
+        uint32_t first, count;
+        leaf_range(S, r.e, first, count);
+        for (uint32_t i = first; i < first + count; ++i) {
+            float lam, b[4];
+            V4 ev;
+            if (STATS) C.shadow_prim_tests++;
+            if (intersect(S.geoms + 5 * (size_t)i, r.o, r.d, tmin, r.tmax, lam, ev, b)) return 1;
+        }
+    } else {
+        const float4 *bx = S.nbox + 4 * (size_t)r.e;
+        uint2 info = S.ninfo[r.e];
+        float t;
+        if (STATS) C.shadow_box_tests += 2;
+        bool hl = box_hit(bx[0], bx[1], r.o, r.inv, tmin, r.tmax, t);
+        bool hr = box_hit(bx[2], bx[3], r.o, r.inv, tmin, r.tmax, t);
+        if (hr && stk.size() < STACK) stk.push(info.y, 0.0f);
+        if (hl && stk.size() < STACK) {
+            r.e = info.x;
+            r.have = true;
+        }
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------------------

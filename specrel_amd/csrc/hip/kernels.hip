@@ -1,5 +1,5 @@
-// Kernels of the gfx950 renderer: per-view star pass, the fused trace/shade kernel, and the
-// bloom pyramid + tone map (bloom.rs:93-125, renderer.rs:1116-1129).
+// Kernels of the gfx950 renderer: per-view star pass and star binning, the primary-ray trace,
+// the shading pass, and the bloom pyramid + tone map (bloom.rs:93-125, renderer.rs:1116-1129).
 #include "kernels.hpp"
 
 namespace dev {
@@ -41,10 +41,45 @@ __device__ __forceinline__ float pix_sy(const ViewDerived &v, uint32_t py) {
     return ndc_y * v.half_h;
 }
 
+// The pixel's ray direction (renderer.rs:369-384 with Tetrad::past/future_light_ray_direction,
+// local_spatial_to_world; minkowski.rs:526-575).
+__device__ __forceinline__ V4 ray_dir(const ViewDerived &v, const specrel_frame &f, uint32_t px, uint32_t py) {
+    float sx = pix_sx(v, px), sy = pix_sy(v, py);
+    V3 local_dir = normalized(V3{sx, sy, 1.0f});
+    if (f.slice_mode == SPECREL_VIEW_PAST_CONE) {
+        V3 n = normalized(local_dir);
+        return -v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
+    }
+    if (f.slice_mode == SPECREL_VIEW_FRONTIER) {
+        V3 n = normalized(local_dir);
+        return v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
+    }
+    return v.e1 * local_dir.x + v.e2 * local_dir.y + v.e3 * local_dir.z;
+}
+
+// Traversal stack of a kernel: SK > 0 keeps SK positions per lane in LDS, SK < 0 keeps the top
+// -SK entries in registers.
+template <int SK, bool = (SK > 0)>
+struct StackSel {
+    using type = LdsStack<SK>;
+    static constexpr int lds_entries = SK * 256;
+    __device__ static void init(type &s, uint2 *lds, uint32_t tid) {
+        s.lds = lds + tid;
+        s.stride = 256;
+        s.n = 0;
+    }
+};
+template <int SK>
+struct StackSel<SK, false> {
+    using type = RegStack<-SK>;
+    static constexpr int lds_entries = 1;
+    __device__ static void init(type &s, uint2 *, uint32_t) { s.n = 0; }
+};
+
 // Star pass: for every star, its cone event and the view-only parts of star_light
-// (star.rs:251-283), plus its direction in the camera rest frame for tile culling.
+// (star.rs:251-283), plus its unit direction in the camera rest frame for the tile culling.
 __global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t num_stars,
-                                                   const specrel_frame *frame, StarPrep *out) {
+                                                   const specrel_frame *frame, StarPrep *out, StarDir *dirs) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= num_stars) return;
     const specrel_frame &f = *frame;
@@ -53,8 +88,8 @@ __global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t
     p.flags = 0;
     p.r = make_float4(0, 0, 0, 0);
     p.bb_cn = make_float4(0, 0, 0, 0);
-    p.dir = make_double4(0, 0, 0, 0);
     p.pad[0] = p.pad[1] = p.pad[2] = 0;
+    StarDir sd = {0.0f, 0.0f, 0.0f, 0u};
     V4 d, u;
     if (cone_event(words, num_stars, i, v.origin, v.e0, v.cone, d, u)) {
         float tau = -mdot(d, v.e0);
@@ -79,147 +114,412 @@ __global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t
         bool finite = isfinite(r.t) && isfinite(r.x) && isfinite(r.y) && isfinite(r.z) && isfinite(bb.x) &&
                       isfinite(bb.y) && isfinite(bb.z) && isfinite(hdr.y) && ln > 0.0;
         if (!finite) p.flags |= 2u;  // evaluated for every pixel, never culled
-        else p.dir = make_double4(lx / ln, ly / ln, lz / ln, 0.0);
+        else sd = StarDir{(float)(lx / ln), (float)(ly / ln), (float)(lz / ln), 0u};
+        sd.flags = p.flags;
     }
     out[i] = p;
+    dirs[i] = sd;
 }
 
-constexpr int TILE = 16;
-constexpr int STAR_CAP = 2048;
+// Bounding cone of the pixel rays (pixel centres) of the image rectangle [xa,xb] x [ya,yb], widened
+// by the largest angle at which a star can still reach a pixel: tan(angle) < 2.4 spread <= 2.4 pitch
+// (sigma = 0.6 spread lambda, cut at 4 sigma; star.rs:261-267), plus 5% and half a pixel of slack.
+// Returned as a float axis and a float cosine bound lowered by 1e-6, so a float dot product
+// (error < 3e-7) can only keep more stars, never fewer.
+__device__ void tile_cone(const ViewDerived &v, uint32_t xa, uint32_t xb, uint32_t ya, uint32_t yb, float &cx,
+                          float &cy, float &cz, float &cos_lim) {
+    double cd[4][3];
+    uint32_t xs[4] = {xa, xb, xa, xb}, ys[4] = {ya, ya, yb, yb};
+    double sx_ = 0, sy_ = 0, sz_ = 0;
+    for (int k = 0; k < 4; ++k) {
+        double sx = pix_sx(v, xs[k]), sy = pix_sy(v, ys[k]);
+        double n = sqrt(sx * sx + sy * sy + 1.0);
+        cd[k][0] = sx / n;
+        cd[k][1] = sy / n;
+        cd[k][2] = 1.0 / n;
+        sx_ += cd[k][0];
+        sy_ += cd[k][1];
+        sz_ += cd[k][2];
+    }
+    double cn = sqrt(sx_ * sx_ + sy_ * sy_ + sz_ * sz_);
+    sx_ /= cn;
+    sy_ /= cn;
+    sz_ /= cn;
+    double cos_a = 1.0;
+    for (int k = 0; k < 4; ++k) cos_a = fmin(cos_a, cd[k][0] * sx_ + cd[k][1] * sy_ + cd[k][2] * sz_);
+    double beta = (1.05 * 2.4 + 0.5) * (double)v.pitch;
+    double ang = acos(fmin(1.0, cos_a)) + beta;
+    cx = (float)sx_;
+    cy = (float)sy_;
+    cz = (float)sz_;
+    cos_lim = ang >= 3.0 ? -2.0f : (float)(cos(ang) - 1e-6);
+}
 
-template <bool STATS>
-__global__ __launch_bounds__(256) void k_render(RenderArgs a) {
+__device__ __forceinline__ bool star_in_cone(const StarDir &s, float cx, float cy, float cz, float cos_lim) {
+    if (s.flags & 2u) return true;
+    if (!(s.flags & 1u)) return false;
+    return s.x * cx + s.y * cy + s.z * cz >= cos_lim;
+}
+
+// Ordered compaction of candidates (in increasing index) into list: every thread of the block
+// calls it with the same `n`; keep(i) decides candidate i. Returns the total kept (which may
+// exceed cap, in which case the list holds only the first cap entries).
+template <class Keep>
+__device__ uint32_t ordered_compact(uint32_t n, Keep keep, uint32_t *list, uint32_t cap, uint32_t *s_wave,
+                                    uint32_t *s_total) {
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (tid == 0) *s_total = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += 256) {
+        uint32_t idx = base + tid;
+        uint32_t item = 0;
+        bool k = idx < n && keep(idx, item);
+        uint64_t mask = __ballot(k);
+        uint32_t before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_wave[wave] = __popcll(mask);
+        __syncthreads();
+        uint32_t off = *s_total;
+        for (uint32_t w = 0; w < wave; ++w) off += s_wave[w];
+        if (k && off + before < cap) list[off + before] = item;
+        __syncthreads();
+        if (tid == 0) *s_total += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        __syncthreads();
+    }
+    return *s_total;
+}
+
+// First culling level: one block per 64x64-pixel super tile keeps, in index order, the stars that
+// can reach any of its pixels.
+__global__ __launch_bounds__(256) void k_star_bin(const specrel_frame *frame, const StarDir *dirs,
+                                                  uint32_t num_stars, uint32_t *lists, uint32_t *counts) {
+    const ViewDerived v = derive(*frame, num_stars);
+    const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
+    const uint32_t st = blockIdx.x, x0 = (st % nsx) * SUPER, y0 = (st / nsx) * SUPER;
+    __shared__ float s_cone[4];
+    __shared__ uint32_t s_wave[4], s_total;
+    if (threadIdx.x == 0)
+        tile_cone(v, x0, min(x0 + SUPER, v.w) - 1, y0, min(y0 + SUPER, v.h) - 1, s_cone[0], s_cone[1], s_cone[2],
+                  s_cone[3]);
+    __syncthreads();
+    const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
+    uint32_t *list = lists + (size_t)st * SUPER_CAP;
+    uint32_t n = ordered_compact(
+        num_stars,
+        [&](uint32_t i, uint32_t &item) {
+            item = i;
+            return star_in_cone(dirs[i], cx, cy, cz, cl);
+        },
+        list, SUPER_CAP, s_wave, &s_total);
+    if (threadIdx.x == 0) counts[st] = n;
+}
+
+// Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT).
+template <bool STATS, int SK>
+__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
+    const specrel_frame &f = *a.frame;
+    const ViewDerived v = derive(f, 0);
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t px = blockIdx.x * TILE + (wave & 1) * 8 + (lane & 7);
+    const uint32_t py = blockIdx.y * TILE + (wave >> 1) * 8 + (lane >> 3);
+    __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
+    typename StackSel<SK>::type stk;
+    StackSel<SK>::init(stk, s_stack, tid);
+    Counters C = {};
+    if (px < v.w && py < v.h) {
+        V4 dir = ray_dir(v, f, px, py);
+        Hit hit;
+        bool found = !a.S.empty && trace<STATS>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+        a.slot[(size_t)py * v.w + px] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
+    }
+    if (STATS) {
+        __shared__ unsigned long long s_c[4];
+        if (tid < 4) s_c[tid] = 0;
+        __syncthreads();
+        atomicAdd(&s_c[0], (unsigned long long)((px < v.w && py < v.h) ? 1 : 0));
+        atomicAdd(&s_c[1], (unsigned long long)C.node_pops);
+        atomicAdd(&s_c[2], (unsigned long long)C.box_tests);
+        atomicAdd(&s_c[3], (unsigned long long)C.prim_tests);
+        __syncthreads();
+        if (tid < 4) atomicAdd(&a.stats[tid], s_c[tid]);
+    }
+}
+
+// Persistent primary-ray kernel: every lane takes the next ray of a global queue (8x8-pixel
+// tiles in row-major order, 64 rays per grab) as soon as its own ray ends, so the waves stay
+// full until the queue drains. Each ray's traversal is trace()'s, step for step.
+template <bool STATS, int SK>
+__global__ __launch_bounds__(256) void k_trace_p(TraceArgs a) {
+    const specrel_frame &f = *a.frame;
+    const ViewDerived v = derive(f, 0);
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
+    typename StackSel<SK>::type stk;
+    StackSel<SK>::init(stk, s_stack, tid);
+    Counters C = {};
+    uint64_t rays = 0;
+    const uint32_t tiles_x = (v.w + 7) / 8, tiles_y = (v.h + 7) / 8;
+    const uint32_t total = tiles_x * tiles_y * 64;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t next = 0, end = 0;
+    bool drained = false, active = false;
+    RayState r;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        if (idle != 0 && !drained) {
+            if (next == end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.queue, 64u);
+                base = __shfl(base, 0);
+                if (base >= total) drained = true;
+                else {
+                    next = base;
+                    end = min(base + 64u, total);
+                }
+            }
+            if (!drained) {
+                uint32_t avail = end - next, rank = __popcll(idle & lt);
+                if (!active && rank < avail) {
+                    uint32_t ray = next + rank, t = ray >> 6, l = ray & 63;
+                    uint32_t px = (t % tiles_x) * 8 + (l & 7), py = (t / tiles_x) * 8 + (l >> 3);
+                    if (px < v.w && py < v.h) {
+                        r.pix = py * v.w + px;
+                        r.o = v.origin;
+                        r.d = ray_dir(v, f, px, py);
+                        r.tmax = 180.0f;
+                        if (STATS) rays++;
+                        if (trace_begin(a.S, r, 1e-3f, stk)) active = true;
+                        else a.slot[r.pix] = SPECREL_NO_HIT;
+                    }
+                }
+                next += min((uint32_t)__popcll(idle), avail);
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active && !trace_step<STATS>(a.S, r, 1e-3f, C, stk)) {
+            a.slot[r.pix] = r.best >= 0 ? (uint32_t)r.best : SPECREL_NO_HIT;
+            active = false;
+        }
+    }
+    if (STATS) {
+        atomicAdd(&a.stats[0], (unsigned long long)rays);
+        atomicAdd(&a.stats[1], (unsigned long long)C.node_pops);
+        atomicAdd(&a.stats[2], (unsigned long long)C.box_tests);
+        atomicAdd(&a.stats[3], (unsigned long long)C.prim_tests);
+    }
+}
+
+// Persistent any-hit kernel over the queued shadow queries; an occluded query sets bit `id`
+// of its pixel's mask.
+template <bool STATS, int SK>
+__global__ __launch_bounds__(256) void k_occlude_p(OccludeArgs a) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
+    typename StackSel<SK>::type stk;
+    StackSel<SK>::init(stk, s_stack, tid);
+    Counters C = {};
+    uint64_t queries = 0;
+    const uint32_t total = min(*a.count, a.cap);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t next = 0, end = 0, pixel = 0, id = 0;
+    float tmin = 0.0f;
+    bool drained = false, active = false;
+    RayState r;
+    while (true) {
+        uint64_t idle = __ballot(!active);
+        if (idle != 0 && !drained) {
+            if (next == end) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.next, 64u);
+                base = __shfl(base, 0);
+                if (base >= total) drained = true;
+                else {
+                    next = base;
+                    end = min(base + 64u, total);
+                }
+            }
+            if (!drained) {
+                uint32_t avail = end - next, rank = __popcll(idle & lt);
+                if (!active && rank < avail) {
+                    const ShadowQuery q = a.queue[next + rank];
+                    r.o = f4v(q.o);
+                    r.d = f4v(q.d);
+                    r.tmax = q.tmax;
+                    tmin = q.tmin;
+                    pixel = q.pixel;
+                    id = q.id;
+                    if (STATS) queries++;
+                    active = trace_begin(a.S, r, tmin, stk);
+                }
+                next += min((uint32_t)__popcll(idle), avail);
+            }
+        }
+        if (__ballot(active) == 0) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            int st = occlude_step<STATS>(a.S, r, tmin, C, stk);
+            if (st == 1) atomicOr(&a.mask[pixel], 1u << id);
+            if (st != 0) active = false;
+        }
+    }
+    if (STATS) {
+        atomicAdd(&a.stats[5], (unsigned long long)queries);
+        atomicAdd(&a.stats[6], (unsigned long long)C.shadow_node_pops);
+        atomicAdd(&a.stats[7], (unsigned long long)C.shadow_box_tests);
+        atomicAdd(&a.stats[8], (unsigned long long)C.shadow_prim_tests);
+    }
+}
+
+// Zeroes the per-frame queue counters (a kernel rather than a memset, so the frame's work can
+// be captured in a graph).
+__global__ void k_reset(uint32_t *counters, uint32_t n) {
+    uint32_t i = threadIdx.x;
+    if (i < n) counters[i] = 0u;
+}
+
+// Shading: the winner's record is intersected again (same inputs, same float operations, so the
+// same lambda, event and barycentrics), interpolated and shaded with shadow rays, haze and the
+// tile's stars; writes the HDR pixel and the optional per-pixel outputs.
+//   PASS 0: shadow rays traced in place.
+//   PASS 1: shadow queries queued (and the pixel's occlusion mask cleared); nothing else written.
+//   PASS 2: shadow queries answered from the occlusion masks.
+template <bool STATS, int SK, int PASS>
+__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const specrel_frame &f = *a.frame;
     const ViewDerived v = derive(f, a.num_stars);
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
     const uint32_t px = x0 + (wave & 1) * 8 + (lane & 7);
     const uint32_t py = y0 + (wave >> 1) * 8 + (lane >> 3);
-
-    __shared__ uint32_t s_list[STAR_CAP];
-    __shared__ uint32_t s_wave[4];
-    __shared__ uint32_t s_total;
+    __shared__ uint2 s_stack[PASS == 0 ? StackSel<SK>::lds_entries : 1];
+    __shared__ uint32_t s_list[PASS == 1 ? 1 : TILE_CAP];
+    __shared__ uint32_t s_wave[4], s_total;
+    typename StackSel<SK>::type stk;
+    if constexpr (PASS == 0) StackSel<SK>::init(stk, s_stack, tid);
     Counters C = {};
 
-    const bool cull = v.draw_stars && a.cull;
-    if (cull) {
-        // Bounding cone of the tile's pixel directions (pixel centres) in the camera frame.
-        uint32_t xa = x0, xb = min(x0 + TILE, v.w) - 1, ya = y0, yb = min(y0 + TILE, v.h) - 1;
-        double cd[4][3];
-        uint32_t cxs[4] = {xa, xb, xa, xb}, cys[4] = {ya, ya, yb, yb};
-        double cx = 0, cy = 0, cz = 0;
-        for (int k = 0; k < 4; ++k) {
-            double sx = pix_sx(v, cxs[k]), sy = pix_sy(v, cys[k]);
-            double n = sqrt(sx * sx + sy * sy + 1.0);
-            cd[k][0] = sx / n;
-            cd[k][1] = sy / n;
-            cd[k][2] = 1.0 / n;
-            cx += cd[k][0];
-            cy += cd[k][1];
-            cz += cd[k][2];
-        }
-        double cn = sqrt(cx * cx + cy * cy + cz * cz);
-        cx /= cn;
-        cy /= cn;
-        cz /= cn;
-        double cos_a = 1.0;
-        for (int k = 0; k < 4; ++k) cos_a = fmin(cos_a, cd[k][0] * cx + cd[k][1] * cy + cd[k][2] * cz);
-        // A star reaches a pixel only if tan(angle to the pixel ray) < 2.4 spread <= 2.4 pitch
-        // (sigma = 0.6 spread lambda, cut = 4 sigma); 5% and a quarter pixel of slack on top.
-        double beta = (1.05 * 2.4 + 0.25) * (double)v.pitch;
-        double ang = acos(fmin(1.0, cos_a)) + beta;
-        double cos_lim = ang >= 3.141592653589793 ? -2.0 : cos(ang);
-        if (tid == 0) s_total = 0;
+    // Star list of this tile: culled from its super tile's list (or from every star when that
+    // list overflowed); when this tile's own list overflows, the super-tile list is used as is.
+    const uint32_t *src = nullptr;
+    uint32_t n_src = 0, n_tile = 0;
+    bool use_tile_list = false;
+    if (PASS != 1 && v.draw_stars && a.cull) {
+        const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
+        const uint32_t st = (y0 / SUPER) * nsx + x0 / SUPER;
+        uint32_t sc = a.super_counts[st];
+        bool all = sc > SUPER_CAP;
+        src = all ? nullptr : a.super_lists + (size_t)st * SUPER_CAP;
+        n_src = all ? a.num_stars : sc;
+        __shared__ float s_cone[4];
+        if (tid == 0)
+            tile_cone(v, x0, min(x0 + TILE, v.w) - 1, y0, min(y0 + TILE, v.h) - 1, s_cone[0], s_cone[1], s_cone[2],
+                      s_cone[3]);
         __syncthreads();
-        for (uint32_t base = 0; base < a.num_stars; base += 256) {
-            uint32_t idx = base + tid;
-            bool keep = false;
-            if (idx < a.num_stars) {
-                const StarPrep &p = a.stars[idx];
-                if (p.flags & 2u) keep = true;
-                else if (p.flags & 1u) keep = p.dir.x * cx + p.dir.y * cy + p.dir.z * cz >= cos_lim - 1e-12;
-            }
-            uint64_t mask = __ballot(keep);
-            uint32_t before = __popcll(mask & ((1ull << lane) - 1ull));
-            if (lane == 0) s_wave[wave] = __popcll(mask);
-            __syncthreads();
-            uint32_t off = s_total;
-            for (uint32_t k = 0; k < wave; ++k) off += s_wave[k];
-            if (keep && off + before < STAR_CAP) s_list[off + before] = idx;
-            __syncthreads();
-            if (tid == 0) s_total += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
-            __syncthreads();
-        }
+        const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
+        n_tile = ordered_compact(
+            n_src,
+            [&](uint32_t k, uint32_t &item) {
+                item = src ? src[k] : k;
+                return star_in_cone(a.dirs[item], cx, cy, cz, cl);
+            },
+            s_list, TILE_CAP, s_wave, &s_total);
+        use_tile_list = n_tile <= TILE_CAP;
     }
 
     if (px < v.w && py < v.h) {
-        float sx = pix_sx(v, px), sy = pix_sy(v, py);
-        V3 local_dir = normalized(V3{sx, sy, 1.0f});
-        V4 dir;
-        if (f.slice_mode == SPECREL_VIEW_PAST_CONE) {
-            V3 n = normalized(local_dir);
-            dir = -v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
-        } else if (f.slice_mode == SPECREL_VIEW_FRONTIER) {
-            V3 n = normalized(local_dir);
-            dir = v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
-        } else {
-            dir = v.e1 * local_dir.x + v.e2 * local_dir.y + v.e3 * local_dir.z;
-        }
-        Hit hit;
-        bool found = !a.S.empty && trace<STATS>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C);
-        V3 color;
+        size_t pi = (size_t)py * v.w + px;
+        if (PASS == 1) a.mask[pi] = 0u;
+        V4 dir = ray_dir(v, f, px, py);
+        uint32_t slot = a.slot[pi];
+        bool found = slot != SPECREL_NO_HIT;
+        V3 color = V3{0.0f, 0.0f, 0.0f};
         Rec rec;
+        float lambda = 0.0f;
         if (found) {
-            interpolate(a.S.attrs + 12 * (size_t)hit.slot, hit, rec);
-            color = shade_hit<STATS>(a.S, a.T, f, dir, rec, C);
-            if (f.num_haze > 0) color = haze(color, rec.event, f);
-        } else {
+            Hit hit;
+            hit.slot = (int32_t)slot;
+            intersect(a.S.geoms + 5 * (size_t)slot, v.origin, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
+            lambda = hit.lambda;
+            interpolate(a.S.attrs + 12 * (size_t)slot, hit, rec);
+            if constexpr (PASS == 0) {
+                TraceShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk};
+                color = shade_hit(a.S, a.T, f, dir, rec, sh);
+            } else if constexpr (PASS == 1) {
+                EnqueueShadow sh{a.queue, a.qcount, a.qcap, (uint32_t)pi};
+                shade_hit(a.S, a.T, f, dir, rec, sh);
+            } else {
+                LookupShadow sh{a.mask[pi]};
+                color = shade_hit(a.S, a.T, f, dir, rec, sh);
+            }
+            if (PASS != 1 && f.num_haze > 0) color = haze(color, rec.event, f);
+        } else if (PASS != 1) {
             color = shade_sky(dir);
         }
-        if (v.draw_stars) {
-            float inv_len = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
-            float spread = v.pitch * inv_len * sqrtf(inv_len);
-            float lmax = found ? hit.lambda : __builtin_inff();
-            V4 n = dir - v.e0 * v.cone;
-            V3 total = V3{0.0f, 0.0f, 0.0f};
-            if (cull && s_total <= (uint32_t)STAR_CAP) {
-                for (uint32_t k = 0; k < s_total; ++k) star_pixel<STATS>(a.stars[s_list[k]], n, spread, lmax, total, C);
-            } else {
-                for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(a.stars[k], n, spread, lmax, total, C);
+        if (PASS != 1) {
+            if (v.draw_stars) {
+                float sx = pix_sx(v, px), sy = pix_sy(v, py);
+                float inv_len = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
+                float spread = v.pitch * inv_len * sqrtf(inv_len);
+                float lmax = found ? lambda : __builtin_inff();
+                V4 n = dir - v.e0 * v.cone;
+                V3 total = V3{0.0f, 0.0f, 0.0f};
+                if (a.cull && use_tile_list) {
+                    for (uint32_t k = 0; k < n_tile; ++k)
+                        star_pixel<STATS>(a.stars[s_list[k]], n, spread, lmax, total, C);
+                } else if (a.cull && src) {
+                    for (uint32_t k = 0; k < n_src; ++k) star_pixel<STATS>(a.stars[src[k]], n, spread, lmax, total, C);
+                } else {
+                    for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(a.stars[k], n, spread, lmax, total, C);
+                }
+                color += total;
             }
-            color += total;
-        }
-        size_t pi = (size_t)py * v.w + px;
-        a.hdr[pi] = make_float4(color.x, color.y, color.z, 0.0f);
-        if (a.want_hits) {
-            a.hit[pi] = found ? a.S.order[hit.slot] : SPECREL_NO_HIT;
-            a.lambda[pi] = found ? hit.lambda : __builtin_inff();
-            a.tau[pi] = found ? rec.tau : 0.0f;
+            a.hdr[pi] = make_float4(color.x, color.y, color.z, 0.0f);
+            if (a.want_hits) {
+                a.hit[pi] = found ? a.S.order[slot] : SPECREL_NO_HIT;
+                a.lambda[pi] = found ? lambda : __builtin_inff();
+                a.tau[pi] = found ? rec.tau : 0.0f;
+            }
+            if (STATS && found) C.hits++;
         }
     }
-
-    if (STATS) {
-        __shared__ unsigned long long s_c[10];
-        if (tid < 10) s_c[tid] = 0;
+    if (STATS && PASS != 1) {
+        __shared__ unsigned long long s_c[6];
+        if (tid < 6) s_c[tid] = 0;
         __syncthreads();
-        bool in = px < v.w && py < v.h;
-        atomicAdd(&s_c[0], (unsigned long long)(in ? 1 : 0));
-        atomicAdd(&s_c[1], (unsigned long long)C.node_pops);
-        atomicAdd(&s_c[2], (unsigned long long)C.box_tests);
-        atomicAdd(&s_c[3], (unsigned long long)C.prim_tests);
-        atomicAdd(&s_c[5], (unsigned long long)C.shadow_rays);
-        atomicAdd(&s_c[6], (unsigned long long)C.shadow_node_pops);
-        atomicAdd(&s_c[7], (unsigned long long)C.shadow_box_tests);
-        atomicAdd(&s_c[8], (unsigned long long)C.shadow_prim_tests);
-        atomicAdd(&s_c[9], (unsigned long long)C.star_tests);
-        if (in && a.want_hits && a.hit[(size_t)py * v.w + px] != SPECREL_NO_HIT) atomicAdd(&s_c[4], 1ull);
+        atomicAdd(&s_c[0], (unsigned long long)C.hits);
+        atomicAdd(&s_c[1], (unsigned long long)C.shadow_rays);
+        atomicAdd(&s_c[2], (unsigned long long)C.shadow_node_pops);
+        atomicAdd(&s_c[3], (unsigned long long)C.shadow_box_tests);
+        atomicAdd(&s_c[4], (unsigned long long)C.shadow_prim_tests);
+        atomicAdd(&s_c[5], (unsigned long long)C.star_tests);
         __syncthreads();
-        if (tid < 10) atomicAdd(&a.stats[tid], s_c[tid]);
+        if (tid < 6) atomicAdd(&a.stats[4 + tid], s_c[tid]);
     }
 }
 
-template __global__ void k_render<false>(RenderArgs);
-template __global__ void k_render<true>(RenderArgs);
+#define SPECREL_INSTANTIATE(K)                                    \
+    template __global__ void k_trace<false, K>(TraceArgs);       \
+    template __global__ void k_trace<true, K>(TraceArgs);        \
+    template __global__ void k_trace_p<false, K>(TraceArgs);     \
+    template __global__ void k_trace_p<true, K>(TraceArgs);      \
+    template __global__ void k_occlude_p<false, K>(OccludeArgs); \
+    template __global__ void k_occlude_p<true, K>(OccludeArgs);  \
+    template __global__ void k_shade<false, K, 0>(ShadeArgs);    \
+    template __global__ void k_shade<true, K, 0>(ShadeArgs);     \
+    template __global__ void k_shade<false, K, 1>(ShadeArgs);    \
+    template __global__ void k_shade<true, K, 1>(ShadeArgs);     \
+    template __global__ void k_shade<false, K, 2>(ShadeArgs);    \
+    template __global__ void k_shade<true, K, 2>(ShadeArgs);
+SPECREL_INSTANTIATE(8)
+SPECREL_INSTANTIATE(16)
+SPECREL_INSTANTIATE(-4)
+SPECREL_INSTANTIATE(-8)
+SPECREL_INSTANTIATE(-12)
+SPECREL_INSTANTIATE(-16)
 
 // ---------------------------------------------------------------------------------------
 // Bloom (bloom.rs:22-125): the pyramid lives after the full-resolution HDR image.

@@ -19,15 +19,16 @@ from . import _ffi as F
 from .scene import build_frame
 
 OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
-OPT_STATS, OPT_STAR_CULL = 1, 2
+OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE = 1, 2, 3, 4, 5
 
 
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
-                                         "trace_ms", "bloom_ms", "composite_ms")]
+                                         "trace_ms", "shade_ms", "bloom_ms", "composite_ms")] + \
+        [("reserved", C.c_float * 3)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
 
 
 class SpecrelError(RuntimeError):

@@ -37,7 +37,7 @@ def test_layouts_match_the_header():
     assert F.Attr.misc.offset == 160 and F.Attr.decided_at.offset == 176
     assert C.sizeof(F.SceneArrays) == 88 and F.SceneArrays.light_pos.offset == 72
     assert C.sizeof(F.Stats) == 128
-    assert C.sizeof(R.Timings) == 32
+    assert C.sizeof(R.Timings) == 48
 
 
 def test_argument_errors_are_codes_not_aborts():

@@ -113,3 +113,37 @@ def test_traversal_counters_match_oracle(gpu_ctx, twin):
     for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops",
               "shadow_box_tests", "shadow_prim_tests"):
         assert st[k] == oc[k], (k, st[k], oc[k])
+
+
+@pytest.mark.parametrize("depth", [8, -4, -8, -16])
+def test_stack_variants_are_exact(gpu_ctx, twin, depth):
+    # LDS stacks (K > 0) and register windows (-N): entries beyond either live in private memory;
+    # a 4-entry window sends most rays there.
+    from specrel_amd.renderer import OPT_STACK
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.3, 0.0, 0.5))
+    fr = S.build_frame(cam, S.RenderConfig(192, 120))
+    gpu_ctx.set_option(OPT_STACK, depth)
+    try:
+        g, o = render_both(gpu_ctx, twin, fr)
+    finally:
+        gpu_ctx.set_option(OPT_STACK, 16)
+    assert_same(g, o, f"lds stack {depth}")
+
+
+@pytest.mark.parametrize("persistent,queue", [(0, 0), (1, 0), (0, 1)])
+def test_traversal_variants_are_exact(gpu_ctx, twin, persistent, queue):
+    # tile-launched vs persistent primary rays; shadow rays in place vs queued (with flashes)
+    from specrel_amd.renderer import OPT_PERSISTENT, OPT_SHADOW_QUEUE
+    cam = twin.default_camera()
+    p = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((p[0] - 6.0, p[1], p[2], p[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    fr = S.build_frame(cam, S.RenderConfig(192, 120), dyn)
+    gpu_ctx.set_option(OPT_PERSISTENT, persistent)
+    gpu_ctx.set_option(OPT_SHADOW_QUEUE, queue)
+    try:
+        g, o = render_both(gpu_ctx, twin, fr)
+    finally:
+        gpu_ctx.set_option(OPT_PERSISTENT, 1)
+        gpu_ctx.set_option(OPT_SHADOW_QUEUE, 1)
+    assert_same(g, o, f"persistent={persistent} queue={queue}")
