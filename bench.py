@@ -7,13 +7,17 @@ constant proper acceleration), 128 time slices each = 15,728,640 tetrahedra, plu
 on a sphere of radius 500 (T log-uniform 3000-30000 K) -- seen by an observer at the origin
 event boosted to rapidity 1.0, 1280x800 past-light-cone rays, null shadows, f32.
 
-A step is one full frame: star pass, fused trace/shade/star kernel, bloom pyramid, tone map to
-RGB8, with every input already resident in HBM and the image left there (no readback).
+A step is one full frame: star pass and star binning, primary-ray trace, shading (with shadow
+rays and the binned stars), bloom pyramid, tone map to RGB8, with every input already resident
+in HBM and the image left there (no readback).
 value = primary rays per second of the whole job = W*H*steps / wall time, in Mrays/s.
 
-Prints ONE JSON line. Extra legs: `roofline` for the dominant kernel (k_render) from the
+Prints ONE JSON line. Extra legs: `roofline` for the dominant kernel (k_trace) from the
 instrumented traversal counters and HIP-event durations; `cpu_baseline` from the CPU oracle
-(a restatement of the reference renderer) timed on this host's cores on a bounded row sample.
+(a restatement of the reference renderer) timed on this host's cores on a bounded row sample;
+`batched`: 64 observers of uneven resolution (160x100 .. 1920x1200) over the same scene in one
+batched render (autotuned launch configuration, graph replay) against a loop of single-view
+renders of the same frames, in views/s.
 """
 import argparse
 import json
@@ -75,9 +79,61 @@ def cpu_baseline(scene, frame, w, h, rows):
     }
 
 
+def bench_batch(ctx, scene, n_views, steps, seed):
+    """Views/s of one batched render of `n_views` observers against a loop of single-view
+    renders of the same frames (each enqueue + wait, outputs left resident either way)."""
+    from specrel_amd import scene as S
+    from specrel_amd.renderer import OPT_GRAPH
+
+    obs = S.batch_observers(S.scene_box(scene), n_views, seed=seed)
+    frames = [S.build_frame(c, r) for c, r in obs]
+    rays = sum(f.size[0] * f.size[1] for f in frames)
+    arr = ctx._frames(frames)
+    log(f"batch: {n_views} views, {rays} rays; autotuning")
+    tuning = ctx.autotune(0, arr, iters=2)
+    log(f"autotune: {tuning}")
+    ctx.render_batch_resident(0, arr)
+    ctx.sync()
+    t0 = time.perf_counter()
+    kern = 0.0
+    for _ in range(steps):
+        ctx.render_batch_resident(0, arr)
+        ctx.sync()
+        kern += ctx.timings()["kernel_ms"]
+    batch_s = (time.perf_counter() - t0) / steps
+    # The loop a caller of the single-view interface writes: one render per view, launched
+    # directly (no graph), with the same launch options otherwise.
+    ctx.set_option(OPT_GRAPH, 0)
+    for f in frames[:4]:
+        ctx.render_resident(0, f)
+        ctx.sync()
+    loops = max(1, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(loops):
+        for f in frames:
+            ctx.render_resident(0, f)
+            ctx.sync()
+    loop_s = (time.perf_counter() - t0) / loops
+    ctx.set_option(OPT_GRAPH, tuning["graph"])
+    return {
+        "views": n_views,
+        "rays": rays,
+        "views_per_s": round(n_views / batch_s, 2),
+        "Mrays_per_s": round(rays / batch_s / 1e6, 2),
+        "ms_per_batch": round(batch_s * 1e3, 3),
+        "kernel_ms_per_batch": round(kern / steps, 3),
+        "loop_views_per_s": round(n_views / loop_s, 2),
+        "loop_ms": round(loop_s * 1e3, 3),
+        "speedup_vs_loop": round(loop_s / batch_s, 3),
+        "tuning": tuning,
+        "workload": f"{n_views} observers, width U[160,1920] x height width*5/8, events U(scene box), "
+                    f"|rapidity| U[0,2], random rotate_local, past-cone/simultaneity mix, seed {seed}",
+    }
+
+
 def copy_probe():
     """A plain copy kernel of known size: 2^28 floats (1 GiB read + 1 GiB written per launch).
-    Its FETCH_SIZE / WRITE_SIZE readings scale the counters of k_render to bytes."""
+    Its FETCH_SIZE / WRITE_SIZE readings scale the counters of the renderer kernels to bytes."""
     import torch
 
     x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -98,7 +154,9 @@ def main():
     ap.add_argument("--objects", type=int, default=128)
     ap.add_argument("--slices", type=int, default=128)
     ap.add_argument("--stars", type=int, default=10000)
-    ap.add_argument("--cpu-rows", type=int, default=32)
+    ap.add_argument("--cpu-rows", type=int, default=400)
+    ap.add_argument("--views", type=int, default=64, help="observers of the batched leg (0: skip it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stack", type=int, default=0,
                     help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
@@ -187,6 +245,11 @@ def main():
                                       "achieved_GBs": round(alg[k] / (times[k] * 1e-3) / 1e9, 2)}
                                   for k in times if k != dom}}
 
+    batched = None
+    if args.views > 0:
+        batched = bench_batch(ctx, scene, args.views, args.batch_steps, seed=2)
+        log(f"batched: {batched}")
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline: oracle on a row sample")
@@ -216,6 +279,7 @@ def main():
         "counters": st,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

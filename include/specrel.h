@@ -47,15 +47,38 @@ typedef struct specrel_timings {
     float reserved[3];
 } specrel_timings;
 
-/* Options for specrel_set_option. */
+/* Options for specrel_set_option. No counterpart in the reference: launch choices of this backend. */
 enum {
     SPECREL_OPT_STATS = 1,      /* 1: run the instrumented trace variant and fill specrel_stats */
     SPECREL_OPT_STAR_CULL = 2,  /* 1 (default): per-tile star culling; 0: every star per pixel */
     SPECREL_OPT_STACK = 3,      /* traversal stack: 8 or 16 (default) positions per lane in LDS, or
                                    -4, -8, -12, -16: that many top entries in registers */
     SPECREL_OPT_PERSISTENT = 4, /* 1: persistent primary-ray kernel with per-lane refill; 0 (default): one tile per block */
-    SPECREL_OPT_SHADOW_QUEUE = 5 /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
+    SPECREL_OPT_SHADOW_QUEUE = 5, /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
+    SPECREL_OPT_GRAPH = 6         /* replay a captured HIP graph of the launches: 0 never, 1 (default) for
+                                     batches, 2 for every render. Graph renders report kernel_ms only. */
 };
+
+/* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are
+ * the view's width*height. */
+typedef struct specrel_view_out {
+    uint32_t *rgb;
+    uint32_t *hit;
+    float *lambda;
+    float *tau;
+} specrel_view_out;
+
+/* Launch configuration chosen by specrel_autotune (it is also applied to the context). */
+typedef struct specrel_tuning {
+    int32_t stack;        /* SPECREL_OPT_STACK */
+    int32_t persistent;   /* SPECREL_OPT_PERSISTENT */
+    int32_t shadow_queue; /* SPECREL_OPT_SHADOW_QUEUE */
+    int32_t graph;        /* SPECREL_OPT_GRAPH */
+    uint32_t candidates;  /* configurations timed */
+    float best_ms;        /* median wall time per render of the batch with the choice */
+    float default_ms;     /* the same with the options the context had before */
+    uint32_t reserved[5];
+} specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
 int specrel_create(int device_ordinal, specrel_ctx **out);
@@ -85,6 +108,26 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
 int specrel_sync(specrel_ctx *ctx);
 int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau);
+
+/* Batched observers: n_views (1..4096) frames of any sizes and modes rendered over one scene
+ * with one launch per stage (each kernel's grid spans the tiles of every view), replayed from a
+ * captured graph (SPECREL_OPT_GRAPH). Every view's outputs are bit-identical to specrel_render of
+ * its frame alone. No counterpart in the reference, whose callers loop over
+ * GpuRenderer::render (gpu.rs:798); `outs` (n_views entries) may be NULL to leave everything
+ * resident. Synchronous, like specrel_render. */
+int specrel_render_batch(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
+                         const specrel_view_out *outs);
+/* The same, returning once enqueued; specrel_download_view copies one view afterwards (view 0
+ * of a single render is what specrel_download copies). */
+int specrel_render_batch_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames,
+                                  uint32_t n_views, int want_hits);
+int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda,
+                          float *out_tau);
+/* Times every launch configuration (stack depth 16/8 in LDS, persistent or tiled primary rays,
+ * queued or in-place shadow rays, graph replay or direct launches) on this batch, `iters`
+ * renders each after one warm-up, and keeps the fastest by median wall time. */
+int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
+                     uint32_t iters, specrel_tuning *out);
 
 int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
 /* Counters of the last render made with SPECREL_OPT_STATS on. */

@@ -2,6 +2,7 @@
 // into the device layout, render, timings. The counterpart of GpuRenderer (src/gpu.rs:432-901).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -39,15 +40,25 @@ struct specrel_ctx {
     std::string name, err;
     hipStream_t stream = nullptr;
     std::vector<SceneSlot> scenes;
-    uint32_t fw = 0, fh = 0;
-    DevBuf hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, frame, tables, stats;
+    // Per-launch view tables (prefix sums, see dev::Views) and the buffers sized for them.
+    std::vector<uint32_t> tab_host;
+    std::vector<uint32_t> sizes;  // w, h per view of the last render
+    uint32_t n_views = 0, total_tiles = 0, total_supers = 0, total_pix = 0, total_pyr = 0;
+    uint32_t stars_for = 0;  // num_stars the star buffers are sized for (per view)
+    DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
     DevBuf counters, mask, squeue;
+    uint64_t generation = 0;  // bumped whenever a buffer a captured graph points to moves
     int cus = 0;
     hipEvent_t ev[7] = {};
+    bool timed_stages = true;
     specrel_timings timings{};
     specrel_stats stats_host{};
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
+    int opt_graph = 1;
     bool have_hits = false;
+    // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint64_t> graph_key;
 };
 
 namespace {
@@ -77,6 +88,13 @@ void dalloc(DevBuf &b, size_t bytes) {
     b.bytes = bytes;
 }
 
+// dalloc for buffers that captured graphs reference: a move invalidates the graph.
+void galloc(specrel_ctx *ctx, DevBuf &b, size_t bytes) {
+    void *before = b.p;
+    dalloc(b, bytes);
+    if (b.p != before) ctx->generation++;
+}
+
 void upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s) {
     dalloc(b, bytes);
     if (bytes) check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
@@ -93,35 +111,6 @@ int guarded(specrel_ctx *ctx, F &&f) {
         if (ctx) ctx->err = e.what();
         return SPECREL_EINVAL;
     }
-}
-
-size_t pyramid_len(uint32_t w, uint32_t h) {  // bloom.rs:47-49
-    size_t n = 0;
-    for (int k = 0; k <= SPECREL_BLOOM_LEVELS; ++k) {
-        n += (size_t)w * h;
-        w = (w + 1) / 2;
-        h = (h + 1) / 2;
-    }
-    return n;
-}
-
-void level_size(uint32_t w, uint32_t h, int k, uint32_t &ow, uint32_t &oh) {
-    for (int i = 0; i < k; ++i) {
-        w = (w + 1) / 2;
-        h = (h + 1) / 2;
-    }
-    ow = w;
-    oh = h;
-}
-
-size_t level_offset(uint32_t w, uint32_t h, int k) {
-    size_t off = 0;
-    for (int i = 0; i < k; ++i) {
-        uint32_t lw, lh;
-        level_size(w, h, i, lw, lh);
-        off += (size_t)lw * lh;
-    }
-    return off;
 }
 
 // Re-pack the reference arrays into the device layout, validating the tree on the way.
@@ -228,27 +217,93 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
 
 uint32_t num_super(uint32_t w, uint32_t h) { return ((w + SUPER - 1) / SUPER) * ((h + SUPER - 1) / SUPER); }
 
-void ensure_frame(specrel_ctx *ctx, uint32_t w, uint32_t h, uint32_t num_stars) {
-    if (ctx->fw != w || ctx->fh != h) {
-        dalloc(ctx->hdr, pyramid_len(w, h) * sizeof(float4));
-        dalloc(ctx->rgb, (size_t)w * h * 4);
-        dalloc(ctx->hit, (size_t)w * h * 4);
-        dalloc(ctx->lam, (size_t)w * h * 4);
-        dalloc(ctx->tau, (size_t)w * h * 4);
-        dalloc(ctx->slot, (size_t)w * h * 4);
-        dalloc(ctx->mask, (size_t)w * h * 4);
-        dalloc(ctx->super_counts, (size_t)num_super(w, h) * 4);
-        dalloc(ctx->super_lists, (size_t)num_super(w, h) * SUPER_CAP * 4);
-        ctx->fw = w;
-        ctx->fh = h;
-    }
-    dalloc(ctx->prep, (size_t)(num_stars ? num_stars : 1) * sizeof(StarPrep));
-    dalloc(ctx->dirs, (size_t)(num_stars ? num_stars : 1) * sizeof(StarDir));
+constexpr int NLEV = SPECREL_BLOOM_LEVELS + 1;
+// Rows of the view table: NLEV tile levels, then super tiles, pixels, pyramid float4s.
+constexpr int ROW_SUPER = NLEV, ROW_PIX = NLEV + 1, ROW_PYR = NLEV + 2, ROWS = NLEV + 3;
+constexpr uint32_t MAX_VIEWS = 4096;
+
+void check_frame(const specrel_frame *f, uint32_t i) {
+    const uint32_t w = f->size[0], h = f->size[1];
+    std::string at = " (view " + std::to_string(i) + ")";
+    if (w == 0 || h == 0 || w > 32768 || h > 32768) throw Fail{SPECREL_EINVAL, "bad frame size" + at};
+    if (f->mode > 3 || f->shadow_mode > 2 || f->slice_mode > 3) throw Fail{SPECREL_EINVAL, "bad render mode" + at};
+    if (f->num_flashes > SPECREL_MAX_FLASHES || f->num_haze > SPECREL_MAX_HAZE)
+        throw Fail{SPECREL_EINVAL, "too many flashes or haze origins" + at};
 }
 
-// The traversal stack is a launch choice (SPECREL_OPT_STACK): K positions per lane in LDS, or
-// the top N entries in registers; deeper entries use private memory either way.
-// Persistent kernels are launched with exactly as many blocks as fit on the device at once.
+// Sizes every per-launch buffer for these views (growing only) and uploads the view table when
+// the shape changed.
+void ensure_views(specrel_ctx *ctx, const specrel_frame *frames, uint32_t n, uint32_t num_stars) {
+    std::vector<uint32_t> sizes(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        sizes[2 * i] = frames[i].size[0];
+        sizes[2 * i + 1] = frames[i].size[1];
+    }
+    if (sizes != ctx->sizes) {
+        const size_t stride = (size_t)n + 1;
+        std::vector<uint32_t> tab((size_t)ROWS * stride, 0u);
+        uint64_t acc[ROWS] = {};
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t w = sizes[2 * i], h = sizes[2 * i + 1];
+            for (int k = 0; k < NLEV; ++k) {
+                uint32_t lw, lh;
+                level_size(w, h, k, lw, lh);
+                acc[k] += (uint64_t)((lw + 15) / 16) * ((lh + 15) / 16);
+            }
+            acc[ROW_SUPER] += num_super(w, h);
+            acc[ROW_PIX] += (uint64_t)w * h;
+            acc[ROW_PYR] += pyramid_len(w, h);
+            for (int r = 0; r < ROWS; ++r) {
+                if (acc[r] >= 0xFFFFFFFFull || acc[0] >= (1ull << 24))
+                    throw Fail{SPECREL_EINVAL, "views too large for one launch"};
+                tab[(size_t)r * stride + i + 1] = (uint32_t)acc[r];
+            }
+        }
+        ctx->total_tiles = (uint32_t)acc[0];
+        ctx->total_supers = (uint32_t)acc[ROW_SUPER];
+        ctx->total_pix = (uint32_t)acc[ROW_PIX];
+        ctx->total_pyr = (uint32_t)acc[ROW_PYR];
+        const size_t np = ctx->total_pix;
+        galloc(ctx, ctx->hdr, (size_t)ctx->total_pyr * sizeof(float4));
+        galloc(ctx, ctx->rgb, np * 4);
+        galloc(ctx, ctx->hit, np * 4);
+        galloc(ctx, ctx->lam, np * 4);
+        galloc(ctx, ctx->tau, np * 4);
+        galloc(ctx, ctx->slot, np * 4);
+        galloc(ctx, ctx->mask, np * 4);
+        galloc(ctx, ctx->super_counts, (size_t)ctx->total_supers * 4);
+        galloc(ctx, ctx->super_lists, (size_t)ctx->total_supers * SUPER_CAP * 4);
+        galloc(ctx, ctx->frames, (size_t)n * sizeof(specrel_frame));
+        galloc(ctx, ctx->tab, tab.size() * 4);
+        ctx->tab_host.swap(tab);
+        check(hipMemcpyAsync(ctx->tab.p, ctx->tab_host.data(), ctx->tab_host.size() * 4, hipMemcpyHostToDevice,
+                             ctx->stream),
+              "view table upload");
+        ctx->sizes.swap(sizes);
+        ctx->n_views = n;
+    }
+    const size_t ns = (size_t)n * (num_stars ? num_stars : 1);
+    galloc(ctx, ctx->prep, ns * sizeof(StarPrep));
+    galloc(ctx, ctx->dirs, ns * sizeof(StarDir));
+}
+
+Views device_views(specrel_ctx *ctx) {
+    Views V;
+    const uint32_t *t = (const uint32_t *)ctx->tab.p;
+    const size_t stride = (size_t)ctx->n_views + 1;
+    V.frames = (const specrel_frame *)ctx->frames.p;
+    for (int k = 0; k < NLEV; ++k) V.level_start[k] = t + k * stride;
+    V.super_start = t + ROW_SUPER * stride;
+    V.pix_start = t + ROW_PIX * stride;
+    V.pyr_start = t + ROW_PYR * stride;
+    V.n = ctx->n_views;
+    return V;
+}
+
+uint32_t host_start(const specrel_ctx *ctx, int row, uint32_t view) {
+    return ctx->tab_host[(size_t)row * (ctx->n_views + 1) + view];
+}
+
 template <class Kern>
 dim3 resident_grid(specrel_ctx *ctx, Kern kern) {
     int per_cu = 0;
@@ -306,63 +361,95 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f, bool want_hits) {
-    if (!f) throw Fail{SPECREL_EINVAL, "frame is NULL"};
+// What a launch over the current views needs, decided on the host from the frames.
+struct Plan {
+    bool stars = false;    // some view is shaded and the scene has stars (renderer.rs:357)
+    bool queries = false;  // some view is shaded with shadows on (renderer.rs:631-641, 680-682)
+    uint32_t max_flashes = 0;
+    bool want_hits = false;
+};
+
+// Validates the frames, sizes the buffers, uploads the frames: everything before the launches.
+Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
+    if (!frames) throw Fail{SPECREL_EINVAL, "frame is NULL"};
+    if (n == 0 || n > MAX_VIEWS) throw Fail{SPECREL_EINVAL, "view count must be 1.." + std::to_string(MAX_VIEWS)};
     if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
         throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
-    const uint32_t w = f->size[0], h = f->size[1];
-    if (w == 0 || h == 0 || w > 32768 || h > 32768) throw Fail{SPECREL_EINVAL, "bad frame size"};
-    if (f->mode > 3 || f->shadow_mode > 2 || f->slice_mode > 3) throw Fail{SPECREL_EINVAL, "bad render mode"};
-    if (f->num_flashes > SPECREL_MAX_FLASHES || f->num_haze > SPECREL_MAX_HAZE)
-        throw Fail{SPECREL_EINVAL, "too many flashes or haze origins"};
     SceneSlot &sc = ctx->scenes[scene_idx];
-    ensure_frame(ctx, w, h, sc.num_stars);
-    hipStream_t s = ctx->stream;
-    auto t0 = std::chrono::steady_clock::now();
-    check(hipMemcpyAsync(ctx->frame.p, f, sizeof(specrel_frame), hipMemcpyHostToDevice, s), "frame upload");
-    const bool stats = ctx->opt_stats != 0;
-    want_hits = want_hits || stats;
-    if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
+    Plan pl;
+    for (uint32_t i = 0; i < n; ++i) {
+        const specrel_frame &f = frames[i];
+        check_frame(&f, i);
+        if (f.mode == SPECREL_MODE_SHADED) {
+            pl.stars = pl.stars || sc.num_stars > 0;
+            if (f.shadow_mode != SPECREL_SHADOW_OFF) {
+                pl.queries = true;
+                pl.max_flashes = std::max(pl.max_flashes, f.num_flashes);
+            }
+        }
+    }
+    pl.want_hits = want_hits || ctx->opt_stats != 0;
+    ensure_views(ctx, frames, n, sc.num_stars);
+    if (pl.queries && ctx->opt_shadow_queue) {
+        uint64_t cap = (uint64_t)ctx->total_pix * (1u + pl.max_flashes);
+        if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "views too large for the shadow queue"};
+        galloc(ctx, ctx->squeue, (size_t)cap * sizeof(ShadowQuery));
+    }
+    check(hipMemcpyAsync(ctx->frames.p, frames, (size_t)n * sizeof(specrel_frame), hipMemcpyHostToDevice,
+                         ctx->stream),
+          "frame upload");
+    return pl;
+}
 
-    k_reset<<<1, 64, 0, s>>>((uint32_t *)ctx->counters.p, 16);
+// The launch sequence of one render of the uploaded views; stage events only when `events`
+// (never inside a graph capture).
+void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool events) {
+    hipStream_t s = ctx->stream;
+    const bool stats = ctx->opt_stats != 0;
+    auto mark = [&](int i) {
+        if (events) check(hipEventRecord(ctx->ev[i], s), "event");
+    };
+    const Views V = device_views(ctx);
+    const uint32_t nv = ctx->n_views;
+    uint32_t *counters = (uint32_t *)ctx->counters.p;
+    k_reset<<<1, 64, 0, s>>>(counters, 16);
     check(hipGetLastError(), "k_reset launch");
-    check(hipEventRecord(ctx->ev[0], s), "event");
-    const bool draw_stars = f->mode == SPECREL_MODE_SHADED && sc.num_stars > 0;
-    const specrel_frame *dframe = (const specrel_frame *)ctx->frame.p;
-    if (draw_stars) {
-        k_star_prep<<<(sc.num_stars + 255) / 256, 256, 0, s>>>((const float4 *)sc.stars.p, sc.num_stars, dframe,
-                                                             (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
+    mark(0);
+    if (pl.stars) {
+        k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, s>>>(
+            (const float4 *)sc.stars.p, sc.num_stars, V, (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
         check(hipGetLastError(), "k_star_prep launch");
         if (ctx->opt_cull) {
-            k_star_bin<<<num_super(w, h), 256, 0, s>>>(dframe, (const StarDir *)ctx->dirs.p, sc.num_stars,
-                                                       (uint32_t *)ctx->super_lists.p, (uint32_t *)ctx->super_counts.p);
+            k_star_bin<<<ctx->total_supers, 256, 0, s>>>(V, (const StarDir *)ctx->dirs.p, sc.num_stars,
+                                                          (uint32_t *)ctx->super_lists.p,
+                                                          (uint32_t *)ctx->super_counts.p);
             check(hipGetLastError(), "k_star_bin launch");
         }
     }
-    check(hipEventRecord(ctx->ev[1], s), "event");
-    uint32_t *counters = (uint32_t *)ctx->counters.p;
-    dim3 tiles((w + TILE - 1) / TILE, (h + TILE - 1) / TILE);
+    mark(1);
+    const dim3 tiles(ctx->total_tiles);
     TraceArgs ta;
     ta.S = sc.S;
-    ta.frame = dframe;
+    ta.V = V;
+    ta.num_tiles = ctx->total_tiles;
     ta.slot = (uint32_t *)ctx->slot.p;
     ta.queue = counters + 0;
     ta.stats = (unsigned long long *)ctx->stats.p;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
-    check(hipEventRecord(ctx->ev[2], s), "event");
+    mark(2);
     ShadeArgs ra;
     ra.S = sc.S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;
     ra.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
-    ra.frame = dframe;
+    ra.V = V;
     ra.stars = (const StarPrep *)ctx->prep.p;
     ra.dirs = (const StarDir *)ctx->dirs.p;
     ra.super_lists = (const uint32_t *)ctx->super_lists.p;
     ra.super_counts = (const uint32_t *)ctx->super_counts.p;
     ra.num_stars = sc.num_stars;
     ra.cull = ctx->opt_cull;
-    ra.want_hits = want_hits ? 1 : 0;
+    ra.want_hits = pl.want_hits ? 1 : 0;
     ra.slot = (const uint32_t *)ctx->slot.p;
     ra.hdr = (float4 *)ctx->hdr.p;
     ra.hit = (uint32_t *)ctx->hit.p;
@@ -373,14 +460,9 @@ int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f,
     ra.qcount = counters + 1;
     ra.qcap = 0;
     ra.stats = (unsigned long long *)ctx->stats.p;
-    // Shadow queries exist only for shaded views with shadows on (renderer.rs:631-641, 680-682).
-    const bool queries = f->mode == SPECREL_MODE_SHADED && f->shadow_mode != SPECREL_SHADOW_OFF;
-    if (queries && ctx->opt_shadow_queue) {
-        uint64_t cap = (uint64_t)w * h * (1u + f->num_flashes);
-        if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "frame too large for the shadow queue"};
-        dalloc(ctx->squeue, (size_t)cap * sizeof(ShadowQuery));
+    if (pl.queries && ctx->opt_shadow_queue) {
         ra.queue = (ShadowQuery *)ctx->squeue.p;
-        ra.qcap = (uint32_t)cap;
+        ra.qcap = (uint32_t)((uint64_t)ctx->total_pix * (1u + pl.max_flashes));
         launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
         check(hipGetLastError(), "k_shade setup launch");
         OccludeArgs oa;
@@ -398,47 +480,87 @@ int enqueue_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *f,
         launch_any(ctx, stats, tiles, s, ShadePass<0>{ra});
     }
     check(hipGetLastError(), "k_shade launch");
-    check(hipEventRecord(ctx->ev[3], s), "event");
+    mark(3);
     float4 *pyr = (float4 *)ctx->hdr.p;
-    for (int k = 1; k <= SPECREL_BLOOM_LEVELS; ++k) {
-        uint32_t sw, sh, dw, dh;
-        level_size(w, h, k - 1, sw, sh);
-        level_size(w, h, k, dw, dh);
-        dim3 g((dw + 15) / 16, (dh + 15) / 16);
-        k_bloom_down<<<g, 256, 0, s>>>(pyr + level_offset(w, h, k - 1), sw, sh, pyr + level_offset(w, h, k), dw, dh);
-    }
-    for (int k = SPECREL_BLOOM_LEVELS - 1; k >= 1; --k) {
-        uint32_t lw, lh, cw, ch;
-        level_size(w, h, k, lw, lh);
-        level_size(w, h, k + 1, cw, ch);
-        dim3 g((lw + 15) / 16, (lh + 15) / 16);
-        k_bloom_up<<<g, 256, 0, s>>>(pyr + level_offset(w, h, k), lw, lh, pyr + level_offset(w, h, k + 1), cw, ch);
-    }
+    for (int k = 1; k <= SPECREL_BLOOM_LEVELS; ++k)
+        k_bloom_down<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
+    for (int k = SPECREL_BLOOM_LEVELS - 1; k >= 1; --k)
+        k_bloom_up<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
     check(hipGetLastError(), "bloom launch");
-    check(hipEventRecord(ctx->ev[4], s), "event");
-    {
-        uint32_t w1, h1;
-        level_size(w, h, 1, w1, h1);
-        dim3 g((w + 15) / 16, (h + 15) / 16);
-        k_composite<<<g, 256, 0, s>>>(pyr, w, h, pyr + level_offset(w, h, 1), w1, h1, (const uint8_t *)ctx->tables.p,
-                                      (uint32_t *)ctx->rgb.p);
-        check(hipGetLastError(), "k_composite launch");
+    mark(4);
+    k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, (uint32_t *)ctx->rgb.p);
+    check(hipGetLastError(), "k_composite launch");
+    mark(5);
+}
+
+void drop_graph(specrel_ctx *ctx) {
+    if (ctx->graph) (void)hipGraphExecDestroy(ctx->graph);
+    ctx->graph = nullptr;
+    ctx->graph_key.clear();
+}
+
+// Enqueues one render of n views. Without stats, batches (or every render, with
+// SPECREL_OPT_GRAPH = 2) replay a captured graph of the launch sequence, captured again only when
+// the view sizes, options or any buffer it references changed.
+int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
+    auto t0 = std::chrono::steady_clock::now();
+    const Plan pl = prepare(ctx, scene_idx, frames, n, want_hits);
+    SceneSlot &sc = ctx->scenes[scene_idx];
+    hipStream_t s = ctx->stream;
+    const bool stats = ctx->opt_stats != 0;
+    if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
+    const bool use_graph = !stats && (ctx->opt_graph == 2 || (ctx->opt_graph == 1 && n > 1));
+    if (use_graph) {
+        std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
+                                     (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
+                                     (uint64_t)ctx->opt_shadow_queue,   pl.stars,         pl.queries,
+                                     pl.max_flashes,    pl.want_hits};
+        key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
+        if (!ctx->graph || key != ctx->graph_key) {
+            drop_graph(ctx);
+            check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                record_launches(ctx, sc, pl, false);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&ctx->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ctx->graph = nullptr;
+                check(e, "hipGraphInstantiate");
+            }
+            ctx->graph_key = key;
+        }
+        check(hipEventRecord(ctx->ev[0], s), "event");
+        check(hipGraphLaunch(ctx->graph, s), "hipGraphLaunch");
+        check(hipEventRecord(ctx->ev[5], s), "event");
+        ctx->timed_stages = false;
+    } else {
+        record_launches(ctx, sc, pl, true);
+        ctx->timed_stages = true;
     }
-    check(hipEventRecord(ctx->ev[5], s), "event");
     auto t1 = std::chrono::steady_clock::now();
     ctx->timings = specrel_timings{};
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
-    ctx->have_hits = want_hits;
+    ctx->have_hits = pl.want_hits;
     return SPECREL_OK;
 }
 
 void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
-    t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
-    t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
-    t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
-    t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
-    t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);
+    if (ctx->timed_stages) {
+        t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
+        t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
+        t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
+        t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
+        t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);
+    }
     t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[5]);
     if (ctx->opt_stats) {
         unsigned long long c[16];
@@ -456,6 +578,19 @@ void collect(specrel_ctx *ctx) {
         st.shadow_prim_tests = c[8];
         st.star_tests = c[9];
     }
+}
+
+// Copies view `view` of the last render to the host (NULL pointers are skipped).
+void download_view(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit, float *lam, float *tau) {
+    if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+    if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
+    if ((hit || lam || tau) && !ctx->have_hits) throw Fail{SPECREL_EINVAL, "the last render did not write hit buffers"};
+    const size_t off = host_start(ctx, ROW_PIX, view);
+    const size_t n = (size_t)ctx->sizes[2 * view] * ctx->sizes[2 * view + 1];
+    if (rgb) check(hipMemcpy(rgb, (uint32_t *)ctx->rgb.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
+    if (hit) check(hipMemcpy(hit, (uint32_t *)ctx->hit.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
+    if (lam) check(hipMemcpy(lam, (float *)ctx->lam.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
+    if (tau) check(hipMemcpy(tau, (float *)ctx->tau.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
 }
 
 }  // namespace
@@ -496,7 +631,6 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         std::vector<uint8_t> bytes(words.size());
         for (size_t i = 0; i < words.size(); ++i) bytes[i] = (uint8_t)words[i];
         upload(ctx->tables, bytes.data(), bytes.size(), ctx->stream);
-        dalloc(ctx->frame, sizeof(specrel_frame));
         dalloc(ctx->stats, 16 * 8);
         dalloc(ctx->counters, 16 * 4);
         ctx->cus = prop.multiProcessorCount;
@@ -523,7 +657,8 @@ void specrel_destroy(specrel_ctx *ctx) {
     dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
     dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
-    dfree(ctx->frame), dfree(ctx->tables), dfree(ctx->stats);
+    dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
+    drop_graph(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -545,7 +680,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         ctx->opt_stack = value;
     } else if (option == SPECREL_OPT_PERSISTENT) ctx->opt_persistent = value != 0;
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
-    else {
+    else if (option == SPECREL_OPT_GRAPH) {
+        if (value < 0 || value > 2) {
+            ctx->err = "graph must be 0 (never), 1 (batches) or 2 (every render)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_graph = value;
+    } else {
         ctx->err = "unknown option";
         return SPECREL_EINVAL;
     }
@@ -562,6 +703,7 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
         // Keep the old device buffers for reuse; rebuild the slot's contents.
         SceneSlot &slot = ctx->scenes[scene_idx];
         slot.used = false;
+        ctx->generation++;  // a captured graph holds the old scene pointers
         build_device_scene(ctx, slot, scene);
         (void)fresh;
         return SPECREL_OK;
@@ -577,15 +719,11 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
         if (!out_rgb) throw Fail{SPECREL_EINVAL, "out_rgb is NULL"};
         check(hipSetDevice(ctx->device), "hipSetDevice");
         bool hits = out_hit || out_lambda || out_tau;
-        enqueue_render(ctx, scene_idx, frame, hits);
+        enqueue_views(ctx, scene_idx, frame, 1, hits);
         auto t0 = std::chrono::steady_clock::now();
         check(hipStreamSynchronize(ctx->stream), "render");
         auto t1 = std::chrono::steady_clock::now();
-        size_t n = (size_t)frame->size[0] * frame->size[1];
-        check(hipMemcpy(out_rgb, ctx->rgb.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_hit) check(hipMemcpy(out_hit, ctx->hit.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_lambda) check(hipMemcpy(out_lambda, ctx->lam.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_tau) check(hipMemcpy(out_tau, ctx->tau.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        download_view(ctx, 0, out_rgb, out_hit, out_lambda, out_tau);
         auto t2 = std::chrono::steady_clock::now();
         collect(ctx);
         ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
@@ -598,7 +736,39 @@ int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
         check(hipSetDevice(ctx->device), "hipSetDevice");
-        return enqueue_render(ctx, scene_idx, frame, want_hits != 0);
+        return enqueue_views(ctx, scene_idx, frame, 1, want_hits != 0);
+    });
+}
+
+int specrel_render_batch(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
+                         const specrel_view_out *outs) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        bool hits = false;
+        if (outs)
+            for (uint32_t i = 0; i < n_views; ++i) hits = hits || outs[i].hit || outs[i].lambda || outs[i].tau;
+        enqueue_views(ctx, scene_idx, frames, n_views, hits);
+        auto t0 = std::chrono::steady_clock::now();
+        check(hipStreamSynchronize(ctx->stream), "render");
+        auto t1 = std::chrono::steady_clock::now();
+        if (outs)
+            for (uint32_t i = 0; i < n_views; ++i)
+                download_view(ctx, i, outs[i].rgb, outs[i].hit, outs[i].lambda, outs[i].tau);
+        auto t2 = std::chrono::steady_clock::now();
+        collect(ctx);
+        ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+        ctx->timings.readback_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
+        return SPECREL_OK;
+    });
+}
+
+int specrel_render_batch_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames,
+                                  uint32_t n_views, int want_hits) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        return enqueue_views(ctx, scene_idx, frames, n_views, want_hits != 0);
     });
 }
 
@@ -615,17 +785,83 @@ int specrel_sync(specrel_ctx *ctx) {
 }
 
 int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau) {
+    return specrel_download_view(ctx, 0, out_rgb, out_hit, out_lambda, out_tau);
+}
+
+int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda,
+                          float *out_tau) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
-        if (ctx->fw == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
-        if ((out_hit || out_lambda || out_tau) && !ctx->have_hits)
-            throw Fail{SPECREL_EINVAL, "the last render did not write hit buffers"};
         check(hipStreamSynchronize(ctx->stream), "download");
-        size_t n = (size_t)ctx->fw * ctx->fh;
-        if (out_rgb) check(hipMemcpy(out_rgb, ctx->rgb.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_hit) check(hipMemcpy(out_hit, ctx->hit.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_lambda) check(hipMemcpy(out_lambda, ctx->lam.p, n * 4, hipMemcpyDeviceToHost), "readback");
-        if (out_tau) check(hipMemcpy(out_tau, ctx->tau.p, n * 4, hipMemcpyDeviceToHost), "readback");
+        download_view(ctx, view, out_rgb, out_hit, out_lambda, out_tau);
+        return SPECREL_OK;
+    });
+}
+
+int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
+                     uint32_t iters, specrel_tuning *out) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
+        if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
+        struct Cand {
+            int stack, persistent, queue, graph;
+        };
+        std::vector<Cand> cands;
+        for (int st : {16, 8})
+            for (int pe : {0, 1})
+                for (int q : {0, 1})
+                    for (int g : {0, 2}) cands.push_back({st, pe, q, g});
+        const Cand saved{ctx->opt_stack, ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph};
+        auto apply = [&](const Cand &c) {
+            ctx->opt_stack = c.stack;
+            ctx->opt_persistent = c.persistent;
+            ctx->opt_shadow_queue = c.queue;
+            ctx->opt_graph = c.graph;
+        };
+        // Median wall time per render of the batch, enqueue to completion.
+        auto time_cand = [&](const Cand &c) {
+            apply(c);
+            enqueue_views(ctx, scene_idx, frames, n_views, false);
+            check(hipStreamSynchronize(ctx->stream), "autotune warmup");
+            std::vector<float> ms;
+            for (uint32_t i = 0; i < iters; ++i) {
+                auto t0 = std::chrono::steady_clock::now();
+                enqueue_views(ctx, scene_idx, frames, n_views, false);
+                check(hipStreamSynchronize(ctx->stream), "autotune");
+                ms.push_back(std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count());
+            }
+            std::sort(ms.begin(), ms.end());
+            return ms[ms.size() / 2];
+        };
+        float base_ms = 0.0f, best_ms = 0.0f;
+        Cand best = saved;
+        try {
+            base_ms = time_cand(saved);
+            best_ms = base_ms;
+            for (const Cand &c : cands) {
+                float t = time_cand(c);
+                if (t < best_ms) {
+                    best_ms = t;
+                    best = c;
+                }
+            }
+        } catch (...) {
+            apply(saved);
+            throw;
+        }
+        apply(best);
+        if (out) {
+            std::memset(out, 0, sizeof *out);
+            out->stack = best.stack;
+            out->persistent = best.persistent;
+            out->shadow_queue = best.queue;
+            out->graph = best.graph;
+            out->candidates = (uint32_t)cands.size() + 1;
+            out->best_ms = best_ms;
+            out->default_ms = base_ms;
+        }
         return SPECREL_OK;
     });
 }

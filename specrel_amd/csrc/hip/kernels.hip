@@ -78,12 +78,15 @@ struct StackSel<SK, false> {
 
 // Star pass: for every star, its cone event and the view-only parts of star_light
 // (star.rs:251-283), plus its unit direction in the camera rest frame for the tile culling.
-__global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t num_stars,
-                                                   const specrel_frame *frame, StarPrep *out, StarDir *dirs) {
+// Grid: (star blocks, views).
+__global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out,
+                                                   StarDir *dirs) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num_stars) return;
-    const specrel_frame &f = *frame;
+    const specrel_frame &f = V.frames[blockIdx.y];
+    if (i >= num_stars || f.mode != SPECREL_MODE_SHADED) return;
     ViewDerived v = derive(f, num_stars);
+    out += (size_t)blockIdx.y * num_stars;
+    dirs += (size_t)blockIdx.y * num_stars;
     StarPrep p;
     p.flags = 0;
     p.r = make_float4(0, 0, 0, 0);
@@ -190,11 +193,16 @@ __device__ uint32_t ordered_compact(uint32_t n, Keep keep, uint32_t *list, uint3
 
 // First culling level: one block per 64x64-pixel super tile keeps, in index order, the stars that
 // can reach any of its pixels.
-__global__ __launch_bounds__(256) void k_star_bin(const specrel_frame *frame, const StarDir *dirs,
-                                                  uint32_t num_stars, uint32_t *lists, uint32_t *counts) {
-    const ViewDerived v = derive(*frame, num_stars);
+// Grid: every super tile of every view.
+__global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists,
+                                                  uint32_t *counts) {
+    const uint32_t gst = blockIdx.x, vi = find_view(V.super_start, V.n, gst);
+    const specrel_frame &f = V.frames[vi];
+    if (f.mode != SPECREL_MODE_SHADED) return;
+    const ViewDerived v = derive(f, num_stars);
+    dirs += (size_t)vi * num_stars;
     const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
-    const uint32_t st = blockIdx.x, x0 = (st % nsx) * SUPER, y0 = (st / nsx) * SUPER;
+    const uint32_t st = gst - V.super_start[vi], x0 = (st % nsx) * SUPER, y0 = (st / nsx) * SUPER;
     __shared__ float s_cone[4];
     __shared__ uint32_t s_wave[4], s_total;
     if (threadIdx.x == 0)
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(256) void k_star_bin(const specrel_frame *frame, co
                   s_cone[3]);
     __syncthreads();
     const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
-    uint32_t *list = lists + (size_t)st * SUPER_CAP;
+    uint32_t *list = lists + (size_t)gst * SUPER_CAP;
     uint32_t n = ordered_compact(
         num_stars,
         [&](uint32_t i, uint32_t &item) {
@@ -210,17 +218,19 @@ __global__ __launch_bounds__(256) void k_star_bin(const specrel_frame *frame, co
             return star_in_cone(dirs[i], cx, cy, cz, cl);
         },
         list, SUPER_CAP, s_wave, &s_total);
-    if (threadIdx.x == 0) counts[st] = n;
+    if (threadIdx.x == 0) counts[gst] = n;
 }
 
 // Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT).
 template <bool STATS, int SK>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
-    const specrel_frame &f = *a.frame;
+    const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
+    const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t px = blockIdx.x * TILE + (wave & 1) * 8 + (lane & 7);
-    const uint32_t py = blockIdx.y * TILE + (wave >> 1) * 8 + (lane >> 3);
+    const uint32_t px = (lt % tiles_x) * TILE + (wave & 1) * 8 + (lane & 7);
+    const uint32_t py = (lt / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
     StackSel<SK>::init(stk, s_stack, tid);
@@ -229,7 +239,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
         bool found = !a.S.empty && trace<STATS>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
-        a.slot[(size_t)py * v.w + px] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
+        a.slot[(size_t)a.V.pix_start[vi] + (size_t)py * v.w + px] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
     }
     if (STATS) {
         __shared__ unsigned long long s_c[4];
@@ -244,23 +254,20 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     }
 }
 
-// Persistent primary-ray kernel: every lane takes the next ray of a global queue (8x8-pixel
-// tiles in row-major order, 64 rays per grab) as soon as its own ray ends, so the waves stay
-// full until the queue drains. Each ray's traversal is trace()'s, step for step.
+// Persistent primary-ray kernel: every lane takes the next ray of a global queue (the 16x16 tiles
+// of every view in order, 64 rays of one 8x8 quarter per grab) as soon as its own ray ends, so
+// the waves stay full until the queue drains. Each ray's traversal is trace()'s, step for step.
 template <bool STATS, int SK>
 __global__ __launch_bounds__(256) void k_trace_p(TraceArgs a) {
-    const specrel_frame &f = *a.frame;
-    const ViewDerived v = derive(f, 0);
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
     StackSel<SK>::init(stk, s_stack, tid);
     Counters C = {};
     uint64_t rays = 0;
-    const uint32_t tiles_x = (v.w + 7) / 8, tiles_y = (v.h + 7) / 8;
-    const uint32_t total = tiles_x * tiles_y * 64;
+    const uint32_t total = a.num_tiles * 256u;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t next = 0, end = 0;
+    uint32_t next = 0, end = 0, gv = 0;
     bool drained = false, active = false;
     RayState r;
     while (true) {
@@ -274,15 +281,20 @@ __global__ __launch_bounds__(256) void k_trace_p(TraceArgs a) {
                 else {
                     next = base;
                     end = min(base + 64u, total);
+                    gv = find_view(a.V.level_start[0], a.V.n, base >> 8);
                 }
             }
             if (!drained) {
                 uint32_t avail = end - next, rank = __popcll(idle & lt);
                 if (!active && rank < avail) {
-                    uint32_t ray = next + rank, t = ray >> 6, l = ray & 63;
-                    uint32_t px = (t % tiles_x) * 8 + (l & 7), py = (t / tiles_x) * 8 + (l >> 3);
+                    const specrel_frame &f = a.V.frames[gv];
+                    const ViewDerived v = derive(f, 0);
+                    uint32_t ray = next + rank, q = (ray >> 6) & 3u, l = ray & 63u;
+                    uint32_t tl = (ray >> 8) - a.V.level_start[0][gv], tiles_x = (v.w + TILE - 1) / TILE;
+                    uint32_t px = (tl % tiles_x) * TILE + (q & 1) * 8 + (l & 7);
+                    uint32_t py = (tl / tiles_x) * TILE + (q >> 1) * 8 + (l >> 3);
                     if (px < v.w && py < v.h) {
-                        r.pix = py * v.w + px;
+                        r.pix = a.V.pix_start[gv] + py * v.w + px;
                         r.o = v.origin;
                         r.d = ray_dir(v, f, px, py);
                         r.tmax = 180.0f;
@@ -389,10 +401,14 @@ __global__ void k_reset(uint32_t *counters, uint32_t n) {
 //   PASS 2: shadow queries answered from the occlusion masks.
 template <bool STATS, int SK, int PASS>
 __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
-    const specrel_frame &f = *a.frame;
+    const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
+    const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, a.num_stars);
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t x0 = blockIdx.x * TILE, y0 = blockIdx.y * TILE;
+    const uint32_t x0 = (lt % tiles_x) * TILE, y0 = (lt / tiles_x) * TILE;
+    const StarPrep *stars = a.stars + (size_t)vi * a.num_stars;
+    const StarDir *dirs = a.dirs + (size_t)vi * a.num_stars;
     const uint32_t px = x0 + (wave & 1) * 8 + (lane & 7);
     const uint32_t py = y0 + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[PASS == 0 ? StackSel<SK>::lds_entries : 1];
@@ -409,7 +425,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     bool use_tile_list = false;
     if (PASS != 1 && v.draw_stars && a.cull) {
         const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
-        const uint32_t st = (y0 / SUPER) * nsx + x0 / SUPER;
+        const uint32_t st = a.V.super_start[vi] + (y0 / SUPER) * nsx + x0 / SUPER;
         uint32_t sc = a.super_counts[st];
         bool all = sc > SUPER_CAP;
         src = all ? nullptr : a.super_lists + (size_t)st * SUPER_CAP;
@@ -424,14 +440,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             n_src,
             [&](uint32_t k, uint32_t &item) {
                 item = src ? src[k] : k;
-                return star_in_cone(a.dirs[item], cx, cy, cz, cl);
+                return star_in_cone(dirs[item], cx, cy, cz, cl);
             },
             s_list, TILE_CAP, s_wave, &s_total);
         use_tile_list = n_tile <= TILE_CAP;
     }
 
     if (px < v.w && py < v.h) {
-        size_t pi = (size_t)py * v.w + px;
+        const size_t li = (size_t)py * v.w + px, pi = a.V.pix_start[vi] + li;
         if (PASS == 1) a.mask[pi] = 0u;
         V4 dir = ray_dir(v, f, px, py);
         uint32_t slot = a.slot[pi];
@@ -469,15 +485,15 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                 V3 total = V3{0.0f, 0.0f, 0.0f};
                 if (a.cull && use_tile_list) {
                     for (uint32_t k = 0; k < n_tile; ++k)
-                        star_pixel<STATS>(a.stars[s_list[k]], n, spread, lmax, total, C);
+                        star_pixel<STATS>(stars[s_list[k]], n, spread, lmax, total, C);
                 } else if (a.cull && src) {
-                    for (uint32_t k = 0; k < n_src; ++k) star_pixel<STATS>(a.stars[src[k]], n, spread, lmax, total, C);
+                    for (uint32_t k = 0; k < n_src; ++k) star_pixel<STATS>(stars[src[k]], n, spread, lmax, total, C);
                 } else {
-                    for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(a.stars[k], n, spread, lmax, total, C);
+                    for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(stars[k], n, spread, lmax, total, C);
                 }
                 color += total;
             }
-            a.hdr[pi] = make_float4(color.x, color.y, color.z, 0.0f);
+            a.hdr[a.V.pyr_start[vi] + li] = make_float4(color.x, color.y, color.z, 0.0f);
             if (a.want_hits) {
                 a.hit[pi] = found ? a.S.order[slot] : SPECREL_NO_HIT;
                 a.lambda[pi] = found ? lambda : __builtin_inff();
@@ -529,10 +545,36 @@ __device__ __forceinline__ V3 ld3(const float4 *p, size_t i) {
     return V3{a.x, a.y, a.z};
 }
 
-__global__ __launch_bounds__(256) void k_bloom_down(const float4 *src, uint32_t sw, uint32_t sh, float4 *dst,
-                                                    uint32_t dw, uint32_t dh) {
-    uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+// Pixel (x, y) of a 16x16 tile of pyramid level k, for the tile's global index over all views.
+struct LevelPix {
+    uint32_t vi, w, h, x, y;
+};
+__device__ __forceinline__ LevelPix level_pix(const Views &V, int k) {
+    LevelPix p;
+    const uint32_t t = blockIdx.x;
+    p.vi = find_view(V.level_start[k], V.n, t);
+    const specrel_frame &f = V.frames[p.vi];
+    p.w = f.size[0];
+    p.h = f.size[1];
+    uint32_t lw, lh;
+    level_size(p.w, p.h, k, lw, lh);
+    const uint32_t lt = t - V.level_start[k][p.vi], tiles_x = (lw + 15) / 16;
+    p.x = (lt % tiles_x) * 16 + (threadIdx.x & 15);
+    p.y = (lt / tiles_x) * 16 + (threadIdx.x >> 4);
+    return p;
+}
+
+// bloom::downsample (bloom.rs:54-75) from level k-1 into level k of every view.
+__global__ __launch_bounds__(256) void k_bloom_down(Views V, int k, float4 *pyr) {
+    const LevelPix p = level_pix(V, k);
+    uint32_t sw, sh, dw, dh;
+    level_size(p.w, p.h, k - 1, sw, sh);
+    level_size(p.w, p.h, k, dw, dh);
+    const uint32_t x = p.x, y = p.y;
     if (x >= dw || y >= dh) return;
+    float4 *base = pyr + V.pyr_start[p.vi];
+    const float4 *src = base + level_offset(p.w, p.h, k - 1);
+    float4 *dst = base + level_offset(p.w, p.h, k);
     const float W[4] = {0.125f, 0.375f, 0.375f, 0.125f};
     V3 acc = V3{0.0f, 0.0f, 0.0f};
     for (int j = 0; j < 4; ++j) {
@@ -561,27 +603,38 @@ __device__ __forceinline__ V3 upsample_at(const float4 *coarse, uint32_t cw, uin
     return r0 * 0.75f + r1 * 0.25f;
 }
 
-__global__ __launch_bounds__(256) void k_bloom_up(float4 *fine, uint32_t w, uint32_t h, const float4 *coarse,
-                                                  uint32_t cw, uint32_t ch) {
-    uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+// One step of the up pass (bloom.rs:110-114): level k += upsample(level k+1), every view.
+__global__ __launch_bounds__(256) void k_bloom_up(Views V, int k, float4 *pyr) {
+    const LevelPix p = level_pix(V, k);
+    uint32_t w, h, cw, ch;
+    level_size(p.w, p.h, k, w, h);
+    level_size(p.w, p.h, k + 1, cw, ch);
+    const uint32_t x = p.x, y = p.y;
     if (x >= w || y >= h) return;
+    float4 *base = pyr + V.pyr_start[p.vi];
+    float4 *fine = base + level_offset(p.w, p.h, k);
+    const float4 *coarse = base + level_offset(p.w, p.h, k + 1);
     size_t i = (size_t)y * w + x;
     V3 c = ld3(fine, i) + upsample_at(coarse, cw, ch, x, y);
     fine[i] = make_float4(c.x, c.y, c.z, 0.0f);
 }
 
-// Final mix (bloom.rs:116-124) and pack_rgb_with_lut (renderer.rs:1116-1129).
-__global__ __launch_bounds__(256) void k_composite(const float4 *hdr, uint32_t w, uint32_t h, const float4 *l1,
-                                                   uint32_t w1, uint32_t h1, const uint8_t *lut, uint32_t *out) {
-    uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+// Final mix (bloom.rs:116-124) and pack_rgb_with_lut (renderer.rs:1116-1129), every view.
+__global__ __launch_bounds__(256) void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out) {
+    const LevelPix p = level_pix(V, 0);
+    const uint32_t w = p.w, h = p.h, x = p.x, y = p.y;
     if (x >= w || y >= h) return;
+    uint32_t w1, h1;
+    level_size(w, h, 1, w1, h1);
+    const float4 *hdr = pyr + V.pyr_start[p.vi];
+    const float4 *l1 = hdr + level_offset(w, h, 1);
     size_t i = (size_t)y * w + x;
     float halo = 0.04f / 6.0f;
     V3 c = ld3(hdr, i) * (1.0f - 0.04f) + upsample_at(l1, w1, h1, x, y) * halo;
     V3 m = clamp3(V3{c.x / (1.0f + c.x * 0.35f), c.y / (1.0f + c.y * 0.35f), c.z / (1.0f + c.z * 0.35f)}, 0.0f, 1.0f);
     const float scale = 4095.0f;
     uint32_t r = lut[f2u_sat(m.x * scale)], g = lut[f2u_sat(m.y * scale)], b = lut[f2u_sat(m.z * scale)];
-    out[i] = (r << 16) | (g << 8) | b;
+    out[V.pix_start[p.vi] + i] = (r << 16) | (g << 8) | b;
 }
 
 }  // namespace dev

@@ -19,13 +19,27 @@ from . import _ffi as F
 from .scene import build_frame
 
 OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
-OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE = 1, 2, 3, 4, 5
+OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH = 1, 2, 3, 4, 5, 6
 
 
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
                                          "trace_ms", "shade_ms", "bloom_ms", "composite_ms")] + \
         [("reserved", C.c_float * 3)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+class ViewOut(C.Structure):
+    _fields_ = [("rgb", C.POINTER(C.c_uint32)), ("hit", C.POINTER(C.c_uint32)), ("lambda_", C.POINTER(C.c_float)),
+                ("tau", C.POINTER(C.c_float))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
+                ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
+                ("reserved", C.c_uint32 * 5)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -55,6 +69,10 @@ SIGNATURES = {
     "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
     "specrel_sync": (C.c_int, [_P]),
     "specrel_download": (C.c_int, [_P, _U32P, _U32P, _FP, _FP]),
+    "specrel_render_batch": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.POINTER(ViewOut)]),
+    "specrel_render_batch_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_int]),
+    "specrel_download_view": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, _FP, _FP]),
+    "specrel_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_uint32, C.POINTER(Tuning)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
 }
@@ -154,6 +172,56 @@ class Context:
         if want_hits:
             out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
         return out
+
+    @staticmethod
+    def _frames(frames):
+        arr = (F.Frame * len(frames))()
+        for i, f in enumerate(frames):
+            arr[i] = f
+        return arr
+
+    def render_batch(self, idx, frames, want_hits=True):
+        """Every view of `frames` in one batched render; returns one dict per view like render_frame."""
+        arr = self._frames(frames)
+        outs = (ViewOut * len(frames))()
+        res = []
+        for i, f in enumerate(frames):
+            w, h = f.size[0], f.size[1]
+            o = {"rgb": np.zeros((h, w), np.uint32)}
+            if want_hits:
+                o.update(hit=np.zeros((h, w), np.uint32), lam=np.zeros((h, w), np.float32),
+                         tau=np.zeros((h, w), np.float32))
+            outs[i].rgb = _ptr(o["rgb"], _U32P)
+            if want_hits:
+                outs[i].hit = _ptr(o["hit"], _U32P)
+                outs[i].lambda_ = _ptr(o["lam"], _FP)
+                outs[i].tau = _ptr(o["tau"], _FP)
+            res.append(o)
+        self._check(self.lib.specrel_render_batch(self._h, idx, arr, len(frames), outs))
+        return res
+
+    def render_batch_resident(self, idx, frames, want_hits=False):
+        arr = frames if isinstance(frames, C.Array) else self._frames(frames)
+        self._check(self.lib.specrel_render_batch_resident(self._h, idx, arr, len(arr), int(want_hits)))
+
+    def download_view(self, view, w, h, want_hits=False):
+        n = w * h
+        rgb = np.zeros(n, np.uint32)
+        hit = np.zeros(n, np.uint32) if want_hits else None
+        lam = np.zeros(n, np.float32) if want_hits else None
+        tau = np.zeros(n, np.float32) if want_hits else None
+        self._check(self.lib.specrel_download_view(self._h, view, _ptr(rgb, _U32P), _ptr(hit, _U32P),
+                                                   _ptr(lam, _FP), _ptr(tau, _FP)))
+        out = {"rgb": rgb.reshape(h, w)}
+        if want_hits:
+            out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
+        return out
+
+    def autotune(self, idx, frames, iters=3):
+        arr = frames if isinstance(frames, C.Array) else self._frames(frames)
+        t = Tuning()
+        self._check(self.lib.specrel_autotune(self._h, idx, arr, len(arr), iters, C.byref(t)))
+        return t.as_dict()
 
     def timings(self):
         t = Timings()

@@ -301,3 +301,39 @@ def build_tables():
     out = (C.c_uint32 * (4096 + 98))()
     F.scene_lib().specrel_build_tables(out)
     return np.frombuffer(out, np.uint32).copy()
+
+
+def scene_box(scene: "Scene"):
+    """The root hyperbox of the scene's BVH as (min[t,x,y,z], max[t,x,y,z])."""
+    n = scene.nodes()
+    if len(n) == 0:
+        return np.zeros(4, np.float32), np.zeros(4, np.float32)
+    return n[0, 0:4].copy(), n[0, 4:8].copy()
+
+
+def batch_observers(box, n, seed, w_min=160, w_max=1920, rapidity_max=2.0, modes=(PastLightCone, CameraSimultaneity),
+                    shade=RelativisticShaded, shadows=ShadowLorentzInvariant):
+    """`n` observers for a batched render: width uniform in [w_min, w_max] with height = width*5/8
+    (160x100 .. 1920x1200), events uniform in the box, boost rapidity |phi| uniform in
+    [0, rapidity_max] along a uniform random direction (Tetrad::boost_local), a random
+    rotate_local (yaw, roll uniform in [-pi, pi), pitch in [-pi/2, pi/2)), and the slice mode
+    drawn from `modes` (real camera = PastLightCone, God's eye = CameraSimultaneity).
+    numpy's PCG64 with `seed` draws everything; returns (Camera4D, RenderConfig) pairs."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.asarray(box[0], np.float64), np.asarray(box[1], np.float64)
+    out = []
+    for _ in range(n):
+        w = int(rng.integers(w_min, w_max + 1))
+        h = max(1, (w * 5) // 8)
+        ev = lo + (hi - lo) * rng.random(4)
+        d = rng.normal(size=3)
+        d /= max(np.linalg.norm(d), 1e-12)
+        phi = d * rng.uniform(0.0, rapidity_max)
+        yaw, roll = rng.uniform(-np.pi, np.pi, 2)
+        pitch = rng.uniform(-np.pi / 2, np.pi / 2)
+        t = Tetrad().boost_local(phi.astype(np.float32)).rotate_local(float(yaw), float(pitch), float(roll))
+        cam = Camera4D.new(tuple(float(x) for x in ev))
+        cam.tetrad = t
+        mode = modes[int(rng.integers(0, len(modes)))]
+        out.append((cam, RenderConfig(w, h, shade, shadows, mode)))
+    return out

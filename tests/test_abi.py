@@ -38,6 +38,7 @@ def test_layouts_match_the_header():
     assert C.sizeof(F.SceneArrays) == 88 and F.SceneArrays.light_pos.offset == 72
     assert C.sizeof(F.Stats) == 128
     assert C.sizeof(R.Timings) == 48
+    assert C.sizeof(R.ViewOut) == 32 and C.sizeof(R.Tuning) == 48
 
 
 def test_argument_errors_are_codes_not_aborts():
@@ -46,6 +47,10 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_upload_scene(None, 0, None) == R.EINVAL
     assert lib.specrel_sync(None) == R.EINVAL
     assert lib.specrel_set_option(None, 1, 1) == R.EINVAL
+    assert lib.specrel_render_batch(None, 0, None, 1, None) == R.EINVAL
+    assert lib.specrel_render_batch_resident(None, 0, None, 1, 0) == R.EINVAL
+    assert lib.specrel_download_view(None, 0, None, None, None, None) == R.EINVAL
+    assert lib.specrel_autotune(None, 0, None, 1, 1, None) == R.EINVAL
 
 
 def test_create_reports_missing_device():
