@@ -17,7 +17,8 @@ instrumented traversal counters and HIP-event durations; `cpu_baseline` from the
 (a restatement of the reference renderer) timed on this host's cores on a bounded row sample;
 `batched`: 64 observers of uneven resolution (160x100 .. 1920x1200) over the same scene in one
 batched render (autotuned launch configuration, graph replay) against a loop of single-view
-renders of the same frames, in views/s.
+renders of the same frames, in views/s; `adjoint`: forward + adjoint times of the image gradient
+on the 32-object subset at 512x320.
 """
 import argparse
 import json
@@ -131,6 +132,58 @@ def bench_batch(ctx, scene, n_views, steps, seed):
     }
 
 
+def bench_adjoint(ctx, steps, slot=1):
+    """Forward render + adjoint on the 32-object subset of the large scene (3,932,160 tets at
+    128 slices) at 512x320, seen by an observer at rest 8 units behind the object with the most
+    neighbours: L2 loss on the linear image against a target rendered at a hidden observer
+    (rapidity 0.8 along a seeded direction, event offset N(0,1)^4), gradient with respect to
+    event (4), rapidity (3), rotation (3) and 32 object boosts (3 each)."""
+    import numpy as np
+    from specrel_amd import adjoint as A
+    from specrel_amd import scene as S
+
+    p = S.gen_params("large", n_objects=32, slices=128, n_stars=10000)
+    t0 = time.perf_counter()
+    sc = S.Scene.generate("large", p)
+    log(f"adjoint scene: {sc.num_prims} tets in {time.perf_counter() - t0:.1f} s")
+    obs, hidden, objs = A.adjoint_problem(sc, seed=7)
+    w, h = 512, 320
+    ctx.upload(slot, sc)
+    ctx.render_resident(slot, hidden.frame(w, h))
+    target = ctx.download_linear(0, w, h)
+    frame = obs.frame(w, h)
+    for _ in range(2):
+        ctx.render_resident(slot, frame)
+        ctx.sync()
+        A.gradient(ctx, obs, target, objs)
+    fwd, adj, wall = [], [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ctx.render_resident(slot, frame)
+        ctx.sync()
+        fwd.append(ctx.timings()["kernel_ms"])
+        g, res = A.gradient(ctx, obs, target, objs)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        adj.append(ctx.timings()["adjoint_ms"])
+    f, a = float(np.median(fwd)), float(np.median(adj))
+    return {
+        "tets": sc.num_prims,
+        "pixels": w * h,
+        "params": int(g.size),
+        "fwd_ms": round(f, 4),
+        "adjoint_ms": round(a, 4),
+        "fwd_plus_adjoint_ms": round(f + a, 4),
+        "ratio_to_fwd": round((f + a) / f, 3),
+        "wall_ms_per_gradient": round(float(np.median(wall)), 3),
+        "valid_pixels": int(res["valid"]),
+        "masked": {k: int(res[k]) for k in ("miss", "starlight", "border", "silhouette", "edge")},
+        "loss": res["loss"],
+        "workload": "32-object subset of the large scene (128 slices), 512x320, observer at rest 8 units behind the "
+                    "busiest object; L2 on the linear pre-bloom image against a hidden observer (rapidity 0.8, event "
+                    "offset N(0,1)^4, seed 7); 106 parameters",
+    }
+
+
 def copy_probe():
     """A plain copy kernel of known size: 2^28 floats (1 GiB read + 1 GiB written per launch).
     Its FETCH_SIZE / WRITE_SIZE readings scale the counters of the renderer kernels to bytes."""
@@ -157,6 +210,7 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=400)
     ap.add_argument("--views", type=int, default=64, help="observers of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--adjoint-steps", type=int, default=10, help="forward+adjoint repetitions (0: skip the leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stack", type=int, default=0,
                     help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
@@ -250,6 +304,11 @@ def main():
         batched = bench_batch(ctx, scene, args.views, args.batch_steps, seed=2)
         log(f"batched: {batched}")
 
+    adjoint = None
+    if args.adjoint_steps > 0:
+        adjoint = bench_adjoint(ctx, args.adjoint_steps)
+        log(f"adjoint: {adjoint}")
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline: oracle on a row sample")
@@ -280,6 +339,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

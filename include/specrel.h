@@ -44,7 +44,8 @@ typedef struct specrel_timings {
     float shade_ms;       /* shading, shadow rays, haze, stars */
     float bloom_ms;       /* pyramid down + up passes */
     float composite_ms;   /* composite + tone map + pack */
-    float reserved[3];
+    float adjoint_ms;     /* the last specrel_adjoint's kernels */
+    float reserved[2];
 } specrel_timings;
 
 /* Options for specrel_set_option. No counterpart in the reference: launch choices of this backend. */
@@ -92,6 +93,21 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value);
 /* GpuRenderer::upload_scene (gpu.rs:622-699): copy (and re-pack) one scene into slot
  * scene_idx, replacing what was there. The arrays are PackedSceneGpuData's plus `order`. */
 int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene);
+/* A range [first, first + count) of node or primitive (leaf-order) indices. */
+typedef struct specrel_range {
+    uint32_t first, count;
+} specrel_range;
+
+/* GpuRenderer::update_scene (gpu.rs:704-731) with the dirty ranges of PackedSceneGpuData::update
+ * (gpu.rs:297-344): `scene` is the slot's scene after an in-place update (same node, primitive,
+ * material and star counts). The bounds of the nodes in node_ranges and the records, attributes
+ * and order entries of the primitives in prim_ranges are copied to the device; materials, stars
+ * and the light are copied whole. When a listed node's children or leaf range differ from the
+ * uploaded tree, or a count differs, everything is uploaded again instead, as the reference does
+ * when the ranges do not apply; *out_full (may be NULL) tells which happened. */
+int specrel_patch_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                        const specrel_range *node_ranges, uint32_t n_node_ranges, const specrel_range *prim_ranges,
+                        uint32_t n_prim_ranges, int *out_full);
 /* Number of scene slots in use (highest uploaded index + 1). */
 uint32_t specrel_num_scenes(const specrel_ctx *ctx);
 
@@ -108,6 +124,10 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
 int specrel_sync(specrel_ctx *ctx);
 int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau);
+
+/* The linear HDR radiance of view `view` of the last render before bloom and tone mapping (the
+ * image the adjoint's loss is defined on): width*height*3 floats, RGB per pixel. */
+int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3);
 
 /* Batched observers: n_views (1..4096) frames of any sizes and modes rendered over one scene
  * with one launch per stage (each kernel's grid spans the tiles of every view), replayed from a
@@ -128,6 +148,40 @@ int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, ui
  * renders each after one warm-up, and keeps the fastest by median wall time. */
 int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                      uint32_t iters, specrel_tuning *out);
+
+/* Differentiation of the last render (one view, RelativisticShaded, PastLightCone, null or no
+ * shadows, no flashes or haze). No counterpart in the reference, which has no gradients.
+ *
+ * Loss: L = sum over valid pixels of |I - target|^2, with I the linear HDR radiance before bloom
+ * (3 floats per pixel; target likewise, width*height*3 floats). Gradients of L with respect to
+ * the observer event, the observer tetrad legs e_a (as passed in the frame) and, per object, the
+ * rapidity vector of a Lorentz boost of the object's whole worldtube about `center`: its
+ * primitives' records, normals and four-velocities are transformed by that boost, which at zero
+ * is the derivative with respect to the object's velocity there.
+ *
+ * Each pixel is recomputed from the render's per-pixel hit (the checkpoint): intersection,
+ * interpolation, shading and its shadow ray, then differentiated by hand. Valid pixels hit an
+ * object, carry no starlight and have 8 neighbours inside the image with the same object, albedo
+ * (without the wireframe blend), shadow state and facet (normal and four-velocity, constant per
+ * tetrahedron); the others (misses, silhouettes, texture, shadow and facet edges) are counted per
+ * class and left out. mask_in (width*height bytes, nonzero =
+ * valid; may be NULL) replaces that test, so a loss can be evaluated on a fixed pixel set;
+ * mask_out (may be NULL) receives the mask used. d_objects receives 3 doubles per object. */
+typedef struct specrel_adj_object {
+    uint32_t prim_first, prim_count; /* input-order primitive range; objects sorted and disjoint */
+    float center[4];                 /* event the boost is about (t, x, y, z) */
+} specrel_adj_object;
+
+typedef struct specrel_adjoint_out {
+    double loss;
+    double d_event[4];     /* dL / d observer event (t, x, y, z) */
+    double d_tetrad[4][4]; /* dL / d e_a (t, x, y, z) */
+    uint64_t valid, miss, starlight, border, silhouette, edge; /* pixels per class */
+} specrel_adjoint_out;
+
+int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
+                    const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out,
+                    double *d_objects);
 
 int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
 /* Counters of the last render made with SPECREL_OPT_STATS on. */

@@ -39,6 +39,12 @@ enum {
     SPECREL_WL_BORN_RAMP = 4     /* d: beta_max, t_min, t_max, steps (mesh4d.rs:1254-1267 profile) */
 };
 
+typedef struct specrel_object {
+    uint32_t prim_first, prim_count;
+    float center[4]; /* (t, x, y, z) */
+    uint32_t material, pad;
+} specrel_object;
+
 typedef struct specrel_wl_sample {  /* WorldlineSample (mesh4d.rs:615-630) */
     float center[4];
     float proper_time;
@@ -107,6 +113,11 @@ int specrel_scene_generate(const char *name, const specrel_gen_params *p, specre
 int specrel_scene_default_camera(const char *name, const specrel_gen_params *p, specrel_camera *out);
 
 int specrel_scene_get_arrays(const specrel_host_scene *s, specrel_scene_arrays *out);
+/* One entry per worldtube added (generators and specrel_builder_add_tube): its primitives are
+ * input indices [prim_first, prim_first + prim_count) (scene order[] maps leaf slots to them);
+ * center is its worldline sample nearest to lab time 0. Returns the object count; copies at
+ * most cap entries to out (which may be NULL). */
+uint32_t specrel_scene_objects(const specrel_host_scene *s, specrel_object *out, uint32_t cap);
 /* Star list as generated (temperature, intensity per star) for tests of pack_stars. */
 int specrel_scene_star_params(const specrel_host_scene *s, float *temp_intensity, uint32_t n);
 void specrel_scene_free(specrel_host_scene *s);

@@ -1180,3 +1180,139 @@ int oracle_render_rows(const specrel_scene_arrays *s, const specrel_frame *f, in
     pthread_mutex_destroy(&J.lock);
     return 0;
 }
+
+/* ---------------------------------------------------------------- forward state for the FP64 adjoint reference */
+static void pixel_state(o_job *J, uint32_t px, uint32_t py, oracle_pixel_state *out) {
+    const specrel_scene_arrays *s = J->base.s;
+    const specrel_frame *f = J->base.f;
+    memset(out, 0, sizeof *out);
+    out->slot = -1;
+    /* the ray of shade_row (past light cone) */
+    float ndc_y = 1.0f - 2.0f * (((float)py + 0.5f) / (float)J->h);
+    float sy = ndc_y * J->half_h;
+    float ndc_x = 2.0f * (((float)px + 0.5f) / (float)J->w) - 1.0f;
+    float sx = ndc_x * J->half_w;
+    o3 local_dir = norm3(V3(sx, sy, 1.0f));
+    o3 n = norm3(local_dir);
+    out->local_dir[0] = n.x;
+    out->local_dir[1] = n.y;
+    out->local_dir[2] = n.z;
+    o4 e0 = ld4(f->e[0]), e1 = ld4(f->e[1]), e2 = ld4(f->e[2]), e3 = ld4(f->e[3]);
+    o4 dir = add4(add4(add4(neg4(e0), mul4(e1, n.x)), mul4(e2, n.y)), mul4(e3, n.z));
+    o_hit h;
+    if (!trace(s, J->ray_origin, dir, 1e-3f, 180.0f, &h, NULL)) return;
+    o_rec rec;
+    interpolate_hit(&s->attrs[h.slot], &h, &rec);
+    o_mat m = load_mat(s, rec.material);
+    out->slot = h.slot;
+    out->material = rec.material;
+    out->pattern = (int32_t)m.pattern;
+    o3 nw;
+    if (m.pattern == SPECREL_PATTERN_CLOCK || m.pattern == SPECREL_PATTERN_LIGHT_CLOCK) {
+        nw = surface_albedo(&J->base, &rec, &m);
+        out->dark = 1.0f;
+    } else {
+        int32_t gu = sat_i32(floorf(rec.uv_u * m.grid)), gv = sat_i32(floorf(rec.uv_v * m.grid));
+        int32_t sum = (int32_t)((uint32_t)gu + (uint32_t)gv);
+        o3 col0 = ((((sum % 2) + 2) % 2) == 0) ? m.base : m.checker;
+        float dark = 1.0f;
+        if (m.pattern != SPECREL_PATTERN_CHECKER) {
+            float ut = remeu(rec.uv_u * m.grid, 1.0f), vt = remeu(rec.uv_v * m.grid, 1.0f);
+            float bd = fmn(fmn(ut, 1.0f - ut), fmn(vt, 1.0f - vt));
+            if (bd < 0.025f) dark = 0.35f;
+        }
+        out->col0[0] = col0.x;
+        out->col0[1] = col0.y;
+        out->col0[2] = col0.z;
+        out->dark = dark;
+        nw = mul3(col0, dark);
+    }
+    out->nw_albedo[0] = nw.x;
+    out->nw_albedo[1] = nw.y;
+    out->nw_albedo[2] = nw.z;
+    /* the f32 branch taken at every kink of the differentiable part */
+    {
+        o3 tb = rec.tri_bary;
+        float mtb = fmn(fmn(tb.x, tb.y), tb.z);
+        out->mtb_k = (tb.x <= tb.y) ? (tb.x <= tb.z ? 0 : 2) : (tb.y <= tb.z ? 1 : 2);
+        if (m.pattern != SPECREL_PATTERN_CLOCK && m.pattern != SPECREL_PATTERN_LIGHT_CLOCK && mtb < m.wire) {
+            float et = mtb / m.wire;
+            if (et > 0.0f && et < 1.0f) out->flags |= ORACLE_PX_WIRE;
+        }
+        o3 light = V3(s->light_pos[1], s->light_pos[2], s->light_pos[3]);
+        o3 s2l = sub3(light, spat(rec.event));
+        float len = len3(s2l), dist = fmx(len, 1e-4f);
+        if (len > 1e-4f) out->flags |= ORACLE_PX_DIST_FREE;
+        float t_emit = rec.event.t - dist;
+        o4 to_light = mul4(sub4(V4(t_emit, light.x, light.y, light.z), rec.event), 1.0f / dist);
+        o4 vs[2] = {to_light, dir};
+        for (int k = 0; k < 2; ++k) {
+            o4 v = vs[k];
+            float sd = mdot(v, rec.four_vel);
+            o4 vp = add4(v, mul4(rec.four_vel, sd));
+            float q = mdot(vp, vp);
+            if (q > 1e-12f) {
+                float r = mdot(vp, rec.normal) / sqrtf(q);
+                if (r > -1.0f && r < 1.0f) out->flags |= k == 0 ? ORACLE_PX_RL_FREE : ORACLE_PX_RV_FREE;
+            }
+        }
+        if (fmx(rest_frame_cos(to_light, rec.normal, rec.four_vel), 0.0f) > 0.0f) out->flags |= ORACLE_PX_CL;
+        if (fmx(-rest_frame_cos(dir, rec.normal, rec.four_vel), 0.0f) > 0.0f) out->flags |= ORACLE_PX_CV;
+        o3 sp = spat(dir);
+        float sl = len3(sp);
+        if (sl > 1e-12f) {
+            o3 su = mul3(sp, 1.0f / sl);
+            float hz = su.y * 0.5f + 0.5f;
+            if (hz > 0.0f && hz < 1.0f) out->flags |= ORACLE_PX_HZ_FREE;
+        }
+    }
+    /* the shadow decision of compute_illumination (null shadows) */
+    if (m.pattern == SPECREL_PATTERN_CHECKER && f->shadow_mode == SPECREL_SHADOW_NULL) {
+        o3 light = V3(s->light_pos[1], s->light_pos[2], s->light_pos[3]);
+        o3 s2l = sub3(light, spat(rec.event));
+        float dist = fmx(len3(s2l), 1e-4f);
+        float t_emit = rec.event.t - dist;
+        o4 to_light = mul4(sub4(V4(t_emit, light.x, light.y, light.z), rec.event), 1.0f / dist);
+        float cl = fmx(rest_frame_cos(to_light, rec.normal, rec.four_vel), 0.0f);
+        if (cl > 0.0f) out->in_shadow = occluded(s, rec.event, to_light, 2e-3f, dist - 2e-3f, NULL);
+    }
+    o3 color = shade_hit(&J->base, dir, &rec);
+    out->color[0] = color.x;
+    out->color[1] = color.y;
+    out->color[2] = color.z;
+}
+
+typedef struct {
+    o_job *J;
+    oracle_pixel_state *out;
+    atomic_uint next;
+} o_state_job;
+
+static void *state_worker(void *arg) {
+    o_state_job *S = (o_state_job *)arg;
+    for (;;) {
+        uint32_t py = atomic_fetch_add(&S->next, 1);
+        if (py >= S->J->h) break;
+        for (uint32_t px = 0; px < S->J->w; ++px) pixel_state(S->J, px, py, &S->out[(size_t)py * S->J->w + px]);
+    }
+    return NULL;
+}
+
+int oracle_pixel_states(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, oracle_pixel_state *out) {
+    if (f->slice_mode != SPECREL_VIEW_PAST_CONE || f->mode != SPECREL_MODE_SHADED) return -1;
+    uint8_t lut[SPECREL_LUT_SIZE];
+    oracle_gamma_lut(lut);
+    o_job J;
+    setup_job(&J, s, f, lut);
+    o_state_job S;
+    S.J = &J;
+    S.out = out;
+    atomic_init(&S.next, 0);
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    for (int i = 0; i < nthreads; ++i) pthread_create(&th[i], NULL, state_worker, &S);
+    for (int i = 0; i < nthreads; ++i) pthread_join(th[i], NULL);
+    free(th);
+    pthread_mutex_destroy(&J.lock);
+    return 0;
+}

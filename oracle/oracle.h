@@ -57,6 +57,34 @@ int oracle_digital_tau(float u, float v, float proper_time);
 float oracle_decision_age(const float event[4], const float origin[4], const float u[4]);
 void oracle_shade_sky(const float dir[4], float out[3]);
 void oracle_haze(const float base[3], const float event[4], const float *origins, uint32_t n, float out[3]);
+/* The forward state of every pixel that the FP64 adjoint reference (adjoint_ref.cpp) takes as
+ * given: the hit's leaf slot (-1 on a miss), the albedo without the wireframe blend, the checker
+ * cell colour and grid-line factor, the shadow decision, the shaded colour before haze and stars,
+ * and the pixel's unit local direction. Shaded past-light-cone views only (else -1). */
+typedef struct oracle_pixel_state {
+    int64_t slot;
+    float nw_albedo[3];
+    float col0[3];
+    float dark;
+    int32_t pattern;
+    int32_t in_shadow;
+    float color[3];
+    float local_dir[3];
+    uint32_t material;
+    uint32_t flags; /* ORACLE_PX_*: the f32 forward's branch at each kink of the shading */
+    int32_t mtb_k;  /* which triangle barycentric is the smallest (0..2) */
+} oracle_pixel_state;
+enum {
+    ORACLE_PX_WIRE = 1,       /* inside the wireframe band, blend not clamped */
+    ORACLE_PX_CL = 2,         /* light cosine > 0 */
+    ORACLE_PX_CV = 4,         /* view cosine > 0 */
+    ORACLE_PX_RL_FREE = 8,    /* light rest-frame cosine neither clamped nor degenerate */
+    ORACLE_PX_RV_FREE = 16,   /* view rest-frame cosine neither clamped nor degenerate */
+    ORACLE_PX_HZ_FREE = 32,   /* sky horizon blend not clamped */
+    ORACLE_PX_DIST_FREE = 64  /* light distance above its 1e-4 floor */
+};
+int oracle_pixel_states(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, oracle_pixel_state *out);
+
 /* f32 helpers as the oracle defines them (checked against high-precision values). */
 float oracle_expf(float x);
 float oracle_sinf(float x);

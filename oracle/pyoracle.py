@@ -61,6 +61,11 @@ def lib():
             "oracle_atan2f": (C.c_float, [C.c_float, C.c_float]),
             "oracle_acosf": (C.c_float, [C.c_float]),
             "oracle_rem_euclid": (C.c_float, [C.c_float, C.c_float]),
+            "oracle_pixel_states": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.c_int, C.c_void_p]),
+            "oracle_adjoint_ref": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.POINTER(C.c_double),
+                                             _FP, C.POINTER(C.c_uint8), _U32P, _U32P, _FP, C.c_uint32, C.c_int, C.c_int,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint8), C.POINTER(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -172,3 +177,53 @@ def cone_event(words, num_stars, star, origin, e0, cone):
     u = np.zeros(4, np.float32)
     ok = lib().oracle_cone_event(wp, num_stars, star, op, ep, cone, d.ctypes.data_as(_FP), u.ctypes.data_as(_FP))
     return (d, u) if ok else None
+
+
+def adjoint_ref(scene, frame, theta, target, objects, mask_in=None, nthreads=None, frame_values=True):
+    """FP64 forward-mode gradient of the masked image loss (adjoint_ref.cpp) in the parameters
+    [event(4), phi(3), rot(3), d_0(3), ...]; `frame` must be the f32 frame of `theta`. With
+    frame_values the model is evaluated at the frame's f32 event and tetrad (the renderer's
+    inputs), else at theta in double."""
+    w, h = frame.size[0], frame.size[1]
+    th = np.ascontiguousarray(theta, np.float64)
+    tgt = np.ascontiguousarray(target, np.float32)
+    n = len(objects)
+    first = np.array([o[0] for o in objects] or [0], np.uint32)
+    count = np.array([o[1] for o in objects] or [0], np.uint32)
+    center = np.array([o[2] for o in objects] or [(0, 0, 0, 0)], np.float32)
+    mi = None if mask_in is None else np.ascontiguousarray(mask_in, np.uint8)
+    mo = np.zeros((h, w), np.uint8)
+    loss = C.c_double()
+    grad = np.zeros(10 + 3 * n, np.float64)
+    cnt = np.zeros(6, np.uint64)
+    err = C.c_double()
+    u8 = C.POINTER(C.c_uint8)
+    rc = lib().oracle_adjoint_ref(C.byref(scene.arrays), C.byref(frame), th.ctypes.data_as(C.POINTER(C.c_double)),
+                                  tgt.ctypes.data_as(_FP), None if mi is None else mi.ctypes.data_as(u8),
+                                  first.ctypes.data_as(_U32P), count.ctypes.data_as(_U32P),
+                                  center.ctypes.data_as(_FP), n, int(frame_values), nthreads or os.cpu_count() or 1,
+                                  C.byref(loss),
+                                  grad.ctypes.data_as(C.POINTER(C.c_double)),
+                                  cnt.ctypes.data_as(C.POINTER(C.c_uint64)), mo.ctypes.data_as(u8), C.byref(err))
+    if rc != 0:
+        raise RuntimeError("oracle_adjoint_ref: unsupported view")
+    names = ("valid", "miss", "starlight", "border", "silhouette", "edge")
+    return {"loss": loss.value, "grad": grad, "counts": dict(zip(names, (int(c) for c in cnt))), "mask": mo,
+            "max_rel_color_err": err.value}
+
+
+PIXEL_STATE = np.dtype([("slot", np.int64), ("nw_albedo", np.float32, 3), ("col0", np.float32, 3),
+                        ("dark", np.float32), ("pattern", np.int32), ("in_shadow", np.int32),
+                        ("color", np.float32, 3), ("local_dir", np.float32, 3), ("material", np.uint32),
+                        ("flags", np.uint32), ("mtb_k", np.int32)])
+
+
+def pixel_states(scene, frame, nthreads=None):
+    """The f32 forward state of every pixel (oracle_pixel_states) as a structured (h, w) array."""
+    w, h = frame.size[0], frame.size[1]
+    out = np.zeros(w * h, PIXEL_STATE)
+    assert out.itemsize == 80
+    if lib().oracle_pixel_states(C.byref(scene.arrays), C.byref(frame), nthreads or os.cpu_count() or 1,
+                                 out.ctypes.data) != 0:
+        raise RuntimeError("oracle_pixel_states: unsupported view")
+    return out.reshape(h, w)

@@ -79,6 +79,11 @@ class DynamicsC(C.Structure):
                 ("num_remote", C.c_uint32), ("remote_origins", C.POINTER(F4))]
 
 
+class Object(C.Structure):
+    _fields_ = [("prim_first", C.c_uint32), ("prim_count", C.c_uint32), ("center", C.c_float * 4),
+                ("material", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("n_objects", C.c_uint32), ("slices", C.c_uint32), ("n_stars", C.c_uint32),
                 ("n_clocks", C.c_uint32), ("icosphere_fraction", C.c_float), ("accel_fraction", C.c_float),
@@ -112,6 +117,7 @@ def _scene_protos(lib):
         "specrel_scene_default_camera": (C.c_int, [C.c_char_p, C.POINTER(GenParams), C.POINTER(Camera)]),
         "specrel_scene_get_arrays": (C.c_int, [_P, C.POINTER(SceneArrays)]),
         "specrel_scene_star_params": (C.c_int, [_P, _FP, _u32]),
+        "specrel_scene_objects": (_u32, [_P, C.POINTER(Object), _u32]),
         "specrel_scene_free": (None, [_P]),
         "specrel_builder_new": (_P, []),
         "specrel_builder_free": (None, [_P]),

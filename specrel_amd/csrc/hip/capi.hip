@@ -25,7 +25,11 @@ struct SceneSlot {
     bool used = false;
     DevBuf nbox, ninfo, big, geoms, attrs, order, mats, stars;
     Scene S{};
-    uint32_t num_stars = 0, num_star_words = 0, num_materials = 0;
+    uint32_t num_stars = 0, num_star_words = 0, num_materials = 0, num_nodes = 0, num_prims = 0;
+    // Host copies for specrel_patch_scene: each node's parent slot (parent << 1 | right side;
+    // ~0u for the root) and the encoded child entries and large leaves as uploaded.
+    std::vector<uint32_t> parent;
+    std::vector<uint2> ninfo_host, big_host;
 };
 
 struct Fail {
@@ -47,9 +51,13 @@ struct specrel_ctx {
     uint32_t stars_for = 0;  // num_stars the star buffers are sized for (per view)
     DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
     DevBuf counters, mask, squeue;
+    DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
+        adj_result;
+    std::vector<specrel_frame> last_frames;
+    uint32_t last_scene = 0;
     uint64_t generation = 0;  // bumped whenever a buffer a captured graph points to moves
     int cus = 0;
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[9] = {};
     bool timed_stages = true;
     specrel_timings timings{};
     specrel_stats stats_host{};
@@ -164,6 +172,7 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
         std::vector<uint8_t> seen(nn, 0);
         std::vector<uint32_t> todo{0};
         seen[0] = 1;
+        slot.parent.assign(nn, ~0u);
         while (!todo.empty()) {
             uint32_t i = todo.back();
             todo.pop_back();
@@ -176,6 +185,8 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
             if (l >= nn || r >= nn) throw Fail{SPECREL_EINVAL, "child index out of range"};
             if (seen[l] || seen[r] || l == r) throw Fail{SPECREL_EINVAL, "node graph is not a tree"};
             seen[l] = seen[r] = 1;
+            slot.parent[l] = i << 1;
+            slot.parent[r] = (i << 1) | 1u;
             todo.push_back(r);
             todo.push_back(l);
             const specrel_node &L = a->nodes[l], &R = a->nodes[r];
@@ -190,6 +201,7 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
         S.root_min = make_float4(root.min[0], root.min[1], root.min[2], root.min[3]);
         S.root_max = make_float4(root.max[0], root.max[1], root.max[2], root.max[3]);
     }
+    if (nn > 0x7FFFFFFFu / 2) throw Fail{SPECREL_EINVAL, "too many nodes"};
     hipStream_t s = ctx->stream;
     upload(slot.nbox, nbox.data(), nbox.size() * sizeof(float4), s);
     upload(slot.ninfo, ninfo.data(), ninfo.size() * sizeof(uint2), s);
@@ -212,7 +224,110 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
     slot.num_stars = a->num_stars;
     slot.num_star_words = a->num_star_words;
     slot.num_materials = a->num_materials;
+    slot.num_nodes = nn;
+    slot.num_prims = np;
+    slot.ninfo_host.swap(ninfo);
+    slot.big_host.swap(big);
     slot.used = true;
+}
+
+// The encoded child entry of node j of `a` as it stands, compared with what was uploaded
+// (`have`): equal when j is the same kind of node with the same children or primitive range.
+bool same_entry(const SceneSlot &slot, const specrel_scene_arrays *a, uint32_t j, uint32_t have) {
+    const specrel_node &n = a->nodes[j];
+    if (n.count == 0) return have == j;
+    if ((have & LEAF) == 0) return false;
+    if (n.count <= COUNT_MAX && n.right_or_first <= FIRST_MASK)
+        return have == (LEAF | (n.count << COUNT_SHIFT) | n.right_or_first);
+    uint32_t c = (have >> COUNT_SHIFT) & COUNT_MAX, idx = have & FIRST_MASK;
+    return c == 0 && idx < slot.big_host.size() && slot.big_host[idx].x == n.right_or_first &&
+           slot.big_host[idx].y == n.count;
+}
+
+// GpuRenderer::update_scene with dirty ranges (gpu.rs:704-731, PackedSceneGpuData::update
+// gpu.rs:297-344): node bounds and primitive records of the given ranges are copied into the
+// uploaded layout. Returns false when the arrays no longer match the uploaded tree (counts,
+// children or leaf ranges), in which case the caller uploads everything again.
+bool patch_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_arrays *a, const specrel_range *nr,
+                        uint32_t n_nr, const specrel_range *pr, uint32_t n_pr) {
+    if (a->num_nodes != slot.num_nodes || a->num_prims != slot.num_prims || a->num_materials != slot.num_materials ||
+        a->num_stars != slot.num_stars || a->num_star_words != slot.num_star_words || slot.num_nodes == 0)
+        return false;
+    if ((n_nr && !nr) || (n_pr && !pr)) throw Fail{SPECREL_EINVAL, "range list is NULL"};
+    const uint32_t nn = slot.num_nodes, np = slot.num_prims;
+    std::vector<uint32_t> pslot;
+    std::vector<float4> bounds;
+    for (uint32_t r = 0; r < n_nr; ++r) {
+        if ((uint64_t)nr[r].first + nr[r].count > nn) throw Fail{SPECREL_EINVAL, "node range out of bounds"};
+        for (uint32_t j = nr[r].first; j < nr[r].first + nr[r].count; ++j) {
+            const specrel_node &n = a->nodes[j];
+            if (n.count == 0) {
+                if (n.left >= nn || n.right_or_first >= nn) return false;
+                if (!same_entry(slot, a, n.left, slot.ninfo_host[j].x) ||
+                    !same_entry(slot, a, n.right_or_first, slot.ninfo_host[j].y))
+                    return false;
+            }
+            if (j == 0) {
+                slot.S.root_min = make_float4(n.min[0], n.min[1], n.min[2], n.min[3]);
+                slot.S.root_max = make_float4(n.max[0], n.max[1], n.max[2], n.max[3]);
+                continue;
+            }
+            if (slot.parent[j] == ~0u) return false;
+            const uint32_t p = slot.parent[j] >> 1;
+            const uint32_t have = (slot.parent[j] & 1u) ? slot.ninfo_host[p].y : slot.ninfo_host[p].x;
+            if (!same_entry(slot, a, j, have)) return false;
+            pslot.push_back(slot.parent[j]);
+            bounds.push_back(make_float4(n.min[0], n.min[1], n.min[2], n.min[3]));
+            bounds.push_back(make_float4(n.max[0], n.max[1], n.max[2], n.max[3]));
+        }
+    }
+    for (uint32_t r = 0; r < n_pr; ++r) {
+        if ((uint64_t)pr[r].first + pr[r].count > np) throw Fail{SPECREL_EINVAL, "primitive range out of bounds"};
+        for (uint32_t i = pr[r].first; i < pr[r].first + pr[r].count; ++i) {
+            uint32_t m;
+            std::memcpy(&m, &a->attrs[i].misc[2], 4);
+            if (m >= slot.num_materials) throw Fail{SPECREL_EINVAL, "patched primitive has material out of range"};
+            if (a->order[i] >= np) throw Fail{SPECREL_EINVAL, "patched order entry out of range"};
+        }
+    }
+    hipStream_t s = ctx->stream;
+    for (uint32_t r = 0; r < n_pr; ++r) {
+        const size_t f = pr[r].first, c = pr[r].count;
+        if (!c) continue;
+        check(hipMemcpyAsync((specrel_geom *)slot.geoms.p + f, a->geoms + f, c * sizeof(specrel_geom),
+                             hipMemcpyHostToDevice, s), "patch geoms");
+        check(hipMemcpyAsync((specrel_attr *)slot.attrs.p + f, a->attrs + f, c * sizeof(specrel_attr),
+                             hipMemcpyHostToDevice, s), "patch attrs");
+        check(hipMemcpyAsync((uint32_t *)slot.order.p + f, a->order + f, c * 4, hipMemcpyHostToDevice, s),
+              "patch order");
+    }
+    if (!pslot.empty()) {
+        DevBuf tmp;
+        const size_t n = pslot.size();
+        try {
+            dalloc(tmp, n * (32 + 4));
+            float4 *d_b = (float4 *)tmp.p;
+            uint32_t *d_ps = (uint32_t *)(d_b + 2 * n);
+            check(hipMemcpyAsync(d_ps, pslot.data(), n * 4, hipMemcpyHostToDevice, s), "patch parents");
+            check(hipMemcpyAsync(d_b, bounds.data(), n * 32, hipMemcpyHostToDevice, s), "patch bounds");
+            k_patch_nodes<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d_ps, d_b, (uint32_t)n, (float4 *)slot.nbox.p);
+            check(hipGetLastError(), "k_patch_nodes launch");
+            check(hipStreamSynchronize(s), "patch");
+        } catch (...) {
+            dfree(tmp);
+            throw;
+        }
+        dfree(tmp);
+    }
+    if (a->num_materials)
+        check(hipMemcpyAsync(slot.mats.p, a->materials, (size_t)a->num_materials * sizeof(specrel_material),
+                             hipMemcpyHostToDevice, s), "patch materials");
+    if (a->num_star_words)
+        check(hipMemcpyAsync(slot.stars.p, a->star_words, (size_t)a->num_star_words * 16, hipMemcpyHostToDevice, s),
+              "patch stars");
+    slot.S.light = make_float4(a->light_pos[0], a->light_pos[1], a->light_pos[2], a->light_pos[3]);
+    check(hipStreamSynchronize(s), "patch");
+    return true;
 }
 
 uint32_t num_super(uint32_t w, uint32_t h) { return ((w + SUPER - 1) / SUPER) * ((h + SUPER - 1) / SUPER); }
@@ -390,6 +505,8 @@ Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, 
     }
     pl.want_hits = want_hits || ctx->opt_stats != 0;
     ensure_views(ctx, frames, n, sc.num_stars);
+    ctx->last_frames.assign(frames, frames + n);
+    ctx->last_scene = scene_idx;
     if (pl.queries && ctx->opt_shadow_queue) {
         uint64_t cap = (uint64_t)ctx->total_pix * (1u + pl.max_flashes);
         if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "views too large for the shadow queue"};
@@ -658,6 +775,9 @@ void specrel_destroy(specrel_ctx *ctx) {
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
     dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
     dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
+    dfree(ctx->adj_target), dfree(ctx->adj_mask_in), dfree(ctx->adj_mask_out), dfree(ctx->adj_obj);
+    dfree(ctx->adj_pix), dfree(ctx->adj_sig), dfree(ctx->adj_objid), dfree(ctx->adj_partials);
+    dfree(ctx->adj_counts), dfree(ctx->adj_result);
     drop_graph(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -706,6 +826,27 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
         ctx->generation++;  // a captured graph holds the old scene pointers
         build_device_scene(ctx, slot, scene);
         (void)fresh;
+        return SPECREL_OK;
+    });
+}
+
+int specrel_patch_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                        const specrel_range *node_ranges, uint32_t n_node_ranges, const specrel_range *prim_ranges,
+                        uint32_t n_prim_ranges, int *out_full) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!scene) throw Fail{SPECREL_EINVAL, "scene is NULL"};
+        if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
+            throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
+        SceneSlot &slot = ctx->scenes[scene_idx];
+        ctx->generation++;  // a captured graph holds the scene by value
+        bool ok = patch_device_scene(ctx, slot, scene, node_ranges, n_node_ranges, prim_ranges, n_prim_ranges);
+        if (!ok) {
+            slot.used = false;
+            build_device_scene(ctx, slot, scene);
+        }
+        if (out_full) *out_full = ok ? 0 : 1;
         return SPECREL_OK;
     });
 }
@@ -798,6 +939,28 @@ int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, ui
     });
 }
 
+int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        if (!out_rgb3) throw Fail{SPECREL_EINVAL, "output is NULL"};
+        if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+        if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(ctx->stream), "download");
+        const size_t n = (size_t)ctx->sizes[2 * view] * ctx->sizes[2 * view + 1];
+        std::vector<float4> tmp(n);
+        check(hipMemcpy(tmp.data(), (float4 *)ctx->hdr.p + host_start(ctx, ROW_PYR, view), n * sizeof(float4),
+                        hipMemcpyDeviceToHost),
+              "readback");
+        for (size_t i = 0; i < n; ++i) {
+            out_rgb3[3 * i] = tmp[i].x;
+            out_rgb3[3 * i + 1] = tmp[i].y;
+            out_rgb3[3 * i + 2] = tmp[i].z;
+        }
+        return SPECREL_OK;
+    });
+}
+
 int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                      uint32_t iters, specrel_tuning *out) {
     if (!ctx) return SPECREL_EINVAL;
@@ -862,6 +1025,114 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->best_ms = best_ms;
             out->default_ms = base_ms;
         }
+        return SPECREL_OK;
+    });
+}
+
+int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
+                    const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out,
+                    double *d_objects) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!target) throw Fail{SPECREL_EINVAL, "target is NULL"};
+        if (n_objects && (!objects || !d_objects)) throw Fail{SPECREL_EINVAL, "objects or d_objects is NULL"};
+        if (n_objects > 1024) throw Fail{SPECREL_EINVAL, "at most 1024 objects"};
+        if (ctx->n_views != 1 || ctx->last_frames.size() != 1) throw Fail{SPECREL_EINVAL, "the last render is not a single view"};
+        const specrel_frame &f = ctx->last_frames[0];
+        if (f.mode != SPECREL_MODE_SHADED || f.slice_mode != SPECREL_VIEW_PAST_CONE ||
+            f.shadow_mode == SPECREL_SHADOW_CAMERA_SLICE || f.num_flashes != 0 || f.num_haze != 0)
+            throw Fail{SPECREL_EINVAL, "the adjoint covers shaded past-light-cone views with null or no shadows, "
+                                       "without flashes or haze"};
+        if (ctx->last_scene >= ctx->scenes.size() || !ctx->scenes[ctx->last_scene].used)
+            throw Fail{SPECREL_EINVAL, "the last render's scene is gone"};
+        SceneSlot &sc = ctx->scenes[ctx->last_scene];
+        std::vector<uint32_t> ofirst(n_objects), ocount(n_objects);
+        std::vector<float4> ocenter(n_objects);
+        for (uint32_t k = 0; k < n_objects; ++k) {
+            const specrel_adj_object &o = objects[k];
+            if (o.prim_count == 0 || (uint64_t)o.prim_first + o.prim_count > sc.num_prims)
+                throw Fail{SPECREL_EINVAL, "object " + std::to_string(k) + " has a bad primitive range"};
+            if (k && o.prim_first < (uint64_t)objects[k - 1].prim_first + objects[k - 1].prim_count)
+                throw Fail{SPECREL_EINVAL, "objects must be sorted by prim_first and disjoint"};
+            ofirst[k] = o.prim_first;
+            ocount[k] = o.prim_count;
+            ocenter[k] = make_float4(o.center[0], o.center[1], o.center[2], o.center[3]);
+        }
+        const uint32_t w = f.size[0], h = f.size[1];
+        const size_t np = (size_t)w * h;
+        const uint32_t tiles = ctx->total_tiles, stride = (uint32_t)(ADJ_CAM + 3 * n_objects);
+        hipStream_t s = ctx->stream;
+        dalloc(ctx->adj_target, np * 3 * sizeof(float));
+        dalloc(ctx->adj_pix, np * ADJ_PIX * sizeof(double));
+        dalloc(ctx->adj_sig, np * 4);
+        dalloc(ctx->adj_objid, np * 4);
+        dalloc(ctx->adj_partials, (size_t)tiles * stride * sizeof(double));
+        dalloc(ctx->adj_counts, (size_t)tiles * ADJ_CLASSES * 4);
+        dalloc(ctx->adj_result, (size_t)stride * sizeof(double) + ADJ_CLASSES * 8);
+        dalloc(ctx->adj_obj, (size_t)n_objects * 24 + 16);
+        if (mask_in) dalloc(ctx->adj_mask_in, np);
+        if (mask_out) dalloc(ctx->adj_mask_out, np);
+        check(hipMemcpyAsync(ctx->adj_target.p, target, np * 3 * sizeof(float), hipMemcpyHostToDevice, s), "target");
+        if (mask_in) check(hipMemcpyAsync(ctx->adj_mask_in.p, mask_in, np, hipMemcpyHostToDevice, s), "mask");
+        float4 *d_center = (float4 *)ctx->adj_obj.p;
+        uint32_t *d_first = (uint32_t *)(d_center + n_objects), *d_count = d_first + n_objects;
+        if (n_objects) {
+            check(hipMemcpyAsync(d_center, ocenter.data(), n_objects * 16, hipMemcpyHostToDevice, s), "objects");
+            check(hipMemcpyAsync(d_first, ofirst.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
+            check(hipMemcpyAsync(d_count, ocount.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
+        }
+        AdjArgs a;
+        a.S = sc.S;
+        a.T.lut = (const uint8_t *)ctx->tables.p;
+        a.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
+        a.V = device_views(ctx);
+        a.slot = (const uint32_t *)ctx->slot.p;
+        a.hdr = (const float4 *)ctx->hdr.p;
+        a.target = (const float *)ctx->adj_target.p;
+        a.mask_in = mask_in ? (const uint8_t *)ctx->adj_mask_in.p : nullptr;
+        a.mask_out = mask_out ? (uint8_t *)ctx->adj_mask_out.p : nullptr;
+        a.obj_first = d_first;
+        a.obj_count = d_count;
+        a.obj_center = d_center;
+        a.n_obj = n_objects;
+        a.pix = (double *)ctx->adj_pix.p;
+        a.sig = (uint32_t *)ctx->adj_sig.p;
+        a.obj = (uint32_t *)ctx->adj_objid.p;
+        a.partials = (double *)ctx->adj_partials.p;
+        a.block_counts = (uint32_t *)ctx->adj_counts.p;
+        a.stride = stride;
+        a.result = (double *)ctx->adj_result.p;
+        a.result_counts = (unsigned long long *)(a.result + stride);
+        check(hipEventRecord(ctx->ev[7], s), "event");
+        k_adj_pixel<16><<<tiles, 256, 0, s>>>(a);
+        check(hipGetLastError(), "k_adj_pixel launch");
+        k_adj_reduce<<<tiles, 256, 0, s>>>(a);
+        check(hipGetLastError(), "k_adj_reduce launch");
+        k_adj_final<<<1, 256, 0, s>>>(a, tiles);
+        check(hipGetLastError(), "k_adj_final launch");
+        check(hipEventRecord(ctx->ev[8], s), "event");
+        check(hipStreamSynchronize(s), "adjoint");
+        std::vector<double> res(stride);
+        uint64_t cnt[ADJ_CLASSES];
+        check(hipMemcpy(res.data(), a.result, stride * sizeof(double), hipMemcpyDeviceToHost), "adjoint readback");
+        check(hipMemcpy(cnt, a.result_counts, sizeof cnt, hipMemcpyDeviceToHost), "adjoint readback");
+        if (mask_out) check(hipMemcpy(mask_out, ctx->adj_mask_out.p, np, hipMemcpyDeviceToHost), "mask readback");
+        ctx->timings.adjoint_ms = ev_ms(ctx->ev[7], ctx->ev[8]);
+        if (out) {
+            std::memset(out, 0, sizeof *out);
+            out->loss = res[0];
+            for (int k = 0; k < 4; ++k) out->d_event[k] = res[1 + k];
+            for (int e = 0; e < 4; ++e)
+                for (int k = 0; k < 4; ++k) out->d_tetrad[e][k] = res[5 + 4 * e + k];
+            out->valid = cnt[0];
+            out->miss = cnt[1];
+            out->starlight = cnt[2];
+            out->border = cnt[3];
+            out->silhouette = cnt[4];
+            out->edge = cnt[5];
+        }
+        for (uint32_t k = 0; k < 3 * n_objects; ++k) d_objects[k] = res[ADJ_CAM + k];
         return SPECREL_OK;
     });
 }

@@ -4,78 +4,6 @@
 
 namespace dev {
 
-// Camera-derived per-view values, computed exactly as render_frame does (renderer.rs:334-358).
-struct ViewDerived {
-    V4 origin, e0, e1, e2, e3;
-    float cone, half_w, half_h, pitch;
-    uint32_t w, h;
-    bool draw_stars;
-};
-
-__device__ __forceinline__ ViewDerived derive(const specrel_frame &f, uint32_t num_stars) {
-    ViewDerived v;
-    v.e0 = lv(f.e[0]);
-    v.e1 = lv(f.e[1]);
-    v.e2 = lv(f.e[2]);
-    v.e3 = lv(f.e[3]);
-    V4 cam = lv(f.cam_pos);
-    if (f.slice_mode == SPECREL_VIEW_FRONTIER) v.origin = cam - v.e0 * f.frontier_view_offset;
-    else if (f.slice_mode == SPECREL_VIEW_PROPHECY) v.origin = cam + v.e0 * f.prophecy_delta;
-    else v.origin = cam;
-    v.cone = f.slice_mode == SPECREL_VIEW_PAST_CONE ? -1.0f : (f.slice_mode == SPECREL_VIEW_FRONTIER ? 1.0f : 0.0f);
-    v.w = f.size[0];
-    v.h = f.size[1];
-    v.half_w = f.half_wh[0];
-    v.half_h = f.half_wh[1];
-    v.pitch = 2.0f * v.half_h / (float)v.h;
-    v.draw_stars = f.mode == SPECREL_MODE_SHADED && num_stars > 0;
-    return v;
-}
-
-__device__ __forceinline__ float pix_sx(const ViewDerived &v, uint32_t px) {
-    float ndc_x = 2.0f * (((float)px + 0.5f) / (float)v.w) - 1.0f;
-    return ndc_x * v.half_w;
-}
-__device__ __forceinline__ float pix_sy(const ViewDerived &v, uint32_t py) {
-    float ndc_y = 1.0f - 2.0f * (((float)py + 0.5f) / (float)v.h);
-    return ndc_y * v.half_h;
-}
-
-// The pixel's ray direction (renderer.rs:369-384 with Tetrad::past/future_light_ray_direction,
-// local_spatial_to_world; minkowski.rs:526-575).
-__device__ __forceinline__ V4 ray_dir(const ViewDerived &v, const specrel_frame &f, uint32_t px, uint32_t py) {
-    float sx = pix_sx(v, px), sy = pix_sy(v, py);
-    V3 local_dir = normalized(V3{sx, sy, 1.0f});
-    if (f.slice_mode == SPECREL_VIEW_PAST_CONE) {
-        V3 n = normalized(local_dir);
-        return -v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
-    }
-    if (f.slice_mode == SPECREL_VIEW_FRONTIER) {
-        V3 n = normalized(local_dir);
-        return v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
-    }
-    return v.e1 * local_dir.x + v.e2 * local_dir.y + v.e3 * local_dir.z;
-}
-
-// Traversal stack of a kernel: SK > 0 keeps SK positions per lane in LDS, SK < 0 keeps the top
-// -SK entries in registers.
-template <int SK, bool = (SK > 0)>
-struct StackSel {
-    using type = LdsStack<SK>;
-    static constexpr int lds_entries = SK * 256;
-    __device__ static void init(type &s, uint2 *lds, uint32_t tid) {
-        s.lds = lds + tid;
-        s.stride = 256;
-        s.n = 0;
-    }
-};
-template <int SK>
-struct StackSel<SK, false> {
-    using type = RegStack<-SK>;
-    static constexpr int lds_entries = 1;
-    __device__ static void init(type &s, uint2 *, uint32_t) { s.n = 0; }
-};
-
 // Star pass: for every star, its cone event and the view-only parts of star_light
 // (star.rs:251-283), plus its unit direction in the camera rest frame for the tile culling.
 // Grid: (star blocks, views).
@@ -384,6 +312,16 @@ __global__ __launch_bounds__(256) void k_occlude_p(OccludeArgs a) {
         atomicAdd(&a.stats[7], (unsigned long long)C.shadow_box_tests);
         atomicAdd(&a.stats[8], (unsigned long long)C.shadow_prim_tests);
     }
+}
+
+// specrel_patch_scene: node i's new bounds go to the child-box pair its parent holds
+// (pslot = parent << 1 | right side).
+__global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint32_t n, float4 *nbox) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    size_t at = 4 * (size_t)(pslot[i] >> 1) + 2 * (pslot[i] & 1u);
+    nbox[at] = bounds[2 * (size_t)i];
+    nbox[at + 1] = bounds[2 * (size_t)i + 1];
 }
 
 // Zeroes the per-frame queue counters (a kernel rather than a memset, so the frame's work can

@@ -35,6 +35,78 @@ __device__ __forceinline__ uint32_t find_view(const uint32_t *start, uint32_t n,
     return lo;
 }
 
+// Camera-derived per-view values, computed exactly as render_frame does (renderer.rs:334-358).
+struct ViewDerived {
+    V4 origin, e0, e1, e2, e3;
+    float cone, half_w, half_h, pitch;
+    uint32_t w, h;
+    bool draw_stars;
+};
+
+__device__ __forceinline__ ViewDerived derive(const specrel_frame &f, uint32_t num_stars) {
+    ViewDerived v;
+    v.e0 = lv(f.e[0]);
+    v.e1 = lv(f.e[1]);
+    v.e2 = lv(f.e[2]);
+    v.e3 = lv(f.e[3]);
+    V4 cam = lv(f.cam_pos);
+    if (f.slice_mode == SPECREL_VIEW_FRONTIER) v.origin = cam - v.e0 * f.frontier_view_offset;
+    else if (f.slice_mode == SPECREL_VIEW_PROPHECY) v.origin = cam + v.e0 * f.prophecy_delta;
+    else v.origin = cam;
+    v.cone = f.slice_mode == SPECREL_VIEW_PAST_CONE ? -1.0f : (f.slice_mode == SPECREL_VIEW_FRONTIER ? 1.0f : 0.0f);
+    v.w = f.size[0];
+    v.h = f.size[1];
+    v.half_w = f.half_wh[0];
+    v.half_h = f.half_wh[1];
+    v.pitch = 2.0f * v.half_h / (float)v.h;
+    v.draw_stars = f.mode == SPECREL_MODE_SHADED && num_stars > 0;
+    return v;
+}
+
+__device__ __forceinline__ float pix_sx(const ViewDerived &v, uint32_t px) {
+    float ndc_x = 2.0f * (((float)px + 0.5f) / (float)v.w) - 1.0f;
+    return ndc_x * v.half_w;
+}
+__device__ __forceinline__ float pix_sy(const ViewDerived &v, uint32_t py) {
+    float ndc_y = 1.0f - 2.0f * (((float)py + 0.5f) / (float)v.h);
+    return ndc_y * v.half_h;
+}
+
+// The pixel's ray direction (renderer.rs:369-384 with Tetrad::past/future_light_ray_direction,
+// local_spatial_to_world; minkowski.rs:526-575).
+__device__ __forceinline__ V4 ray_dir(const ViewDerived &v, const specrel_frame &f, uint32_t px, uint32_t py) {
+    float sx = pix_sx(v, px), sy = pix_sy(v, py);
+    V3 local_dir = normalized(V3{sx, sy, 1.0f});
+    if (f.slice_mode == SPECREL_VIEW_PAST_CONE) {
+        V3 n = normalized(local_dir);
+        return -v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
+    }
+    if (f.slice_mode == SPECREL_VIEW_FRONTIER) {
+        V3 n = normalized(local_dir);
+        return v.e0 + v.e1 * n.x + v.e2 * n.y + v.e3 * n.z;
+    }
+    return v.e1 * local_dir.x + v.e2 * local_dir.y + v.e3 * local_dir.z;
+}
+
+// Traversal stack of a kernel: SK > 0 keeps SK positions per lane in LDS, SK < 0 keeps the top
+// -SK entries in registers.
+template <int SK, bool = (SK > 0)>
+struct StackSel {
+    using type = LdsStack<SK>;
+    static constexpr int lds_entries = SK * 256;
+    __device__ static void init(type &s, uint2 *lds, uint32_t tid) {
+        s.lds = lds + tid;
+        s.stride = 256;
+        s.n = 0;
+    }
+};
+template <int SK>
+struct StackSel<SK, false> {
+    using type = RegStack<-SK>;
+    static constexpr int lds_entries = 1;
+    __device__ static void init(type &s, uint2 *, uint32_t) { s.n = 0; }
+};
+
 struct TraceArgs {
     Scene S;
     Views V;
@@ -79,6 +151,37 @@ struct ShadeArgs {
 
 __global__ void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out, StarDir *dirs);
 __global__ void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists, uint32_t *counts);
+// Adjoint of the shaded image (adjoint.hip): per-pixel floats, camera sums, pixel classes.
+constexpr int ADJ_PIX = 12;     // d origin[4], d direction[4], d object boost[3], loss term
+constexpr int ADJ_CAM = 21;     // loss, d event[4], d e0..e3[16]
+constexpr int ADJ_CLASSES = 6;  // valid, miss, starlight, border, silhouette, shading edge
+struct AdjArgs {
+    Scene S;
+    Tables T;
+    Views V;  // one view
+    const uint32_t *slot;
+    const float4 *hdr;
+    const float *target;  // 3 floats per pixel
+    const uint8_t *mask_in;
+    uint8_t *mask_out;
+    const uint32_t *obj_first;  // objects sorted by first input index, disjoint
+    const uint32_t *obj_count;
+    const float4 *obj_center;
+    uint32_t n_obj;
+    double *pix;     // ADJ_PIX per pixel
+    uint32_t *sig;   // per pixel
+    uint32_t *obj;   // per pixel
+    double *partials;       // per block: ADJ_CAM + 3 n_obj
+    uint32_t *block_counts; // per block: ADJ_CLASSES
+    uint32_t stride;        // ADJ_CAM + 3 n_obj
+    double *result;
+    unsigned long long *result_counts;
+};
+template <int SK>
+__global__ void k_adj_pixel(AdjArgs a);
+__global__ void k_adj_reduce(AdjArgs a);
+__global__ void k_adj_final(AdjArgs a, uint32_t nblocks);
+
 template <bool STATS, int SK>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
@@ -88,6 +191,7 @@ __global__ void k_occlude_p(OccludeArgs a);
 template <bool STATS, int SK, int PASS>
 __global__ void k_shade(ShadeArgs a);
 __global__ void k_reset(uint32_t *counters, uint32_t n);
+__global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint32_t n, float4 *nbox);
 __global__ void k_bloom_down(Views V, int level, float4 *pyr);
 __global__ void k_bloom_up(Views V, int level, float4 *pyr);
 __global__ void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);

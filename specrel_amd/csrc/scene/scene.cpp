@@ -1020,6 +1020,7 @@ struct Rng {
 // ----------------------------------------------------------------------------------------
 struct specrel_builder {
     Prims prims;
+    std::vector<specrel_object> objects;
     std::vector<Star> stars;
     std::vector<specrel_material> materials = common_materials();
     V3 light = v3(2.0f, 6.5f, 2.0f);
@@ -1033,6 +1034,7 @@ struct specrel_host_scene {
     std::vector<specrel_material> materials;
     std::vector<float> star_words;
     std::vector<float> star_ti;
+    std::vector<specrel_object> objects;
     uint32_t num_stars = 0;
     float light[4] = {0, 0, 0, 0};
 };
@@ -1086,6 +1088,7 @@ void finish_into(specrel_builder &b, specrel_host_scene &s) {
         s.attrs[i] = b.prims.attrs[order[i]];
     }
     s.materials = b.materials;
+    s.objects = b.objects;
     // pack_stars (gpu.rs:106-119)
     s.star_words.clear();
     s.star_ti.clear();
@@ -1121,7 +1124,24 @@ void add_tube(specrel_builder &b, uint32_t mesh_kind, const float *mp, uint32_t 
     std::vector<Sample> s = sample_worldline(wl_kind, wp);
     if (do_sync) sync_proper_time_at(s, sync_t);
     if (decided && n_decided != s.size()) throw Error("one decided_at entry per worldline sample");
+    size_t first = b.prims.size();
     extrude(m, s, decided, material, b.prims);
+    // The object's reference event: its worldline sample nearest to lab time 0.
+    specrel_object o;
+    std::memset(&o, 0, sizeof o);
+    o.prim_first = (uint32_t)first;
+    o.prim_count = (uint32_t)(b.prims.size() - first);
+    size_t best = 0;
+    for (size_t i = 1; i < s.size(); ++i)
+        if (std::fabs(s[i].center.t) < std::fabs(s[best].center.t)) best = i;
+    if (!s.empty()) {
+        o.center[0] = s[best].center.t;
+        o.center[1] = s[best].center.x;
+        o.center[2] = s[best].center.y;
+        o.center[3] = s[best].center.z;
+    }
+    o.material = material;
+    b.objects.push_back(o);
 }
 
 specrel_gen_params defaults(const std::string &name) {
@@ -1437,6 +1457,12 @@ int specrel_scene_get_arrays(const specrel_host_scene *s, specrel_scene_arrays *
     o->num_star_words = (uint32_t)(s->star_words.size() / 4);
     for (int i = 0; i < 4; ++i) o->light_pos[i] = s->light[i];
     return 0;
+}
+
+uint32_t specrel_scene_objects(const specrel_host_scene *s, specrel_object *out, uint32_t cap) {
+    uint32_t n = (uint32_t)s->objects.size();
+    if (out) std::memcpy(out, s->objects.data(), sizeof(specrel_object) * std::min(n, cap));
+    return n;
 }
 
 int specrel_scene_star_params(const specrel_host_scene *s, float *ti, uint32_t n) {

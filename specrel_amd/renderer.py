@@ -24,8 +24,8 @@ OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH
 
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
-                                         "trace_ms", "shade_ms", "bloom_ms", "composite_ms")] + \
-        [("reserved", C.c_float * 3)]
+                                         "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "adjoint_ms")] + \
+        [("reserved", C.c_float * 2)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -34,6 +34,19 @@ class Timings(C.Structure):
 class ViewOut(C.Structure):
     _fields_ = [("rgb", C.POINTER(C.c_uint32)), ("hit", C.POINTER(C.c_uint32)), ("lambda_", C.POINTER(C.c_float)),
                 ("tau", C.POINTER(C.c_float))]
+
+
+class AdjObject(C.Structure):
+    _fields_ = [("prim_first", C.c_uint32), ("prim_count", C.c_uint32), ("center", C.c_float * 4)]
+
+
+class AdjointOut(C.Structure):
+    _fields_ = [("loss", C.c_double), ("d_event", C.c_double * 4), ("d_tetrad", (C.c_double * 4) * 4)] + \
+        [(n, C.c_uint64) for n in ("valid", "miss", "starlight", "border", "silhouette", "edge")]
+
+
+class Range(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("count", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -64,6 +77,8 @@ SIGNATURES = {
     "specrel_device_name": (C.c_char_p, [_P]),
     "specrel_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "specrel_upload_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays)]),
+    "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
+                                     C.POINTER(Range), C.c_uint32, C.POINTER(C.c_int)]),
     "specrel_num_scenes": (C.c_uint32, [_P]),
     "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP]),
     "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
@@ -72,7 +87,10 @@ SIGNATURES = {
     "specrel_render_batch": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.POINTER(ViewOut)]),
     "specrel_render_batch_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_int]),
     "specrel_download_view": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, _FP, _FP]),
+    "specrel_download_linear": (C.c_int, [_P, C.c_uint32, _FP]),
     "specrel_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_uint32, C.POINTER(Tuning)]),
+    "specrel_adjoint": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(AdjObject),
+                                  C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
 }
@@ -139,6 +157,16 @@ class Context:
 
     def upload(self, idx, scene):
         self._check(self.lib.specrel_upload_scene(self._h, idx, C.byref(scene.arrays)))
+
+    def patch(self, idx, scene, node_ranges=(), prim_ranges=()):
+        """specrel_patch_scene: returns True when the ranges were applied, False when the library
+        uploaded the whole scene instead."""
+        nr = (Range * max(1, len(node_ranges)))(*[Range(int(a), int(c)) for a, c in node_ranges])
+        pr = (Range * max(1, len(prim_ranges)))(*[Range(int(a), int(c)) for a, c in prim_ranges])
+        full = C.c_int(0)
+        self._check(self.lib.specrel_patch_scene(self._h, idx, C.byref(scene.arrays), nr, len(node_ranges), pr,
+                                                 len(prim_ranges), C.byref(full)))
+        return full.value == 0
 
     def render_frame(self, idx, frame, want_hits=True):
         w, h = frame.size[0], frame.size[1]
@@ -217,11 +245,44 @@ class Context:
             out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
         return out
 
+    def download_linear(self, view, w, h):
+        """The view's pre-bloom linear HDR image of the last render, (h, w, 3) float32."""
+        out = np.zeros((h, w, 3), np.float32)
+        self._check(self.lib.specrel_download_linear(self._h, view, out.ctypes.data_as(_FP)))
+        return out
+
     def autotune(self, idx, frames, iters=3):
         arr = frames if isinstance(frames, C.Array) else self._frames(frames)
         t = Tuning()
         self._check(self.lib.specrel_autotune(self._h, idx, arr, len(arr), iters, C.byref(t)))
         return t.as_dict()
+
+    def adjoint(self, target, objects=(), mask_in=None, want_mask=False):
+        """specrel_adjoint on the last render. target: (h, w, 3) float32 linear HDR; objects:
+        (prim_first, prim_count, center[4]) tuples. Returns a dict with loss, d_event (4),
+        d_tetrad (4x4), d_objects (n x 3), the pixel-class counts and optionally the mask."""
+        tgt = np.ascontiguousarray(target, np.float32)
+        h, w = tgt.shape[0], tgt.shape[1]
+        objs = (AdjObject * max(1, len(objects)))()
+        for i, (first, count, center) in enumerate(objects):
+            objs[i].prim_first, objs[i].prim_count = int(first), int(count)
+            for k in range(4):
+                objs[i].center[k] = center[k]
+        mi = None if mask_in is None else np.ascontiguousarray(mask_in, np.uint8)
+        mo = np.zeros((h, w), np.uint8) if want_mask else None
+        out = AdjointOut()
+        dobj = np.zeros((max(1, len(objects)), 3), np.float64)
+        u8 = C.POINTER(C.c_uint8)
+        self._check(self.lib.specrel_adjoint(self._h, tgt.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), objs,
+                                             len(objects), C.byref(out),
+                                             dobj.ctypes.data_as(C.POINTER(C.c_double))))
+        res = {"loss": out.loss, "d_event": np.array(out.d_event[:]),
+               "d_tetrad": np.array([[out.d_tetrad[a][k] for k in range(4)] for a in range(4)]),
+               "d_objects": dobj[:len(objects)].copy()}
+        res.update({n: getattr(out, n) for n in ("valid", "miss", "starlight", "border", "silhouette", "edge")})
+        if want_mask:
+            res["mask"] = mo
+        return res
 
     def timings(self):
         t = Timings()
@@ -257,10 +318,13 @@ class GpuRenderer:
     def adapter_name(self):
         return self.ctx.device_name()
 
-    def update_scene(self, scene_idx, scene):
-        # The reference patches dirty ranges when it can (gpu.rs:704-731); a full re-upload
-        # is its fallback and gives the same device contents.
-        self.ctx.upload(scene_idx, scene)
+    def update_scene(self, scene_idx, scene, node_ranges=None, prim_ranges=None):
+        # Dirty ranges are patched in place when given (gpu.rs:704-731); otherwise, or when they
+        # no longer fit the uploaded tree, the whole scene is uploaded again.
+        if node_ranges is None and prim_ranges is None:
+            self.ctx.upload(scene_idx, scene)
+        else:
+            self.ctx.patch(scene_idx, scene, node_ranges or (), prim_ranges or ())
 
     def render(self, scene_idx, camera, cfg, dynamics, framebuffer):
         if framebuffer.size != cfg.width * cfg.height:

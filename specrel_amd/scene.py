@@ -194,6 +194,15 @@ class Scene:
     def num_stars(self):
         return self.arrays.num_stars
 
+    def objects(self):
+        """Worldtubes of the scene: list of (prim_first, prim_count, center[t,x,y,z], material), with
+        primitives as input indices (order[] maps leaf slots to them)."""
+        lib = F.scene_lib()
+        n = lib.specrel_scene_objects(self._h, None, 0)
+        arr = (F.Object * max(n, 1))()
+        lib.specrel_scene_objects(self._h, arr, n)
+        return [(o.prim_first, o.prim_count, tuple(o.center), o.material) for o in arr[:n]]
+
     def _view(self, ptr, n, dtype, width):
         if n == 0:
             return np.zeros((0, width), dtype)

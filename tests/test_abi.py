@@ -39,6 +39,8 @@ def test_layouts_match_the_header():
     assert C.sizeof(F.Stats) == 128
     assert C.sizeof(R.Timings) == 48
     assert C.sizeof(R.ViewOut) == 32 and C.sizeof(R.Tuning) == 48
+    assert C.sizeof(R.AdjObject) == 24 and C.sizeof(R.AdjointOut) == 216 and C.sizeof(R.Range) == 8
+    assert C.sizeof(F.Object) == 32
 
 
 def test_argument_errors_are_codes_not_aborts():
@@ -51,6 +53,9 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_render_batch_resident(None, 0, None, 1, 0) == R.EINVAL
     assert lib.specrel_download_view(None, 0, None, None, None, None) == R.EINVAL
     assert lib.specrel_autotune(None, 0, None, 1, 1, None) == R.EINVAL
+    assert lib.specrel_adjoint(None, None, None, None, None, 0, None, None) == R.EINVAL
+    assert lib.specrel_patch_scene(None, 0, None, None, 0, None, 0, None) == R.EINVAL
+    assert lib.specrel_download_linear(None, 0, None) == R.EINVAL
 
 
 def test_create_reports_missing_device():

@@ -1,0 +1,138 @@
+"""CPU checks of the adjoint's host side and of its FP64 forward-mode reference
+(oracle/adjoint_ref.cpp): the observer tetrad against closed forms, its complex-step Jacobian
+against differences, object boosts against the BVH, and the reference's dual-number gradient
+against central differences of its own double-precision loss (camera and object parameters)."""
+import numpy as np
+import pytest
+
+from specrel_amd import adjoint as A
+from specrel_amd import scene as S
+from oracle import pyoracle as O
+
+ETA = np.diag([-1.0, 1.0, 1.0, 1.0])
+
+
+def test_observer_tetrad_closed_forms():
+    e = A.observer_tetrad(np.array([0.7, 0.0, 0.0]), np.zeros(3))
+    assert np.allclose(e[0], [np.cosh(0.7), np.sinh(0.7), 0, 0])
+    assert np.allclose(e[1], [np.sinh(0.7), np.cosh(0.7), 0, 0])
+    assert np.allclose(e[2], [0, 0, 1, 0]) and np.allclose(e[3], [0, 0, 0, 1])
+    e = A.observer_tetrad(np.zeros(3), np.array([0.0, 0.0, np.pi / 2]))  # quarter turn about z
+    assert np.allclose(e[1], [0, 0, 1, 0]) and np.allclose(e[2], [0, -1, 0, 0])
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        e = A.observer_tetrad(rng.normal(size=3), rng.normal(size=3))
+        assert np.allclose(e @ ETA @ e.T, ETA, atol=1e-12)  # orthonormal: e_a . e_b = eta_ab
+
+
+def test_tetrad_jacobian_matches_differences():
+    rng = np.random.default_rng(4)
+    phi, w = rng.normal(size=3) * 0.6, rng.normal(size=3) * 0.4
+    J = A.tetrad_jacobian(phi, w)
+    h = 1e-6
+    for p in range(6):
+        x = np.concatenate([phi, w])
+        xp, xm = x.copy(), x.copy()
+        xp[p] += h
+        xm[p] -= h
+        fd = (A.observer_tetrad(xp[:3], xp[3:]) - A.observer_tetrad(xm[:3], xm[3:])) / (2 * h)
+        assert np.allclose(J[:, :, p], fd, atol=1e-8)
+    # at the identity rotation the series branch is taken
+    J0 = A.tetrad_jacobian(phi, np.zeros(3))
+    assert np.isfinite(J0).all() and np.abs(J0[:, :, 3:]).max() > 0.5
+
+
+def test_boost_matrix_is_a_lorentz_transform():
+    L = A.boost_matrix(np.array([0.3, -0.8, 0.5]))
+    assert np.allclose(L.T @ ETA @ L, ETA, atol=1e-12)
+    assert np.allclose(A.boost_matrix(np.array([0.3, -0.8, 0.5])) @ A.boost_matrix(-np.array([0.3, -0.8, 0.5])),
+                       np.eye(4), atol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def twin():
+    return S.Scene.generate("twin")
+
+
+def setup_view(scene, w=96, h=60, seed=0):
+    cam = scene.default_camera()
+    rng = np.random.default_rng(seed)
+    obs = A.Observer(np.array(cam.position, np.float64), rng.normal(size=3) * 0.1, rng.normal(size=3) * 0.05)
+    tgt_obs = A.Observer(obs.event + rng.normal(size=4) * 0.2, obs.phi + rng.normal(size=3) * 0.05, obs.rot)
+    target = O.render(scene, tgt_obs.frame(w, h), want_hdr=True)["hdr"]
+    return obs, target
+
+
+def ref_loss(scene, obs, w, h, target, objs, mask):
+    return O.adjoint_ref(scene, obs.frame(w, h), obs.theta, target, objs, mask_in=mask, frame_values=False)["loss"]
+
+
+def test_reference_gradient_matches_its_differences_camera(twin):
+    w, h = 96, 60
+    obs, target = setup_view(twin, w, h, seed=1)
+    objs = A.scene_objects(twin)
+    r = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs, frame_values=False)
+    assert r["counts"]["valid"] > 300
+    mask = r["mask"]
+    step = 1e-6
+    for p in range(10):
+        tp, tm = obs.theta.copy(), obs.theta.copy()
+        tp[p] += step
+        tm[p] -= step
+        fd = (ref_loss(twin, A.Observer.from_theta(tp), w, h, target, objs, mask) -
+              ref_loss(twin, A.Observer.from_theta(tm), w, h, target, objs, mask)) / (2 * step)
+        g = r["grad"][p]
+        assert abs(g - fd) <= 2e-4 * max(abs(g), 1e-3 * np.abs(r["grad"][:10]).max()), (p, g, fd)
+
+
+def test_reference_gradient_matches_its_differences_objects(twin):
+    w, h = 96, 60
+    obs, target = setup_view(twin, w, h, seed=2)
+    objs = A.scene_objects(twin)
+    frame = obs.frame(w, h)
+    r = O.adjoint_ref(twin, frame, obs.theta, target, objs)
+    mask = r["mask"]
+    seen = np.nonzero(np.abs(r["grad"][10:]).reshape(-1, 3).sum(axis=1) > 0)[0]
+    assert seen.size >= 1
+    booster = A.ObjectBooster(twin)
+    checked = 0
+    for j in seen[:4]:
+        for ax in range(3):
+            # The boosted records are stored in f32, so a small step is noisy, while a large one
+            # moves far parts of a big object (the ground) across texture cells: the difference
+            # must agree at one of the steps.
+            errs = []
+            g = r["grad"][10 + 3 * j + ax]
+            scale = max(abs(g), 1e-3 * np.abs(r["grad"][10:]).max())
+            for step in (1e-4, 1e-5, 3e-6):
+                vals = []
+                for sgn in (1.0, -1.0):
+                    rap = np.zeros(3)
+                    rap[ax] = sgn * step
+                    booster.boost(objs[j][0], objs[j][1], objs[j][2], rap)
+                    vals.append(O.adjoint_ref(twin, frame, obs.theta, target, objs, mask_in=mask)["loss"])
+                    booster.undo()
+                errs.append(abs(g - (vals[0] - vals[1]) / (2 * step)) / scale)
+            assert min(errs) <= 1e-3, (j, ax, g, errs)
+            checked += 1
+    assert checked >= 6
+
+
+def test_object_boost_keeps_the_bvh_valid_and_undo_restores(twin):
+    booster = A.ObjectBooster(twin)
+    objs = A.scene_objects(twin)
+    before = (twin.geoms().copy(), twin.attrs().copy(), twin.nodes().copy())
+    first, count, center = objs[0]
+    nr, pr = booster.boost(first, count, center, np.array([0.05, -0.02, 0.01]))
+    assert sum(c for _, c in pr) == count and len(nr) >= 1
+    rng = np.random.default_rng(5)
+    cam = twin.default_camera()
+    for _ in range(60):  # BVH traversal still finds what a linear scan finds
+        d = np.concatenate([[-1.0], rng.normal(size=3)])
+        d[1:] /= np.linalg.norm(d[1:])
+        a = O.trace(twin, cam.position, d)
+        b = O.trace(twin, cam.position, d, brute=True)
+        assert a[0] == b[0]
+    booster.undo()
+    assert np.array_equal(twin.geoms(), before[0]) and np.array_equal(twin.attrs(), before[1])
+    assert np.array_equal(twin.nodes(), before[2])
