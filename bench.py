@@ -18,7 +18,8 @@ instrumented traversal counters and HIP-event durations; `cpu_baseline` from the
 `batched`: 64 observers of uneven resolution (160x100 .. 1920x1200) over the same scene in one
 batched render (autotuned launch configuration, graph replay) against a loop of single-view
 renders of the same frames, in views/s; `adjoint`: forward + adjoint times of the image gradient
-on the 32-object subset at 512x320.
+on the 32-object subset at 512x320; `batched_adjoint`: the same for 32 observers of uneven
+resolution over the accelerated-bodies scene in one batch.
 """
 import argparse
 import json
@@ -184,6 +185,49 @@ def bench_adjoint(ctx, steps, slot=1):
     }
 
 
+def bench_batch_adjoint(ctx, steps, slot=2, n_views=32):
+    """Batched forward + adjoint: 32 observers of uneven resolution (128x80 .. 1024x640), each with
+    its own parameters and gradient, over the accelerated-bodies scene (64 Born-rigid bodies with
+    proper acceleration U[0.05, 0.5], 16 light clocks, 256 slices), one launch per pass."""
+    import numpy as np
+    from specrel_amd import adjoint as A
+    from specrel_amd import scene as S
+
+    t0 = time.perf_counter()
+    sc = S.Scene.generate("accel_clocks", S.gen_params("accel_clocks"))
+    log(f"accelerated scene: {sc.num_prims} tets, {len(sc.objects())} objects in {time.perf_counter() - t0:.1f} s")
+    obs, hidden, sizes, objs = A.batch_problem(sc, n_views, seed=11)
+    ctx.upload(slot, sc)
+    targets = A.batch_targets(ctx, slot, hidden, sizes)
+    frames = ctx._frames([o.frame(w, h) for o, (w, h) in zip(obs, sizes)])
+    for _ in range(2):
+        ctx.render_batch_resident(slot, frames)
+        ctx.sync()
+        ctx.adjoint_batch(targets, objs)
+    fwd, adj = [], []
+    for _ in range(steps):
+        ctx.render_batch_resident(slot, frames)
+        ctx.sync()
+        fwd.append(ctx.timings()["kernel_ms"])
+        res = ctx.adjoint_batch(targets, objs)
+        adj.append(ctx.timings()["adjoint_ms"])
+    f, a = float(np.median(fwd)), float(np.median(adj))
+    return {
+        "tets": sc.num_prims,
+        "views": n_views,
+        "rays": int(sum(w * h for w, h in sizes)),
+        "params_per_view": 10 + 3 * len(objs),
+        "fwd_ms": round(f, 4),
+        "adjoint_ms": round(a, 4),
+        "fwd_plus_adjoint_ms": round(f + a, 4),
+        "ratio_to_fwd": round((f + a) / f, 3),
+        "valid_pixels": int(sum(r["valid"] for r in res)),
+        "workload": f"{n_views} observers framing the bodies (width U[128,1024], height width*5/8), each with "
+                    f"its own target (hidden event offset N(0,0.5)^4, rapidity offset N(0,0.1)^3), seed 11; "
+                    f"64 accelerated bodies + 16 light clocks, 256 slices",
+    }
+
+
 def copy_probe():
     """A plain copy kernel of known size: 2^28 floats (1 GiB read + 1 GiB written per launch).
     Its FETCH_SIZE / WRITE_SIZE readings scale the counters of the renderer kernels to bytes."""
@@ -309,6 +353,11 @@ def main():
         adjoint = bench_adjoint(ctx, args.adjoint_steps)
         log(f"adjoint: {adjoint}")
 
+    batch_adjoint = None
+    if args.adjoint_steps > 0:
+        batch_adjoint = bench_batch_adjoint(ctx, args.adjoint_steps)
+        log(f"batched adjoint: {batch_adjoint}")
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline: oracle on a row sample")
@@ -340,6 +389,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "batched_adjoint": batch_adjoint,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

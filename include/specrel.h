@@ -182,6 +182,14 @@ typedef struct specrel_adjoint_out {
 int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
                     const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out,
                     double *d_objects);
+/* The same for every view of the last render (a batch of uneven views, each with its own
+ * observer): one launch per pass over all views' tiles, one gradient per view. targets, masks_in
+ * and masks_out hold the views' pixels one after another (view 0 first, width*height each, 3
+ * floats per pixel for targets); outs receives n_views results and d_objects n_views * 3 *
+ * n_objects doubles (view-major). */
+int specrel_adjoint_batch(specrel_ctx *ctx, const float *targets, const uint8_t *masks_in, uint8_t *masks_out,
+                          const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *outs,
+                          double *d_objects);
 
 int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
 /* Counters of the last render made with SPECREL_OPT_STATS on. */

@@ -287,3 +287,39 @@ def adjoint_problem(scene, seed=7, distance=8.0):
     d = rng.normal(size=3)
     hidden = Observer(obs.event + rng.normal(size=4), 0.8 * d / np.linalg.norm(d), obs.rot.copy())
     return obs, hidden, objs
+
+
+def batch_problem(scene, n_views, seed, w_min=128, w_max=1024, distance=(6.0, 14.0)):
+    """Observers for a batched adjoint, each framing an object of the scene (object v mod count,
+    at a seeded distance, with a small seeded boost), with a width uniform in [w_min, w_max]
+    (height width*5/8), and for each a hidden observer: the event offset by N(0, 0.5)^4 and the
+    rapidity by N(0, 0.1)^3. Returns (observers, hidden observers, sizes, objects)."""
+    objs = scene_objects(scene)
+    rng = np.random.default_rng(seed)
+    obs, hidden, sizes = [], [], []
+    for v in range(n_views):
+        wv = int(rng.integers(w_min, w_max + 1))
+        o = framing_observer(scene, objs, v % len(objs), float(rng.uniform(*distance)), phi=rng.normal(size=3) * 0.05)
+        obs.append(o)
+        hidden.append(Observer(o.event + rng.normal(size=4) * 0.5, o.phi + rng.normal(size=3) * 0.1, o.rot.copy()))
+        sizes.append((wv, max(1, (wv * 5) // 8)))
+    return obs, hidden, sizes, objs
+
+
+def batch_targets(ctx, slot, hidden, sizes):
+    """Renders the hidden observers as one batch and returns their linear images."""
+    frames = [o.frame(w, h) for o, (w, h) in zip(hidden, sizes)]
+    ctx.render_batch_resident(slot, frames)
+    ctx.sync()
+    return [ctx.download_linear(v, w, h) for v, (w, h) in enumerate(sizes)]
+
+
+def batch_gradients(ctx, slot, obs, sizes, targets, objects, masks_in=None, want_mask=False):
+    """Batched forward render of the observers, then the batched adjoint: one gradient vector
+    [event(4), phi(3), rot(3), d_0(3), ...] and result dict per observer."""
+    frames = [o.frame(w, h) for o, (w, h) in zip(obs, sizes)]
+    ctx.render_batch_resident(slot, frames)
+    res = ctx.adjoint_batch(targets, objects, masks_in=masks_in, want_mask=want_mask)
+    grads = [np.concatenate([observer_gradient(r, o), np.asarray(r["d_objects"]).reshape(-1)])
+             for r, o in zip(res, obs)]
+    return grads, res

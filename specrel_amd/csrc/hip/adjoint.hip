@@ -91,21 +91,22 @@ __device__ __forceinline__ uint32_t find_object(const uint32_t *first, const uin
 // for the mask.
 template <int SK>
 __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
-    const uint32_t t = blockIdx.x;
-    const specrel_frame &f = a.V.frames[0];
+    const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
+    const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
-    const uint32_t tiles_x = (v.w + TILE - 1) / TILE, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t px = (t % tiles_x) * TILE + (wave & 1) * 8 + (lane & 7);
-    const uint32_t py = (t / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint32_t px = (lt % tiles_x) * TILE + (wave & 1) * 8 + (lane & 7);
+    const uint32_t py = (lt / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
     StackSel<SK>::init(stk, s_stack, tid);
     if (px >= v.w || py >= v.h) return;
-    const size_t li = (size_t)py * v.w + px;
+    const size_t lp = (size_t)py * v.w + px, li = a.V.pix_start[vi] + lp;  // in the view, in the batch
     double out[ADJ_PIX];
     for (int k = 0; k < ADJ_PIX; ++k) out[k] = 0.0;
     uint32_t sig = SIG_MISS, obj = NO_OBJ;
-    const float4 hv = a.hdr[li];
+    const float4 hv = a.hdr[a.V.pyr_start[vi] + lp];
     const V3 hdr = V3{hv.x, hv.y, hv.z};
     const V3 tgt = V3{a.target[3 * li], a.target[3 * li + 1], a.target[3 * li + 2]};
     const V3 res = hdr - tgt;
@@ -357,15 +358,16 @@ __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
 
 template __global__ void k_adj_pixel<16>(AdjArgs);
 
-// Pass 2: one block per 16x16 tile. A pixel is valid when it hit, carries no starlight and its
+// Pass 2: one block per 16x16 tile of every view. A pixel is valid when it hit, carries no starlight and its
 // 8 neighbours lie in the image with the same signature (or, with a caller's mask, when the mask
 // says so). Valid pixels' terms are summed in double in a fixed order per block.
 __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
-    const uint32_t t = blockIdx.x;
-    const specrel_frame &f = a.V.frames[0];
+    const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
+    const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
-    const uint32_t tiles_x = (v.w + TILE - 1) / TILE, tid = threadIdx.x;
-    const uint32_t px = (t % tiles_x) * TILE + (tid & 15), py = (t / tiles_x) * TILE + (tid >> 4);
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE, tid = threadIdx.x;
+    const uint32_t px = (lt % tiles_x) * TILE + (tid & 15), py = (lt / tiles_x) * TILE + (tid >> 4);
+    const size_t base = a.V.pix_start[vi];
     __shared__ double s_red[256];
     __shared__ uint32_t s_obj[256];
     __shared__ double s_g[256 * 3];
@@ -377,7 +379,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
     double go[3] = {0.0, 0.0, 0.0};
     __syncthreads();
     if (px < v.w && py < v.h) {
-        const size_t li = (size_t)py * v.w + px;
+        const size_t li = base + (size_t)py * v.w + px;
         const uint32_t sig = a.sig[li];
         int cls;  // 0 valid, 1 miss, 2 starlight, 3 image border, 4 silhouette, 5 shading edge
         if (a.mask_in) {
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
             const uint32_t mine = a.obj[li];
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx) {
-                    const size_t ni = (size_t)(py + dy) * v.w + (px + dx);
+                    const size_t ni = base + (size_t)(py + dy) * v.w + (px + dx);
                     const uint32_t s2 = a.sig[ni];
                     if (s2 == sig) continue;
                     if (s2 == SIG_MISS || s2 == SIG_STAR || a.obj[ni] != mine) cls = 4;
@@ -452,17 +454,19 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
     if (tid < ADJ_CLASSES) a.block_counts[(size_t)t * ADJ_CLASSES + tid] = s_cnt[tid];
 }
 
-// Pass 3: the blocks' partial sums added in block order (one thread per component).
-__global__ __launch_bounds__(256) void k_adj_final(AdjArgs a, uint32_t nblocks) {
+// Pass 3: one block per view adds its tiles' partial sums in tile order (one thread per
+// component).
+__global__ __launch_bounds__(256) void k_adj_final(AdjArgs a) {
+    const uint32_t v = blockIdx.x, b0 = a.V.level_start[0][v], b1 = a.V.level_start[0][v + 1];
     for (uint32_t c = threadIdx.x; c < a.stride; c += blockDim.x) {
         double s = 0.0;
-        for (uint32_t b = 0; b < nblocks; ++b) s += a.partials[(size_t)b * a.stride + c];
-        a.result[c] = s;
+        for (uint32_t b = b0; b < b1; ++b) s += a.partials[(size_t)b * a.stride + c];
+        a.result[(size_t)v * a.stride + c] = s;
     }
     if (threadIdx.x < ADJ_CLASSES) {
         uint64_t s = 0;
-        for (uint32_t b = 0; b < nblocks; ++b) s += a.block_counts[(size_t)b * ADJ_CLASSES + threadIdx.x];
-        a.result_counts[threadIdx.x] = s;
+        for (uint32_t b = b0; b < b1; ++b) s += a.block_counts[(size_t)b * ADJ_CLASSES + threadIdx.x];
+        a.result_counts[(size_t)v * ADJ_CLASSES + threadIdx.x] = s;
     }
 }
 

@@ -710,6 +710,119 @@ void download_view(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit
     if (tau) check(hipMemcpy(tau, (float *)ctx->tau.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
 }
 
+// specrel_adjoint / specrel_adjoint_batch on the views of the last render: per-view gradients,
+// one launch per pass over every view's tiles.
+int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
+                  const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out, double *d_objects) {
+    check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (!target) throw Fail{SPECREL_EINVAL, "target is NULL"};
+    if (n_objects && (!objects || !d_objects)) throw Fail{SPECREL_EINVAL, "objects or d_objects is NULL"};
+    if (n_objects > 1024) throw Fail{SPECREL_EINVAL, "at most 1024 objects"};
+    const uint32_t nv = ctx->n_views;
+    if (nv == 0 || ctx->last_frames.size() != nv) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+    for (uint32_t v = 0; v < nv; ++v) {
+        const specrel_frame &f = ctx->last_frames[v];
+        if (f.mode != SPECREL_MODE_SHADED || f.slice_mode != SPECREL_VIEW_PAST_CONE ||
+            f.shadow_mode == SPECREL_SHADOW_CAMERA_SLICE || f.num_flashes != 0 || f.num_haze != 0)
+            throw Fail{SPECREL_EINVAL, "view " + std::to_string(v) + ": the adjoint covers shaded past-light-cone "
+                                       "views with null or no shadows, without flashes or haze"};
+    }
+    if (ctx->last_scene >= ctx->scenes.size() || !ctx->scenes[ctx->last_scene].used)
+        throw Fail{SPECREL_EINVAL, "the last render's scene is gone"};
+    SceneSlot &sc = ctx->scenes[ctx->last_scene];
+    std::vector<uint32_t> ofirst(n_objects), ocount(n_objects);
+    std::vector<float4> ocenter(n_objects);
+    for (uint32_t k = 0; k < n_objects; ++k) {
+        const specrel_adj_object &o = objects[k];
+        if (o.prim_count == 0 || (uint64_t)o.prim_first + o.prim_count > sc.num_prims)
+            throw Fail{SPECREL_EINVAL, "object " + std::to_string(k) + " has a bad primitive range"};
+        if (k && o.prim_first < (uint64_t)objects[k - 1].prim_first + objects[k - 1].prim_count)
+            throw Fail{SPECREL_EINVAL, "objects must be sorted by prim_first and disjoint"};
+        ofirst[k] = o.prim_first;
+        ocount[k] = o.prim_count;
+        ocenter[k] = make_float4(o.center[0], o.center[1], o.center[2], o.center[3]);
+    }
+    const size_t np = ctx->total_pix;
+    const uint32_t tiles = ctx->total_tiles, stride = (uint32_t)(ADJ_CAM + 3 * n_objects);
+    hipStream_t s = ctx->stream;
+    dalloc(ctx->adj_target, np * 3 * sizeof(float));
+    dalloc(ctx->adj_pix, np * ADJ_PIX * sizeof(double));
+    dalloc(ctx->adj_sig, np * 4);
+    dalloc(ctx->adj_objid, np * 4);
+    dalloc(ctx->adj_partials, (size_t)tiles * stride * sizeof(double));
+    dalloc(ctx->adj_counts, (size_t)tiles * ADJ_CLASSES * 4);
+    dalloc(ctx->adj_result, (size_t)nv * (stride * sizeof(double) + ADJ_CLASSES * 8));
+    dalloc(ctx->adj_obj, (size_t)n_objects * 24 + 16);
+    if (mask_in) dalloc(ctx->adj_mask_in, np);
+    if (mask_out) dalloc(ctx->adj_mask_out, np);
+    check(hipMemcpyAsync(ctx->adj_target.p, target, np * 3 * sizeof(float), hipMemcpyHostToDevice, s), "target");
+    if (mask_in) check(hipMemcpyAsync(ctx->adj_mask_in.p, mask_in, np, hipMemcpyHostToDevice, s), "mask");
+    float4 *d_center = (float4 *)ctx->adj_obj.p;
+    uint32_t *d_first = (uint32_t *)(d_center + n_objects), *d_count = d_first + n_objects;
+    if (n_objects) {
+        check(hipMemcpyAsync(d_center, ocenter.data(), n_objects * 16, hipMemcpyHostToDevice, s), "objects");
+        check(hipMemcpyAsync(d_first, ofirst.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
+        check(hipMemcpyAsync(d_count, ocount.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
+    }
+    AdjArgs a;
+    a.S = sc.S;
+    a.T.lut = (const uint8_t *)ctx->tables.p;
+    a.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
+    a.V = device_views(ctx);
+    a.slot = (const uint32_t *)ctx->slot.p;
+    a.hdr = (const float4 *)ctx->hdr.p;
+    a.target = (const float *)ctx->adj_target.p;
+    a.mask_in = mask_in ? (const uint8_t *)ctx->adj_mask_in.p : nullptr;
+    a.mask_out = mask_out ? (uint8_t *)ctx->adj_mask_out.p : nullptr;
+    a.obj_first = d_first;
+    a.obj_count = d_count;
+    a.obj_center = d_center;
+    a.n_obj = n_objects;
+    a.pix = (double *)ctx->adj_pix.p;
+    a.sig = (uint32_t *)ctx->adj_sig.p;
+    a.obj = (uint32_t *)ctx->adj_objid.p;
+    a.partials = (double *)ctx->adj_partials.p;
+    a.block_counts = (uint32_t *)ctx->adj_counts.p;
+    a.stride = stride;
+    a.result = (double *)ctx->adj_result.p;
+    a.result_counts = (unsigned long long *)(a.result + (size_t)nv * stride);
+    check(hipEventRecord(ctx->ev[7], s), "event");
+    k_adj_pixel<16><<<tiles, 256, 0, s>>>(a);
+    check(hipGetLastError(), "k_adj_pixel launch");
+    k_adj_reduce<<<tiles, 256, 0, s>>>(a);
+    check(hipGetLastError(), "k_adj_reduce launch");
+    k_adj_final<<<nv, 256, 0, s>>>(a);
+    check(hipGetLastError(), "k_adj_final launch");
+    check(hipEventRecord(ctx->ev[8], s), "event");
+    check(hipStreamSynchronize(s), "adjoint");
+    std::vector<double> res((size_t)nv * stride);
+    std::vector<uint64_t> cnt((size_t)nv * ADJ_CLASSES);
+    check(hipMemcpy(res.data(), a.result, res.size() * sizeof(double), hipMemcpyDeviceToHost), "adjoint readback");
+    check(hipMemcpy(cnt.data(), a.result_counts, cnt.size() * 8, hipMemcpyDeviceToHost), "adjoint readback");
+    if (mask_out) check(hipMemcpy(mask_out, ctx->adj_mask_out.p, np, hipMemcpyDeviceToHost), "mask readback");
+    ctx->timings.adjoint_ms = ev_ms(ctx->ev[7], ctx->ev[8]);
+    for (uint32_t v = 0; v < nv; ++v) {
+        const double *r = res.data() + (size_t)v * stride;
+        const uint64_t *c = cnt.data() + (size_t)v * ADJ_CLASSES;
+        if (out) {
+            specrel_adjoint_out &o = out[v];
+            std::memset(&o, 0, sizeof o);
+            o.loss = r[0];
+            for (int k = 0; k < 4; ++k) o.d_event[k] = r[1 + k];
+            for (int e = 0; e < 4; ++e)
+                for (int k = 0; k < 4; ++k) o.d_tetrad[e][k] = r[5 + 4 * e + k];
+            o.valid = c[0];
+            o.miss = c[1];
+            o.starlight = c[2];
+            o.border = c[3];
+            o.silhouette = c[4];
+            o.edge = c[5];
+        }
+        for (uint32_t k = 0; k < 3 * n_objects; ++k) d_objects[(size_t)v * 3 * n_objects + k] = r[ADJ_CAM + k];
+    }
+    return SPECREL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1034,106 +1147,17 @@ int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_i
                     double *d_objects) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        if (!target) throw Fail{SPECREL_EINVAL, "target is NULL"};
-        if (n_objects && (!objects || !d_objects)) throw Fail{SPECREL_EINVAL, "objects or d_objects is NULL"};
-        if (n_objects > 1024) throw Fail{SPECREL_EINVAL, "at most 1024 objects"};
-        if (ctx->n_views != 1 || ctx->last_frames.size() != 1) throw Fail{SPECREL_EINVAL, "the last render is not a single view"};
-        const specrel_frame &f = ctx->last_frames[0];
-        if (f.mode != SPECREL_MODE_SHADED || f.slice_mode != SPECREL_VIEW_PAST_CONE ||
-            f.shadow_mode == SPECREL_SHADOW_CAMERA_SLICE || f.num_flashes != 0 || f.num_haze != 0)
-            throw Fail{SPECREL_EINVAL, "the adjoint covers shaded past-light-cone views with null or no shadows, "
-                                       "without flashes or haze"};
-        if (ctx->last_scene >= ctx->scenes.size() || !ctx->scenes[ctx->last_scene].used)
-            throw Fail{SPECREL_EINVAL, "the last render's scene is gone"};
-        SceneSlot &sc = ctx->scenes[ctx->last_scene];
-        std::vector<uint32_t> ofirst(n_objects), ocount(n_objects);
-        std::vector<float4> ocenter(n_objects);
-        for (uint32_t k = 0; k < n_objects; ++k) {
-            const specrel_adj_object &o = objects[k];
-            if (o.prim_count == 0 || (uint64_t)o.prim_first + o.prim_count > sc.num_prims)
-                throw Fail{SPECREL_EINVAL, "object " + std::to_string(k) + " has a bad primitive range"};
-            if (k && o.prim_first < (uint64_t)objects[k - 1].prim_first + objects[k - 1].prim_count)
-                throw Fail{SPECREL_EINVAL, "objects must be sorted by prim_first and disjoint"};
-            ofirst[k] = o.prim_first;
-            ocount[k] = o.prim_count;
-            ocenter[k] = make_float4(o.center[0], o.center[1], o.center[2], o.center[3]);
-        }
-        const uint32_t w = f.size[0], h = f.size[1];
-        const size_t np = (size_t)w * h;
-        const uint32_t tiles = ctx->total_tiles, stride = (uint32_t)(ADJ_CAM + 3 * n_objects);
-        hipStream_t s = ctx->stream;
-        dalloc(ctx->adj_target, np * 3 * sizeof(float));
-        dalloc(ctx->adj_pix, np * ADJ_PIX * sizeof(double));
-        dalloc(ctx->adj_sig, np * 4);
-        dalloc(ctx->adj_objid, np * 4);
-        dalloc(ctx->adj_partials, (size_t)tiles * stride * sizeof(double));
-        dalloc(ctx->adj_counts, (size_t)tiles * ADJ_CLASSES * 4);
-        dalloc(ctx->adj_result, (size_t)stride * sizeof(double) + ADJ_CLASSES * 8);
-        dalloc(ctx->adj_obj, (size_t)n_objects * 24 + 16);
-        if (mask_in) dalloc(ctx->adj_mask_in, np);
-        if (mask_out) dalloc(ctx->adj_mask_out, np);
-        check(hipMemcpyAsync(ctx->adj_target.p, target, np * 3 * sizeof(float), hipMemcpyHostToDevice, s), "target");
-        if (mask_in) check(hipMemcpyAsync(ctx->adj_mask_in.p, mask_in, np, hipMemcpyHostToDevice, s), "mask");
-        float4 *d_center = (float4 *)ctx->adj_obj.p;
-        uint32_t *d_first = (uint32_t *)(d_center + n_objects), *d_count = d_first + n_objects;
-        if (n_objects) {
-            check(hipMemcpyAsync(d_center, ocenter.data(), n_objects * 16, hipMemcpyHostToDevice, s), "objects");
-            check(hipMemcpyAsync(d_first, ofirst.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
-            check(hipMemcpyAsync(d_count, ocount.data(), n_objects * 4, hipMemcpyHostToDevice, s), "objects");
-        }
-        AdjArgs a;
-        a.S = sc.S;
-        a.T.lut = (const uint8_t *)ctx->tables.p;
-        a.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
-        a.V = device_views(ctx);
-        a.slot = (const uint32_t *)ctx->slot.p;
-        a.hdr = (const float4 *)ctx->hdr.p;
-        a.target = (const float *)ctx->adj_target.p;
-        a.mask_in = mask_in ? (const uint8_t *)ctx->adj_mask_in.p : nullptr;
-        a.mask_out = mask_out ? (uint8_t *)ctx->adj_mask_out.p : nullptr;
-        a.obj_first = d_first;
-        a.obj_count = d_count;
-        a.obj_center = d_center;
-        a.n_obj = n_objects;
-        a.pix = (double *)ctx->adj_pix.p;
-        a.sig = (uint32_t *)ctx->adj_sig.p;
-        a.obj = (uint32_t *)ctx->adj_objid.p;
-        a.partials = (double *)ctx->adj_partials.p;
-        a.block_counts = (uint32_t *)ctx->adj_counts.p;
-        a.stride = stride;
-        a.result = (double *)ctx->adj_result.p;
-        a.result_counts = (unsigned long long *)(a.result + stride);
-        check(hipEventRecord(ctx->ev[7], s), "event");
-        k_adj_pixel<16><<<tiles, 256, 0, s>>>(a);
-        check(hipGetLastError(), "k_adj_pixel launch");
-        k_adj_reduce<<<tiles, 256, 0, s>>>(a);
-        check(hipGetLastError(), "k_adj_reduce launch");
-        k_adj_final<<<1, 256, 0, s>>>(a, tiles);
-        check(hipGetLastError(), "k_adj_final launch");
-        check(hipEventRecord(ctx->ev[8], s), "event");
-        check(hipStreamSynchronize(s), "adjoint");
-        std::vector<double> res(stride);
-        uint64_t cnt[ADJ_CLASSES];
-        check(hipMemcpy(res.data(), a.result, stride * sizeof(double), hipMemcpyDeviceToHost), "adjoint readback");
-        check(hipMemcpy(cnt, a.result_counts, sizeof cnt, hipMemcpyDeviceToHost), "adjoint readback");
-        if (mask_out) check(hipMemcpy(mask_out, ctx->adj_mask_out.p, np, hipMemcpyDeviceToHost), "mask readback");
-        ctx->timings.adjoint_ms = ev_ms(ctx->ev[7], ctx->ev[8]);
-        if (out) {
-            std::memset(out, 0, sizeof *out);
-            out->loss = res[0];
-            for (int k = 0; k < 4; ++k) out->d_event[k] = res[1 + k];
-            for (int e = 0; e < 4; ++e)
-                for (int k = 0; k < 4; ++k) out->d_tetrad[e][k] = res[5 + 4 * e + k];
-            out->valid = cnt[0];
-            out->miss = cnt[1];
-            out->starlight = cnt[2];
-            out->border = cnt[3];
-            out->silhouette = cnt[4];
-            out->edge = cnt[5];
-        }
-        for (uint32_t k = 0; k < 3 * n_objects; ++k) d_objects[k] = res[ADJ_CAM + k];
-        return SPECREL_OK;
+        if (ctx->n_views != 1) throw Fail{SPECREL_EINVAL, "the last render is not a single view"};
+        return adjoint_views(ctx, target, mask_in, mask_out, objects, n_objects, out, d_objects);
+    });
+}
+
+int specrel_adjoint_batch(specrel_ctx *ctx, const float *targets, const uint8_t *masks_in, uint8_t *masks_out,
+                          const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *outs,
+                          double *d_objects) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        return adjoint_views(ctx, targets, masks_in, masks_out, objects, n_objects, outs, d_objects);
     });
 }
 

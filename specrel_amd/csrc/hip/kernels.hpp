@@ -158,9 +158,9 @@ constexpr int ADJ_CLASSES = 6;  // valid, miss, starlight, border, silhouette, s
 struct AdjArgs {
     Scene S;
     Tables T;
-    Views V;  // one view
+    Views V;
     const uint32_t *slot;
-    const float4 *hdr;
+    const float4 *hdr;    // the pyramids (pyr_start); every other per-pixel array by pix_start
     const float *target;  // 3 floats per pixel
     const uint8_t *mask_in;
     uint8_t *mask_out;
@@ -171,16 +171,16 @@ struct AdjArgs {
     double *pix;     // ADJ_PIX per pixel
     uint32_t *sig;   // per pixel
     uint32_t *obj;   // per pixel
-    double *partials;       // per block: ADJ_CAM + 3 n_obj
-    uint32_t *block_counts; // per block: ADJ_CLASSES
+    double *partials;       // per tile: ADJ_CAM + 3 n_obj
+    uint32_t *block_counts; // per tile: ADJ_CLASSES
     uint32_t stride;        // ADJ_CAM + 3 n_obj
-    double *result;
-    unsigned long long *result_counts;
+    double *result;         // per view: stride
+    unsigned long long *result_counts;  // per view: ADJ_CLASSES
 };
 template <int SK>
 __global__ void k_adj_pixel(AdjArgs a);
 __global__ void k_adj_reduce(AdjArgs a);
-__global__ void k_adj_final(AdjArgs a, uint32_t nblocks);
+__global__ void k_adj_final(AdjArgs a);
 
 template <bool STATS, int SK>
 __global__ void k_trace(TraceArgs a);

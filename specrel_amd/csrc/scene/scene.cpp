@@ -556,7 +556,10 @@ void extrude(const Mesh &mesh, const std::vector<Sample> &samples, const float *
     for (const Sample &s : samples) {
         for (const V3 &p : mesh.v) {
             float rindler = 1.0f + dot(s.accel_local, p);
-            if (!(rindler > 0.0f)) throw Error("Born-rigid worldtube crosses its Rindler horizon");
+            if (!(rindler > 0.0f))
+                throw Error("Born-rigid worldtube crosses its Rindler horizon (sample t " +
+                            std::to_string(s.center.t) + ", proper acceleration " +
+                            std::to_string(length(s.accel_local)) + ", body extent " + std::to_string(length(p)) + ")");
             emb.push_back(s.center + local_spatial_to_world(s.tetrad, p));
         }
     }
@@ -1202,9 +1205,13 @@ void gen_large(specrel_builder &b, const specrel_gen_params &p, bool clocks_scen
         double pick = rng.uniform();
         if (k < p.first_object) continue;
         uint32_t material = pick < 0.5 ? mats_checker[(int)(pick * 6.0) % 3] : mats_clock[(int)((pick - 0.5) * 10.0) % 5];
+        double window = 3.0;
         if (clocks_scene) {
-            // Keep the Born-rigid body inside its Rindler horizon: a * r < 1 (mesh4d.rs:742-749).
-            radius = std::min(radius, 0.8 / a);
+            // Keep the Born-rigid body inside its Rindler horizon, a * r < 1 (mesh4d.rs:742-749),
+            // with room for the f32 finite-difference proper acceleration of the sampler, whose
+            // error grows like gamma^2: a * r <= 0.6 and gamma <= 2.24 (window |a dt| <= 2).
+            radius = std::min(radius, 0.6 / a);
+            window = 2.0;
         }
         float mp[3];
         uint32_t mk;
@@ -1223,7 +1230,7 @@ void gen_large(specrel_builder &b, const specrel_gen_params &p, bool clocks_scen
         if (accel) {
             double t_turn = rng.range(-60.0, 0.0);
             (void)speed;
-            double wp[12] = {c[0], c[1], c[2], dir[0], dir[1], dir[2], a, t_turn, 3.0, (double)p.t_min,
+            double wp[12] = {c[0], c[1], c[2], dir[0], dir[1], dir[2], a, t_turn, window, (double)p.t_min,
                              (double)p.t_max, (double)p.slices};
             add_tube(b, mk, mp, SPECREL_WL_HYPERBOLIC, wp, true, 0.0f, material, nullptr, 0);
         } else {

@@ -91,6 +91,8 @@ SIGNATURES = {
     "specrel_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_uint32, C.POINTER(Tuning)]),
     "specrel_adjoint": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(AdjObject),
                                   C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
+    "specrel_adjoint_batch": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(AdjObject),
+                                        C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
 }
@@ -282,6 +284,39 @@ class Context:
         res.update({n: getattr(out, n) for n in ("valid", "miss", "starlight", "border", "silhouette", "edge")})
         if want_mask:
             res["mask"] = mo
+        return res
+
+    def adjoint_batch(self, targets, objects=(), masks_in=None, want_mask=False):
+        """specrel_adjoint_batch on the last (batched) render: targets and masks are lists with one
+        (h, w, 3) / (h, w) array per view. Returns one result dict per view, as adjoint()."""
+        tg = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).reshape(-1) for t in targets]))
+        shapes = [np.asarray(t).shape[:2] for t in targets]
+        objs = (AdjObject * max(1, len(objects)))()
+        for i, (first, count, center) in enumerate(objects):
+            objs[i].prim_first, objs[i].prim_count = int(first), int(count)
+            for k in range(4):
+                objs[i].center[k] = center[k]
+        mi = None if masks_in is None else np.ascontiguousarray(
+            np.concatenate([np.asarray(m, np.uint8).reshape(-1) for m in masks_in]))
+        npx = sum(h * w for h, w in shapes)
+        mo = np.zeros(npx, np.uint8) if want_mask else None
+        outs = (AdjointOut * len(targets))()
+        dobj = np.zeros((len(targets), max(1, len(objects)), 3), np.float64)
+        u8 = C.POINTER(C.c_uint8)
+        self._check(self.lib.specrel_adjoint_batch(self._h, tg.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), objs,
+                                                   len(objects), outs,
+                                                   dobj.ctypes.data_as(C.POINTER(C.c_double))))
+        res, off = [], 0
+        for v, (h, w) in enumerate(shapes):
+            o = outs[v]
+            r = {"loss": o.loss, "d_event": np.array(o.d_event[:]),
+                 "d_tetrad": np.array([[o.d_tetrad[a][k] for k in range(4)] for a in range(4)]),
+                 "d_objects": dobj[v, :len(objects)].copy()}
+            r.update({n: getattr(o, n) for n in ("valid", "miss", "starlight", "border", "silhouette", "edge")})
+            if want_mask:
+                r["mask"] = mo[off:off + h * w].reshape(h, w)
+            off += h * w
+            res.append(r)
         return res
 
     def timings(self):

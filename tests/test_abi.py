@@ -54,6 +54,7 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_download_view(None, 0, None, None, None, None) == R.EINVAL
     assert lib.specrel_autotune(None, 0, None, 1, 1, None) == R.EINVAL
     assert lib.specrel_adjoint(None, None, None, None, None, 0, None, None) == R.EINVAL
+    assert lib.specrel_adjoint_batch(None, None, None, None, None, 0, None, None) == R.EINVAL
     assert lib.specrel_patch_scene(None, 0, None, None, 0, None, 0, None) == R.EINVAL
     assert lib.specrel_download_linear(None, 0, None) == R.EINVAL
 
