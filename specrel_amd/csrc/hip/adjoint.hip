@@ -454,19 +454,29 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
     if (tid < ADJ_CLASSES) a.block_counts[(size_t)t * ADJ_CLASSES + tid] = s_cnt[tid];
 }
 
-// Pass 3: one block per view adds its tiles' partial sums in tile order (one thread per
-// component).
+// Pass 3: each view's tiles added up: a block takes 16 components of one view, 16 threads per
+// component stride over the tiles in order, and their 16 partial sums are added in order
+// (the result does not depend on scheduling). Components past `stride` are the pixel classes.
 __global__ __launch_bounds__(256) void k_adj_final(AdjArgs a) {
-    const uint32_t v = blockIdx.x, b0 = a.V.level_start[0][v], b1 = a.V.level_start[0][v + 1];
-    for (uint32_t c = threadIdx.x; c < a.stride; c += blockDim.x) {
-        double s = 0.0;
-        for (uint32_t b = b0; b < b1; ++b) s += a.partials[(size_t)b * a.stride + c];
-        a.result[(size_t)v * a.stride + c] = s;
+    const uint32_t v = blockIdx.x, tid = threadIdx.x, c = blockIdx.y * 16 + (tid & 15), lane = tid >> 4;
+    const uint32_t b0 = a.V.level_start[0][v], b1 = a.V.level_start[0][v + 1];
+    __shared__ double s_part[256];
+    double acc = 0.0;
+    if (c < a.stride) {
+        for (uint32_t b = b0 + lane; b < b1; b += 16) acc += a.partials[(size_t)b * a.stride + c];
+    } else if (c < a.stride + ADJ_CLASSES) {
+        uint64_t n = 0;
+        for (uint32_t b = b0 + lane; b < b1; b += 16) n += a.block_counts[(size_t)b * ADJ_CLASSES + (c - a.stride)];
+        acc = (double)n;  // exact: pixel counts stay far below 2^53
     }
-    if (threadIdx.x < ADJ_CLASSES) {
-        uint64_t s = 0;
-        for (uint32_t b = b0; b < b1; ++b) s += a.block_counts[(size_t)b * ADJ_CLASSES + threadIdx.x];
-        a.result_counts[(size_t)v * ADJ_CLASSES + threadIdx.x] = s;
+    s_part[tid] = acc;
+    __syncthreads();
+    if (lane == 0) {
+        double sum = 0.0;
+        for (uint32_t j = 0; j < 16; ++j) sum += s_part[(tid & 15) + 16 * j];
+        if (c < a.stride) a.result[(size_t)v * a.stride + c] = sum;
+        else if (c < a.stride + ADJ_CLASSES)
+            a.result_counts[(size_t)v * ADJ_CLASSES + (c - a.stride)] = (unsigned long long)sum;
     }
 }
 

@@ -791,7 +791,7 @@ int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in,
     check(hipGetLastError(), "k_adj_pixel launch");
     k_adj_reduce<<<tiles, 256, 0, s>>>(a);
     check(hipGetLastError(), "k_adj_reduce launch");
-    k_adj_final<<<nv, 256, 0, s>>>(a);
+    k_adj_final<<<dim3(nv, (stride + ADJ_CLASSES + 15) / 16), 256, 0, s>>>(a);
     check(hipGetLastError(), "k_adj_final launch");
     check(hipEventRecord(ctx->ev[8], s), "event");
     check(hipStreamSynchronize(s), "adjoint");
