@@ -260,6 +260,7 @@ def main():
                     help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
     ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent primary-ray kernel (-1: default)")
     ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
+    ap.add_argument("--prefetch", type=int, default=-1, help="1/0: child prefetch in the primary trace (-1: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -273,7 +274,7 @@ def main():
 
     import numpy as np
     from specrel_amd import scene as S
-    from specrel_amd.renderer import OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, Context
+    from specrel_amd.renderer import OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, Context
 
     w, h = args.width, args.height
     p = S.gen_params("large", n_objects=args.objects, slices=args.slices, n_stars=args.stars)
@@ -292,6 +293,8 @@ def main():
         ctx.set_option(OPT_PERSISTENT, args.persistent)
     if args.shadow_queue >= 0:
         ctx.set_option(OPT_SHADOW_QUEUE, args.shadow_queue)
+    if args.prefetch >= 0:
+        ctx.set_option(OPT_PREFETCH, args.prefetch)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")

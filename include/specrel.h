@@ -56,8 +56,10 @@ enum {
                                    -4, -8, -12, -16: that many top entries in registers */
     SPECREL_OPT_PERSISTENT = 4, /* 1: persistent primary-ray kernel with per-lane refill; 0 (default): one tile per block */
     SPECREL_OPT_SHADOW_QUEUE = 5, /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
-    SPECREL_OPT_GRAPH = 6         /* replay a captured HIP graph of the launches: 0 never, 1 (default) for
+    SPECREL_OPT_GRAPH = 6,        /* replay a captured HIP graph of the launches: 0 never, 1 (default) for
                                      batches, 2 for every render. Graph renders report kernel_ms only. */
+    SPECREL_OPT_PREFETCH = 7      /* 1: primary rays load both interior children of a node while its boxes
+                                     are tested (LDS stacks only); 0 (default): load a node when visited */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are

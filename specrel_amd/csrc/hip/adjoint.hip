@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
     const uint32_t py = (lt / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
-    StackSel<SK>::init(stk, s_stack, tid);
+    typename StackSel<SK>::Private stk_priv;
+    StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     if (px >= v.w || py >= v.h) return;
     const size_t lp = (size_t)py * v.w + px, li = a.V.pix_start[vi] + lp;  // in the view, in the batch
     double out[ADJ_PIX];

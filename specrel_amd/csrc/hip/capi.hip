@@ -63,6 +63,7 @@ struct specrel_ctx {
     specrel_stats stats_host{};
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
     int opt_graph = 1;
+    int opt_prefetch = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
     hipGraphExec_t graph = nullptr;
@@ -431,9 +432,17 @@ void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const Tra
     if (ctx->opt_persistent) {
         if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
         else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
+    } else if constexpr (K > 0) {
+        if (ctx->opt_prefetch) {
+            if (stats) k_trace<true, K, true><<<tiles, 256, 0, s>>>(ta);
+            else k_trace<false, K, true><<<tiles, 256, 0, s>>>(ta);
+        } else {
+            if (stats) k_trace<true, K, false><<<tiles, 256, 0, s>>>(ta);
+            else k_trace<false, K, false><<<tiles, 256, 0, s>>>(ta);
+        }
     } else {
-        if (stats) k_trace<true, K><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K><<<tiles, 256, 0, s>>>(ta);
+        if (stats) k_trace<true, K, false><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, false><<<tiles, 256, 0, s>>>(ta);
     }
 }
 template <int K>
@@ -913,6 +922,7 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         ctx->opt_stack = value;
     } else if (option == SPECREL_OPT_PERSISTENT) ctx->opt_persistent = value != 0;
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
+    else if (option == SPECREL_OPT_PREFETCH) ctx->opt_prefetch = value != 0;
     else if (option == SPECREL_OPT_GRAPH) {
         if (value < 0 || value > 2) {
             ctx->err = "graph must be 0 (never), 1 (batches) or 2 (every render)";

@@ -144,23 +144,42 @@ constexpr int STACK = 64;  // the reference's stack size (bvh4d.rs:454, 537), ov
 //   RegStack<N>: the top N entries in registers (static-index shifts, no memory on push/pop),
 //                older ones spilled to private memory; a spilled entry is restored when it
 //                re-enters the window, N pops before it is needed, so its load latency hides.
+// A pointer into LDS as such (address space 3): accesses compile to ds_read/ds_write. Through a
+// generic pointer, a choice between an LDS slot and a private slot becomes one flat access.
+typedef __attribute__((address_space(3))) uint2 lds_uint2;
+
+// The private part is an array the kernel declares (`deep`), so that the stack object itself
+// holds only scalars and lives in registers; the LDS read is volatile so that it is not merged
+// with the private read into one access through a generic pointer.
 template <int K>
 struct LdsStack {
-    uint2 *lds;       // this lane's column: lds[pos * stride]
+    static constexpr int DEEP = STACK - K;
+    lds_uint2 *lds;   // this lane's column: lds[pos * stride]
     uint32_t stride;  // lanes per block
     int n;
-    uint2 deep[STACK - K];
+    uint2 *deep;      // DEEP entries in private memory
     __device__ __forceinline__ void reset() { n = 0; }
     __device__ __forceinline__ int size() const { return n; }
     __device__ __forceinline__ void push(uint32_t e, float t) {
-        uint2 v = make_uint2(e, __float_as_uint(t));
-        if (n < K) lds[(uint32_t)n * stride] = v;
-        else deep[n - K] = v;
+        if (n < K) {
+            lds_uint2 *p = lds + (uint32_t)n * stride;
+            p->x = e;
+            p->y = __float_as_uint(t);
+        } else {
+            deep[n - K] = make_uint2(e, __float_as_uint(t));
+        }
         n += 1;
     }
     __device__ __forceinline__ void pop(uint32_t &e, float &t) {
         n -= 1;
-        uint2 v = n < K ? lds[(uint32_t)n * stride] : deep[n - K];
+        uint2 v;
+        if (n < K) {
+            const volatile lds_uint2 *p = lds + (uint32_t)n * stride;
+            v.x = p->x;
+            v.y = p->y;
+        } else {
+            v = deep[n - K];
+        }
         e = v.x;
         t = __uint_as_float(v.y);
     }
@@ -206,7 +225,20 @@ struct RegStack {
 // pushed and popped again but traversed at once: the reference pops it right after pushing it,
 // with nothing in between, so the visit order, the t_enter >= t_max skips and the stack-size
 // limit (both children only when sp + 2 <= 64) are the same.
-template <bool STATS, class ST>
+// An interior node's two child boxes and child entries (the 64 + 8 bytes one step loads).
+struct NodeData {
+    float4 lmn, lmx, rmn, rmx;
+    uint2 info;
+};
+__device__ __forceinline__ NodeData load_node(const Scene &S, uint32_t e) {
+    const float4 *bx = S.nbox + 4 * (size_t)e;
+    return NodeData{bx[0], bx[1], bx[2], bx[3], S.ninfo[e]};
+}
+
+// Bvh4D::trace (bvh4d.rs:443-520). With PF, an interior node's two interior children are loaded
+// while its boxes are tested, so the step into the near child finds its data already on the way;
+// the visiting order, and with it every result and counter, is the same either way.
+template <bool STATS, class ST, bool PF = false>
 __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk) {
     hit.slot = -1;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
@@ -214,11 +246,13 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
     uint32_t e = S.root_entry;
     stk.reset();
-    bool have = true;
+    bool have = true, have_data = false;
+    NodeData D;
     while (true) {
         if (!have) {
             if (stk.size() == 0) break;
             stk.pop(e, t_enter);
+            have_data = false;
         }
         have = false;
         if (t_enter >= tmax) continue;
@@ -242,37 +276,51 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
                 }
             }
         } else {
-            const float4 *bx = S.nbox + 4 * (size_t)e;
-            float4 lmn = bx[0], lmx = bx[1], rmn = bx[2], rmx = bx[3];
-            uint2 info = S.ninfo[e];
+            if (!PF || !have_data) D = load_node(S, e);
+            NodeData DL, DR;
+            bool l_int = false, r_int = false;
+            if (PF) {
+                l_int = (D.info.x & LEAF) == 0;
+                r_int = (D.info.y & LEAF) == 0;
+                if (l_int) DL = load_node(S, D.info.x);
+                if (r_int) DR = load_node(S, D.info.y);
+            }
+            const uint2 info = D.info;
             float tl, tr;
             if (STATS) C.box_tests += 2;
-            bool hl = box_hit(lmn, lmx, o, inv, tmin, tmax, tl);
-            bool hr = box_hit(rmn, rmx, o, inv, tmin, tmax, tr);
+            bool hl = box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl);
+            bool hr = box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr);
+            bool go_left = false, go_right = false;
             if (hl && hr) {
                 if (stk.size() + 2 <= STACK) {
                     if (tl <= tr) {
                         stk.push(info.y, tr);
-                        e = info.x;
-                        t_enter = tl;
+                        go_left = true;
                     } else {
                         stk.push(info.x, tl);
-                        e = info.y;
-                        t_enter = tr;
+                        go_right = true;
                     }
-                    have = true;
                 }
             } else if (hl) {
-                if (stk.size() < STACK) {
-                    e = info.x;
-                    t_enter = tl;
-                    have = true;
-                }
+                go_left = stk.size() < STACK;
             } else if (hr) {
-                if (stk.size() < STACK) {
-                    e = info.y;
-                    t_enter = tr;
-                    have = true;
+                go_right = stk.size() < STACK;
+            }
+            if (go_left) {
+                e = info.x;
+                t_enter = tl;
+                have = true;
+                if (PF) {
+                    D = DL;
+                    have_data = l_int;
+                }
+            } else if (go_right) {
+                e = info.y;
+                t_enter = tr;
+                have = true;
+                if (PF) {
+                    D = DR;
+                    have_data = r_int;
                 }
             }
         }

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
 }
 
 // Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT).
-template <bool STATS, int SK>
+template <bool STATS, int SK, bool PF>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
@@ -161,12 +161,14 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const uint32_t py = (lt / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
-    StackSel<SK>::init(stk, s_stack, tid);
+    typename StackSel<SK>::Private stk_priv;
+    StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
     if (px < v.w && py < v.h) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
-        bool found = !a.S.empty && trace<STATS>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+        bool found = !a.S.empty && trace<STATS, typename StackSel<SK>::type, PF>(a.S, v.origin, dir, 1e-3f, 180.0f,
+                                                                                hit, C, stk);
         a.slot[(size_t)a.V.pix_start[vi] + (size_t)py * v.w + px] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
     }
     if (STATS) {
@@ -190,7 +192,8 @@ __global__ __launch_bounds__(256) void k_trace_p(TraceArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
-    StackSel<SK>::init(stk, s_stack, tid);
+    typename StackSel<SK>::Private stk_priv;
+    StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
     uint64_t rays = 0;
     const uint32_t total = a.num_tiles * 256u;
@@ -258,7 +261,8 @@ __global__ __launch_bounds__(256) void k_occlude_p(OccludeArgs a) {
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
     typename StackSel<SK>::type stk;
-    StackSel<SK>::init(stk, s_stack, tid);
+    typename StackSel<SK>::Private stk_priv;
+    StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
     uint64_t queries = 0;
     const uint32_t total = min(*a.count, a.cap);
@@ -353,7 +357,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     __shared__ uint32_t s_list[PASS == 1 ? 1 : TILE_CAP];
     __shared__ uint32_t s_wave[4], s_total;
     typename StackSel<SK>::type stk;
-    if constexpr (PASS == 0) StackSel<SK>::init(stk, s_stack, tid);
+    typename StackSel<SK>::Private stk_priv;
+    if constexpr (PASS == 0) StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
 
     // Star list of this tile: culled from its super tile's list (or from every star when that
@@ -456,8 +461,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
 }
 
 #define SPECREL_INSTANTIATE(K)                                    \
-    template __global__ void k_trace<false, K>(TraceArgs);       \
-    template __global__ void k_trace<true, K>(TraceArgs);        \
+    template __global__ void k_trace<false, K, false>(TraceArgs); \
+    template __global__ void k_trace<true, K, false>(TraceArgs);  \
     template __global__ void k_trace_p<false, K>(TraceArgs);     \
     template __global__ void k_trace_p<true, K>(TraceArgs);      \
     template __global__ void k_occlude_p<false, K>(OccludeArgs); \
@@ -470,6 +475,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     template __global__ void k_shade<true, K, 2>(ShadeArgs);
 SPECREL_INSTANTIATE(8)
 SPECREL_INSTANTIATE(16)
+template __global__ void k_trace<false, 8, true>(TraceArgs);
+template __global__ void k_trace<true, 8, true>(TraceArgs);
+template __global__ void k_trace<false, 16, true>(TraceArgs);
+template __global__ void k_trace<true, 16, true>(TraceArgs);
 SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)

@@ -90,21 +90,27 @@ __device__ __forceinline__ V4 ray_dir(const ViewDerived &v, const specrel_frame 
 
 // Traversal stack of a kernel: SK > 0 keeps SK positions per lane in LDS, SK < 0 keeps the top
 // -SK entries in registers.
+// Kernels declare `typename StackSel<SK>::Private priv;` next to the stack and pass it to init.
 template <int SK, bool = (SK > 0)>
 struct StackSel {
     using type = LdsStack<SK>;
     static constexpr int lds_entries = SK * 256;
-    __device__ static void init(type &s, uint2 *lds, uint32_t tid) {
-        s.lds = lds + tid;
+    struct Private {
+        uint2 deep[type::DEEP];
+    };
+    __device__ static void init(type &s, uint2 *lds, uint32_t tid, Private &p) {
+        s.lds = (lds_uint2 *)(lds + tid);  // lds is a __shared__ array
         s.stride = 256;
         s.n = 0;
+        s.deep = p.deep;
     }
 };
 template <int SK>
 struct StackSel<SK, false> {
     using type = RegStack<-SK>;
     static constexpr int lds_entries = 1;
-    __device__ static void init(type &s, uint2 *, uint32_t) { s.n = 0; }
+    struct Private {};
+    __device__ static void init(type &s, uint2 *, uint32_t, Private &) { s.n = 0; }
 };
 
 struct TraceArgs {
@@ -182,7 +188,7 @@ __global__ void k_adj_pixel(AdjArgs a);
 __global__ void k_adj_reduce(AdjArgs a);
 __global__ void k_adj_final(AdjArgs a);
 
-template <bool STATS, int SK>
+template <bool STATS, int SK, bool PF>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
 __global__ void k_trace_p(TraceArgs a);

@@ -19,7 +19,7 @@ from . import _ffi as F
 from .scene import build_frame
 
 OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
-OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH = 1, 2, 3, 4, 5, 6
+OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH, OPT_PREFETCH = 1, 2, 3, 4, 5, 6, 7
 
 
 class Timings(C.Structure):
