@@ -290,22 +290,15 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
             if (STATS) C.box_tests += 2;
             bool hl = box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl);
             bool hr = box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr);
-            bool go_left = false, go_right = false;
-            if (hl && hr) {
-                if (stk.size() + 2 <= STACK) {
-                    if (tl <= tr) {
-                        stk.push(info.y, tr);
-                        go_left = true;
-                    } else {
-                        stk.push(info.x, tl);
-                        go_right = true;
-                    }
-                }
-            } else if (hl) {
-                go_left = stk.size() < STACK;
-            } else if (hr) {
-                go_right = stk.size() < STACK;
-            }
+            // Both hit: push the far one, continue with the near one (ties to the left), if two
+            // more entries fit; one hit: continue with it if one more fits (bvh4d.rs:488-512).
+            // One push site with selects instead of two branches.
+            const int sz = stk.size();
+            const bool both = hl && hr && sz + 2 <= STACK;
+            const bool near_left = tl <= tr;
+            if (both) stk.push(near_left ? info.y : info.x, near_left ? tr : tl);
+            const bool go_left = both ? near_left : (hl && !hr && sz < STACK);
+            const bool go_right = both ? !near_left : (hr && !hl && sz < STACK);
             if (go_left) {
                 e = info.x;
                 t_enter = tl;
