@@ -38,17 +38,24 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(st, w, h, n_stars):
+def algorithmic_bytes(st, w, h, n_stars, in_trace):
     """Algorithmic bytes per launch of the two per-pixel kernels, priced with the reference's
     record sizes: 48 B per node pop (BvhNode4D / GpuNode), 32 B per child box tested, 80 B per
     hot record tested (SimplexGeom4D), 192 + 48 B per hit (cold attributes + material).
-      k_trace: the primary traversal, plus 4 B per pixel for the hit slot written.
+      k_trace: the primary traversal, plus 4 B per pixel for the hit slot written; with the shadow
+               queries traced there (in_trace), also the shadow traversal, the hit's 192 + 48 B
+               and 4 B per pixel of occlusion mask written.
       k_shade: 4 B per pixel of hit slot read, per hit the 80 B record intersected again plus
-               192 + 48 B, the shadow queries priced like the primary ones, 64 B per star of the
-               view, and 16 B per pixel of HDR written."""
+               192 + 48 B, the shadow queries priced like the primary ones (or 4 B per pixel of
+               occlusion mask read), 64 B per star of the view, and 16 B per pixel of HDR written."""
+    shadow = 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"] + 80 * st["shadow_prim_tests"]
     trace = 48 * st["node_pops"] + 32 * st["box_tests"] + 80 * st["prim_tests"] + 4 * w * h
-    shade = (4 * w * h + (80 + 240) * st["hits"] + 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"]
-             + 80 * st["shadow_prim_tests"] + 64 * n_stars + 16 * w * h)
+    shade = 4 * w * h + (80 + 240) * st["hits"] + 64 * n_stars + 16 * w * h
+    if in_trace:
+        trace += shadow + 240 * st["hits"] + 4 * w * h
+        shade += 4 * w * h
+    else:
+        shade += shadow
     return {"k_trace": trace, "k_shade": shade}
 
 
@@ -261,6 +268,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent primary-ray kernel (-1: default)")
     ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
     ap.add_argument("--prefetch", type=int, default=-1, help="1/0: child prefetch in the primary trace (-1: default)")
+    ap.add_argument("--shadow-trace", type=int, default=-1,
+                    help="1/0: shadow queries traced by the primary-ray kernel (-1: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -274,7 +283,8 @@ def main():
 
     import numpy as np
     from specrel_amd import scene as S
-    from specrel_amd.renderer import OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, Context
+    from specrel_amd.renderer import (OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
+                                      OPT_STACK, OPT_STATS, SHADOWS_IN_TRACE, Context)
 
     w, h = args.width, args.height
     p = S.gen_params("large", n_objects=args.objects, slices=args.slices, n_stars=args.stars)
@@ -295,6 +305,8 @@ def main():
         ctx.set_option(OPT_SHADOW_QUEUE, args.shadow_queue)
     if args.prefetch >= 0:
         ctx.set_option(OPT_PREFETCH, args.prefetch)
+    if args.shadow_trace >= 0:
+        ctx.set_option(OPT_SHADOW_IN_TRACE, args.shadow_trace)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")
@@ -304,6 +316,7 @@ def main():
     ctx.render_resident(0, frame, True)
     ctx.sync()
     st = ctx.stats()
+    in_trace = ctx.timings()["shadow_path"] == SHADOWS_IN_TRACE
     ctx.set_option(OPT_STATS, 0)
     log(f"counters: {st}")
 
@@ -329,7 +342,7 @@ def main():
     ctx.render_frame(0, frame, want_hits=False)
     e2e_ms = (time.perf_counter() - t0) * 1e3
 
-    alg = algorithmic_bytes(st, w, h, scene.num_stars)
+    alg = algorithmic_bytes(st, w, h, scene.num_stars, in_trace)
     times = {"k_trace": parts["trace_ms"], "k_shade": parts["shade_ms"]}
     dom = max(times, key=times.get)
     achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
@@ -386,6 +399,7 @@ def main():
                    "width": w, "height": h, "tets": scene.num_prims, "nodes": scene.num_nodes,
                    "stars": scene.num_stars},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in parts.items()},
+        "shadow_rays_traced_in": "k_trace" if in_trace else "k_shade",
         "e2e_ms_with_readback": round(e2e_ms, 3),
         "counters": st,
         "roofline": roofline,

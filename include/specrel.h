@@ -45,8 +45,16 @@ typedef struct specrel_timings {
     float bloom_ms;       /* pyramid down + up passes */
     float composite_ms;   /* composite + tone map + pack */
     float adjoint_ms;     /* the last specrel_adjoint's kernels */
-    float reserved[2];
+    uint32_t shadow_path; /* where the last render traced its shadow queries: SPECREL_SHADOWS_* */
+    float reserved;
 } specrel_timings;
+
+enum {
+    SPECREL_SHADOWS_NONE = 0,     /* no shaded view with shadows on */
+    SPECREL_SHADOWS_IN_SHADE = 1, /* by the shading kernel, per pixel */
+    SPECREL_SHADOWS_QUEUED = 2,   /* queued by the shading kernel, traced by a dense any-hit pass */
+    SPECREL_SHADOWS_IN_TRACE = 3  /* by the primary-ray kernel, right after the pixel's closest hit */
+};
 
 /* Options for specrel_set_option. No counterpart in the reference: launch choices of this backend. */
 enum {
@@ -58,8 +66,11 @@ enum {
     SPECREL_OPT_SHADOW_QUEUE = 5, /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
     SPECREL_OPT_GRAPH = 6,        /* replay a captured HIP graph of the launches: 0 never, 1 (default) for
                                      batches, 2 for every render. Graph renders report kernel_ms only. */
-    SPECREL_OPT_PREFETCH = 7      /* 1: primary rays load both interior children of a node while its boxes
+    SPECREL_OPT_PREFETCH = 7,     /* 1: primary rays load both interior children of a node while its boxes
                                      are tested (LDS stacks only); 0 (default): load a node when visited */
+    SPECREL_OPT_SHADOW_IN_TRACE = 8 /* 1 (default): a hit's shadow queries are traced by the primary-ray
+                                       kernel right after its ray (LDS stacks, tiled rays, no queue);
+                                       0: by the shading pass */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are
@@ -80,7 +91,8 @@ typedef struct specrel_tuning {
     uint32_t candidates;  /* configurations timed */
     float best_ms;        /* median wall time per render of the batch with the choice */
     float default_ms;     /* the same with the options the context had before */
-    uint32_t reserved[5];
+    int32_t shadow_in_trace; /* SPECREL_OPT_SHADOW_IN_TRACE */
+    uint32_t reserved[4];
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
@@ -146,8 +158,9 @@ int specrel_render_batch_resident(specrel_ctx *ctx, uint32_t scene_idx, const sp
 int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda,
                           float *out_tau);
 /* Times every launch configuration (stack depth 16/8 in LDS, persistent or tiled primary rays,
- * queued or in-place shadow rays, graph replay or direct launches) on this batch, `iters`
- * renders each after one warm-up, and keeps the fastest by median wall time. */
+ * shadow rays in the shading pass, queued or in the primary-ray kernel, graph replay or direct
+ * launches) on this batch, `iters` renders each after one warm-up, and keeps the fastest by
+ * median wall time. */
 int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                      uint32_t iters, specrel_tuning *out);
 

@@ -64,6 +64,7 @@ struct specrel_ctx {
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
     int opt_graph = 1;
     int opt_prefetch = 0;
+    int opt_shadow_trace = 1;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
     hipGraphExec_t graph = nullptr;
@@ -433,16 +434,19 @@ void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const Tra
         if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
         else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
     } else if constexpr (K > 0) {
-        if (ctx->opt_prefetch) {
-            if (stats) k_trace<true, K, true><<<tiles, 256, 0, s>>>(ta);
-            else k_trace<false, K, true><<<tiles, 256, 0, s>>>(ta);
+        if (ta.mask) {  // shadow queries traced here
+            if (stats) k_trace<true, K, false, true><<<tiles, 256, 0, s>>>(ta);
+            else k_trace<false, K, false, true><<<tiles, 256, 0, s>>>(ta);
+        } else if (ctx->opt_prefetch) {
+            if (stats) k_trace<true, K, true, false><<<tiles, 256, 0, s>>>(ta);
+            else k_trace<false, K, true, false><<<tiles, 256, 0, s>>>(ta);
         } else {
-            if (stats) k_trace<true, K, false><<<tiles, 256, 0, s>>>(ta);
-            else k_trace<false, K, false><<<tiles, 256, 0, s>>>(ta);
+            if (stats) k_trace<true, K, false, false><<<tiles, 256, 0, s>>>(ta);
+            else k_trace<false, K, false, false><<<tiles, 256, 0, s>>>(ta);
         }
     } else {
-        if (stats) k_trace<true, K, false><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K, false><<<tiles, 256, 0, s>>>(ta);
+        if (stats) k_trace<true, K, false, false><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, false, false><<<tiles, 256, 0, s>>>(ta);
     }
 }
 template <int K>
@@ -492,6 +496,15 @@ struct Plan {
     uint32_t max_flashes = 0;
     bool want_hits = false;
 };
+
+// Where a render's shadow queries are traced (specrel_timings.shadow_path): queued for a dense
+// pass, in the primary-ray kernel (LDS stacks, tiled rays), or by the shading pass itself.
+uint32_t shadow_path(const specrel_ctx *ctx, const Plan &pl) {
+    if (!pl.queries) return SPECREL_SHADOWS_NONE;
+    if (ctx->opt_shadow_queue) return SPECREL_SHADOWS_QUEUED;
+    if (ctx->opt_shadow_trace && ctx->opt_stack > 0 && !ctx->opt_persistent) return SPECREL_SHADOWS_IN_TRACE;
+    return SPECREL_SHADOWS_IN_SHADE;
+}
 
 // Validates the frames, sizes the buffers, uploads the frames: everything before the launches.
 Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
@@ -554,11 +567,14 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     }
     mark(1);
     const dim3 tiles(ctx->total_tiles);
+    const uint32_t path = shadow_path(ctx, pl);
+    const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
     TraceArgs ta;
     ta.S = sc.S;
     ta.V = V;
     ta.num_tiles = ctx->total_tiles;
     ta.slot = (uint32_t *)ctx->slot.p;
+    ta.mask = in_trace ? (uint32_t *)ctx->mask.p : nullptr;
     ta.queue = counters + 0;
     ta.stats = (unsigned long long *)ctx->stats.p;
     launch_any(ctx, stats, tiles, s, ta);
@@ -586,7 +602,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ra.qcount = counters + 1;
     ra.qcap = 0;
     ra.stats = (unsigned long long *)ctx->stats.p;
-    if (pl.queries && ctx->opt_shadow_queue) {
+    if (queued) {
         ra.queue = (ShadowQuery *)ctx->squeue.p;
         ra.qcap = (uint32_t)((uint64_t)ctx->total_pix * (1u + pl.max_flashes));
         launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
@@ -601,6 +617,8 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         oa.stats = ra.stats;
         launch_any(ctx, stats, tiles, s, oa);
         check(hipGetLastError(), "k_occlude launch");
+        launch_any(ctx, stats, tiles, s, ShadePass<2>{ra});
+    } else if (in_trace) {
         launch_any(ctx, stats, tiles, s, ShadePass<2>{ra});
     } else {
         launch_any(ctx, stats, tiles, s, ShadePass<0>{ra});
@@ -639,7 +657,8 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     if (use_graph) {
         std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
-                                     (uint64_t)ctx->opt_shadow_queue,   pl.stars,         pl.queries,
+                                     (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
+                                     (uint64_t)ctx->opt_prefetch,       pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
         if (!ctx->graph || key != ctx->graph_key) {
@@ -674,6 +693,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     auto t1 = std::chrono::steady_clock::now();
     ctx->timings = specrel_timings{};
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    ctx->timings.shadow_path = shadow_path(ctx, pl);
     ctx->have_hits = pl.want_hits;
     return SPECREL_OK;
 }
@@ -923,6 +943,7 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     } else if (option == SPECREL_OPT_PERSISTENT) ctx->opt_persistent = value != 0;
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
     else if (option == SPECREL_OPT_PREFETCH) ctx->opt_prefetch = value != 0;
+    else if (option == SPECREL_OPT_SHADOW_IN_TRACE) ctx->opt_shadow_trace = value != 0;
     else if (option == SPECREL_OPT_GRAPH) {
         if (value < 0 || value > 2) {
             ctx->err = "graph must be 0 (never), 1 (batches) or 2 (every render)";
@@ -1092,19 +1113,21 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph;
+            int stack, persistent, queue, graph, in_trace;
         };
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
-                for (int q : {0, 1})
-                    for (int g : {0, 2}) cands.push_back({st, pe, q, g});
-        const Cand saved{ctx->opt_stack, ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph};
+                for (int sh : {0, 1, 2})  // shadows in the shading pass, queued, in the trace kernel
+                    for (int g : {0, 2}) cands.push_back({st, pe, sh == 1, g, sh == 2});
+        const Cand saved{ctx->opt_stack, ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
+                         ctx->opt_shadow_trace};
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
             ctx->opt_shadow_queue = c.queue;
             ctx->opt_graph = c.graph;
+            ctx->opt_shadow_trace = c.in_trace;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -1144,6 +1167,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->persistent = best.persistent;
             out->shadow_queue = best.queue;
             out->graph = best.graph;
+            out->shadow_in_trace = best.in_trace;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

@@ -787,6 +787,22 @@ struct EnqueueShadow {
     }
 };
 
+// Shadow queries traced where they are asked, their answers kept as bit `id` of `mask` for a
+// later shading pass that looks them up (LookupShadow).
+template <bool STATS, class ST>
+struct RecordShadow {
+    const Scene &S;
+    Counters &C;
+    ST &stk;
+    uint32_t mask;
+    __device__ bool query(uint32_t id, V4 o, V4 d, float tmin, float tmax) {
+        if (STATS) C.shadow_rays++;
+        bool occ = occluded<STATS>(S, o, d, tmin, tmax, C, stk);
+        if (occ) mask |= 1u << id;
+        return occ;
+    }
+};
+
 // Shadow queries answered from the occlusion pass: bit `id` of the pixel's mask.
 struct LookupShadow {
     uint32_t mask;

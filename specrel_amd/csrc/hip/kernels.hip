@@ -149,8 +149,11 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
     if (threadIdx.x == 0) counts[gst] = n;
 }
 
-// Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT).
-template <bool STATS, int SK, bool PF>
+// Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT). With SH, a shaded
+// view's hit also traces its shadow queries at once (the ones compute_illumination and
+// compute_flash_illumination make, renderer.rs:585-688, from the same record the shading pass
+// rebuilds) and stores their answers for the shading pass (k_shade PASS 2).
+template <bool STATS, int SK, bool PF, bool SH>
 __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
     const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
@@ -169,7 +172,21 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         Hit hit;
         bool found = !a.S.empty && trace<STATS, typename StackSel<SK>::type, PF>(a.S, v.origin, dir, 1e-3f, 180.0f,
                                                                                 hit, C, stk);
-        a.slot[(size_t)a.V.pix_start[vi] + (size_t)py * v.w + px] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
+        const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
+        a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
+        if constexpr (SH) {
+            uint32_t bits = 0u;
+            if (found && f.mode == SPECREL_MODE_SHADED && f.shadow_mode != SPECREL_SHADOW_OFF) {
+                Rec rec;
+                interpolate(a.S.attrs + 12 * (size_t)hit.slot, hit, rec);
+                const Mat m = load_mat(a.S, rec.material);
+                RecordShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk, 0u};
+                illumination(a.S, f, dir, rec, m, sh);
+                if (f.num_flashes > 0) flashes(a.S, f, rec, sh);
+                bits = sh.mask;
+            }
+            a.mask[pi] = bits;
+        }
     }
     if (STATS) {
         __shared__ unsigned long long s_c[4];
@@ -181,6 +198,12 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
         atomicAdd(&s_c[3], (unsigned long long)C.prim_tests);
         __syncthreads();
         if (tid < 4) atomicAdd(&a.stats[tid], s_c[tid]);
+        if constexpr (SH) {
+            atomicAdd(&a.stats[5], (unsigned long long)C.shadow_rays);
+            atomicAdd(&a.stats[6], (unsigned long long)C.shadow_node_pops);
+            atomicAdd(&a.stats[7], (unsigned long long)C.shadow_box_tests);
+            atomicAdd(&a.stats[8], (unsigned long long)C.shadow_prim_tests);
+        }
     }
 }
 
@@ -460,9 +483,9 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     }
 }
 
-#define SPECREL_INSTANTIATE(K)                                    \
-    template __global__ void k_trace<false, K, false>(TraceArgs); \
-    template __global__ void k_trace<true, K, false>(TraceArgs);  \
+#define SPECREL_INSTANTIATE(K)                                           \
+    template __global__ void k_trace<false, K, false, false>(TraceArgs); \
+    template __global__ void k_trace<true, K, false, false>(TraceArgs);  \
     template __global__ void k_trace_p<false, K>(TraceArgs);     \
     template __global__ void k_trace_p<true, K>(TraceArgs);      \
     template __global__ void k_occlude_p<false, K>(OccludeArgs); \
@@ -475,10 +498,14 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     template __global__ void k_shade<true, K, 2>(ShadeArgs);
 SPECREL_INSTANTIATE(8)
 SPECREL_INSTANTIATE(16)
-template __global__ void k_trace<false, 8, true>(TraceArgs);
-template __global__ void k_trace<true, 8, true>(TraceArgs);
-template __global__ void k_trace<false, 16, true>(TraceArgs);
-template __global__ void k_trace<true, 16, true>(TraceArgs);
+template __global__ void k_trace<false, 8, true, false>(TraceArgs);
+template __global__ void k_trace<true, 8, true, false>(TraceArgs);
+template __global__ void k_trace<false, 16, true, false>(TraceArgs);
+template __global__ void k_trace<true, 16, true, false>(TraceArgs);
+template __global__ void k_trace<false, 8, false, true>(TraceArgs);
+template __global__ void k_trace<true, 8, false, true>(TraceArgs);
+template __global__ void k_trace<false, 16, false, true>(TraceArgs);
+template __global__ void k_trace<true, 16, false, true>(TraceArgs);
 SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)

@@ -118,6 +118,7 @@ struct TraceArgs {
     Views V;
     uint32_t num_tiles;
     uint32_t *slot;
+    uint32_t *mask;   // with shadows traced here: per-pixel occlusion bits (bit 0 light, 1 + i flash i)
     uint32_t *queue;  // persistent variant: ray-queue counter (zeroed per frame)
     unsigned long long *stats;
 };
@@ -188,7 +189,7 @@ __global__ void k_adj_pixel(AdjArgs a);
 __global__ void k_adj_reduce(AdjArgs a);
 __global__ void k_adj_final(AdjArgs a);
 
-template <bool STATS, int SK, bool PF>
+template <bool STATS, int SK, bool PF, bool SH>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
 __global__ void k_trace_p(TraceArgs a);

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
-from specrel_amd.renderer import OPT_GRAPH, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS
+from specrel_amd.renderer import OPT_GRAPH, OPT_PERSISTENT, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS
 from oracle import pyoracle as O
 from tests.helpers import assert_same
 
@@ -128,13 +128,14 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
     try:
         t = gpu_ctx.autotune(0, frames, iters=2)
         assert t["stack"] in (8, 16) and t["persistent"] in (0, 1) and t["shadow_queue"] in (0, 1)
-        assert t["candidates"] == 17 and 0 < t["best_ms"] <= t["default_ms"]
+        assert t["candidates"] == 25 and 0 < t["best_ms"] <= t["default_ms"]
         assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "autotuned")
     finally:
         gpu_ctx.set_option(OPT_STACK, 16)
         gpu_ctx.set_option(OPT_PERSISTENT, 0)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)
         gpu_ctx.set_option(OPT_GRAPH, 1)
+        gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
 
 
 def test_large_scene_random_observers(gpu_ctx):

@@ -131,19 +131,43 @@ def test_stack_variants_are_exact(gpu_ctx, twin, depth):
     assert_same(g, o, f"lds stack {depth}")
 
 
-@pytest.mark.parametrize("persistent,queue", [(0, 0), (1, 0), (0, 1)])
-def test_traversal_variants_are_exact(gpu_ctx, twin, persistent, queue):
-    # tile-launched vs persistent primary rays; shadow rays in place vs queued (with flashes)
-    from specrel_amd.renderer import OPT_PERSISTENT, OPT_SHADOW_QUEUE
+@pytest.mark.parametrize("persistent,queue,in_trace", [(0, 0, 0), (0, 0, 1), (1, 0, 1), (0, 1, 0)])
+def test_traversal_variants_are_exact(gpu_ctx, twin, persistent, queue, in_trace):
+    # tile-launched vs persistent primary rays; shadow rays traced by the shading pass, queued, or
+    # by the primary-ray kernel (with flashes: several queries per pixel)
+    from specrel_amd.renderer import (OPT_PERSISTENT, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, SHADOWS_IN_SHADE,
+                                      SHADOWS_IN_TRACE, SHADOWS_QUEUED)
     cam = twin.default_camera()
     p = cam.position
     dyn = S.FrameDynamics(flashes=[S.FlashEvent((p[0] - 6.0, p[1], p[2], p[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
     fr = S.build_frame(cam, S.RenderConfig(192, 120), dyn)
     gpu_ctx.set_option(OPT_PERSISTENT, persistent)
     gpu_ctx.set_option(OPT_SHADOW_QUEUE, queue)
+    gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, in_trace)
     try:
         g, o = render_both(gpu_ctx, twin, fr)
+        path = gpu_ctx.timings()["shadow_path"]
     finally:
-        gpu_ctx.set_option(OPT_PERSISTENT, 1)
-        gpu_ctx.set_option(OPT_SHADOW_QUEUE, 1)
-    assert_same(g, o, f"persistent={persistent} queue={queue}")
+        gpu_ctx.set_option(OPT_PERSISTENT, 0)
+        gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)
+        gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
+    assert_same(g, o, f"persistent={persistent} queue={queue} in_trace={in_trace}")
+    assert path == (SHADOWS_QUEUED if queue else SHADOWS_IN_TRACE if in_trace and not persistent else SHADOWS_IN_SHADE)
+
+
+def test_counters_with_shadows_in_the_shading_pass(gpu_ctx, twin):
+    from oracle import pyoracle as O
+    from specrel_amd.renderer import OPT_SHADOW_IN_TRACE, OPT_STATS
+    fr = S.build_frame(twin.default_camera(), S.RenderConfig(160, 100))
+    gpu_ctx.upload(0, twin)
+    gpu_ctx.set_option(OPT_STATS, 1)
+    gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 0)
+    try:
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
+    oc = O.render(twin, fr, counts=True)["counts"]
+    for k in ("shadow_rays", "shadow_node_pops", "shadow_box_tests", "shadow_prim_tests"):
+        assert st[k] == oc[k], (k, st[k], oc[k])
