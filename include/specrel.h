@@ -60,7 +60,7 @@ enum {
 enum {
     SPECREL_OPT_STATS = 1,      /* 1: run the instrumented trace variant and fill specrel_stats */
     SPECREL_OPT_STAR_CULL = 2,  /* 1 (default): per-tile star culling; 0: every star per pixel */
-    SPECREL_OPT_STACK = 3,      /* traversal stack: 8 or 16 (default) positions per lane in LDS, or
+    SPECREL_OPT_STACK = 3,      /* traversal stack: 8, 10, 12 or 16 (default) positions per lane in LDS, or
                                    -4, -8, -12, -16: that many top entries in registers */
     SPECREL_OPT_PERSISTENT = 4, /* 1: persistent primary-ray kernel with per-lane refill; 0 (default): one tile per block */
     SPECREL_OPT_SHADOW_QUEUE = 5, /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
@@ -68,9 +68,11 @@ enum {
                                      batches, 2 for every render. Graph renders report kernel_ms only. */
     SPECREL_OPT_PREFETCH = 7,     /* 1: primary rays load both interior children of a node while its boxes
                                      are tested (LDS stacks only); 0 (default): load a node when visited */
-    SPECREL_OPT_SHADOW_IN_TRACE = 8 /* 1 (default): a hit's shadow queries are traced by the primary-ray
+    SPECREL_OPT_SHADOW_IN_TRACE = 8, /* 1 (default): a hit's shadow queries are traced by the primary-ray
                                        kernel right after its ray (LDS stacks, tiled rays, no queue);
                                        0: by the shading pass */
+    SPECREL_OPT_TRACE_WAVES = 9 /* 1, 2 or 4 (default): waves per block of the primary-ray kernel with
+                                   shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are

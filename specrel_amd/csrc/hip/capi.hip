@@ -65,6 +65,7 @@ struct specrel_ctx {
     int opt_graph = 1;
     int opt_prefetch = 0;
     int opt_shadow_trace = 1;
+    int opt_trace_waves = 4;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
     hipGraphExec_t graph = nullptr;
@@ -434,7 +435,16 @@ void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const Tra
         if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
         else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
     } else if constexpr (K > 0) {
-        if (ta.mask) {  // shadow queries traced here
+        if (ta.mask && (K == 8 || K == 16) && ctx->opt_trace_waves != 4) {  // shadow queries traced here
+            const dim3 g(tiles.x * 4 / ctx->opt_trace_waves);
+            if (ctx->opt_trace_waves == 1) {
+                if (stats) k_trace<true, K == 8 ? 8 : 16, false, true, 1><<<g, 64, 0, s>>>(ta);
+                else k_trace<false, K == 8 ? 8 : 16, false, true, 1><<<g, 64, 0, s>>>(ta);
+            } else {
+                if (stats) k_trace<true, K == 8 ? 8 : 16, false, true, 2><<<g, 128, 0, s>>>(ta);
+                else k_trace<false, K == 8 ? 8 : 16, false, true, 2><<<g, 128, 0, s>>>(ta);
+            }
+        } else if (ta.mask) {  // shadow queries traced here
             if (stats) k_trace<true, K, false, true><<<tiles, 256, 0, s>>>(ta);
             else k_trace<false, K, false, true><<<tiles, 256, 0, s>>>(ta);
         } else if (ctx->opt_prefetch) {
@@ -475,6 +485,8 @@ template <class A>
 void launch_any(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const A &args) {
     switch (ctx->opt_stack) {
         case 8: launch_kk<8>(ctx, stats, tiles, s, args); break;
+        case 10: launch_kk<10>(ctx, stats, tiles, s, args); break;
+        case 12: launch_kk<12>(ctx, stats, tiles, s, args); break;
         case -4: launch_kk<-4>(ctx, stats, tiles, s, args); break;
         case -8: launch_kk<-8>(ctx, stats, tiles, s, args); break;
         case -12: launch_kk<-12>(ctx, stats, tiles, s, args); break;
@@ -658,7 +670,8 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
         std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
-                                     (uint64_t)ctx->opt_prefetch,       pl.stars,         pl.queries,
+                                     (uint64_t)ctx->opt_prefetch,       (uint64_t)ctx->opt_trace_waves,
+                                     pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
         if (!ctx->graph || key != ctx->graph_key) {
@@ -935,8 +948,9 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     if (option == SPECREL_OPT_STATS) ctx->opt_stats = value != 0;
     else if (option == SPECREL_OPT_STAR_CULL) ctx->opt_cull = value != 0;
     else if (option == SPECREL_OPT_STACK) {
-        if (value != 8 && value != 16 && value != -4 && value != -8 && value != -12 && value != -16) {
-            ctx->err = "stack must be 8 or 16 (LDS positions) or -4, -8, -12, -16 (register entries)";
+        if (value != 8 && value != 10 && value != 12 && value != 16 && value != -4 && value != -8 && value != -12 &&
+            value != -16) {
+            ctx->err = "stack must be 8, 10, 12 or 16 (LDS positions) or -4, -8, -12, -16 (register entries)";
             return SPECREL_EINVAL;
         }
         ctx->opt_stack = value;
@@ -944,6 +958,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
     else if (option == SPECREL_OPT_PREFETCH) ctx->opt_prefetch = value != 0;
     else if (option == SPECREL_OPT_SHADOW_IN_TRACE) ctx->opt_shadow_trace = value != 0;
+    else if (option == SPECREL_OPT_TRACE_WAVES) {
+        if (value != 1 && value != 2 && value != 4) {
+            ctx->err = "trace waves must be 1, 2 or 4";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_trace_waves = value;
+    }
     else if (option == SPECREL_OPT_GRAPH) {
         if (value < 0 || value > 2) {
             ctx->err = "graph must be 0 (never), 1 (batches) or 2 (every render)";

@@ -153,25 +153,27 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
 // view's hit also traces its shadow queries at once (the ones compute_illumination and
 // compute_flash_illumination make, renderer.rs:585-688, from the same record the shading pass
 // rebuilds) and stores their answers for the shading pass (k_shade PASS 2).
-template <bool STATS, int SK, bool PF, bool SH>
-__global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
-    const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
+template <bool STATS, int SK, bool PF, bool SH, int W>
+__global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
+    static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
+    using SS = StackSel<SK, 64 * W>;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    const uint32_t quarter = blockIdx.x * W + (tid >> 6);  // 8x8 quarters, four per tile
+    const uint32_t t = quarter >> 2, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
-    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const uint32_t px = (lt % tiles_x) * TILE + (wave & 1) * 8 + (lane & 7);
-    const uint32_t py = (lt / tiles_x) * TILE + (wave >> 1) * 8 + (lane >> 3);
-    __shared__ uint2 s_stack[StackSel<SK>::lds_entries];
-    typename StackSel<SK>::type stk;
-    typename StackSel<SK>::Private stk_priv;
-    StackSel<SK>::init(stk, s_stack, tid, stk_priv);
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE, q = quarter & 3;
+    const uint32_t px = (lt % tiles_x) * TILE + (q & 1) * 8 + (lane & 7);
+    const uint32_t py = (lt / tiles_x) * TILE + (q >> 1) * 8 + (lane >> 3);
+    __shared__ uint2 s_stack[SS::lds_entries];
+    typename SS::type stk;
+    typename SS::Private stk_priv;
+    SS::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
     if (px < v.w && py < v.h) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
-        bool found = !a.S.empty && trace<STATS, typename StackSel<SK>::type, PF>(a.S, v.origin, dir, 1e-3f, 180.0f,
-                                                                                hit, C, stk);
+        bool found = !a.S.empty && trace<STATS, typename SS::type, PF>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
         a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
         if constexpr (SH) {
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void k_trace(TraceArgs a) {
                 Rec rec;
                 interpolate(a.S.attrs + 12 * (size_t)hit.slot, hit, rec);
                 const Mat m = load_mat(a.S, rec.material);
-                RecordShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk, 0u};
+                RecordShadow<STATS, typename SS::type> sh{a.S, C, stk, 0u};
                 illumination(a.S, f, dir, rec, m, sh);
                 if (f.num_flashes > 0) flashes(a.S, f, rec, sh);
                 bits = sh.mask;
@@ -497,7 +499,25 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     template __global__ void k_shade<false, K, 2>(ShadeArgs);    \
     template __global__ void k_shade<true, K, 2>(ShadeArgs);
 SPECREL_INSTANTIATE(8)
+SPECREL_INSTANTIATE(10)
+SPECREL_INSTANTIATE(12)
 SPECREL_INSTANTIATE(16)
+template __global__ void k_trace<false, 10, true, false>(TraceArgs);
+template __global__ void k_trace<true, 10, true, false>(TraceArgs);
+template __global__ void k_trace<false, 12, true, false>(TraceArgs);
+template __global__ void k_trace<true, 12, true, false>(TraceArgs);
+template __global__ void k_trace<false, 16, false, true, 1>(TraceArgs);
+template __global__ void k_trace<true, 16, false, true, 1>(TraceArgs);
+template __global__ void k_trace<false, 16, false, true, 2>(TraceArgs);
+template __global__ void k_trace<true, 16, false, true, 2>(TraceArgs);
+template __global__ void k_trace<false, 8, false, true, 1>(TraceArgs);
+template __global__ void k_trace<true, 8, false, true, 1>(TraceArgs);
+template __global__ void k_trace<false, 8, false, true, 2>(TraceArgs);
+template __global__ void k_trace<true, 8, false, true, 2>(TraceArgs);
+template __global__ void k_trace<false, 10, false, true>(TraceArgs);
+template __global__ void k_trace<true, 10, false, true>(TraceArgs);
+template __global__ void k_trace<false, 12, false, true>(TraceArgs);
+template __global__ void k_trace<true, 12, false, true>(TraceArgs);
 template __global__ void k_trace<false, 8, true, false>(TraceArgs);
 template __global__ void k_trace<true, 8, true, false>(TraceArgs);
 template __global__ void k_trace<false, 16, true, false>(TraceArgs);

@@ -91,22 +91,23 @@ __device__ __forceinline__ V4 ray_dir(const ViewDerived &v, const specrel_frame 
 // Traversal stack of a kernel: SK > 0 keeps SK positions per lane in LDS, SK < 0 keeps the top
 // -SK entries in registers.
 // Kernels declare `typename StackSel<SK>::Private priv;` next to the stack and pass it to init.
-template <int SK, bool = (SK > 0)>
+// LANES is the block size (the LDS array holds one column per lane).
+template <int SK, int LANES = 256, bool = (SK > 0)>
 struct StackSel {
     using type = LdsStack<SK>;
-    static constexpr int lds_entries = SK * 256;
+    static constexpr int lds_entries = SK * LANES;
     struct Private {
         uint2 deep[type::DEEP];
     };
     __device__ static void init(type &s, uint2 *lds, uint32_t tid, Private &p) {
         s.lds = (lds_uint2 *)(lds + tid);  // lds is a __shared__ array
-        s.stride = 256;
+        s.stride = LANES;
         s.n = 0;
         s.deep = p.deep;
     }
 };
-template <int SK>
-struct StackSel<SK, false> {
+template <int SK, int LANES>
+struct StackSel<SK, LANES, false> {
     using type = RegStack<-SK>;
     static constexpr int lds_entries = 1;
     struct Private {};
@@ -189,7 +190,8 @@ __global__ void k_adj_pixel(AdjArgs a);
 __global__ void k_adj_reduce(AdjArgs a);
 __global__ void k_adj_final(AdjArgs a);
 
-template <bool STATS, int SK, bool PF, bool SH>
+// W: waves per block (each wave one 8x8 quarter of a 16x16 tile), 1, 2 or 4.
+template <bool STATS, int SK, bool PF, bool SH, int W = 4>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
 __global__ void k_trace_p(TraceArgs a);

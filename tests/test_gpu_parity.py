@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
+from oracle import pyoracle as O
 from tests.helpers import assert_same, frame_for, render_both
 
 pytestmark = pytest.mark.gpu
@@ -115,7 +116,7 @@ def test_traversal_counters_match_oracle(gpu_ctx, twin):
         assert st[k] == oc[k], (k, st[k], oc[k])
 
 
-@pytest.mark.parametrize("depth", [8, -4, -8, -16])
+@pytest.mark.parametrize("depth", [8, 10, 12, -4, -8, -16])
 def test_stack_variants_are_exact(gpu_ctx, twin, depth):
     # LDS stacks (K > 0) and register windows (-N): entries beyond either live in private memory;
     # a 4-entry window sends most rays there.
@@ -153,6 +154,30 @@ def test_traversal_variants_are_exact(gpu_ctx, twin, persistent, queue, in_trace
         gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
     assert_same(g, o, f"persistent={persistent} queue={queue} in_trace={in_trace}")
     assert path == (SHADOWS_QUEUED if queue else SHADOWS_IN_TRACE if in_trace and not persistent else SHADOWS_IN_SHADE)
+
+
+@pytest.mark.parametrize("waves,depth", [(1, 16), (2, 16), (1, 8), (2, 8)])
+def test_primary_blocks_of_fewer_waves_are_exact(gpu_ctx, twin, waves, depth):
+    from specrel_amd.renderer import OPT_STACK, OPT_STATS, OPT_TRACE_WAVES
+    cam = twin.default_camera()
+    p = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((p[0] - 6.0, p[1], p[2], p[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    fr = S.build_frame(cam, S.RenderConfig(200, 120), dyn)
+    gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+    gpu_ctx.set_option(OPT_STACK, depth)
+    try:
+        g, o = render_both(gpu_ctx, twin, fr)
+        gpu_ctx.set_option(OPT_STATS, 1)
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 4)
+        gpu_ctx.set_option(OPT_STACK, 16)
+    assert_same(g, o, f"{waves} waves per block, stack {depth}")
+    oc = O.render(twin, fr, counts=True)["counts"]
+    for k in ("rays", "node_pops", "box_tests", "prim_tests", "shadow_rays", "shadow_node_pops"):
+        assert st[k] == oc[k], (k, st[k], oc[k])
 
 
 def test_counters_with_shadows_in_the_shading_pass(gpu_ctx, twin):
