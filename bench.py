@@ -26,6 +26,7 @@ import json
 import os
 import sys
 import time
+import types
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -72,6 +73,12 @@ def cpu_baseline(scene, frame, w, h, rows):
     t0 = time.perf_counter()
     O.render_rows(scene, frame, step // 2, step, rows, nthreads=cores)
     t_rows = time.perf_counter() - t0
+    # The same rows without the star loop: the split between traversal + shading and stars.
+    no_stars = types.SimpleNamespace(arrays=type(scene.arrays).from_buffer_copy(scene.arrays))
+    no_stars.arrays.num_stars = 0
+    t0 = time.perf_counter()
+    O.render_rows(no_stars, frame, step // 2, step, rows, nthreads=cores)
+    t_trace = time.perf_counter() - t0
     hdr = np.full((h, w, 3), 0.25, np.float32)
     t0 = time.perf_counter()
     O.pack(O.bloom(hdr))
@@ -85,6 +92,8 @@ def cpu_baseline(scene, frame, w, h, rows):
         "sample": f"{rows} of {h} rows (every {step}th, {rows * w} rays incl. the full star loop and shadow rays) "
                   f"in {t_rows:.2f} s, plus bloom+tone map of a full {w}x{h} frame in {t_post:.3f} s; "
                   f"extrapolated frame time {frame_s:.2f} s",
+        "split_s_per_frame": {"traversal_shading_shadows": round(t_trace * h / rows, 3),
+                              "stars": round((t_rows - t_trace) * h / rows, 3), "bloom_tonemap": round(t_post, 3)},
     }
 
 

@@ -109,6 +109,11 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value);
 /* GpuRenderer::upload_scene (gpu.rs:622-699): copy (and re-pack) one scene into slot
  * scene_idx, replacing what was there. The arrays are PackedSceneGpuData's plus `order`. */
 int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene);
+/* The same from a scene file (include/specrel_file.h) written by any producer, in place of the
+ * arrays PackedSceneGpuData::from_scene (gpu.rs:247-289) hands GpuRenderer::new in process (and
+ * web.rs:431-479 hands WebGPU). The file is mapped read-only and copied to the device; verify != 0
+ * checks the section checksums first. SPECREL_EINVAL names what is wrong with a bad file. */
+int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *path, int verify);
 /* A range [first, first + count) of node or primitive (leaf-order) indices. */
 typedef struct specrel_range {
     uint32_t first, count;

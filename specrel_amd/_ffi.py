@@ -91,6 +91,27 @@ class GenParams(C.Structure):
                 ("first_object", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class FileInfo(C.Structure):
+    _fields_ = [("version", C.c_uint32), ("flags", C.c_uint32), ("bvh_layout", C.c_uint64),
+                ("bvh_edit", C.c_uint64), ("file_bytes", C.c_uint64), ("num_objects", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class FileHeader(C.Structure):
+    _fields_ = [("magic", C.c_char * 8), ("version", C.c_uint32), ("byte_order", C.c_uint32),
+                ("header_bytes", C.c_uint32), ("num_sections", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("num_prims", C.c_uint32), ("num_materials", C.c_uint32), ("num_stars", C.c_uint32),
+                ("num_star_words", C.c_uint32), ("num_objects", C.c_uint32), ("light_pos", C.c_float * 4),
+                ("bvh_layout", C.c_uint64), ("bvh_edit", C.c_uint64), ("file_bytes", C.c_uint64),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+
+
+class FileSection(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("elem_bytes", C.c_uint32), ("count", C.c_uint64), ("offset", C.c_uint64),
+                ("checksum", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+assert C.sizeof(FileHeader) == 128 and C.sizeof(FileSection) == 32 and C.sizeof(FileInfo) == 40
 assert C.sizeof(Node) == 48 and C.sizeof(Geom) == 80 and C.sizeof(Attr) == 192
 assert C.sizeof(Material) == 48 and C.sizeof(Frame) == 800
 
@@ -141,6 +162,12 @@ def _scene_protos(lib):
                                           C.POINTER(Frame)]),
         "specrel_common_materials": (_u32, [C.POINTER(Material), _u32]),
         "specrel_build_tables": (None, [C.POINTER(_u32)]),
+        "specrel_scene_file_write": (C.c_int, [C.c_char_p, C.POINTER(SceneArrays), C.POINTER(Object), _u32,
+                                               C.c_uint64, C.c_uint64]),
+        "specrel_scene_file_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P), C.POINTER(SceneArrays),
+                                              C.POINTER(FileInfo)]),
+        "specrel_scene_file_objects": (C.POINTER(Object), [_P, C.POINTER(_u32)]),
+        "specrel_scene_file_close": (None, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

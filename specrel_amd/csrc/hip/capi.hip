@@ -8,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "../core/scene_file.hpp"
 #include "../core/tables.h"
 #include "kernels.hpp"
 #include "specrel.h"
@@ -991,6 +992,27 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
         ctx->generation++;  // a captured graph holds the old scene pointers
         build_device_scene(ctx, slot, scene);
         (void)fresh;
+        return SPECREL_OK;
+    });
+}
+
+int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *path, int verify) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (scene_idx > 4096) throw Fail{SPECREL_EINVAL, "scene index too large"};
+        sr::file::Mapped m;
+        specrel_scene_arrays a;
+        try {
+            sr::file::open(path, verify != 0, m, a);
+        } catch (const sr::file::Error &e) {
+            throw Fail{SPECREL_EINVAL, e.msg};
+        }
+        if (ctx->scenes.size() <= scene_idx) ctx->scenes.resize(scene_idx + 1);
+        SceneSlot &slot = ctx->scenes[scene_idx];
+        slot.used = false;
+        ctx->generation++;
+        build_device_scene(ctx, slot, &a);  // copies out of the mapping before it is unmapped
         return SPECREL_OK;
     });
 }

@@ -27,7 +27,9 @@
 #include <vector>
 
 #include "../core/minkowski.h"
+#include "../core/scene_file.hpp"
 #include "../core/tables.h"
+#include "specrel_file.h"
 #include "specrel_scene.h"
 
 using namespace sr;
@@ -1629,3 +1631,55 @@ uint32_t specrel_common_materials(specrel_material *out, uint32_t cap) {
 void specrel_build_tables(uint32_t *out) { build_tables(out); }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Scene file (include/specrel_file.h)
+// ---------------------------------------------------------------------------------------
+struct specrel_scene_file {
+    sr::file::Mapped m;
+};
+
+int specrel_scene_file_write(const char *path, const specrel_scene_arrays *s, const specrel_object *objects,
+                             uint32_t n_objects, uint64_t bvh_layout, uint64_t bvh_edit) {
+    return guarded([&] {
+        if (!s) throw std::invalid_argument("scene is NULL");
+        try {
+            sr::file::write(path, *s, objects, n_objects, bvh_layout, bvh_edit);
+        } catch (const sr::file::Error &e) {
+            throw std::runtime_error(e.msg);
+        }
+        return 0;
+    });
+}
+
+int specrel_scene_file_open(const char *path, int verify, specrel_scene_file **out, specrel_scene_arrays *arrays,
+                            specrel_file_info *info) {
+    return guarded([&] {
+        if (!out || !arrays) throw std::invalid_argument("out or arrays is NULL");
+        *out = nullptr;
+        std::unique_ptr<specrel_scene_file> f(new specrel_scene_file);
+        try {
+            sr::file::open(path, verify != 0, f->m, *arrays);
+        } catch (const sr::file::Error &e) {
+            throw std::runtime_error(e.msg);
+        }
+        if (info) {
+            std::memset(info, 0, sizeof *info);
+            info->version = f->m.h.version;
+            info->flags = f->m.h.flags;
+            info->bvh_layout = f->m.h.bvh_layout;
+            info->bvh_edit = f->m.h.bvh_edit;
+            info->file_bytes = f->m.h.file_bytes;
+            info->num_objects = f->m.h.num_objects;
+        }
+        *out = f.release();
+        return 0;
+    });
+}
+
+const specrel_object *specrel_scene_file_objects(const specrel_scene_file *f, uint32_t *count) {
+    if (count) *count = f ? f->m.h.num_objects : 0;
+    return f ? (const specrel_object *)f->m.objects : nullptr;
+}
+
+void specrel_scene_file_close(specrel_scene_file *f) { delete f; }

@@ -80,6 +80,7 @@ SIGNATURES = {
     "specrel_device_name": (C.c_char_p, [_P]),
     "specrel_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "specrel_upload_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays)]),
+    "specrel_upload_scene_file": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_int]),
     "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
                                      C.POINTER(Range), C.c_uint32, C.POINTER(C.c_int)]),
     "specrel_num_scenes": (C.c_uint32, [_P]),
@@ -162,6 +163,10 @@ class Context:
 
     def upload(self, idx, scene):
         self._check(self.lib.specrel_upload_scene(self._h, idx, C.byref(scene.arrays)))
+
+    def upload_file(self, idx, path, verify=True):
+        """specrel_upload_scene_file: a scene file written by any producer (scene.Scene.save)."""
+        self._check(self.lib.specrel_upload_scene_file(self._h, idx, os.fsencode(path), 1 if verify else 0))
 
     def patch(self, idx, scene, node_ranges=(), prim_ranges=()):
         """specrel_patch_scene: returns True when the ranges were applied, False when the library

@@ -6,6 +6,7 @@ Mirrors the reference's Rust types that feed `GpuRenderer`: `Scene4D` (src/scene
 worldlines, extrusion, SAH BVH, packing) runs in libspecrel_scene.so.
 """
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -224,11 +225,60 @@ class Scene:
     def star_words(self):
         return self._view(self.arrays.star_words, self.arrays.num_star_words, np.float32, 4)
 
+    def save(self, path, bvh_layout=0, bvh_edit=0):
+        """Writes the versioned scene file (include/specrel_file.h) with the object table."""
+        lib = F.scene_lib()
+        n = lib.specrel_scene_objects(self._h, None, 0) if self._h.value else 0
+        objs = (F.Object * max(n, 1))()
+        if n:
+            lib.specrel_scene_objects(self._h, objs, n)
+        _check(lib.specrel_scene_file_write(os.fsencode(path), C.byref(self.arrays), objs, n, bvh_layout, bvh_edit))
+
     def star_params(self):
         out = np.zeros((self.num_stars, 2), np.float32)
         if self.num_stars:
             F.scene_lib().specrel_scene_star_params(self._h, _fptr(out), self.num_stars)
         return out
+
+
+class SceneFile(Scene):
+    """A scene read from a scene file: the arrays point into a read-only mapping of the file,
+    kept until the object is closed or collected. Usable wherever a Scene is (upload, oracle)."""
+
+    def __init__(self, path, verify=True):
+        self.name = os.path.basename(os.fspath(path))
+        self.arrays = F.SceneArrays()
+        self.info = F.FileInfo()
+        h = C.c_void_p()
+        _check(F.scene_lib().specrel_scene_file_open(os.fsencode(path), 1 if verify else 0, C.byref(h),
+                                                     C.byref(self.arrays), C.byref(self.info)))
+        self._f = h
+
+    def close(self):
+        if getattr(self, "_f", None) and self._f.value:
+            F.scene_lib().specrel_scene_file_close(self._f)
+            self._f = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def objects(self):
+        n = C.c_uint32(0)
+        p = F.scene_lib().specrel_scene_file_objects(self._f, C.byref(n))
+        return [(o.prim_first, o.prim_count, tuple(o.center), o.material) for o in p[:n.value]] if n.value else []
+
+    def save(self, path, bvh_layout=None, bvh_edit=None):
+        objs = self.objects()
+        arr = (F.Object * max(len(objs), 1))()
+        for i, (a, c, cen, m) in enumerate(objs):
+            arr[i] = F.Object(a, c, (C.c_float * 4)(*cen), m, 0)
+        _check(F.scene_lib().specrel_scene_file_write(
+            os.fsencode(path), C.byref(self.arrays), arr, len(objs),
+            self.info.bvh_layout if bvh_layout is None else bvh_layout,
+            self.info.bvh_edit if bvh_edit is None else bvh_edit))
+
+    def star_params(self):
+        raise NotImplementedError("a scene file holds packed stars only")
 
 
 def gen_params(name, **overrides) -> F.GenParams:
