@@ -277,6 +277,7 @@ def main():
     ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent primary-ray kernel (-1: default)")
     ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
     ap.add_argument("--prefetch", type=int, default=-1, help="1/0: child prefetch in the primary trace (-1: default)")
+    ap.add_argument("--star-overlap", type=int, default=-1, help="1/0: star pass beside the primary rays (-1: default)")
     ap.add_argument("--trace-waves", type=int, default=0, help="1, 2, 4: waves per primary-ray block (0: default)")
     ap.add_argument("--shadow-trace", type=int, default=-1,
                     help="1/0: shadow queries traced by the primary-ray kernel (-1: default)")
@@ -294,7 +295,8 @@ def main():
     import numpy as np
     from specrel_amd import scene as S
     from specrel_amd.renderer import (OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
-                                      OPT_STACK, OPT_STATS, OPT_TRACE_WAVES, SHADOWS_IN_TRACE, Context)
+                                      OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, SHADOWS_IN_TRACE,
+                                      Context)
 
     w, h = args.width, args.height
     p = S.gen_params("large", n_objects=args.objects, slices=args.slices, n_stars=args.stars)
@@ -319,6 +321,8 @@ def main():
         ctx.set_option(OPT_SHADOW_IN_TRACE, args.shadow_trace)
     if args.trace_waves:
         ctx.set_option(OPT_TRACE_WAVES, args.trace_waves)
+    if args.star_overlap >= 0:
+        ctx.set_option(OPT_STAR_OVERLAP, args.star_overlap)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")

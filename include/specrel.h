@@ -71,8 +71,10 @@ enum {
     SPECREL_OPT_SHADOW_IN_TRACE = 8, /* 1 (default): a hit's shadow queries are traced by the primary-ray
                                        kernel right after its ray (LDS stacks, tiled rays, no queue);
                                        0: by the shading pass */
-    SPECREL_OPT_TRACE_WAVES = 9 /* 1, 2 or 4 (default): waves per block of the primary-ray kernel with
-                                   shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
+    SPECREL_OPT_TRACE_WAVES = 9, /* 1, 2 or 4 (default): waves per block of the primary-ray kernel with
+                                    shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
+    SPECREL_OPT_STAR_OVERLAP = 10 /* 1: the star pass runs on a second stream beside the primary
+                                     rays; 0 (default): before them on the same stream */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are

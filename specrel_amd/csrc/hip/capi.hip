@@ -58,8 +58,12 @@ struct specrel_ctx {
     uint32_t last_scene = 0;
     uint64_t generation = 0;  // bumped whenever a buffer a captured graph points to moves
     int cus = 0;
-    hipEvent_t ev[9] = {};
-    bool timed_stages = true;
+    hipEvent_t ev[11] = {};  // 0-5 stages on the main stream, 7-8 adjoint, 9-10 star pass on the side stream
+    // Second stream (default priority) for the star pass, which the primary rays do not need;
+    // fork / join order it against the main stream (also inside a graph capture).
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool timed_stages = true, stars_overlapped = false;
     specrel_timings timings{};
     specrel_stats stats_host{};
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
@@ -67,6 +71,7 @@ struct specrel_ctx {
     int opt_prefetch = 0;
     int opt_shadow_trace = 1;
     int opt_trace_waves = 4;
+    int opt_star_overlap = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
     hipGraphExec_t graph = nullptr;
@@ -567,15 +572,30 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     k_reset<<<1, 64, 0, s>>>(counters, 16);
     check(hipGetLastError(), "k_reset launch");
     mark(0);
+    // The star pass runs beside the primary rays on the side stream when overlapped; the shading
+    // pass waits for both.
+    const bool overlap = pl.stars && ctx->opt_star_overlap;
+    ctx->stars_overlapped = overlap;
     if (pl.stars) {
-        k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, s>>>(
+        hipStream_t ss = s;
+        if (overlap) {
+            ss = ctx->side;
+            check(hipEventRecord(ctx->fork, s), "fork");
+            check(hipStreamWaitEvent(ss, ctx->fork, 0), "fork wait");
+            if (events) check(hipEventRecord(ctx->ev[9], ss), "event");
+        }
+        k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
             (const float4 *)sc.stars.p, sc.num_stars, V, (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
         check(hipGetLastError(), "k_star_prep launch");
         if (ctx->opt_cull) {
-            k_star_bin<<<ctx->total_supers, 256, 0, s>>>(V, (const StarDir *)ctx->dirs.p, sc.num_stars,
-                                                          (uint32_t *)ctx->super_lists.p,
-                                                          (uint32_t *)ctx->super_counts.p);
+            k_star_bin<<<ctx->total_supers, 256, 0, ss>>>(V, (const StarDir *)ctx->dirs.p, sc.num_stars,
+                                                           (uint32_t *)ctx->super_lists.p,
+                                                           (uint32_t *)ctx->super_counts.p);
             check(hipGetLastError(), "k_star_bin launch");
+        }
+        if (overlap) {
+            if (events) check(hipEventRecord(ctx->ev[10], ss), "event");
+            check(hipEventRecord(ctx->join, ss), "join");
         }
     }
     mark(1);
@@ -593,6 +613,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(2);
+    if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
     ra.S = sc.S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;
@@ -672,6 +693,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
                                      (uint64_t)ctx->opt_prefetch,       (uint64_t)ctx->opt_trace_waves,
+                                     (uint64_t)ctx->opt_star_overlap,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -715,7 +737,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
 void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
     if (ctx->timed_stages) {
-        t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
+        t.star_prep_ms = ctx->stars_overlapped ? ev_ms(ctx->ev[9], ctx->ev[10]) : ev_ms(ctx->ev[0], ctx->ev[1]);
         t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
         t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
         t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
@@ -899,6 +921,9 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         check(hipSetDevice(device_ordinal), "hipSetDevice");
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto &e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
+        check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming), "hipEventCreate");
         std::vector<uint32_t> words(SPECREL_LUT_SIZE + 14 * 7);
         sr::build_tables(words.data());
         std::vector<uint8_t> bytes(words.size());
@@ -938,6 +963,9 @@ void specrel_destroy(specrel_ctx *ctx) {
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side), (void)hipStreamDestroy(ctx->side);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
     delete ctx;
 }
 
@@ -959,6 +987,7 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
     else if (option == SPECREL_OPT_PREFETCH) ctx->opt_prefetch = value != 0;
     else if (option == SPECREL_OPT_SHADOW_IN_TRACE) ctx->opt_shadow_trace = value != 0;
+    else if (option == SPECREL_OPT_STAR_OVERLAP) ctx->opt_star_overlap = value != 0;
     else if (option == SPECREL_OPT_TRACE_WAVES) {
         if (value != 1 && value != 2 && value != 4) {
             ctx->err = "trace waves must be 1, 2 or 4";

@@ -105,6 +105,19 @@ def test_batch_launch_variants(gpu_ctx, twin, stack, persistent, queue):
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)
 
 
+def test_star_pass_on_a_second_stream_gives_the_same_views(gpu_ctx, twin):
+    from specrel_amd.renderer import OPT_STAR_OVERLAP
+    gpu_ctx.upload(0, twin)
+    frames = twin_frames(twin, 23)
+    ref = singles(gpu_ctx, frames)  # star pass before the primary rays (default)
+    try:
+        gpu_ctx.set_option(OPT_STAR_OVERLAP, 1)
+        assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "star pass on a second stream")
+        assert_views_equal(singles(gpu_ctx, frames), ref, "single views, star pass on a second stream")
+    finally:
+        gpu_ctx.set_option(OPT_STAR_OVERLAP, 0)
+
+
 def test_batch_stats_are_the_sum_of_single_views(gpu_ctx, twin):
     gpu_ctx.upload(0, twin)
     frames = twin_frames(twin, 31)
