@@ -201,6 +201,47 @@ def bench_adjoint(ctx, steps, slot=1):
     }
 
 
+def bench_extrude(ctx, p, steps):
+    """Worldtube extrusion of the large scene's tubes on the GPU (specrel_extrude_tubes) against
+    the host extrusion of the same descriptors on all cores; records compared byte for byte."""
+    import numpy as np
+    from specrel_amd import scene as S
+
+    d = S.DeferredScene("large", p)
+    tubes = d.tubes()
+    _, _, _, counts = ctx.extrude_tubes(tubes, download=False)  # warm-up (allocations)
+    ms = []
+    for _ in range(steps):
+        ctx.extrude_tubes(tubes, download=False)
+        ms.append(ctx.timings()["extrude_ms"])
+    ms.sort()
+    kernel_ms = ms[len(ms) // 2]
+    t0 = time.perf_counter()
+    got = ctx.extrude_tubes(tubes)
+    e2e_s = time.perf_counter() - t0
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    ref = S.extrude_tubes_host(tubes)
+    host_s = time.perf_counter() - t0
+    same = all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(got[:3], ref[:3])) and \
+        np.array_equal(got[3], ref[3])
+    n = int(counts.sum())
+    out_bytes = n * (80 + 192 + 32)
+    return {"tubes": len(tubes), "tets": n, "kernel_ms": round(kernel_ms, 3),
+            "Mtets_per_s": round(n / kernel_ms / 1e3, 1),
+            "achieved_GBs": round(out_bytes / (kernel_ms * 1e-3) / 1e9, 1),
+            "algorithmic_bytes": out_bytes,
+            "e2e_ms_with_download": round(e2e_s * 1e3, 1),
+            "host_ms": round(host_s * 1e3, 1), "host_cores": cores,
+            "speedup_kernel_vs_host": round(host_s * 1e3 / kernel_ms, 1),
+            "bit_identical_to_host": bool(same),
+            "workload": "the large scene's worldtubes (icospheres, 320 triangles, inertial and accelerated), "
+                        "every prism of every slice; outputs GpuGeom + GpuAttr + padded bounds"}
+
+
 def bench_batch_adjoint(ctx, steps, slot=2, n_views=32):
     """Batched forward + adjoint: 32 observers of uneven resolution (128x80 .. 1024x640), each with
     its own parameters and gradient, over the accelerated-bodies scene (64 Born-rigid bodies with
@@ -271,6 +312,7 @@ def main():
     ap.add_argument("--views", type=int, default=64, help="observers of the batched leg (0: skip it)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--adjoint-steps", type=int, default=10, help="forward+adjoint repetitions (0: skip the leg)")
+    ap.add_argument("--extrude-steps", type=int, default=5, help="GPU extrusion repetitions (0: skip the leg)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stack", type=int, default=0,
                     help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
@@ -390,6 +432,11 @@ def main():
         batch_adjoint = bench_batch_adjoint(ctx, args.adjoint_steps)
         log(f"batched adjoint: {batch_adjoint}")
 
+    extrude = None
+    if args.extrude_steps > 0:
+        extrude = bench_extrude(ctx, p, args.extrude_steps)
+        log(f"extrusion: {extrude}")
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline: oracle on a row sample")
@@ -423,6 +470,7 @@ def main():
         "batched": batched,
         "adjoint": adjoint,
         "batched_adjoint": batch_adjoint,
+        "extrude": extrude,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

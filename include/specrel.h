@@ -46,7 +46,7 @@ typedef struct specrel_timings {
     float composite_ms;   /* composite + tone map + pack */
     float adjoint_ms;     /* the last specrel_adjoint's kernels */
     uint32_t shadow_path; /* where the last render traced its shadow queries: SPECREL_SHADOWS_* */
-    float reserved;
+    float extrude_ms;     /* the last specrel_extrude_tubes' kernels */
 } specrel_timings;
 
 enum {
@@ -116,6 +116,18 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
  * web.rs:431-479 hands WebGPU). The file is mapped read-only and copied to the device; verify != 0
  * checks the section checksums first. SPECREL_EINVAL names what is wrong with a bad file. */
 int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *path, int verify);
+/* extend_worldtube_simplices (mesh4d.rs:721-854) for n tubes on the device: one thread per prism
+ * (a triangle swept over one worldline step), its three tetrahedra built by Simplex4D::new
+ * (mesh4d.rs:312-362) and packed as GpuGeom / GpuAttr (gpu.rs:68-92, 358-381), degenerate ones
+ * dropped; the records of tube i follow those of tube i-1, bit-identical to the scene library's
+ * specrel_extrude_tubes_host. *n_out receives the record count and counts[i] (may be NULL) the
+ * records of tube i. geoms / attrs / boxes (padded bounds: min t,x,y,z, max t,x,y,z; any may be
+ * NULL) are host arrays of cap records; with all three NULL the records stay on the device and only
+ * the counts come back. SPECREL_EINVAL for a tube crossing its Rindler horizon, a bad index or too
+ * small a cap. specrel_timings.extrude_ms holds the kernels' time. */
+int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t n_tubes, specrel_geom *geoms,
+                          specrel_attr *attrs, float *boxes, uint64_t cap, uint64_t *n_out, uint64_t *counts);
+
 /* A range [first, first + count) of node or primitive (leaf-order) indices. */
 typedef struct specrel_range {
     uint32_t first, count;

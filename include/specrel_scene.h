@@ -45,15 +45,6 @@ typedef struct specrel_object {
     uint32_t material, pad;
 } specrel_object;
 
-typedef struct specrel_wl_sample {  /* WorldlineSample (mesh4d.rs:615-630) */
-    float center[4];
-    float proper_time;
-    float phi[3];
-    float proper_accel_local[3];
-    float pad;
-    float e[4][4];
-} specrel_wl_sample;
-
 typedef struct specrel_camera {    /* Camera4D (renderer.rs:45-54) */
     float position[4];
     float e[4][4];
@@ -150,6 +141,24 @@ int specrel_sync_proper_time(specrel_wl_sample *s, uint32_t n, float t_sync);
 int specrel_extrude(uint32_t mesh_kind, const float *mesh_params, const specrel_wl_sample *s,
                     uint32_t n_samples, uint32_t material, specrel_geom *geoms, specrel_attr *attrs,
                     uint32_t cap, uint32_t *n_out);
+/* extend_worldtube_simplices (mesh4d.rs:721-854) for n tubes on the host, the records of tube i
+ * following those of tube i-1: the CPU counterpart of specrel_extrude_tubes (include/specrel.h).
+ * boxes (may be NULL) receive each record's padded bounds (min t,x,y,z, max t,x,y,z); counts
+ * (may be NULL) the records per tube. Fails with the reference's message when a Born-rigid tube
+ * crosses its Rindler horizon, or when more than cap records would be written. */
+int specrel_extrude_tubes_host(const specrel_tube *tubes, uint32_t n, specrel_geom *geoms, specrel_attr *attrs,
+                               float *boxes, uint64_t cap, uint64_t *n_out, uint64_t *counts);
+
+/* Deferred extrusion: the named scene's generator runs with every tube recorded instead of
+ * extruded; specrel_builder_pending_tubes hands out their descriptors (valid while the builder
+ * lives), and the records come back through specrel_builder_set_tube_records (for example from
+ * specrel_extrude_tubes on the GPU) or specrel_builder_extrude_pending (host), in tube order.
+ * specrel_builder_finish then builds the BVH as usual; the result equals specrel_scene_generate's. */
+int specrel_scene_generate_deferred(const char *name, const specrel_gen_params *p, specrel_builder **out);
+uint32_t specrel_builder_pending_tubes(const specrel_builder *b, specrel_tube *out, uint32_t cap);
+int specrel_builder_set_tube_records(specrel_builder *b, const specrel_geom *geoms, const specrel_attr *attrs,
+                                     const float *boxes, uint64_t n, const uint64_t *counts);
+int specrel_builder_extrude_pending(specrel_builder *b);
 
 /* Tetrad operations (minkowski.rs:424-596); e is e[0..4] as (t,x,y,z). */
 void specrel_tetrad_rest(float e[16]);

@@ -83,6 +83,27 @@ typedef struct specrel_material {  /* GpuMaterial, gpu.rs:96-100 */
 } specrel_material;
 
 /* The arrays `specrel_upload_scene` copies to the device (PackedSceneGpuData, gpu.rs:232-244). */
+typedef struct specrel_wl_sample {  /* WorldlineSample (mesh4d.rs:615-630) */
+    float center[4];
+    float proper_time;
+    float phi[3];
+    float proper_accel_local[3];
+    float pad;
+    float e[4][4];
+} specrel_wl_sample;
+
+/* One worldtube to extrude: a triangle mesh in its rest frame swept along a sampled worldline
+ * (extend_worldtube_simplices, mesh4d.rs:721-854; the mesh as Mesh3D, mesh4d.rs:437-451). */
+typedef struct specrel_tube {
+    const float *vertices;          /* 3 per vertex (x, y, z), rest frame */
+    const uint32_t *tri_index;      /* 3 per triangle */
+    const float *tri_uv;            /* 6 per triangle: (u, v) of each corner */
+    const float *tri_normal;        /* 3 per triangle: outward rest-frame normal */
+    const specrel_wl_sample *samples;
+    const float *decided_at;        /* one per sample, or NULL: DECIDED_AT_STATIC (-1e9) */
+    uint32_t n_vertices, n_triangles, n_samples, material;
+} specrel_tube;
+
 typedef struct specrel_scene_arrays {
     const specrel_node *nodes;
     uint32_t num_nodes;

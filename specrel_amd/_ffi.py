@@ -91,6 +91,15 @@ class GenParams(C.Structure):
                 ("first_object", C.c_uint32), ("pad", C.c_uint32)]
 
 
+class Tube(C.Structure):
+    """specrel_tube: a rest-frame triangle mesh swept along a sampled worldline."""
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("tri_index", C.POINTER(C.c_uint32)),
+                ("tri_uv", C.POINTER(C.c_float)), ("tri_normal", C.POINTER(C.c_float)),
+                ("samples", C.POINTER(WlSample)), ("decided_at", C.POINTER(C.c_float)),
+                ("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32), ("n_samples", C.c_uint32),
+                ("material", C.c_uint32)]
+
+
 class FileInfo(C.Structure):
     _fields_ = [("version", C.c_uint32), ("flags", C.c_uint32), ("bvh_layout", C.c_uint64),
                 ("bvh_edit", C.c_uint64), ("file_bytes", C.c_uint64), ("num_objects", C.c_uint32),
@@ -111,6 +120,7 @@ class FileSection(C.Structure):
                 ("checksum", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+assert C.sizeof(Tube) == 64 and C.sizeof(WlSample) == 112
 assert C.sizeof(FileHeader) == 128 and C.sizeof(FileSection) == 32 and C.sizeof(FileInfo) == 40
 assert C.sizeof(Node) == 48 and C.sizeof(Geom) == 80 and C.sizeof(Attr) == 192
 assert C.sizeof(Material) == 48 and C.sizeof(Frame) == 800
@@ -162,6 +172,13 @@ def _scene_protos(lib):
                                           C.POINTER(Frame)]),
         "specrel_common_materials": (_u32, [C.POINTER(Material), _u32]),
         "specrel_build_tables": (None, [C.POINTER(_u32)]),
+        "specrel_extrude_tubes_host": (C.c_int, [C.POINTER(Tube), _u32, C.POINTER(Geom), C.POINTER(Attr), _FP,
+                                                 C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+        "specrel_scene_generate_deferred": (C.c_int, [C.c_char_p, C.POINTER(GenParams), C.POINTER(_P)]),
+        "specrel_builder_pending_tubes": (_u32, [_P, C.POINTER(Tube), _u32]),
+        "specrel_builder_set_tube_records": (C.c_int, [_P, C.POINTER(Geom), C.POINTER(Attr), _FP, C.c_uint64,
+                                                       C.POINTER(C.c_uint64)]),
+        "specrel_builder_extrude_pending": (C.c_int, [_P]),
         "specrel_scene_file_write": (C.c_int, [C.c_char_p, C.POINTER(SceneArrays), C.POINTER(Object), _u32,
                                                C.c_uint64, C.c_uint64]),
         "specrel_scene_file_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P), C.POINTER(SceneArrays),

@@ -1,0 +1,212 @@
+// Worldtube extrusion, shared by the scene producer (host C++) and k_extrude (HIP), so both
+// emit the same records bit for bit: TubeTriangle::new (mesh4d.rs:662-718), the prism step of
+// extend_worldtube_simplices (mesh4d.rs:721-854) and Simplex4D::new (mesh4d.rs:312-362) with the
+// GpuGeom / GpuAttr packing (gpu.rs:68-92, 358-381).
+#pragma once
+#include <stdint.h>
+
+#include "minkowski.h"
+#include "specrel_types.h"
+
+namespace sr {
+
+// WorldlineSample (mesh4d.rs:615-630).
+struct Sample {
+    V4 center;
+    float proper_time;
+    V3 phi;
+    V3 accel_local;
+    Tetrad tetrad;
+};
+
+// TubeTriangle: the corners sorted by vertex index (stable), their rest positions, UVs and
+// standard barycentric corners, and the UV->position gradients tu, tv.
+struct TubeTri {
+    uint32_t idx[3];
+    V3 p[3];
+    float uv[3][2];
+    V3 bary[3];
+    V3 tu, tv, normal;
+};
+
+struct Box4 {
+    V4 mn, mx;
+};
+
+const float DECIDED_AT_STATIC = -1.0e9f;
+
+// TubeTriangle::new from a triangle's vertex indices, rest positions, corner UVs and normal.
+SR_HD TubeTri tube_tri(const uint32_t idx[3], const V3 pos[3], const float uv[3][2], V3 normal) {
+    const V3 std_bary[3] = {v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, 1)};
+    // slice::sort_by_key on the vertex index: a stable sort of three corners.
+    int perm[3] = {0, 1, 2};
+    for (int i = 1; i < 3; ++i)
+        for (int j = i; j > 0 && idx[perm[j]] < idx[perm[j - 1]]; --j) {
+            int t = perm[j];
+            perm[j] = perm[j - 1];
+            perm[j - 1] = t;
+        }
+    TubeTri t;
+    for (int k = 0; k < 3; ++k) {
+        t.idx[k] = idx[perm[k]];
+        t.p[k] = pos[perm[k]];
+        t.uv[k][0] = uv[perm[k]][0];
+        t.uv[k][1] = uv[perm[k]][1];
+        t.bary[k] = std_bary[perm[k]];
+    }
+    V3 dp1 = t.p[1] - t.p[0], dp2 = t.p[2] - t.p[0];
+    float du1 = t.uv[1][0] - t.uv[0][0], dv1 = t.uv[1][1] - t.uv[0][1];
+    float du2 = t.uv[2][0] - t.uv[0][0], dv2 = t.uv[2][1] - t.uv[0][1];
+    float uv_det = du1 * dv2 - dv1 * du2;
+    if (fabsf(uv_det) > 1e-10f) {
+        float inv_det = 1.0f / uv_det;
+        t.tu = (dp1 * dv2 - dp2 * dv1) * inv_det;
+        t.tv = (dp2 * du1 - dp1 * du2) * inv_det;
+    } else {
+        t.tu = v3(0, 0, 0);
+        t.tv = v3(0, 0, 0);
+    }
+    t.normal = normal;
+    return t;
+}
+
+SR_HD void put4(float *dst, V4 v) {
+    dst[0] = v.t;
+    dst[1] = v.x;
+    dst[2] = v.y;
+    dst[3] = v.z;
+}
+
+// Simplex4D::new + packing. Returns false for a degenerate 3-simplex (no record).
+SR_HD bool make_simplex(const V4 verts[4], const float uv[4][2], const V3 tri_bary[4], const float tau[4],
+                        const float grad[4][2], float uvws0, float uvws1, V4 normal, V4 four_vel,
+                        const float decided[4], uint32_t material, specrel_geom &g, specrel_attr &a, Box4 &box) {
+    V4 v0 = verts[0];
+    V4 e1 = verts[1] - v0, e2 = verts[2] - v0, e3 = verts[3] - v0;
+    V4 n = cross4(e1, e2, e3);
+    float n_len_sq = edot(n, n);
+    if (n_len_sq < 1e-16f) return false;
+    float inv_n_len = 1.0f / sqrtf(n_len_sq);
+    V4 unit_n = n * inv_n_len;
+    V4 raw_b1 = cross4(e2, e3, unit_n);
+    float d1 = edot(raw_b1, e1);
+    V4 raw_b2 = cross4(e3, e1, unit_n);
+    float d2 = edot(raw_b2, e2);
+    V4 raw_b3 = cross4(e1, e2, unit_n);
+    float d3 = edot(raw_b3, e3);
+    if (fabsf(d1) < 1e-12f || fabsf(d2) < 1e-12f || fabsf(d3) < 1e-12f) return false;
+    const float inf = __builtin_inff();
+    box.mn = splat4(inf);
+    box.mx = splat4(-inf);
+    for (int i = 0; i < 4; ++i) {
+        box.mn = vmin(box.mn, verts[i]);
+        box.mx = vmax(box.mx, verts[i]);
+    }
+    V4 eps = splat4(1e-4f);
+    box.mn -= eps;
+    box.mx += eps;
+    put4(g.v0, v0);
+    put4(g.n, unit_n);
+    put4(g.b1, raw_b1 * (1.0f / d1));
+    put4(g.b2, raw_b2 * (1.0f / d2));
+    put4(g.b3, raw_b3 * (1.0f / d3));
+    for (int i = 0; i < 4; ++i) {
+        a.uv_u[i] = uv[i][0];
+        a.uv_v[i] = uv[i][1];
+        a.tb_x[i] = tri_bary[i].x;
+        a.tb_y[i] = tri_bary[i].y;
+        a.tb_z[i] = tri_bary[i].z;
+        a.tau[i] = tau[i];
+        a.grad_u[i] = grad[i][0];
+        a.grad_v[i] = grad[i][1];
+        a.decided_at[i] = decided[i];
+    }
+    put4(a.normal, normal);
+    put4(a.four_vel, four_vel);
+    a.misc[0] = uvws0;
+    a.misc[1] = uvws1;
+    __builtin_memcpy(&a.misc[2], &material, 4);
+    a.misc[3] = 0.0f;
+    return true;
+}
+
+// The event of a rest-frame body point at a worldline sample (mesh4d.rs:749).
+SR_HD V4 embed(const Sample &s, V3 p) { return s.center + local_spatial_to_world(s.tetrad, p); }
+
+// The three tetrahedra of one prism (triangle `tri` swept from sample sc to sample sn,
+// mesh4d.rs:755-849), in order: sink(ok, geom, attr, box) once per tetrahedron, ok false for a
+// degenerate one (geom, attr and box then unset).
+template <class Sink>
+SR_HD void prism(const TubeTri &tri, const Sample &sc, const Sample &sn, float dec0, float dec1, uint32_t material,
+                 Sink &&sink) {
+    V4 u_raw = sc.tetrad.e[0] + sn.tetrad.e[0];
+    float u_norm = sqrtf(fmax_(-mdot(u_raw, u_raw), 1e-12f));
+    V4 u_avg = u_raw * (1.0f / u_norm);
+    V4 a0 = embed(sc, tri.p[0]), b0 = embed(sc, tri.p[1]), c0 = embed(sc, tri.p[2]);
+    V4 a1 = embed(sn, tri.p[0]), b1 = embed(sn, tri.p[1]), c1 = embed(sn, tri.p[2]);
+    V3 pa = tri.p[0], pb = tri.p[1], pc = tri.p[2];
+    float tau_a0 = sc.proper_time + dot(pa, sc.phi);
+    float tau_b0 = sc.proper_time + dot(pb, sc.phi);
+    float tau_c0 = sc.proper_time + dot(pc, sc.phi);
+    float tau_a1 = sn.proper_time + dot(pa, sn.phi);
+    float tau_b1 = sn.proper_time + dot(pb, sn.phi);
+    float tau_c1 = sn.proper_time + dot(pc, sn.phi);
+    float uvws0 = length(tri.tu), uvws1 = length(tri.tv);
+    float g0[2] = {dot(tri.tu, sc.phi), dot(tri.tv, sc.phi)};
+    float g1[2] = {dot(tri.tu, sn.phi), dot(tri.tv, sn.phi)};
+    V3 ba = tri.bary[0], bb = tri.bary[1], bc = tri.bary[2];
+    V4 n4c = local_spatial_to_world(sc.tetrad, tri.normal);
+    V4 n4n = local_spatial_to_world(sn.tetrad, tri.normal);
+    V4 n4_raw = n4c + n4n;
+    float n4_len = sqrtf(fmax_(mdot(n4_raw, n4_raw), 1e-12f));
+    V4 normal = n4_raw * (1.0f / n4_len);
+    const float *uva = tri.uv[0], *uvb = tri.uv[1], *uvc = tri.uv[2];
+    specrel_geom g;
+    specrel_attr a;
+    Box4 box;
+    // Prism (A0,B0,C0)->(A1,B1,C1) split into 3 tetrahedra (mesh4d.rs:801-830).
+    {
+        V4 vv[4] = {a0, b0, c0, c1};
+        float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}, {uvc[0], uvc[1]}};
+        V3 tb[4] = {ba, bb, bc, bc};
+        float ta[4] = {tau_a0, tau_b0, tau_c0, tau_c1};
+        float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}};
+        float dd[4] = {dec0, dec0, dec0, dec1};
+        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
+        sink(ok, g, a, box);
+    }
+    {
+        V4 vv[4] = {a0, b0, b1, c1};
+        float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
+        V3 tb[4] = {ba, bb, bb, bc};
+        float ta[4] = {tau_a0, tau_b0, tau_b1, tau_c1};
+        float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
+        float dd[4] = {dec0, dec0, dec1, dec1};
+        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
+        sink(ok, g, a, box);
+    }
+    {
+        V4 vv[4] = {a0, a1, b1, c1};
+        float uv[4][2] = {{uva[0], uva[1]}, {uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
+        V3 tb[4] = {ba, ba, bb, bc};
+        float ta[4] = {tau_a0, tau_a1, tau_b1, tau_c1};
+        float gg[4][2] = {{g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
+        float dd[4] = {dec0, dec1, dec1, dec1};
+        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
+        sink(ok, g, a, box);
+    }
+}
+
+// Sample from the C ABI layout (specrel_wl_sample).
+SR_HD Sample sample_from(const float center[4], float proper_time, const float phi[3], const float accel[3],
+                         const float e[4][4]) {
+    Sample s;
+    s.center = v4(center[0], center[1], center[2], center[3]);
+    s.proper_time = proper_time;
+    s.phi = v3(phi[0], phi[1], phi[2]);
+    s.accel_local = v3(accel[0], accel[1], accel[2]);
+    for (int k = 0; k < 4; ++k) s.tetrad.e[k] = v4(e[k][0], e[k][1], e[k][2], e[k][3]);
+    return s;
+}
+
+}  // namespace sr

@@ -10,6 +10,7 @@
 
 #include "../core/scene_file.hpp"
 #include "../core/tables.h"
+#include "extrude_dev.hpp"
 #include "kernels.hpp"
 #include "specrel.h"
 
@@ -52,6 +53,8 @@ struct specrel_ctx {
     uint32_t stars_for = 0;  // num_stars the star buffers are sized for (per view)
     DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
     DevBuf counters, mask, squeue;
+    // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
+    DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
     DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
         adj_result;
     std::vector<specrel_frame> last_frames;
@@ -956,6 +959,9 @@ void specrel_destroy(specrel_ctx *ctx) {
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
     dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
     dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
+    dfree(ctx->ex_tubes), dfree(ctx->ex_tris), dfree(ctx->ex_samples), dfree(ctx->ex_decided);
+    dfree(ctx->ex_counts), dfree(ctx->ex_offsets), dfree(ctx->ex_first), dfree(ctx->ex_geoms);
+    dfree(ctx->ex_attrs), dfree(ctx->ex_boxes);
     dfree(ctx->adj_target), dfree(ctx->adj_mask_in), dfree(ctx->adj_mask_out), dfree(ctx->adj_obj);
     dfree(ctx->adj_pix), dfree(ctx->adj_sig), dfree(ctx->adj_objid), dfree(ctx->adj_partials);
     dfree(ctx->adj_counts), dfree(ctx->adj_result);
@@ -1042,6 +1048,138 @@ int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *
         slot.used = false;
         ctx->generation++;
         build_device_scene(ctx, slot, &a);  // copies out of the mapping before it is unmapped
+        return SPECREL_OK;
+    });
+}
+
+int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t n_tubes, specrel_geom *geoms,
+                          specrel_attr *attrs, float *boxes, uint64_t cap, uint64_t *n_out, uint64_t *counts) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if ((n_tubes && !tubes) || !n_out) throw Fail{SPECREL_EINVAL, "tubes or n_out is NULL"};
+        // Host tables: the tubes' prepared triangles (TubeTriangle::new), samples, decided_at.
+        std::vector<dev::ExTube> desc(n_tubes);
+        std::vector<sr::TubeTri> tris;
+        std::vector<sr::Sample> samples;
+        std::vector<float> decided;
+        uint64_t n_prisms = 0;
+        for (uint32_t i = 0; i < n_tubes; ++i) {
+            const specrel_tube &t = tubes[i];
+            const std::string who = "tube " + std::to_string(i) + ": ";
+            dev::ExTube &d = desc[i];
+            d.prism_first = n_prisms;
+            d.tri_first = (uint32_t)tris.size();
+            d.n_tris = t.n_triangles;
+            d.sample_first = (uint32_t)samples.size();
+            d.n_samples = t.n_samples;
+            d.decided_first = ~0u;
+            d.material = t.material;
+            if (t.n_samples < 2 || t.n_triangles == 0) {  // nothing to extrude (mesh4d.rs:728)
+                d.n_tris = std::max(1u, t.n_triangles);
+                continue;
+            }
+            if (!t.vertices || !t.tri_index || !t.tri_uv || !t.tri_normal || !t.samples)
+                throw Fail{SPECREL_EINVAL, who + "arrays are NULL"};
+            std::vector<sr::V3> verts(t.n_vertices);
+            for (uint32_t k = 0; k < t.n_vertices; ++k)
+                verts[k] = sr::v3(t.vertices[3 * k], t.vertices[3 * k + 1], t.vertices[3 * k + 2]);
+            for (uint32_t k = 0; k < t.n_samples; ++k) {
+                const specrel_wl_sample &w = t.samples[k];
+                sr::Sample sm = sr::sample_from(w.center, w.proper_time, w.phi, w.proper_accel_local, w.e);
+                for (const sr::V3 &p : verts)  // extend_worldtube_simplices' horizon check (mesh4d.rs:738-747)
+                    if (!(1.0f + sr::dot(sm.accel_local, p) > 0.0f))
+                        throw Fail{SPECREL_EINVAL, who + "Born-rigid worldtube crosses its Rindler horizon at sample " +
+                                                       std::to_string(k)};
+                samples.push_back(sm);
+            }
+            for (uint32_t k = 0; k < t.n_triangles; ++k) {
+                uint32_t idx[3];
+                sr::V3 pos[3];
+                float uv[3][2];
+                for (int c = 0; c < 3; ++c) {
+                    idx[c] = t.tri_index[3 * k + c];
+                    if (idx[c] >= t.n_vertices) throw Fail{SPECREL_EINVAL, who + "triangle vertex index out of range"};
+                    pos[c] = verts[idx[c]];
+                    uv[c][0] = t.tri_uv[6 * k + 2 * c];
+                    uv[c][1] = t.tri_uv[6 * k + 2 * c + 1];
+                }
+                tris.push_back(sr::tube_tri(
+                    idx, pos, uv, sr::v3(t.tri_normal[3 * k], t.tri_normal[3 * k + 1], t.tri_normal[3 * k + 2])));
+            }
+            if (t.decided_at) {
+                d.decided_first = (uint32_t)decided.size();
+                decided.insert(decided.end(), t.decided_at, t.decided_at + t.n_samples);
+            }
+            n_prisms += (uint64_t)(t.n_samples - 1) * t.n_triangles;
+        }
+        const uint64_t n_blocks = (n_prisms + 255) / 256;
+        if (n_blocks > 0x7FFFFFFFull) throw Fail{SPECREL_EINVAL, "too many prisms for one launch"};
+        hipStream_t s = ctx->stream;
+        upload(ctx->ex_tubes, desc.data(), desc.size() * sizeof(dev::ExTube), s);
+        upload(ctx->ex_tris, tris.data(), tris.size() * sizeof(sr::TubeTri), s);
+        upload(ctx->ex_samples, samples.data(), samples.size() * sizeof(sr::Sample), s);
+        upload(ctx->ex_decided, decided.data(), decided.size() * sizeof(float), s);
+        dalloc(ctx->ex_counts, std::max<uint64_t>(1, n_blocks) * 4);
+        dalloc(ctx->ex_offsets, (n_blocks + 1) * 8);
+        dalloc(ctx->ex_first, std::max<uint32_t>(1, n_tubes) * 8);
+        dev::ExArgs a;
+        a.tubes = (const dev::ExTube *)ctx->ex_tubes.p;
+        a.n_tubes = n_tubes;
+        a.n_prisms = n_prisms;
+        a.tris = (const sr::TubeTri *)ctx->ex_tris.p;
+        a.samples = (const sr::Sample *)ctx->ex_samples.p;
+        a.decided = (const float *)ctx->ex_decided.p;
+        uint64_t total = 0;
+        float ms = 0.0f;
+        if (n_prisms) {
+            check(hipEventRecord(ctx->ev[7], s), "event");
+            dev::k_extrude_count<<<dim3((unsigned)n_blocks), 256, 0, s>>>(a, (uint32_t *)ctx->ex_counts.p);
+            check(hipGetLastError(), "k_extrude_count launch");
+            dev::k_extrude_scan<<<1, 1024, 0, s>>>((uint32_t *)ctx->ex_counts.p, (uint32_t)n_blocks,
+                                                   (uint64_t *)ctx->ex_offsets.p);
+            check(hipGetLastError(), "k_extrude_scan launch");
+            check(hipEventRecord(ctx->ev[8], s), "event");
+            check(hipMemcpyAsync(&total, (const uint64_t *)ctx->ex_offsets.p + n_blocks, 8, hipMemcpyDeviceToHost, s),
+                  "total readback");
+            check(hipStreamSynchronize(s), "extrude count");
+            ms = ev_ms(ctx->ev[7], ctx->ev[8]);
+            dalloc(ctx->ex_geoms, total * sizeof(specrel_geom));
+            dalloc(ctx->ex_attrs, total * sizeof(specrel_attr));
+            dalloc(ctx->ex_boxes, total * 32);
+            check(hipEventRecord(ctx->ev[7], s), "event");
+            dev::k_extrude_emit<<<dim3((unsigned)n_blocks), 256, 0, s>>>(
+                a, (const uint64_t *)ctx->ex_offsets.p, (specrel_geom *)ctx->ex_geoms.p, (specrel_attr *)ctx->ex_attrs.p,
+                (float4 *)ctx->ex_boxes.p, (uint64_t *)ctx->ex_first.p);
+            check(hipGetLastError(), "k_extrude_emit launch");
+            check(hipEventRecord(ctx->ev[8], s), "event");
+            check(hipStreamSynchronize(s), "extrude emit");
+            ms += ev_ms(ctx->ev[7], ctx->ev[8]);
+        }
+        ctx->timings.extrude_ms = ms;
+        *n_out = total;
+        std::vector<uint64_t> first(n_tubes, 0);
+        if (n_tubes && n_prisms)
+            check(hipMemcpy(first.data(), ctx->ex_first.p, n_tubes * 8, hipMemcpyDeviceToHost), "tube offsets");
+        if (counts) {
+            // Records of tube i: from its first record to the next tube with prisms (or the end).
+            uint64_t next = total;
+            for (uint32_t i = n_tubes; i-- > 0;) {
+                const bool has = tubes[i].n_samples >= 2 && tubes[i].n_triangles > 0;
+                counts[i] = has ? next - first[i] : 0;
+                if (has) next = first[i];
+            }
+        }
+        if (geoms || attrs || boxes) {
+            if (total > cap)
+                throw Fail{SPECREL_EINVAL, "output buffers hold " + std::to_string(cap) + " records, " +
+                                               std::to_string(total) + " needed"};
+            if (geoms && total)
+                check(hipMemcpy(geoms, ctx->ex_geoms.p, total * sizeof(specrel_geom), hipMemcpyDeviceToHost), "geoms");
+            if (attrs && total)
+                check(hipMemcpy(attrs, ctx->ex_attrs.p, total * sizeof(specrel_attr), hipMemcpyDeviceToHost), "attrs");
+            if (boxes && total) check(hipMemcpy(boxes, ctx->ex_boxes.p, total * 32, hipMemcpyDeviceToHost), "boxes");
+        }
         return SPECREL_OK;
     });
 }

@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "../core/extrude.h"
 #include "../core/minkowski.h"
 #include "../core/scene_file.hpp"
 #include "../core/tables.h"
@@ -221,14 +222,6 @@ Mesh make_mesh(uint32_t kind, const float *p) {
 // ----------------------------------------------------------------------------------------
 // Worldlines (mesh4d.rs:613-1087)
 // ----------------------------------------------------------------------------------------
-struct Sample {
-    V4 center;
-    float proper_time;
-    V3 phi;
-    V3 accel_local;
-    Tetrad tetrad;
-};
-
 std::vector<Sample> sample_inertial(V3 pos, V3 beta, float t_min, float t_max, size_t steps) {
     Tetrad T = tetrad_from_beta(beta);
     double inv_gamma = 1.0 / (double)T.e[0].t;
@@ -425,9 +418,7 @@ std::vector<Sample> sample_worldline(uint32_t kind, const double *d) {
 // ----------------------------------------------------------------------------------------
 // Simplices (mesh4d.rs:214-371) and worldtube extrusion (mesh4d.rs:662-854)
 // ----------------------------------------------------------------------------------------
-struct Aabb {
-    V4 mn, mx;
-};
+using Aabb = Box4;
 inline Aabb aabb_empty() {
     const float inf = __builtin_inff();
     return Aabb{splat4(inf), splat4(-inf)};
@@ -446,190 +437,61 @@ struct Prims {
     }
 };
 
-inline void put4(float *dst, V4 v) {
-    dst[0] = v.t;
-    dst[1] = v.x;
-    dst[2] = v.y;
-    dst[3] = v.z;
-}
-
-// Simplex4D::new + GpuGeom/GpuAttr packing. Returns false for a degenerate 3-simplex.
-bool make_simplex(const V4 verts[4], const float uv[4][2], const V3 tri_bary[4], const float tau[4],
-                  const float grad[4][2], float uvws0, float uvws1, V4 normal, V4 four_vel,
-                  const float decided[4], uint32_t material, Prims &out) {
-    V4 v0 = verts[0];
-    V4 e1 = verts[1] - v0, e2 = verts[2] - v0, e3 = verts[3] - v0;
-    V4 n = cross4(e1, e2, e3);
-    float n_len_sq = edot(n, n);
-    if (n_len_sq < 1e-16f) return false;
-    float inv_n_len = 1.0f / sqrtf(n_len_sq);
-    V4 unit_n = n * inv_n_len;
-    V4 raw_b1 = cross4(e2, e3, unit_n);
-    float d1 = edot(raw_b1, e1);
-    V4 raw_b2 = cross4(e3, e1, unit_n);
-    float d2 = edot(raw_b2, e2);
-    V4 raw_b3 = cross4(e1, e2, unit_n);
-    float d3 = edot(raw_b3, e3);
-    if (fabsf(d1) < 1e-12f || fabsf(d2) < 1e-12f || fabsf(d3) < 1e-12f) return false;
-    Aabb box = aabb_empty();
-    for (int i = 0; i < 4; ++i) {
-        box.mn = vmin(box.mn, verts[i]);
-        box.mx = vmax(box.mx, verts[i]);
-    }
-    V4 eps = splat4(1e-4f);
-    box.mn -= eps;
-    box.mx += eps;
-
-    specrel_geom g;
-    put4(g.v0, v0);
-    put4(g.n, unit_n);
-    put4(g.b1, raw_b1 * (1.0f / d1));
-    put4(g.b2, raw_b2 * (1.0f / d2));
-    put4(g.b3, raw_b3 * (1.0f / d3));
-    specrel_attr a;
-    for (int i = 0; i < 4; ++i) {
-        a.uv_u[i] = uv[i][0];
-        a.uv_v[i] = uv[i][1];
-        a.tb_x[i] = tri_bary[i].x;
-        a.tb_y[i] = tri_bary[i].y;
-        a.tb_z[i] = tri_bary[i].z;
-        a.tau[i] = tau[i];
-        a.grad_u[i] = grad[i][0];
-        a.grad_v[i] = grad[i][1];
-        a.decided_at[i] = decided[i];
-    }
-    put4(a.normal, normal);
-    put4(a.four_vel, four_vel);
-    a.misc[0] = uvws0;
-    a.misc[1] = uvws1;
-    std::memcpy(&a.misc[2], &material, 4);
-    a.misc[3] = 0.0f;
-    out.geoms.push_back(g);
-    out.attrs.push_back(a);
-    out.boxes.push_back(box);
-    return true;
-}
-
-const float DECIDED_AT_STATIC = -1.0e9f;
-
-struct TubeTri {
-    size_t idx[3];
-    V3 p[3];
-    float uv[3][2];
-    V3 bary[3];
-    V3 tu, tv, normal;
-};
-
-TubeTri tube_tri(const Mesh &m, const Tri &tri) {
-    const V3 std_bary[3] = {v3(1, 0, 0), v3(0, 1, 0), v3(0, 0, 1)};
-    int perm[3] = {0, 1, 2};
-    // Stable sort of the corner numbers by vertex index (slice::sort_by_key).
-    std::stable_sort(perm, perm + 3, [&](int a, int b) { return tri.idx[a] < tri.idx[b]; });
-    TubeTri t;
+TubeTri tube_tri_of(const Mesh &m, const Tri &tri) {
+    uint32_t idx[3];
+    V3 pos[3];
     for (int k = 0; k < 3; ++k) {
-        t.idx[k] = tri.idx[perm[k]];
-        t.p[k] = m.v[t.idx[k]];
-        t.uv[k][0] = tri.uv[perm[k]][0];
-        t.uv[k][1] = tri.uv[perm[k]][1];
-        t.bary[k] = std_bary[perm[k]];
+        idx[k] = (uint32_t)tri.idx[k];
+        pos[k] = m.v[tri.idx[k]];
     }
-    V3 dp1 = t.p[1] - t.p[0], dp2 = t.p[2] - t.p[0];
-    float du1 = t.uv[1][0] - t.uv[0][0], dv1 = t.uv[1][1] - t.uv[0][1];
-    float du2 = t.uv[2][0] - t.uv[0][0], dv2 = t.uv[2][1] - t.uv[0][1];
-    float uv_det = du1 * dv2 - dv1 * du2;
-    if (fabsf(uv_det) > 1e-10f) {
-        float inv_det = 1.0f / uv_det;
-        t.tu = (dp1 * dv2 - dp2 * dv1) * inv_det;
-        t.tv = (dp2 * du1 - dp1 * du2) * inv_det;
-    } else {
-        t.tu = v3(0, 0, 0);
-        t.tv = v3(0, 0, 0);
-    }
-    t.normal = tri.normal;
-    return t;
+    return tube_tri(idx, pos, tri.uv, tri.normal);
 }
 
-void extrude(const Mesh &mesh, const std::vector<Sample> &samples, const float *decided, uint32_t material,
-             Prims &out) {
-    if (samples.size() < 2) return;
-    size_t nv = mesh.v.size(), n_steps = samples.size() - 1;
-    std::vector<V4> emb;
-    emb.reserve(samples.size() * nv);
-    for (const Sample &s : samples) {
-        for (const V3 &p : mesh.v) {
+// The Rindler-horizon check of extend_worldtube_simplices (mesh4d.rs:738-747).
+void check_rindler(const std::vector<V3> &verts, const Sample *samples, size_t n) {
+    for (size_t i = 0; i < n; ++i) {
+        const Sample &s = samples[i];
+        for (const V3 &p : verts) {
             float rindler = 1.0f + dot(s.accel_local, p);
             if (!(rindler > 0.0f))
                 throw Error("Born-rigid worldtube crosses its Rindler horizon (sample t " +
                             std::to_string(s.center.t) + ", proper acceleration " +
                             std::to_string(length(s.accel_local)) + ", body extent " + std::to_string(length(p)) + ")");
-            emb.push_back(s.center + local_spatial_to_world(s.tetrad, p));
         }
     }
-    std::vector<TubeTri> tris;
-    tris.reserve(mesh.t.size());
-    for (const Tri &t : mesh.t) tris.push_back(tube_tri(mesh, t));
+}
+
+// extend_worldtube_simplices (mesh4d.rs:721-854) over prepared triangles: steps in order,
+// triangles in order, the three tetrahedra of each prism in order; degenerate ones dropped.
+void extrude_tris(const std::vector<TubeTri> &tris, const Sample *samples, size_t n_samples, const float *decided,
+                  uint32_t material, Prims &out) {
+    if (n_samples < 2) return;
+    size_t n_steps = n_samples - 1;
     size_t need = out.size() + n_steps * 3 * tris.size();
     if (need > out.geoms.capacity()) out.reserve(std::max(need, 2 * out.geoms.capacity()));
     for (size_t step = 0; step < n_steps; ++step) {
-        const V4 *vc = &emb[step * nv];
-        const V4 *vn = &emb[(step + 1) * nv];
         const Sample &sc = samples[step], &sn = samples[step + 1];
         float dec0 = decided ? decided[step] : DECIDED_AT_STATIC;
         float dec1 = decided ? decided[step + 1] : DECIDED_AT_STATIC;
-        V4 u_raw = sc.tetrad.e[0] + sn.tetrad.e[0];
-        float u_norm = sqrtf(fmax_(-mdot(u_raw, u_raw), 1e-12f));
-        V4 u_avg = u_raw * (1.0f / u_norm);
-        for (const TubeTri &tri : tris) {
-            V4 a0 = vc[tri.idx[0]], b0 = vc[tri.idx[1]], c0 = vc[tri.idx[2]];
-            V4 a1 = vn[tri.idx[0]], b1 = vn[tri.idx[1]], c1 = vn[tri.idx[2]];
-            V3 pa = tri.p[0], pb = tri.p[1], pc = tri.p[2];
-            float tau_a0 = sc.proper_time + dot(pa, sc.phi);
-            float tau_b0 = sc.proper_time + dot(pb, sc.phi);
-            float tau_c0 = sc.proper_time + dot(pc, sc.phi);
-            float tau_a1 = sn.proper_time + dot(pa, sn.phi);
-            float tau_b1 = sn.proper_time + dot(pb, sn.phi);
-            float tau_c1 = sn.proper_time + dot(pc, sn.phi);
-            float uvws0 = length(tri.tu), uvws1 = length(tri.tv);
-            float g0[2] = {dot(tri.tu, sc.phi), dot(tri.tv, sc.phi)};
-            float g1[2] = {dot(tri.tu, sn.phi), dot(tri.tv, sn.phi)};
-            V3 ba = tri.bary[0], bb = tri.bary[1], bc = tri.bary[2];
-            V4 n4c = local_spatial_to_world(sc.tetrad, tri.normal);
-            V4 n4n = local_spatial_to_world(sn.tetrad, tri.normal);
-            V4 n4_raw = n4c + n4n;
-            float n4_len = sqrtf(fmax_(mdot(n4_raw, n4_raw), 1e-12f));
-            V4 normal = n4_raw * (1.0f / n4_len);
-            const float *uva = tri.uv[0], *uvb = tri.uv[1], *uvc = tri.uv[2];
-
-            // Prism (A0,B0,C0)->(A1,B1,C1) split into 3 tetrahedra (mesh4d.rs:801-830).
-            {
-                V4 vv[4] = {a0, b0, c0, c1};
-                float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}, {uvc[0], uvc[1]}};
-                V3 tb[4] = {ba, bb, bc, bc};
-                float ta[4] = {tau_a0, tau_b0, tau_c0, tau_c1};
-                float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}};
-                float dd[4] = {dec0, dec0, dec0, dec1};
-                make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, out);
-            }
-            {
-                V4 vv[4] = {a0, b0, b1, c1};
-                float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
-                V3 tb[4] = {ba, bb, bb, bc};
-                float ta[4] = {tau_a0, tau_b0, tau_b1, tau_c1};
-                float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
-                float dd[4] = {dec0, dec0, dec1, dec1};
-                make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, out);
-            }
-            {
-                V4 vv[4] = {a0, a1, b1, c1};
-                float uv[4][2] = {{uva[0], uva[1]}, {uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
-                V3 tb[4] = {ba, ba, bb, bc};
-                float ta[4] = {tau_a0, tau_a1, tau_b1, tau_c1};
-                float gg[4][2] = {{g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
-                float dd[4] = {dec0, dec1, dec1, dec1};
-                make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, out);
-            }
-        }
+        for (const TubeTri &tri : tris)
+            prism(tri, sc, sn, dec0, dec1, material,
+                  [&](bool ok, const specrel_geom &g, const specrel_attr &a, const Box4 &box) {
+                      if (!ok) return;
+                      out.geoms.push_back(g);
+                      out.attrs.push_back(a);
+                      out.boxes.push_back(box);
+                  });
     }
+}
+
+void extrude(const Mesh &mesh, const std::vector<Sample> &samples, const float *decided, uint32_t material,
+             Prims &out) {
+    if (samples.size() < 2) return;
+    check_rindler(mesh.v, samples.data(), samples.size());
+    std::vector<TubeTri> tris;
+    tris.reserve(mesh.t.size());
+    for (const Tri &t : mesh.t) tris.push_back(tube_tri_of(mesh, t));
+    extrude_tris(tris, samples.data(), samples.size(), decided, material, out);
 }
 
 // ----------------------------------------------------------------------------------------
@@ -1023,8 +885,19 @@ struct Rng {
 // ----------------------------------------------------------------------------------------
 // Builder + host scene
 // ----------------------------------------------------------------------------------------
+// A tube recorded for deferred extrusion: its mesh and samples in the C ABI layout.
+struct PendingTube {
+    std::vector<float> verts, uv, normal, decided;
+    std::vector<uint32_t> idx;
+    std::vector<specrel_wl_sample> samples;
+    uint32_t material = 0;
+    size_t object = 0;
+};
+
 struct specrel_builder {
     Prims prims;
+    bool deferred = false;
+    std::vector<PendingTube> pending;
     std::vector<specrel_object> objects;
     std::vector<Star> stars;
     std::vector<specrel_material> materials = common_materials();
@@ -1123,6 +996,68 @@ void finish_into(specrel_builder &b, specrel_host_scene &s) {
     s.light[3] = b.light.z;
 }
 
+specrel_wl_sample to_c(const Sample &s) {
+    specrel_wl_sample o;
+    std::memset(&o, 0, sizeof o);
+    put4(o.center, s.center);
+    o.proper_time = s.proper_time;
+    o.phi[0] = s.phi.x;
+    o.phi[1] = s.phi.y;
+    o.phi[2] = s.phi.z;
+    o.proper_accel_local[0] = s.accel_local.x;
+    o.proper_accel_local[1] = s.accel_local.y;
+    o.proper_accel_local[2] = s.accel_local.z;
+    for (int k = 0; k < 4; ++k) put4(o.e[k], s.tetrad.e[k]);
+    return o;
+}
+
+// Extrusion of one tube descriptor (the mesh and samples in the C ABI layout), appending to out.
+void extrude_tube(const specrel_tube &t, Prims &out) {
+    if (t.n_samples < 2) return;
+    if (!t.vertices || !t.tri_index || !t.tri_uv || !t.tri_normal || !t.samples)
+        throw Error("tube arrays are NULL");
+    std::vector<V3> verts(t.n_vertices);
+    for (uint32_t i = 0; i < t.n_vertices; ++i)
+        verts[i] = v3(t.vertices[3 * i], t.vertices[3 * i + 1], t.vertices[3 * i + 2]);
+    std::vector<Sample> samples(t.n_samples);
+    for (uint32_t i = 0; i < t.n_samples; ++i) {
+        const specrel_wl_sample &w = t.samples[i];
+        samples[i] = sample_from(w.center, w.proper_time, w.phi, w.proper_accel_local, w.e);
+    }
+    check_rindler(verts, samples.data(), samples.size());
+    std::vector<TubeTri> tris(t.n_triangles);
+    for (uint32_t k = 0; k < t.n_triangles; ++k) {
+        uint32_t idx[3];
+        V3 pos[3];
+        float uv[3][2];
+        for (int c = 0; c < 3; ++c) {
+            idx[c] = t.tri_index[3 * k + c];
+            if (idx[c] >= t.n_vertices) throw Error("triangle vertex index out of range");
+            pos[c] = verts[idx[c]];
+            uv[c][0] = t.tri_uv[6 * k + 2 * c];
+            uv[c][1] = t.tri_uv[6 * k + 2 * c + 1];
+        }
+        tris[k] = tube_tri(idx, pos, uv, v3(t.tri_normal[3 * k], t.tri_normal[3 * k + 1], t.tri_normal[3 * k + 2]));
+    }
+    extrude_tris(tris, samples.data(), samples.size(), t.decided_at, t.material, out);
+}
+
+specrel_tube descriptor(const PendingTube &p) {
+    specrel_tube t;
+    std::memset(&t, 0, sizeof t);
+    t.vertices = p.verts.data();
+    t.tri_index = p.idx.data();
+    t.tri_uv = p.uv.data();
+    t.tri_normal = p.normal.data();
+    t.samples = p.samples.data();
+    t.decided_at = p.decided.empty() ? nullptr : p.decided.data();
+    t.n_vertices = (uint32_t)(p.verts.size() / 3);
+    t.n_triangles = (uint32_t)(p.idx.size() / 3);
+    t.n_samples = (uint32_t)p.samples.size();
+    t.material = p.material;
+    return t;
+}
+
 void add_tube(specrel_builder &b, uint32_t mesh_kind, const float *mp, uint32_t wl_kind, const double *wp,
               bool do_sync, float sync_t, uint32_t material, const float *decided, uint32_t n_decided) {
     Mesh m = make_mesh(mesh_kind, mp);
@@ -1130,7 +1065,26 @@ void add_tube(specrel_builder &b, uint32_t mesh_kind, const float *mp, uint32_t 
     if (do_sync) sync_proper_time_at(s, sync_t);
     if (decided && n_decided != s.size()) throw Error("one decided_at entry per worldline sample");
     size_t first = b.prims.size();
-    extrude(m, s, decided, material, b.prims);
+    if (b.deferred) {
+        if (s.size() >= 2) check_rindler(m.v, s.data(), s.size());
+        PendingTube t;
+        for (const V3 &v : m.v) t.verts.insert(t.verts.end(), {v.x, v.y, v.z});
+        for (const Tri &tr : m.t) {
+            for (int c = 0; c < 3; ++c) {
+                t.idx.push_back((uint32_t)tr.idx[c]);
+                t.uv.push_back(tr.uv[c][0]);
+                t.uv.push_back(tr.uv[c][1]);
+            }
+            t.normal.insert(t.normal.end(), {tr.normal.x, tr.normal.y, tr.normal.z});
+        }
+        for (const Sample &x : s) t.samples.push_back(to_c(x));
+        if (decided) t.decided.assign(decided, decided + n_decided);
+        t.material = material;
+        t.object = b.objects.size();
+        b.pending.push_back(std::move(t));
+    } else {
+        extrude(m, s, decided, material, b.prims);
+    }
     // The object's reference event: its worldline sample nearest to lab time 0.
     specrel_object o;
     std::memset(&o, 0, sizeof o);
@@ -1340,6 +1294,7 @@ int specrel_builder_add_simplex(specrel_builder *b, const float verts[16], const
                                 const float uvws[2], const float normal[4], const float four_vel[4],
                                 const float decided[4], uint32_t material) {
     return guarded([&] {
+        if (b->deferred) throw Error("explicit simplices cannot be mixed with deferred tubes");
         V4 vv[4];
         float uvv[4][2], gg[4][2];
         V3 tb[4];
@@ -1354,9 +1309,17 @@ int specrel_builder_add_simplex(specrel_builder *b, const float verts[16], const
         float dd[4] = {DECIDED_AT_STATIC, DECIDED_AT_STATIC, DECIDED_AT_STATIC, DECIDED_AT_STATIC};
         if (decided)
             for (int i = 0; i < 4; ++i) dd[i] = decided[i];
+        specrel_geom g;
+        specrel_attr a;
+        Box4 box;
         bool ok = make_simplex(vv, uvv, tb, tau, gg, uvws[0], uvws[1],
                                v4(normal[0], normal[1], normal[2], normal[3]),
-                               v4(four_vel[0], four_vel[1], four_vel[2], four_vel[3]), dd, material, b->prims);
+                               v4(four_vel[0], four_vel[1], four_vel[2], four_vel[3]), dd, material, g, a, box);
+        if (ok) {
+            b->prims.geoms.push_back(g);
+            b->prims.attrs.push_back(a);
+            b->prims.boxes.push_back(box);
+        }
         return ok ? 1 : 0;
     });
 }
@@ -1388,6 +1351,7 @@ int specrel_builder_set_materials(specrel_builder *b, const specrel_material *m,
 
 int specrel_builder_finish(specrel_builder *b, specrel_host_scene **out) {
     return guarded([&] {
+        if (!b->pending.empty()) throw Error("the builder has tubes waiting for their records");
         auto s = std::make_unique<specrel_host_scene>();
         finish_into(*b, *s);
         *out = s.release();
@@ -1395,11 +1359,10 @@ int specrel_builder_finish(specrel_builder *b, specrel_host_scene **out) {
     });
 }
 
-int specrel_scene_generate(const char *name, const specrel_gen_params *pp, specrel_host_scene **out) {
-    return guarded([&] {
+namespace {
+void generate_into(specrel_builder &b, const char *name, const specrel_gen_params *pp) {
         std::string n = name ? name : "";
         specrel_gen_params p = pp ? *pp : defaults(n);
-        specrel_builder b;
         if (n == "cube") {
             // One 12-triangle cube (half extent 0.5) at (0,0,5) on an inertial worldline, v = 0.6c
             // along +x over t in [-20, 20], 64 steps: 12 * 64 * 3 = 2304 tetrahedra.
@@ -1418,9 +1381,135 @@ int specrel_scene_generate(const char *name, const specrel_gen_params *pp, specr
         } else {
             throw Error("unknown scene name: " + n);
         }
+}
+}  // namespace
+
+int specrel_scene_generate(const char *name, const specrel_gen_params *pp, specrel_host_scene **out) {
+    return guarded([&] {
+        specrel_builder b;
+        generate_into(b, name, pp);
         auto s = std::make_unique<specrel_host_scene>();
         finish_into(b, *s);
         *out = s.release();
+        return 0;
+    });
+}
+
+int specrel_scene_generate_deferred(const char *name, const specrel_gen_params *pp, specrel_builder **out) {
+    return guarded([&] {
+        if (!out) throw Error("out is NULL");
+        auto b = std::make_unique<specrel_builder>();
+        b->deferred = true;
+        generate_into(*b, name, pp);
+        *out = b.release();
+        return 0;
+    });
+}
+
+uint32_t specrel_builder_pending_tubes(const specrel_builder *b, specrel_tube *out, uint32_t cap) {
+    if (!b) return 0;
+    uint32_t n = (uint32_t)b->pending.size();
+    for (uint32_t i = 0; out && i < n && i < cap; ++i) out[i] = descriptor(b->pending[i]);
+    return n;
+}
+
+namespace {
+// Appends the records of every pending tube (counts[i] of tube i, in order) and closes them.
+void take_records(specrel_builder &b, const specrel_geom *g, const specrel_attr *a, const float *boxes, uint64_t n,
+                  const uint64_t *counts) {
+    uint64_t total = 0;
+    for (size_t i = 0; i < b.pending.size(); ++i) total += counts[i];
+    if (total != n) throw Error("tube record counts do not add up to n");
+    if (n && (!g || !a || !boxes)) throw Error("record arrays are NULL");
+    uint64_t at = 0;
+    for (size_t i = 0; i < b.pending.size(); ++i) {
+        specrel_object &o = b.objects[b.pending[i].object];
+        o.prim_first = (uint32_t)b.prims.size();
+        o.prim_count = (uint32_t)counts[i];
+        for (uint64_t k = at; k < at + counts[i]; ++k) {
+            b.prims.geoms.push_back(g[k]);
+            b.prims.attrs.push_back(a[k]);
+            const float *x = boxes + 8 * k;
+            b.prims.boxes.push_back(Box4{v4(x[0], x[1], x[2], x[3]), v4(x[4], x[5], x[6], x[7])});
+        }
+        at += counts[i];
+    }
+    b.pending.clear();
+}
+}  // namespace
+
+int specrel_builder_set_tube_records(specrel_builder *b, const specrel_geom *geoms, const specrel_attr *attrs,
+                                     const float *boxes, uint64_t n, const uint64_t *counts) {
+    return guarded([&] {
+        if (!b || (!counts && !b->pending.empty())) throw Error("builder or counts is NULL");
+        take_records(*b, geoms, attrs, boxes, n, counts);
+        return 0;
+    });
+}
+
+int specrel_builder_extrude_pending(specrel_builder *b) {
+    return guarded([&] {
+        if (!b) throw Error("builder is NULL");
+        std::vector<uint64_t> counts;
+        Prims all;
+        for (const PendingTube &t : b->pending) {
+            size_t before = all.size();
+            extrude_tube(descriptor(t), all);
+            counts.push_back(all.size() - before);
+        }
+        std::vector<float> boxes(8 * all.size());
+        for (size_t i = 0; i < all.size(); ++i) {
+            put4(&boxes[8 * i], all.boxes[i].mn);
+            put4(&boxes[8 * i + 4], all.boxes[i].mx);
+        }
+        take_records(*b, all.geoms.data(), all.attrs.data(), boxes.data(), all.size(), counts.data());
+        return 0;
+    });
+}
+
+int specrel_extrude_tubes_host(const specrel_tube *tubes, uint32_t n, specrel_geom *geoms, specrel_attr *attrs,
+                               float *boxes, uint64_t cap, uint64_t *n_out, uint64_t *counts) {
+    return guarded([&] {
+        if ((n && !tubes) || !n_out) throw Error("tubes or n_out is NULL");
+        // Tubes are independent: extrude them on a pool of threads, then concatenate in order.
+        std::vector<Prims> parts(n);
+        std::atomic<uint32_t> next{0};
+        std::vector<std::string> errs(n);
+        auto work = [&] {
+            for (uint32_t i; (i = next.fetch_add(1)) < n;) {
+                try {
+                    extrude_tube(tubes[i], parts[i]);
+                } catch (const std::exception &e) {
+                    errs[i] = e.what();
+                }
+            }
+        };
+        unsigned nt = std::max(1u, std::min(n, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned i = 1; i < nt; ++i) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        for (uint32_t i = 0; i < n; ++i)
+            if (!errs[i].empty()) throw Error("tube " + std::to_string(i) + ": " + errs[i]);
+        uint64_t total = 0;
+        for (const Prims &p : parts) total += p.size();
+        *n_out = total;
+        if ((geoms || attrs || boxes) && total > cap)
+            throw Error("output buffers hold " + std::to_string(cap) + " records, " + std::to_string(total) +
+                        " needed");
+        uint64_t at = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const Prims &p = parts[i];
+            if (geoms) std::memcpy(geoms + at, p.geoms.data(), p.size() * sizeof(specrel_geom));
+            if (attrs) std::memcpy(attrs + at, p.attrs.data(), p.size() * sizeof(specrel_attr));
+            if (boxes)
+                for (size_t k = 0; k < p.size(); ++k) {
+                    put4(boxes + 8 * (at + k), p.boxes[k].mn);
+                    put4(boxes + 8 * (at + k) + 4, p.boxes[k].mx);
+                }
+            if (counts) counts[i] = p.size();
+            at += p.size();
+        }
         return 0;
     });
 }

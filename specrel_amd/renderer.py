@@ -28,7 +28,7 @@ SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
                                          "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "adjoint_ms")] + \
-        [("shadow_path", C.c_uint32), ("reserved", C.c_float)]
+        [("shadow_path", C.c_uint32), ("extrude_ms", C.c_float)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -81,6 +81,8 @@ SIGNATURES = {
     "specrel_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "specrel_upload_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays)]),
     "specrel_upload_scene_file": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_int]),
+    "specrel_extrude_tubes": (C.c_int, [_P, C.POINTER(F.Tube), C.c_uint32, C.POINTER(F.Geom), C.POINTER(F.Attr),
+                                        _FP, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
                                      C.POINTER(Range), C.c_uint32, C.POINTER(C.c_int)]),
     "specrel_num_scenes": (C.c_uint32, [_P]),
@@ -167,6 +169,25 @@ class Context:
     def upload_file(self, idx, path, verify=True):
         """specrel_upload_scene_file: a scene file written by any producer (scene.Scene.save)."""
         self._check(self.lib.specrel_upload_scene_file(self._h, idx, os.fsencode(path), 1 if verify else 0))
+
+    def extrude_tubes(self, tubes, download=True):
+        """specrel_extrude_tubes over a sequence of F.Tube: (geoms (n,20), attrs (n,48), boxes (n,8)
+        float32 arrays, per-tube counts), or (None, None, None, counts) with download=False."""
+        arr = (F.Tube * max(1, len(tubes)))(*tubes)
+        n = C.c_uint64(0)
+        counts = np.zeros(max(1, len(tubes)), np.uint64)
+        cptr = counts.ctypes.data_as(C.POINTER(C.c_uint64))
+        self._check(self.lib.specrel_extrude_tubes(self._h, arr, len(tubes), None, None, None, 0, C.byref(n), cptr))
+        if not download:
+            return None, None, None, counts[:len(tubes)]
+        total = n.value
+        g = np.zeros((total, 20), np.float32)
+        a = np.zeros((total, 48), np.float32)
+        b = np.zeros((total, 8), np.float32)
+        self._check(self.lib.specrel_extrude_tubes(
+            self._h, arr, len(tubes), g.ctypes.data_as(C.POINTER(F.Geom)), a.ctypes.data_as(C.POINTER(F.Attr)),
+            b.ctypes.data_as(_FP), total, C.byref(n), cptr))
+        return g, a, b, counts[:len(tubes)]
 
     def patch(self, idx, scene, node_ranges=(), prim_ranges=()):
         """specrel_patch_scene: returns True when the ranges were applied, False when the library

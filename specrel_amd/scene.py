@@ -347,6 +347,67 @@ class Builder:
         return Scene(h.value, name)
 
 
+class DeferredScene:
+    """A named scene generated with its worldtubes recorded, not extruded
+    (specrel_scene_generate_deferred): tubes() are the descriptors, whose records come back from
+    the GPU (specrel_extrude_tubes) or the host (extrude_on_host), then finish() builds the BVH."""
+
+    def __init__(self, name, params: Optional[F.GenParams] = None):
+        self.name = name
+        self.params = params
+        h = C.c_void_p()
+        _check(F.scene_lib().specrel_scene_generate_deferred(name.encode(), C.byref(params) if params is not None
+                                                             else None, C.byref(h)))
+        self._b = h
+
+    def __del__(self):
+        if getattr(self, "_b", None) and self._b.value:
+            F.scene_lib().specrel_builder_free(self._b)
+
+    def tubes(self):
+        lib = F.scene_lib()
+        n = lib.specrel_builder_pending_tubes(self._b, None, 0)
+        arr = (F.Tube * max(1, n))()
+        lib.specrel_builder_pending_tubes(self._b, arr, n)
+        return list(arr[:n])
+
+    def set_records(self, geoms, attrs, boxes, counts):
+        g = np.ascontiguousarray(geoms, np.float32)
+        a = np.ascontiguousarray(attrs, np.float32)
+        b = np.ascontiguousarray(boxes, np.float32)
+        c = np.ascontiguousarray(counts, np.uint64)
+        _check(F.scene_lib().specrel_builder_set_tube_records(
+            self._b, g.ctypes.data_as(C.POINTER(F.Geom)), a.ctypes.data_as(C.POINTER(F.Attr)), _fptr(b), len(g),
+            c.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def extrude_on_host(self):
+        _check(F.scene_lib().specrel_builder_extrude_pending(self._b))
+
+    def finish(self) -> Scene:
+        h = C.c_void_p()
+        _check(F.scene_lib().specrel_builder_finish(self._b, C.byref(h)))
+        s = Scene(h.value, self.name)
+        s.params = self.params
+        return s
+
+
+def extrude_tubes_host(tubes):
+    """specrel_extrude_tubes_host: (geoms (n,20), attrs (n,48), boxes (n,8), per-tube counts)."""
+    lib = F.scene_lib()
+    arr = (F.Tube * max(1, len(tubes)))(*tubes)
+    n = C.c_uint64(0)
+    counts = np.zeros(max(1, len(tubes)), np.uint64)
+    cptr = counts.ctypes.data_as(C.POINTER(C.c_uint64))
+    _check(lib.specrel_extrude_tubes_host(arr, len(tubes), None, None, None, 0, C.byref(n), cptr))
+    total = n.value
+    g = np.zeros((total, 20), np.float32)
+    a = np.zeros((total, 48), np.float32)
+    b = np.zeros((total, 8), np.float32)
+    _check(lib.specrel_extrude_tubes_host(arr, len(tubes), g.ctypes.data_as(C.POINTER(F.Geom)),
+                                          a.ctypes.data_as(C.POINTER(F.Attr)), _fptr(b), total, C.byref(n), cptr))
+    return g, a, b, counts[:len(tubes)]
+
+
 def worldline(kind, params, cap=4096):
     out = (F.WlSample * cap)()
     n = C.c_uint32()
