@@ -133,66 +133,78 @@ SR_HD bool make_simplex(const V4 verts[4], const float uv[4][2], const V3 tri_ba
 // The event of a rest-frame body point at a worldline sample (mesh4d.rs:749).
 SR_HD V4 embed(const Sample &s, V3 p) { return s.center + local_spatial_to_world(s.tetrad, p); }
 
-// The three tetrahedra of one prism (triangle `tri` swept from sample sc to sample sn,
-// mesh4d.rs:755-849), in order: sink(ok, geom, attr, box) once per tetrahedron, ok false for a
-// degenerate one (geom, attr and box then unset).
-template <class Sink>
-SR_HD void prism(const TubeTri &tri, const Sample &sc, const Sample &sn, float dec0, float dec1, uint32_t material,
-                 Sink &&sink) {
+// What the three tetrahedra of one prism share (triangle `tri` swept from sample sc to sample
+// sn, mesh4d.rs:755-799): the six corner events, proper times, UV gradients of tau, the step's
+// surface normal and average four-velocity.
+struct PrismShared {
+    V4 u_avg, normal;
+    V4 ev[2][3];       // [time][corner a, b, c]
+    float tau[2][3];   // [time][corner]
+    float grad[2][2];  // [time][u, v]
+    float uvws0, uvws1;
+};
+
+SR_HD PrismShared prism_shared(const TubeTri &tri, const Sample &sc, const Sample &sn) {
+    PrismShared q;
     V4 u_raw = sc.tetrad.e[0] + sn.tetrad.e[0];
     float u_norm = sqrtf(fmax_(-mdot(u_raw, u_raw), 1e-12f));
-    V4 u_avg = u_raw * (1.0f / u_norm);
-    V4 a0 = embed(sc, tri.p[0]), b0 = embed(sc, tri.p[1]), c0 = embed(sc, tri.p[2]);
-    V4 a1 = embed(sn, tri.p[0]), b1 = embed(sn, tri.p[1]), c1 = embed(sn, tri.p[2]);
-    V3 pa = tri.p[0], pb = tri.p[1], pc = tri.p[2];
-    float tau_a0 = sc.proper_time + dot(pa, sc.phi);
-    float tau_b0 = sc.proper_time + dot(pb, sc.phi);
-    float tau_c0 = sc.proper_time + dot(pc, sc.phi);
-    float tau_a1 = sn.proper_time + dot(pa, sn.phi);
-    float tau_b1 = sn.proper_time + dot(pb, sn.phi);
-    float tau_c1 = sn.proper_time + dot(pc, sn.phi);
-    float uvws0 = length(tri.tu), uvws1 = length(tri.tv);
-    float g0[2] = {dot(tri.tu, sc.phi), dot(tri.tv, sc.phi)};
-    float g1[2] = {dot(tri.tu, sn.phi), dot(tri.tv, sn.phi)};
-    V3 ba = tri.bary[0], bb = tri.bary[1], bc = tri.bary[2];
+    q.u_avg = u_raw * (1.0f / u_norm);
+    for (int c = 0; c < 3; ++c) {
+        q.ev[0][c] = embed(sc, tri.p[c]);
+        q.ev[1][c] = embed(sn, tri.p[c]);
+    }
+    for (int c = 0; c < 3; ++c) {
+        q.tau[0][c] = sc.proper_time + dot(tri.p[c], sc.phi);
+        q.tau[1][c] = sn.proper_time + dot(tri.p[c], sn.phi);
+    }
+    q.uvws0 = length(tri.tu);
+    q.uvws1 = length(tri.tv);
+    q.grad[0][0] = dot(tri.tu, sc.phi);
+    q.grad[0][1] = dot(tri.tv, sc.phi);
+    q.grad[1][0] = dot(tri.tu, sn.phi);
+    q.grad[1][1] = dot(tri.tv, sn.phi);
     V4 n4c = local_spatial_to_world(sc.tetrad, tri.normal);
     V4 n4n = local_spatial_to_world(sn.tetrad, tri.normal);
     V4 n4_raw = n4c + n4n;
     float n4_len = sqrtf(fmax_(mdot(n4_raw, n4_raw), 1e-12f));
-    V4 normal = n4_raw * (1.0f / n4_len);
-    const float *uva = tri.uv[0], *uvb = tri.uv[1], *uvc = tri.uv[2];
+    q.normal = n4_raw * (1.0f / n4_len);
+    return q;
+}
+
+// Tetrahedron k of the prism split (A0,B0,C0)->(A1,B1,C1) (mesh4d.rs:801-830):
+//   0: (A0, B0, C0, C1)   1: (A0, B0, B1, C1)   2: (A0, A1, B1, C1)
+// Vertex 0 is always A0 and vertex 3 C1; vertex 1 is B0 for k < 2, else A1; vertex 2 is C0 for
+// k = 0, else B1. Every attribute follows its vertex's (time, corner).
+SR_HD bool prism_tet(const PrismShared &q, const TubeTri &tri, float dec0, float dec1, uint32_t material, int k,
+                     specrel_geom &g, specrel_attr &a, Box4 &box) {
+    // Selects, not indexed arrays, so a thread's k costs no private memory on the device.
+    const bool b1 = k < 2, b2 = k == 0;  // vertex 1 is B0 (else A1); vertex 2 is C0 (else B1)
+    const V4 vv[4] = {q.ev[0][0], b1 ? q.ev[0][1] : q.ev[1][0], b2 ? q.ev[0][2] : q.ev[1][1], q.ev[1][2]};
+    const float uv[4][2] = {{tri.uv[0][0], tri.uv[0][1]},
+                            {b1 ? tri.uv[1][0] : tri.uv[0][0], b1 ? tri.uv[1][1] : tri.uv[0][1]},
+                            {b2 ? tri.uv[2][0] : tri.uv[1][0], b2 ? tri.uv[2][1] : tri.uv[1][1]},
+                            {tri.uv[2][0], tri.uv[2][1]}};
+    const V3 tb[4] = {tri.bary[0], b1 ? tri.bary[1] : tri.bary[0], b2 ? tri.bary[2] : tri.bary[1], tri.bary[2]};
+    const float ta[4] = {q.tau[0][0], b1 ? q.tau[0][1] : q.tau[1][0], b2 ? q.tau[0][2] : q.tau[1][1], q.tau[1][2]};
+    const float gg[4][2] = {{q.grad[0][0], q.grad[0][1]},
+                            {b1 ? q.grad[0][0] : q.grad[1][0], b1 ? q.grad[0][1] : q.grad[1][1]},
+                            {b2 ? q.grad[0][0] : q.grad[1][0], b2 ? q.grad[0][1] : q.grad[1][1]},
+                            {q.grad[1][0], q.grad[1][1]}};
+    const float dd[4] = {dec0, b1 ? dec0 : dec1, b2 ? dec0 : dec1, dec1};
+    return make_simplex(vv, uv, tb, ta, gg, q.uvws0, q.uvws1, q.normal, q.u_avg, dd, material, g, a, box);
+}
+
+// The three tetrahedra of one prism in order: sink(ok, geom, attr, box) once per tetrahedron, ok
+// false for a degenerate one (geom, attr and box then unset).
+template <class Sink>
+SR_HD void prism(const TubeTri &tri, const Sample &sc, const Sample &sn, float dec0, float dec1, uint32_t material,
+                 Sink &&sink) {
+    const PrismShared q = prism_shared(tri, sc, sn);
     specrel_geom g;
     specrel_attr a;
     Box4 box;
-    // Prism (A0,B0,C0)->(A1,B1,C1) split into 3 tetrahedra (mesh4d.rs:801-830).
-    {
-        V4 vv[4] = {a0, b0, c0, c1};
-        float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}, {uvc[0], uvc[1]}};
-        V3 tb[4] = {ba, bb, bc, bc};
-        float ta[4] = {tau_a0, tau_b0, tau_c0, tau_c1};
-        float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}};
-        float dd[4] = {dec0, dec0, dec0, dec1};
-        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
-        sink(ok, g, a, box);
-    }
-    {
-        V4 vv[4] = {a0, b0, b1, c1};
-        float uv[4][2] = {{uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
-        V3 tb[4] = {ba, bb, bb, bc};
-        float ta[4] = {tau_a0, tau_b0, tau_b1, tau_c1};
-        float gg[4][2] = {{g0[0], g0[1]}, {g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
-        float dd[4] = {dec0, dec0, dec1, dec1};
-        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
-        sink(ok, g, a, box);
-    }
-    {
-        V4 vv[4] = {a0, a1, b1, c1};
-        float uv[4][2] = {{uva[0], uva[1]}, {uva[0], uva[1]}, {uvb[0], uvb[1]}, {uvc[0], uvc[1]}};
-        V3 tb[4] = {ba, ba, bb, bc};
-        float ta[4] = {tau_a0, tau_a1, tau_b1, tau_c1};
-        float gg[4][2] = {{g0[0], g0[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}, {g1[0], g1[1]}};
-        float dd[4] = {dec0, dec1, dec1, dec1};
-        bool ok = make_simplex(vv, uv, tb, ta, gg, uvws0, uvws1, normal, u_avg, dd, material, g, a, box);
+    for (int k = 0; k < 3; ++k) {
+        bool ok = prism_tet(q, tri, dec0, dec1, material, k, g, a, box);
         sink(ok, g, a, box);
     }
 }

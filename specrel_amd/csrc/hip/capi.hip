@@ -1132,34 +1132,55 @@ int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t 
         a.decided = (const float *)ctx->ex_decided.p;
         uint64_t total = 0;
         float ms = 0.0f;
+        bool direct = false;
         if (n_prisms) {
+            // Optimistic pass: records at slot 3p + k; when all are proper these are final.
+            const uint64_t slots = 3 * n_prisms, slot_blocks = (slots + 255) / 256;
+            if (slot_blocks > 0x7FFFFFFFull) throw Fail{SPECREL_EINVAL, "too many prisms for one launch"};
+            dalloc(ctx->ex_geoms, slots * sizeof(specrel_geom));
+            dalloc(ctx->ex_attrs, slots * sizeof(specrel_attr));
+            dalloc(ctx->ex_boxes, slots * 32);
+            dalloc(ctx->ex_counts, slot_blocks * 4);
+            dalloc(ctx->ex_offsets, (slot_blocks + 1) * 8);
             check(hipEventRecord(ctx->ev[7], s), "event");
-            dev::k_extrude_count<<<dim3((unsigned)n_blocks), 256, 0, s>>>(a, (uint32_t *)ctx->ex_counts.p);
-            check(hipGetLastError(), "k_extrude_count launch");
-            dev::k_extrude_scan<<<1, 1024, 0, s>>>((uint32_t *)ctx->ex_counts.p, (uint32_t)n_blocks,
+            dev::k_extrude_direct<<<dim3((unsigned)slot_blocks), 256, 0, s>>>(
+                a, (uint32_t *)ctx->ex_counts.p, (specrel_geom *)ctx->ex_geoms.p, (specrel_attr *)ctx->ex_attrs.p,
+                (float4 *)ctx->ex_boxes.p);
+            check(hipGetLastError(), "k_extrude_direct launch");
+            dev::k_extrude_scan<<<1, 1024, 0, s>>>((uint32_t *)ctx->ex_counts.p, (uint32_t)slot_blocks,
                                                    (uint64_t *)ctx->ex_offsets.p);
             check(hipGetLastError(), "k_extrude_scan launch");
             check(hipEventRecord(ctx->ev[8], s), "event");
-            check(hipMemcpyAsync(&total, (const uint64_t *)ctx->ex_offsets.p + n_blocks, 8, hipMemcpyDeviceToHost, s),
+            check(hipMemcpyAsync(&total, (const uint64_t *)ctx->ex_offsets.p + slot_blocks, 8, hipMemcpyDeviceToHost,
+                                 s),
                   "total readback");
-            check(hipStreamSynchronize(s), "extrude count");
+            check(hipStreamSynchronize(s), "extrude");
             ms = ev_ms(ctx->ev[7], ctx->ev[8]);
-            dalloc(ctx->ex_geoms, total * sizeof(specrel_geom));
-            dalloc(ctx->ex_attrs, total * sizeof(specrel_attr));
-            dalloc(ctx->ex_boxes, total * 32);
-            check(hipEventRecord(ctx->ev[7], s), "event");
-            dev::k_extrude_emit<<<dim3((unsigned)n_blocks), 256, 0, s>>>(
-                a, (const uint64_t *)ctx->ex_offsets.p, (specrel_geom *)ctx->ex_geoms.p, (specrel_attr *)ctx->ex_attrs.p,
-                (float4 *)ctx->ex_boxes.p, (uint64_t *)ctx->ex_first.p);
-            check(hipGetLastError(), "k_extrude_emit launch");
-            check(hipEventRecord(ctx->ev[8], s), "event");
-            check(hipStreamSynchronize(s), "extrude emit");
-            ms += ev_ms(ctx->ev[7], ctx->ev[8]);
+            direct = total == slots;
+            if (!direct) {
+                // Some tetrahedra were degenerate: count per block of prisms, scan, and write each
+                // record again at its index after the drops.
+                check(hipEventRecord(ctx->ev[7], s), "event");
+                dev::k_extrude_count<<<dim3((unsigned)n_blocks), 256, 0, s>>>(a, (uint32_t *)ctx->ex_counts.p);
+                check(hipGetLastError(), "k_extrude_count launch");
+                dev::k_extrude_scan<<<1, 1024, 0, s>>>((uint32_t *)ctx->ex_counts.p, (uint32_t)n_blocks,
+                                                       (uint64_t *)ctx->ex_offsets.p);
+                check(hipGetLastError(), "k_extrude_scan launch");
+                dev::k_extrude_emit<<<dim3((unsigned)n_blocks), 256, 0, s>>>(
+                    a, (const uint64_t *)ctx->ex_offsets.p, (specrel_geom *)ctx->ex_geoms.p,
+                    (specrel_attr *)ctx->ex_attrs.p, (float4 *)ctx->ex_boxes.p, (uint64_t *)ctx->ex_first.p);
+                check(hipGetLastError(), "k_extrude_emit launch");
+                check(hipEventRecord(ctx->ev[8], s), "event");
+                check(hipStreamSynchronize(s), "extrude emit");
+                ms += ev_ms(ctx->ev[7], ctx->ev[8]);
+            }
         }
         ctx->timings.extrude_ms = ms;
         *n_out = total;
         std::vector<uint64_t> first(n_tubes, 0);
-        if (n_tubes && n_prisms)
+        if (direct)
+            for (uint32_t i = 0; i < n_tubes; ++i) first[i] = 3 * desc[i].prism_first;
+        else if (n_tubes && n_prisms)
             check(hipMemcpy(first.data(), ctx->ex_first.p, n_tubes * 8, hipMemcpyDeviceToHost), "tube offsets");
         if (counts) {
             // Records of tube i: from its first record to the next tube with prisms (or the end).

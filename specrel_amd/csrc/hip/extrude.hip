@@ -1,8 +1,10 @@
 // Worldtube extrusion on the device (extend_worldtube_simplices, mesh4d.rs:721-854): one thread
 // per prism (a triangle swept over one worldline step), the prism's three tetrahedra computed by
 // the same source the scene producer runs on the host (core/extrude.h), so the records are
-// bit-identical. Degenerate tetrahedra are dropped in order: a counting pass, a scan of the
-// per-block counts, and an emitting pass that writes each record at its final index.
+// bit-identical. A first pass writes each tetrahedron's record at its slot 3p + k and counts the
+// proper ones; when a degenerate tetrahedron was dropped, a counting pass per block of prisms, a
+// scan of those counts and an emitting pass write every record again at its index after the
+// drops (in order).
 #include <hip/hip_runtime.h>
 
 #include "extrude_dev.hpp"
@@ -60,6 +62,7 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t *s_wave, u
     return before + x - v;
 }
 
+// Proper tetrahedra per block of 256 prisms (the compaction's first pass).
 __global__ __launch_bounds__(256) void k_extrude_count(ExArgs a, uint32_t *block_counts) {
     const uint64_t p = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     uint32_t c = 0, tube;
@@ -117,6 +120,61 @@ __global__ __launch_bounds__(256) void k_extrude_emit(ExArgs a, const uint64_t *
         boxes[2 * o + 1] = make_float4(box.mx.t, box.mx.x, box.mx.y, box.mx.z);
         ++o;
     });
+}
+
+// Optimistic single pass, one thread per tetrahedron slot q = 3p + k: the record goes to slot q
+// and each block counts its proper ones. When every tetrahedron is proper (the usual case) the
+// slots are the final indices and no compaction is needed. The block's records are staged in LDS
+// and stored as contiguous runs (one component array at a time), so every store instruction
+// writes whole cache lines.
+__global__ __launch_bounds__(256) void k_extrude_direct(ExArgs a, uint32_t *block_counts, specrel_geom *geoms,
+                                                        specrel_attr *attrs, float4 *boxes) {
+    const uint64_t slot0 = (uint64_t)blockIdx.x * 256, q = slot0 + threadIdx.x, n_slots = 3 * a.n_prisms;
+    const uint32_t tid = threadIdx.x;
+    bool ok = false;
+    specrel_geom g;
+    specrel_attr at;
+    sr::Box4 box;
+    if (q < n_slots) {
+        const uint64_t p = q / 3;
+        const int k = (int)(q - 3 * p);
+        const uint32_t tube = find_tube(a.tubes, a.n_tubes, p);
+        const ExTube &t = a.tubes[tube];
+        const uint64_t local = p - t.prism_first;
+        const uint32_t step = (uint32_t)(local / t.n_tris), tri = (uint32_t)(local % t.n_tris);
+        const sr::Sample sc = a.samples[t.sample_first + step], sn = a.samples[t.sample_first + step + 1];
+        const float dec0 = t.decided_first == ~0u ? sr::DECIDED_AT_STATIC : a.decided[t.decided_first + step];
+        const float dec1 = t.decided_first == ~0u ? sr::DECIDED_AT_STATIC : a.decided[t.decided_first + step + 1];
+        const sr::TubeTri &tt = a.tris[t.tri_first + tri];
+        const sr::PrismShared sh = sr::prism_shared(tt, sc, sn);
+        ok = sr::prism_tet(sh, tt, dec0, dec1, t.material, k, g, at, box);
+    }
+    __shared__ uint32_t s_wave[4];
+    uint32_t total;
+    block_prefix(ok ? 1u : 0u, s_wave, total);
+    if (tid == 0) block_counts[blockIdx.x] = total;
+    // Staged stores: geoms (5 float4 per record), attrs (12), boxes (2).
+    __shared__ float4 s_buf[256 * 12];
+    const uint32_t n_here = (uint32_t)min((uint64_t)256, n_slots - slot0);
+    const float4 *gv = reinterpret_cast<const float4 *>(&g);
+    const float4 *av = reinterpret_cast<const float4 *>(&at);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) s_buf[tid * 5 + j] = gv[j];
+    __syncthreads();
+    float4 *go = reinterpret_cast<float4 *>(geoms) + slot0 * 5;
+    for (uint32_t i = tid; i < n_here * 5; i += 256) go[i] = s_buf[i];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 12; ++j) s_buf[tid * 12 + j] = av[j];
+    __syncthreads();
+    float4 *ao = reinterpret_cast<float4 *>(attrs) + slot0 * 12;
+    for (uint32_t i = tid; i < n_here * 12; i += 256) ao[i] = s_buf[i];
+    __syncthreads();
+    s_buf[tid * 2] = make_float4(box.mn.t, box.mn.x, box.mn.y, box.mn.z);
+    s_buf[tid * 2 + 1] = make_float4(box.mx.t, box.mx.x, box.mx.y, box.mx.z);
+    __syncthreads();
+    float4 *bo = boxes + slot0 * 2;
+    for (uint32_t i = tid; i < n_here * 2; i += 256) bo[i] = s_buf[i];
 }
 
 }  // namespace dev

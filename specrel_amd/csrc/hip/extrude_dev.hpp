@@ -25,6 +25,8 @@ struct ExArgs {
 };
 
 __global__ void k_extrude_count(ExArgs a, uint32_t *block_counts);
+__global__ void k_extrude_direct(ExArgs a, uint32_t *block_counts, specrel_geom *geoms, specrel_attr *attrs,
+                                 float4 *boxes);
 __global__ void k_extrude_scan(uint32_t *counts, uint32_t n, uint64_t *offsets);
 __global__ void k_extrude_emit(ExArgs a, const uint64_t *offsets, specrel_geom *geoms, specrel_attr *attrs,
                                float4 *boxes, uint64_t *tube_first);
