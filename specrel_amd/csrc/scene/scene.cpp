@@ -677,12 +677,14 @@ size_t split(BuildRef *refs, size_t count, Aabb &bounds_out) {
         }
     }
     if (mid == 0 || mid == count) {
-        // select_nth_unstable_by fallback: deterministic nth_element by centroid on the axis.
+        // select_nth_unstable_by fallback (bvh4d.rs:418-425): any permutation that puts the
+        // count/2 smallest centroids on the axis first satisfies it; the one used here is a
+        // stable sort by that centroid (ties keep their order), which the device builder
+        // reproduces exactly.
         mid = count / 2;
         int ax = best_axis;
-        std::nth_element(refs, refs + mid, refs + count, [ax](const BuildRef &x, const BuildRef &y) {
-            float cx = axis(x.c, ax), cy = axis(y.c, ax);
-            return cx < cy;
+        std::stable_sort(refs, refs + count, [ax](const BuildRef &x, const BuildRef &y) {
+            return axis(x.c, ax) < axis(y.c, ax);
         });
     }
     return mid;
