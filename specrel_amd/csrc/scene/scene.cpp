@@ -26,6 +26,7 @@
 #include <thread>
 #include <vector>
 
+#include "../core/bvh_split.h"
 #include "../core/extrude.h"
 #include "../core/minkowski.h"
 #include "../core/scene_file.hpp"
@@ -419,11 +420,6 @@ std::vector<Sample> sample_worldline(uint32_t kind, const double *d) {
 // Simplices (mesh4d.rs:214-371) and worldtube extrusion (mesh4d.rs:662-854)
 // ----------------------------------------------------------------------------------------
 using Aabb = Box4;
-inline Aabb aabb_empty() {
-    const float inf = __builtin_inff();
-    return Aabb{splat4(inf), splat4(-inf)};
-}
-inline Aabb aabb_union(Aabb a, Aabb b) { return Aabb{vmin(a.mn, b.mn), vmax(a.mx, b.mx)}; }
 
 struct Prims {
     std::vector<specrel_geom> geoms;
@@ -497,13 +493,6 @@ void extrude(const Mesh &mesh, const std::vector<Sample> &samples, const float *
 // ----------------------------------------------------------------------------------------
 // Bvh4D::build (bvh4d.rs:8-13, 122-141, 254-427)
 // ----------------------------------------------------------------------------------------
-inline float box_measure(const Aabb &b) {
-    V4 e = b.mx - b.mn;
-    float spatial_sa = e.x * e.y + e.y * e.z + e.z * e.x;
-    float temporal_sa = 0.22f * e.t * (e.x + e.y + e.z);
-    return spatial_sa + temporal_sa;
-}
-
 struct BuildRef {
     Aabb box;
     V4 c;
@@ -517,7 +506,6 @@ struct Node {
 
 const size_t PARALLEL_MIN = 4096;
 const size_t FOLD_CHUNK = 1u << 16;
-const int BINS = 8;
 
 std::atomic<int> g_threads_live{0};
 int g_thread_budget = 1;
@@ -546,129 +534,66 @@ void parallel_chunks(size_t n, const F &f) {
 
 // The node over refs: bounds, and 0 for a leaf or the split point (refs reordered).
 size_t split(BuildRef *refs, size_t count, Aabb &bounds_out) {
-    Aabb bounds = aabb_empty(), cb = aabb_empty();
+    Aabb bounds = box_empty(), cb = box_empty();
     if (count > FOLD_CHUNK) {
         size_t nchunks = (count + FOLD_CHUNK - 1) / FOLD_CHUNK;
-        std::vector<Aabb> pb(nchunks, aabb_empty()), pc(nchunks, aabb_empty());
+        std::vector<Aabb> pb(nchunks, box_empty()), pc(nchunks, box_empty());
         parallel_chunks(count, [&](size_t b, size_t e, size_t c) {
-            Aabb x = aabb_empty(), y = aabb_empty();
+            Aabb x = box_empty(), y = box_empty();
             for (size_t i = b; i < e; ++i) {
-                x = aabb_union(x, refs[i].box);
-                y.mn = vmin(y.mn, refs[i].c);
-                y.mx = vmax(y.mx, refs[i].c);
+                x = box_union(x, refs[i].box);
+                box_add_point(y, refs[i].c);
             }
             pb[c] = x;
             pc[c] = y;
         });
         for (size_t c = 0; c < nchunks; ++c) {
-            bounds = aabb_union(bounds, pb[c]);
-            cb = aabb_union(cb, pc[c]);
+            bounds = box_union(bounds, pb[c]);
+            cb = box_union(cb, pc[c]);
         }
     } else {
         for (size_t i = 0; i < count; ++i) {
-            bounds = aabb_union(bounds, refs[i].box);
-            cb.mn = vmin(cb.mn, refs[i].c);
-            cb.mx = vmax(cb.mx, refs[i].c);
+            bounds = box_union(bounds, refs[i].box);
+            box_add_point(cb, refs[i].c);
         }
     }
     bounds_out = bounds;
     if (count <= 3) return 0;
 
-    V4 ext = cb.mx - cb.mn;
-    int best_axis = 0;
-    float best_split_pos = 0.0f;
-    float best_cost = __builtin_inff();
-    bool live[4];
-    float axis_min[4], inv_ext[4];
-    for (int a = 0; a < 4; ++a) {
-        live[a] = axis(ext, a) >= 1e-4f;
-        axis_min[a] = axis(cb.mn, a);
-        inv_ext[a] = (float)BINS / axis(ext, a);
-    }
-    struct Bins {
-        Aabb b[4][BINS];
-        uint32_t n[4][BINS];
-        void clear() {
-            for (int a = 0; a < 4; ++a)
-                for (int i = 0; i < BINS; ++i) {
-                    b[a][i] = aabb_empty();
-                    n[a][i] = 0;
-                }
-        }
-        void merge(const Bins &o) {
-            for (int a = 0; a < 4; ++a)
-                for (int i = 0; i < BINS; ++i) {
-                    b[a][i] = aabb_union(b[a][i], o.b[a][i]);
-                    n[a][i] += o.n[a][i];
-                }
-        }
-    };
+    const BinFrame f = bin_frame(cb);
     auto bin_range = [&](Bins &B, size_t b, size_t e) {
         for (size_t i = b; i < e; ++i) {
             const BuildRef &r = refs[i];
             for (int a = 0; a < 4; ++a) {
-                if (!live[a]) continue;
-                float c = axis(r.c, a);
-                uint64_t k = f2u_sat((c - axis_min[a]) * inv_ext[a]);
-                if (k > (uint64_t)(BINS - 1)) k = BINS - 1;
-                B.b[a][k] = aabb_union(B.b[a][k], r.box);
+                if (!f.live[a]) continue;
+                uint32_t k = bin_of(f, a, axis(r.c, a));
+                B.b[a][k] = box_union(B.b[a][k], r.box);
                 B.n[a][k] += 1;
             }
         }
     };
     Bins bins;
-    bins.clear();
+    bins_clear(bins);
     if (count > FOLD_CHUNK) {
         size_t nchunks = (count + FOLD_CHUNK - 1) / FOLD_CHUNK;
         std::vector<Bins> part(nchunks);
         parallel_chunks(count, [&](size_t b, size_t e, size_t c) {
-            part[c].clear();
+            bins_clear(part[c]);
             bin_range(part[c], b, e);
         });
-        for (size_t c = 0; c < nchunks; ++c) bins.merge(part[c]);
+        for (size_t c = 0; c < nchunks; ++c)
+            for (int a = 0; a < 4; ++a)
+                for (int i = 0; i < BVH_BINS; ++i) {
+                    bins.b[a][i] = box_union(bins.b[a][i], part[c].b[a][i]);
+                    bins.n[a][i] += part[c].n[a][i];
+                }
     } else {
         bin_range(bins, 0, count);
     }
-
-    for (int a = 0; a < 4; ++a) {
-        if (!live[a]) continue;
-        float axis_ext = axis(ext, a);
-        Aabb left_box[BINS - 1];
-        uint32_t left_cnt[BINS - 1];
-        Aabb cur_b = aabb_empty();
-        uint32_t cur_c = 0;
-        for (int i = 0; i < BINS - 1; ++i) {
-            cur_b = aabb_union(cur_b, bins.b[a][i]);
-            cur_c += bins.n[a][i];
-            left_box[i] = cur_b;
-            left_cnt[i] = cur_c;
-        }
-        Aabb right_b = aabb_empty();
-        uint32_t right_c = 0;
-        for (int i = BINS - 2; i >= 0; --i) {
-            right_b = aabb_union(right_b, bins.b[a][i + 1]);
-            right_c += bins.n[a][i + 1];
-            if (left_cnt[i] > 0 && right_c > 0) {
-                float cost = (float)left_cnt[i] * box_measure(left_box[i]) + (float)right_c * box_measure(right_b);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_axis = a;
-                    best_split_pos = axis_min[a] + ((float)(i + 1) / (float)BINS) * axis_ext;
-                }
-            }
-        }
-    }
-    if (best_cost == __builtin_inff()) {
-        float max_e = -1.0f;
-        for (int a = 0; a < 4; ++a) {
-            if (axis(ext, a) > max_e) {
-                max_e = axis(ext, a);
-                best_axis = a;
-            }
-        }
-        if (max_e < 1e-5f) return 0;
-        best_split_pos = axis((cb.mn + cb.mx) * 0.5f, best_axis);
-    }
+    int best_axis;
+    float best_split_pos;
+    if (choose_split(cb, f, bins, best_axis, best_split_pos) == SPLIT_LEAF) return 0;
+    // Partition by swapping (bvh4d.rs:409-416): the left side keeps its order.
     size_t mid = 0;
     for (size_t i = 0; i < count; ++i) {
         if (axis(refs[i].c, best_axis) < best_split_pos) {
