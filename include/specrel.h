@@ -128,6 +128,23 @@ int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *
 int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t n_tubes, specrel_geom *geoms,
                           specrel_attr *attrs, float *boxes, uint64_t cap, uint64_t *n_out, uint64_t *counts);
 
+/* Bvh4D::build (bvh4d.rs:122-427) on the device, emitting the same tree as the scene library's
+ * host build (specrel_builder_finish) bit for bit: nodes (capacity 2n - 1) in preorder and the leaf
+ * order (n), for n records whose padded bounds are boxes (8 floats per record: min t,x,y,z, max
+ * t,x,y,z; host memory), or, with boxes NULL, the records of the last specrel_extrude_tubes still
+ * on the device. *n_nodes receives the node count; stats (may be NULL) what the build did. */
+typedef struct specrel_bvh_stats {
+    float ms;                     /* wall time of the build, device work and host decisions */
+    uint32_t levels;              /* breadth-first levels of large segments */
+    uint32_t large_segments;      /* segments partitioned level-parallel */
+    uint32_t small_subtrees;      /* subtrees of at most 256 records, one per thread */
+    uint32_t fallback_segments;   /* partitions that fell back to the centroid sort */
+    uint32_t jump_passes;         /* pointer-jumping passes of the partitions */
+    uint32_t reserved[2];
+} specrel_bvh_stats;
+int specrel_build_bvh(specrel_ctx *ctx, const float *boxes, uint64_t n, specrel_node *nodes, uint32_t *order,
+                      uint32_t *n_nodes, specrel_bvh_stats *stats);
+
 /* A range [first, first + count) of node or primitive (leaf-order) indices. */
 typedef struct specrel_range {
     uint32_t first, count;

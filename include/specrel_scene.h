@@ -159,6 +159,14 @@ uint32_t specrel_builder_pending_tubes(const specrel_builder *b, specrel_tube *o
 int specrel_builder_set_tube_records(specrel_builder *b, const specrel_geom *geoms, const specrel_attr *attrs,
                                      const float *boxes, uint64_t n, const uint64_t *counts);
 int specrel_builder_extrude_pending(specrel_builder *b);
+/* Bvh4D::build (bvh4d.rs:122-427) on the host over n records' padded bounds (8 floats each: min
+ * t,x,y,z, max t,x,y,z): nodes (capacity 2n - 1) in preorder, the leaf order (n) and the node count;
+ * the CPU counterpart of specrel_build_bvh (include/specrel.h). */
+int specrel_build_bvh_host(const float *boxes, uint64_t n, specrel_node *nodes, uint32_t *order, uint32_t *n_nodes);
+/* specrel_builder_finish with a tree built elsewhere (specrel_build_bvh over the builder's records,
+ * in their order): the records are put in the tree's leaf order, nothing is rebuilt. */
+int specrel_builder_finish_with_tree(specrel_builder *b, const specrel_node *nodes, uint32_t n_nodes,
+                                     const uint32_t *order, specrel_host_scene **out);
 
 /* Tetrad operations (minkowski.rs:424-596); e is e[0..4] as (t,x,y,z). */
 void specrel_tetrad_rest(float e[16]);

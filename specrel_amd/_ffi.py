@@ -179,6 +179,10 @@ def _scene_protos(lib):
         "specrel_builder_set_tube_records": (C.c_int, [_P, C.POINTER(Geom), C.POINTER(Attr), _FP, C.c_uint64,
                                                        C.POINTER(C.c_uint64)]),
         "specrel_builder_extrude_pending": (C.c_int, [_P]),
+        "specrel_build_bvh_host": (C.c_int, [_FP, C.c_uint64, C.POINTER(Node), C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32)]),
+        "specrel_builder_finish_with_tree": (C.c_int, [_P, C.POINTER(Node), C.c_uint32, C.POINTER(C.c_uint32),
+                                                       C.POINTER(_P)]),
         "specrel_scene_file_write": (C.c_int, [C.c_char_p, C.POINTER(SceneArrays), C.POINTER(Object), _u32,
                                                C.c_uint64, C.c_uint64]),
         "specrel_scene_file_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P), C.POINTER(SceneArrays),

@@ -41,7 +41,7 @@ def hip_cmd(out=None):
     return [hipcc, f"--offload-arch={HIP_ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", *FP_FLAGS,
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-I" + INC,
             "-o", out, os.path.join(CSRC, "hip", "kernels.hip"), os.path.join(CSRC, "hip", "adjoint.hip"),
-            os.path.join(CSRC, "hip", "extrude.hip"),
+            os.path.join(CSRC, "hip", "extrude.hip"), os.path.join(CSRC, "hip", "bvh_build.hip"),
             os.path.join(CSRC, "hip", "capi.hip")]
 
 

@@ -1,0 +1,805 @@
+// Device BVH build emitting the host builder's tree bit for bit (Bvh4D::build, bvh4d.rs:122-427,
+// as restated in scene.cpp, sharing core/bvh_split.h for every split decision).
+//
+// Large segments (more than SMALL references) are processed breadth first, all segments of a
+// level together: bounds and centroid bounds per chunk of CHUNK references, merged per segment;
+// 8 bins per axis per chunk (one wave per axis), merged per segment; the split chosen by one
+// thread per segment. The partition must be the host's in-place swap partition (bvh4d.rs:409-416):
+// references going left keep their order; those going right end up in the order of a queue in
+// which each right-going reference is appended as it is met and each left-going reference met
+// after the first right-going one moves the queue's front to its back. The final queue is the last
+// F entries of the stream of appends, and each re-append copies an earlier stream entry, so the
+// stream is filled by pointer jumping (a few passes) after a scan of the left flags. Segments at
+// or below SMALL references are then built one per thread, sequentially, exactly as the host
+// does. Node indices are the host's preorder: subtree sizes are summed bottom-up over the large
+// nodes (few) on the host, the small subtrees are copied into place with their offsets.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../core/bvh_split.h"
+#include "bvh_build.hpp"
+#include "extrude_dev.hpp"
+
+namespace dev {
+namespace {
+
+constexpr uint32_t CHUNK = 4096;  // references per chunk of a large segment
+constexpr uint32_t SMALL = 256;   // subtrees at or below this size are built by one thread
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+enum { K_SAH = sr::SPLIT_SAH, K_MID = sr::SPLIT_MIDPOINT, K_LEAF = sr::SPLIT_LEAF, K_FALLBACK = 3 };
+enum { ERR_E = 1, ERR_JUMP = 2, ERR_SCATTER = 4, ERR_SMALL_STACK = 8, ERR_SMALL_NODES = 16, ERR_PLACE = 32 };
+
+struct Refs {
+    float4 *mn, *mx, *c;
+    uint32_t *idx;
+};
+struct Seg {
+    uint32_t begin, count, chunk_first, n_chunks;
+};
+struct Chunk {
+    uint32_t seg, begin, end, pad;
+};
+struct SegInfo {
+    float4 bmn, bmx, cmn, cmx;
+    float axis_min[4], inv_ext[4], ext[4];
+    uint32_t live;  // bit a: axis a has extent
+    int32_t kind, axis;
+    float pos;
+    uint32_t first_false, nT, pad[2];
+};
+struct BinPart {
+    float4 mn[32], mx[32];
+    uint32_t n[32];
+};
+struct SmallTask {
+    uint32_t begin, count;
+};
+struct Place {
+    uint32_t scratch, count, dest, pad;
+};
+struct PutNode {
+    uint32_t dest, pad[3];
+    specrel_node node;
+};
+
+__device__ __host__ __forceinline__ sr::V4 v(float4 f) { return sr::v4(f.x, f.y, f.z, f.w); }
+__device__ __host__ __forceinline__ float4 f4(sr::V4 a) { return make_float4(a.t, a.x, a.y, a.z); }
+__device__ __forceinline__ float comp(float4 f, int a) { return a == 0 ? f.x : a == 1 ? f.y : a == 2 ? f.z : f.w; }
+
+__device__ __forceinline__ sr::BinFrame frame_of(const SegInfo &s) {
+    sr::BinFrame f;
+    for (int a = 0; a < 4; ++a) {
+        f.live[a] = (s.live >> a) & 1u;
+        f.axis_min[a] = s.axis_min[a];
+        f.inv_ext[a] = s.inv_ext[a];
+        f.ext[a] = s.ext[a];
+    }
+    return f;
+}
+
+__global__ __launch_bounds__(256) void k_bb_init(const float4 *boxes, uint32_t n, Refs r) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = boxes[2 * (size_t)i], b = boxes[2 * (size_t)i + 1];
+    r.mn[i] = a;
+    r.mx[i] = b;
+    r.c[i] = f4((v(a) + v(b)) * 0.5f);  // BuildRef centroid (scene.cpp finish_into)
+    r.idx[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_bb_bounds(const Chunk *chunks, Refs r, float4 *part) {
+    const Chunk ch = chunks[blockIdx.x];
+    const uint32_t tid = threadIdx.x;
+    sr::Box4 b = sr::box_empty(), cb = sr::box_empty();
+    for (uint32_t i = ch.begin + tid; i < ch.end; i += 256) {
+        b = sr::box_union(b, sr::Box4{v(r.mn[i]), v(r.mx[i])});
+        sr::box_add_point(cb, v(r.c[i]));
+    }
+    __shared__ float4 s[4][256];
+    s[0][tid] = f4(b.mn);
+    s[1][tid] = f4(b.mx);
+    s[2][tid] = f4(cb.mn);
+    s[3][tid] = f4(cb.mx);
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            s[0][tid] = f4(sr::vmin(v(s[0][tid]), v(s[0][tid + w])));
+            s[1][tid] = f4(sr::vmax(v(s[1][tid]), v(s[1][tid + w])));
+            s[2][tid] = f4(sr::vmin(v(s[2][tid]), v(s[2][tid + w])));
+            s[3][tid] = f4(sr::vmax(v(s[3][tid]), v(s[3][tid + w])));
+        }
+        __syncthreads();
+    }
+    if (tid < 4) part[4 * (size_t)blockIdx.x + tid] = s[tid][0];
+}
+
+__global__ __launch_bounds__(256) void k_bb_frame(const Seg *segs, uint32_t nsegs, const float4 *part,
+                                                  SegInfo *info) {
+    const uint32_t si = blockIdx.x * 256 + threadIdx.x;
+    if (si >= nsegs) return;
+    const Seg sg = segs[si];
+    sr::Box4 b = sr::box_empty(), cb = sr::box_empty();
+    for (uint32_t c = sg.chunk_first; c < sg.chunk_first + sg.n_chunks; ++c) {
+        b = sr::box_union(b, sr::Box4{v(part[4 * (size_t)c]), v(part[4 * (size_t)c + 1])});
+        cb = sr::box_union(cb, sr::Box4{v(part[4 * (size_t)c + 2]), v(part[4 * (size_t)c + 3])});
+    }
+    SegInfo o;
+    o.bmn = f4(b.mn);
+    o.bmx = f4(b.mx);
+    o.cmn = f4(cb.mn);
+    o.cmx = f4(cb.mx);
+    const sr::BinFrame f = sr::bin_frame(cb);
+    o.live = 0;
+    for (int a = 0; a < 4; ++a) {
+        o.axis_min[a] = f.axis_min[a];
+        o.inv_ext[a] = f.inv_ext[a];
+        o.ext[a] = f.ext[a];
+        o.live |= (f.live[a] ? 1u : 0u) << a;
+    }
+    o.kind = -1;
+    o.axis = 0;
+    o.pos = 0.0f;
+    o.first_false = NONE;
+    o.nT = 0;
+    o.pad[0] = o.pad[1] = 0;
+    info[si] = o;
+}
+
+// Bins of one chunk: wave a bins axis a; each lane keeps 8 bins in registers, then the wave
+// reduces them (min / max / sum: the order does not matter).
+__global__ __launch_bounds__(256) void k_bb_bins(const Chunk *chunks, Refs r, const SegInfo *info, BinPart *bp) {
+    const Chunk ch = chunks[blockIdx.x];
+    const SegInfo &si = info[ch.seg];
+    const int a = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const float inf = __builtin_inff();
+    float4 bmn[8], bmx[8];
+    uint32_t bn[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        bmn[k] = make_float4(inf, inf, inf, inf);
+        bmx[k] = make_float4(-inf, -inf, -inf, -inf);
+        bn[k] = 0;
+    }
+    if ((si.live >> a) & 1u) {
+        const sr::BinFrame f = frame_of(si);
+        for (uint32_t i = ch.begin + lane; i < ch.end; i += 64) {
+            const uint32_t k = sr::bin_of(f, a, comp(r.c[i], a));
+            const float4 mn = r.mn[i], mx = r.mx[i];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (k == (uint32_t)q) {
+                    bmn[q] = f4(sr::vmin(v(bmn[q]), v(mn)));
+                    bmx[q] = f4(sr::vmax(v(bmx[q]), v(mx)));
+                    bn[q] += 1;
+                }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        for (int off = 32; off > 0; off >>= 1) {
+            float4 m = make_float4(__shfl_xor(bmn[q].x, off), __shfl_xor(bmn[q].y, off), __shfl_xor(bmn[q].z, off),
+                                   __shfl_xor(bmn[q].w, off));
+            float4 M = make_float4(__shfl_xor(bmx[q].x, off), __shfl_xor(bmx[q].y, off), __shfl_xor(bmx[q].z, off),
+                                   __shfl_xor(bmx[q].w, off));
+            bmn[q] = f4(sr::vmin(v(bmn[q]), v(m)));
+            bmx[q] = f4(sr::vmax(v(bmx[q]), v(M)));
+            bn[q] += __shfl_xor(bn[q], off);
+        }
+    }
+    if (lane == 0) {
+        BinPart &o = bp[blockIdx.x];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            o.mn[a * 8 + q] = bmn[q];
+            o.mx[a * 8 + q] = bmx[q];
+            o.n[a * 8 + q] = bn[q];
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bb_split(const Seg *segs, uint32_t nsegs, const BinPart *bp, SegInfo *info) {
+    const uint32_t si = blockIdx.x * 64 + threadIdx.x;
+    if (si >= nsegs) return;
+    const Seg sg = segs[si];
+    SegInfo &o = info[si];
+    sr::Bins B;
+    sr::bins_clear(B);
+    for (uint32_t c = sg.chunk_first; c < sg.chunk_first + sg.n_chunks; ++c)
+        for (int a = 0; a < 4; ++a)
+            for (int q = 0; q < 8; ++q) {
+                B.b[a][q] = sr::box_union(B.b[a][q], sr::Box4{v(bp[c].mn[a * 8 + q]), v(bp[c].mx[a * 8 + q])});
+                B.n[a][q] += bp[c].n[a * 8 + q];
+            }
+    int axis;
+    float pos;
+    const sr::Box4 cb{v(o.cmn), v(o.cmx)};
+    o.kind = sg.count <= 3 ? K_LEAF : sr::choose_split(cb, frame_of(o), B, axis, pos);
+    o.axis = o.kind == K_LEAF ? 0 : axis;
+    o.pos = o.kind == K_LEAF ? 0.0f : pos;
+}
+
+__global__ __launch_bounds__(256) void k_bb_flags(const Chunk *chunks, const Seg *segs, Refs r, SegInfo *info,
+                                                  uint32_t *flag) {
+    const Chunk ch = chunks[blockIdx.x];
+    SegInfo &si = info[ch.seg];
+    const uint32_t begin = segs[ch.seg].begin;
+    const bool part = si.kind == K_SAH || si.kind == K_MID;
+    const int a = si.axis;
+    const float pos = si.pos;
+    uint32_t first = NONE;
+    for (uint32_t i = ch.begin + threadIdx.x; i < ch.end; i += 256) {
+        const bool f = part && comp(r.c[i], a) < pos;
+        flag[i] = f ? 1u : 0u;
+        if (part && !f) first = min(first, i - begin);
+    }
+    if (first != NONE) atomicMin(&si.first_false, first);
+}
+
+// Exclusive prefix of v over the block (256 threads).
+__device__ __forceinline__ uint32_t block_excl(uint32_t x, uint32_t *s_wave, uint32_t &total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, v0 = x;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += s_wave[w];
+    total = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    return before + x - v0;
+}
+
+// Block-local exclusive scan of 1024 flags per block (4 per thread) and the block totals.
+__global__ __launch_bounds__(256) void k_bb_scan_local(const uint32_t *flag, uint32_t n, uint32_t *scanv,
+                                                       uint32_t *block_sums) {
+    const uint32_t base = blockIdx.x * 1024 + threadIdx.x * 4;
+    uint32_t f[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        f[k] = base + k < n ? flag[base + k] : 0u;
+        s += f[k];
+    }
+    __shared__ uint32_t s_wave[4];
+    uint32_t total;
+    uint32_t run = block_excl(s, s_wave, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) scanv[base + k] = run;
+        run += f[k];
+    }
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_bb_scan_add(uint32_t *scanv, uint32_t n, const uint64_t *offsets,
+                                                     uint32_t nblocks) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) scanv[i] += (uint32_t)offsets[i / 1024];
+    if (i == 0) scanv[n] = (uint32_t)offsets[nblocks];
+}
+
+__global__ __launch_bounds__(256) void k_bb_counts(const Seg *segs, uint32_t nsegs, const uint32_t *scanv,
+                                                   SegInfo *info) {
+    const uint32_t si = blockIdx.x * 256 + threadIdx.x;
+    if (si >= nsegs) return;
+    const Seg sg = segs[si];
+    SegInfo &o = info[si];
+    const uint32_t nT = scanv[sg.begin + sg.count] - scanv[sg.begin];
+    o.nT = nT;
+    if ((o.kind == K_SAH || o.kind == K_MID) && (nT == 0 || nT == sg.count)) o.kind = K_FALLBACK;
+}
+
+// The stream of queue appends of each partitioned segment, at [begin, begin + L):
+// right-going reference with m right-goers and R queue rotations before it -> entry m + R holds
+// its position; the j-th rotation, met after A right-goers -> entry A + j copies entry j.
+__global__ __launch_bounds__(256) void k_bb_stream(const Chunk *chunks, const Seg *segs, const SegInfo *info,
+                                                   const uint32_t *flag, const uint32_t *scanv, uint32_t *src,
+                                                   uint32_t *val, uint32_t *err) {
+    const Chunk ch = chunks[blockIdx.x];
+    const SegInfo &si = info[ch.seg];
+    if (si.kind != K_SAH && si.kind != K_MID) return;
+    const Seg sg = segs[ch.seg];
+    const uint32_t base = scanv[sg.begin], t0 = si.first_false, nT = si.nT;
+    const uint32_t L = (sg.count - nT) + (nT - t0);
+    for (uint32_t p = ch.begin + threadIdx.x; p < ch.end; p += 256) {
+        const uint32_t i = p - sg.begin, tb = scanv[p] - base, fb = i - tb;
+        uint32_t e, s, x;
+        if (!flag[p]) {
+            e = fb + (tb - t0);
+            s = NONE;
+            x = p;
+        } else if (fb > 0) {
+            e = fb + (tb - t0);
+            s = sg.begin + (tb - t0);
+            x = NONE;
+        } else {
+            continue;
+        }
+        if (e >= L || tb < t0) {
+            atomicOr(err, ERR_E);
+            continue;
+        }
+        src[sg.begin + e] = s;
+        val[sg.begin + e] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bb_jump(const Chunk *chunks, const Seg *segs, const SegInfo *info,
+                                                 uint32_t *src, uint32_t *val, uint32_t *unresolved, uint32_t *err) {
+    const Chunk ch = chunks[blockIdx.x];
+    const SegInfo &si = info[ch.seg];
+    if (si.kind != K_SAH && si.kind != K_MID) return;
+    const Seg sg = segs[ch.seg];
+    const uint32_t L = (sg.count - si.nT) + (si.nT - si.first_false);
+    uint32_t left = 0;
+    for (uint32_t p = ch.begin + threadIdx.x; p < ch.end && p - sg.begin < L; p += 256) {
+        if (val[p] != NONE) continue;
+        const uint32_t s = src[p];
+        if (s < sg.begin || s >= p) {
+            atomicOr(err, ERR_JUMP);
+            continue;
+        }
+        const uint32_t x = val[s];
+        if (x != NONE) {
+            val[p] = x;
+        } else {
+            src[p] = src[s];
+            ++left;
+        }
+    }
+    if (left) atomicAdd(unresolved, left);
+}
+
+__device__ __forceinline__ void move_ref(const Refs &from, uint32_t s, const Refs &to, uint32_t d) {
+    to.mn[d] = from.mn[s];
+    to.mx[d] = from.mx[s];
+    to.c[d] = from.c[s];
+    to.idx[d] = from.idx[s];
+}
+
+__global__ __launch_bounds__(256) void k_bb_scatter(const Chunk *chunks, const Seg *segs, const SegInfo *info,
+                                                    const uint32_t *flag, const uint32_t *scanv, const uint32_t *val,
+                                                    Refs cur, Refs nxt, Refs fin, uint32_t *err) {
+    const Chunk ch = chunks[blockIdx.x];
+    const SegInfo &si = info[ch.seg];
+    const Seg sg = segs[ch.seg];
+    if (si.kind == K_LEAF) {
+        for (uint32_t p = ch.begin + threadIdx.x; p < ch.end; p += 256) move_ref(cur, p, fin, p);
+        return;
+    }
+    if (si.kind != K_SAH && si.kind != K_MID) return;
+    const uint32_t base = scanv[sg.begin], nT = si.nT, t0 = si.first_false;
+    const uint32_t r = nT - t0, nF = sg.count - nT, end = sg.begin + sg.count;
+    const Refs &lo = nT <= SMALL ? fin : nxt, &hi = nF <= SMALL ? fin : nxt;
+    for (uint32_t p = ch.begin + threadIdx.x; p < ch.end; p += 256) {
+        const uint32_t i = p - sg.begin;
+        if (flag[p]) {
+            const uint32_t d = sg.begin + (scanv[p] - base);
+            if (d >= sg.begin + nT) atomicOr(err, ERR_SCATTER);
+            else move_ref(cur, p, lo, d);
+        }
+        if (i >= r && i < r + nF) {
+            const uint32_t q = val[p], d = sg.begin + nT + (i - r);
+            if (q < sg.begin || q >= end || d >= end) atomicOr(err, ERR_SCATTER);
+            else move_ref(cur, q, hi, d);
+        }
+    }
+}
+
+// Fallback partition of one segment: dest position begin + k takes the reference at begin + perm[k].
+__global__ __launch_bounds__(256) void k_bb_permute(uint32_t begin, uint32_t count, uint32_t mid,
+                                                    const uint32_t *perm, Refs cur, Refs nxt, Refs fin,
+                                                    uint32_t *err) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t s = perm[k];
+    if (s >= count) {
+        atomicOr(err, ERR_SCATTER);
+        return;
+    }
+    const bool small = k < mid ? mid <= SMALL : count - mid <= SMALL;
+    move_ref(cur, begin + s, small ? fin : nxt, begin + k);
+}
+
+// A subtree of at most SMALL references, built sequentially by one thread exactly as the host
+// builds it (scene.cpp split / build_recursive), in place over fin, nodes in preorder to
+// scratch[2 * begin ...] with subtree-local child indices.
+__global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_t ntasks, Refs f,
+                                                 specrel_node *scratch, uint32_t *task_nodes, uint32_t *err) {
+    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    if (t >= ntasks) return;
+    const SmallTask task = tasks[t];
+    specrel_node *out = scratch + 2 * (size_t)task.begin;
+    const uint32_t cap = 2 * task.count;
+    struct Ent {
+        uint32_t b, c, parent;
+    };
+    Ent stack[SMALL];
+    int sp = 0;
+    stack[sp++] = Ent{task.begin, task.count, NONE};
+    uint32_t nn = 0;
+    while (sp > 0) {
+        const Ent e = stack[--sp];
+        const uint32_t me = nn++;
+        if (me >= cap) {
+            atomicOr(err, ERR_SMALL_NODES);
+            return;
+        }
+        if (e.parent != NONE) out[e.parent].right_or_first = me;
+        sr::Box4 bounds = sr::box_empty(), cb = sr::box_empty();
+        for (uint32_t i = e.b; i < e.b + e.c; ++i) {
+            bounds = sr::box_union(bounds, sr::Box4{v(f.mn[i]), v(f.mx[i])});
+            sr::box_add_point(cb, v(f.c[i]));
+        }
+        specrel_node nd;
+        sr::put4(nd.min, bounds.mn);
+        sr::put4(nd.max, bounds.mx);
+        nd.pad = 0;
+        int kind = K_LEAF, axis = 0;
+        float pos = 0.0f;
+        if (e.c > 3) {
+            const sr::BinFrame fr = sr::bin_frame(cb);
+            sr::Bins B;
+            sr::bins_clear(B);
+            for (uint32_t i = e.b; i < e.b + e.c; ++i) {
+                const float4 c = f.c[i];
+                const sr::Box4 bx{v(f.mn[i]), v(f.mx[i])};
+                for (int a = 0; a < 4; ++a) {
+                    if (!fr.live[a]) continue;
+                    const uint32_t k = sr::bin_of(fr, a, comp(c, a));
+                    B.b[a][k] = sr::box_union(B.b[a][k], bx);
+                    B.n[a][k] += 1;
+                }
+            }
+            kind = sr::choose_split(cb, fr, B, axis, pos);
+        }
+        if (kind == K_LEAF) {
+            nd.right_or_first = e.b;
+            nd.count = e.c;
+            nd.left = 0;
+            out[me] = nd;
+            continue;
+        }
+        uint32_t mid = 0;
+        for (uint32_t i = e.b; i < e.b + e.c; ++i) {
+            if (comp(f.c[i], axis) < pos) {
+                const uint32_t j = e.b + mid;
+                const float4 a0 = f.mn[i], a1 = f.mx[i], a2 = f.c[i];
+                const uint32_t a3 = f.idx[i];
+                f.mn[i] = f.mn[j];
+                f.mx[i] = f.mx[j];
+                f.c[i] = f.c[j];
+                f.idx[i] = f.idx[j];
+                f.mn[j] = a0;
+                f.mx[j] = a1;
+                f.c[j] = a2;
+                f.idx[j] = a3;
+                mid += 1;
+            }
+        }
+        if (mid == 0 || mid == e.c) {
+            // stable insertion sort by the centroid on the axis
+            for (uint32_t i = e.b + 1; i < e.b + e.c; ++i) {
+                const float4 k0 = f.mn[i], k1 = f.mx[i], k2 = f.c[i];
+                const uint32_t k3 = f.idx[i];
+                const float key = comp(k2, axis);
+                uint32_t j = i;
+                while (j > e.b && key < comp(f.c[j - 1], axis)) {
+                    f.mn[j] = f.mn[j - 1];
+                    f.mx[j] = f.mx[j - 1];
+                    f.c[j] = f.c[j - 1];
+                    f.idx[j] = f.idx[j - 1];
+                    --j;
+                }
+                f.mn[j] = k0;
+                f.mx[j] = k1;
+                f.c[j] = k2;
+                f.idx[j] = k3;
+            }
+            mid = e.c / 2;
+        }
+        nd.right_or_first = 0;
+        nd.count = 0;
+        nd.left = me + 1;
+        out[me] = nd;
+        if (sp + 2 > (int)SMALL) {
+            atomicOr(err, ERR_SMALL_STACK);
+            return;
+        }
+        stack[sp++] = Ent{e.b + mid, e.c - mid, me};
+        stack[sp++] = Ent{e.b, mid, NONE};
+    }
+    task_nodes[t] = nn;
+}
+
+__global__ __launch_bounds__(256) void k_bb_place(const Place *places, uint32_t nplaces, const specrel_node *scratch,
+                                                  specrel_node *nodes, uint32_t total, uint32_t *err) {
+    const Place pl = places[blockIdx.x];
+    for (uint32_t j = threadIdx.x; j < pl.count; j += 256) {
+        specrel_node nd = scratch[pl.scratch + j];
+        if (nd.count == 0) {
+            nd.left += pl.dest;
+            nd.right_or_first += pl.dest;
+        }
+        if (pl.dest + j >= total) atomicOr(err, ERR_PLACE);
+        else nodes[pl.dest + j] = nd;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bb_put(const PutNode *puts, uint32_t n, specrel_node *nodes, uint32_t total,
+                                                uint32_t *err) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    if (puts[i].dest >= total) atomicOr(err, ERR_PLACE);
+    else nodes[puts[i].dest] = puts[i].node;
+}
+
+// Device allocations of one build, freed on every exit.
+struct Mem {
+    std::vector<void *> ptrs;
+    ~Mem() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    T *get(size_t count) {
+        void *p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            throw std::string("hipMalloc failed in the BVH build");
+        }
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+};
+
+void ok(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::string(what) + ": " + hipGetErrorString(e);
+}
+
+template <class T>
+void put(T *d, const std::vector<T> &h, hipStream_t s) {
+    if (!h.empty()) ok(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+struct BigNode {
+    int kind;  // 0 interior, 1 leaf (large), 2 small subtree
+    uint32_t left, right, first, count, task;
+    float4 mn, mx;
+};
+
+}  // namespace
+
+std::string build_bvh(const float4 *d_boxes, uint32_t n, specrel_node *d_nodes, uint32_t *d_order, uint32_t *n_nodes,
+                      hipStream_t s, BvhBuildStats *stats) {
+    BvhBuildStats st{};
+    auto t_start = std::chrono::steady_clock::now();
+    try {
+        if (n == 0) {
+            *n_nodes = 0;
+            return "";
+        }
+        if (n > 0x7FFFFFFFu) return "too many records for the device build";
+        Mem m;
+        Refs A{m.get<float4>(n), m.get<float4>(n), m.get<float4>(n), m.get<uint32_t>(n)};
+        Refs B{m.get<float4>(n), m.get<float4>(n), m.get<float4>(n), m.get<uint32_t>(n)};
+        Refs Fn{m.get<float4>(n), m.get<float4>(n), m.get<float4>(n), d_order};
+        uint32_t *flag = m.get<uint32_t>(n), *scanv = m.get<uint32_t>(n + 1);
+        uint32_t *src = m.get<uint32_t>(n), *val = m.get<uint32_t>(n);
+        const uint32_t nsb = (n + 1023) / 1024;
+        uint32_t *bsum = m.get<uint32_t>(nsb);
+        uint64_t *boff = m.get<uint64_t>(nsb + 1);
+        uint32_t *d_err = m.get<uint32_t>(2);  // [0] error bits, [1] unresolved count
+        specrel_node *scratch = m.get<specrel_node>(2 * (size_t)n);
+        ok(hipMemsetAsync(d_err, 0, 8, s), "clear");
+        k_bb_init<<<(n + 255) / 256, 256, 0, s>>>(d_boxes, n, A);
+        ok(hipGetLastError(), "k_bb_init");
+
+        std::vector<BigNode> nodes;  // breadth-first ids
+        std::vector<SmallTask> small;
+        struct Pending {
+            uint32_t id, begin, count;
+        };
+        std::vector<Pending> segs;
+        nodes.push_back(BigNode{});
+        if (n <= SMALL) {
+            nodes[0].kind = 2;
+            nodes[0].task = 0;
+            small.push_back(SmallTask{0, n});
+            ok(hipMemcpyAsync(Fn.mn, A.mn, (size_t)n * 16, hipMemcpyDeviceToDevice, s), "copy");
+            ok(hipMemcpyAsync(Fn.mx, A.mx, (size_t)n * 16, hipMemcpyDeviceToDevice, s), "copy");
+            ok(hipMemcpyAsync(Fn.c, A.c, (size_t)n * 16, hipMemcpyDeviceToDevice, s), "copy");
+            ok(hipMemcpyAsync(Fn.idx, A.idx, (size_t)n * 4, hipMemcpyDeviceToDevice, s), "copy");
+        } else {
+            segs.push_back(Pending{0, 0, n});
+        }
+        Refs cur = A, nxt = B;
+        // Per-level device tables, sized for the worst level.
+        const size_t max_chunks = n / CHUNK + n / SMALL + 2, max_segs = n / SMALL + 1;
+        Seg *d_segs = m.get<Seg>(max_segs);
+        Chunk *d_chunks = m.get<Chunk>(max_chunks);
+        float4 *d_part = m.get<float4>(4 * max_chunks);
+        BinPart *d_bins = m.get<BinPart>(max_chunks);
+        SegInfo *d_info = m.get<SegInfo>(max_segs);
+        uint32_t *d_perm = m.get<uint32_t>(n);
+        std::vector<Seg> hs;
+        std::vector<Chunk> hc;
+        std::vector<SegInfo> hi;
+        while (!segs.empty()) {
+            st.levels += 1;
+            st.large_segments += (uint32_t)segs.size();
+            hs.clear();
+            hc.clear();
+            for (uint32_t k = 0; k < segs.size(); ++k) {
+                const Pending &p = segs[k];
+                Seg sg{p.begin, p.count, (uint32_t)hc.size(), (p.count + CHUNK - 1) / CHUNK};
+                for (uint32_t c = 0; c < sg.n_chunks; ++c)
+                    hc.push_back(Chunk{k, p.begin + c * CHUNK, std::min(p.begin + p.count, p.begin + (c + 1) * CHUNK), 0});
+                hs.push_back(sg);
+            }
+            const uint32_t ns = (uint32_t)hs.size(), nc = (uint32_t)hc.size();
+            put(d_segs, hs, s);
+            put(d_chunks, hc, s);
+            k_bb_bounds<<<nc, 256, 0, s>>>(d_chunks, cur, d_part);
+            k_bb_frame<<<(ns + 255) / 256, 256, 0, s>>>(d_segs, ns, d_part, d_info);
+            k_bb_bins<<<nc, 256, 0, s>>>(d_chunks, cur, d_info, d_bins);
+            k_bb_split<<<(ns + 63) / 64, 64, 0, s>>>(d_segs, ns, d_bins, d_info);
+            k_bb_flags<<<nc, 256, 0, s>>>(d_chunks, d_segs, cur, d_info, flag);
+            k_bb_scan_local<<<nsb, 256, 0, s>>>(flag, n, scanv, bsum);
+            k_extrude_scan<<<1, 1024, 0, s>>>(bsum, nsb, boff);
+            k_bb_scan_add<<<(n + 255) / 256, 256, 0, s>>>(scanv, n, boff, nsb);
+            k_bb_counts<<<(ns + 255) / 256, 256, 0, s>>>(d_segs, ns, scanv, d_info);
+            k_bb_stream<<<nc, 256, 0, s>>>(d_chunks, d_segs, d_info, flag, scanv, src, val, d_err);
+            ok(hipGetLastError(), "level kernels");
+            // Pointer jumping until every stream entry is resolved.
+            for (int round = 0;; ++round) {
+                if (round >= 16) throw std::string("partition streams did not resolve");
+                for (int k = 0; k < 4; ++k) {
+                    if (k == 3) ok(hipMemsetAsync(d_err + 1, 0, 4, s), "clear");  // count the last pass only
+                    k_bb_jump<<<nc, 256, 0, s>>>(d_chunks, d_segs, d_info, src, val, d_err + 1, d_err);
+                    st.jump_passes += 1;
+                }
+                uint32_t left = 0;
+                ok(hipMemcpyAsync(&left, d_err + 1, 4, hipMemcpyDeviceToHost, s), "readback");
+                ok(hipStreamSynchronize(s), "jump");
+                if (left == 0) break;
+            }
+            k_bb_scatter<<<nc, 256, 0, s>>>(d_chunks, d_segs, d_info, flag, scanv, val, cur, nxt, Fn, d_err);
+            ok(hipGetLastError(), "k_bb_scatter");
+            hi.resize(ns);
+            ok(hipMemcpyAsync(hi.data(), d_info, ns * sizeof(SegInfo), hipMemcpyDeviceToHost, s), "readback");
+            ok(hipStreamSynchronize(s), "level");
+            std::vector<Pending> next;
+            for (uint32_t k = 0; k < ns; ++k) {
+                const Pending &p = segs[k];
+                const SegInfo &o = hi[k];
+                BigNode &bn = nodes[p.id];
+                bn.mn = o.bmn;
+                bn.mx = o.bmx;
+                if (o.kind == K_LEAF) {
+                    bn.kind = 1;
+                    bn.first = p.begin;
+                    bn.count = p.count;
+                    continue;
+                }
+                uint32_t mid = o.nT;
+                if (o.kind == K_FALLBACK) {
+                    // Stable sort by the centroid on the axis (scene.cpp split), on the host.
+                    st.fallback_segments += 1;
+                    std::vector<float4> c(p.count);
+                    ok(hipMemcpy(c.data(), cur.c + p.begin, p.count * sizeof(float4), hipMemcpyDeviceToHost), "fallback");
+                    std::vector<uint32_t> perm(p.count);
+                    std::iota(perm.begin(), perm.end(), 0u);
+                    const int ax = o.axis;
+                    auto key = [&](uint32_t i) { return sr::axis(v(c[i]), ax); };
+                    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) { return key(x) < key(y); });
+                    mid = p.count / 2;
+                    ok(hipMemcpyAsync(d_perm, perm.data(), p.count * 4, hipMemcpyHostToDevice, s), "fallback");
+                    k_bb_permute<<<(p.count + 255) / 256, 256, 0, s>>>(p.begin, p.count, mid, d_perm, cur, nxt, Fn,
+                                                                       d_err);
+                    ok(hipStreamSynchronize(s), "fallback");
+                }
+                bn.kind = 0;
+                bn.left = (uint32_t)nodes.size();
+                bn.right = bn.left + 1;
+                const uint32_t lb = p.begin, lc = mid, rb = p.begin + mid, rc = p.count - mid, lid = bn.left,
+                               rid = bn.right;
+                nodes.push_back(BigNode{});
+                nodes.push_back(BigNode{});
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t id = side ? rid : lid, b = side ? rb : lb, c = side ? rc : lc;
+                    if (c <= SMALL) {
+                        nodes[id].kind = 2;
+                        nodes[id].task = (uint32_t)small.size();
+                        small.push_back(SmallTask{b, c});
+                    } else {
+                        next.push_back(Pending{id, b, c});
+                    }
+                }
+            }
+            segs.swap(next);
+            std::swap(cur, nxt);
+        }
+        // Small subtrees.
+        const uint32_t nt = (uint32_t)small.size();
+        st.small_subtrees = nt;
+        SmallTask *d_tasks = m.get<SmallTask>(nt);
+        uint32_t *d_task_nodes = m.get<uint32_t>(nt);
+        put(d_tasks, small, s);
+        k_bb_small<<<(nt + 63) / 64, 64, 0, s>>>(d_tasks, nt, Fn, scratch, d_task_nodes, d_err);
+        ok(hipGetLastError(), "k_bb_small");
+        std::vector<uint32_t> tn(nt);
+        ok(hipMemcpyAsync(tn.data(), d_task_nodes, nt * 4, hipMemcpyDeviceToHost, s), "readback");
+        uint32_t herr[2] = {0, 0};
+        ok(hipMemcpyAsync(herr, d_err, 8, hipMemcpyDeviceToHost, s), "readback");
+        ok(hipStreamSynchronize(s), "small subtrees");
+        if (herr[0]) return "device BVH build consistency check failed (code " + std::to_string(herr[0]) + ")";
+        // Preorder numbering: children have larger breadth-first ids than their parents.
+        const size_t nb = nodes.size();
+        std::vector<uint64_t> size(nb), pre(nb);
+        for (size_t k = nb; k-- > 0;) {
+            const BigNode &b = nodes[k];
+            size[k] = b.kind == 0 ? 1 + size[b.left] + size[b.right] : b.kind == 1 ? 1 : tn[b.task];
+        }
+        const uint64_t total = size[0];
+        if (total > 2 * (uint64_t)n) return "device BVH build produced too many nodes";
+        pre[0] = 0;
+        std::vector<PutNode> puts;
+        std::vector<Place> places;
+        for (size_t k = 0; k < nb; ++k) {
+            const BigNode &b = nodes[k];
+            if (b.kind == 0) {
+                pre[b.left] = pre[k] + 1;
+                pre[b.right] = pre[k] + 1 + size[b.left];
+            }
+            if (b.kind == 2) {
+                places.push_back(Place{2 * small[b.task].begin, tn[b.task], (uint32_t)pre[k], 0});
+                continue;
+            }
+            PutNode pn;
+            std::memset(&pn, 0, sizeof pn);
+            pn.dest = (uint32_t)pre[k];
+            sr::put4(pn.node.min, v(b.mn));
+            sr::put4(pn.node.max, v(b.mx));
+            if (b.kind == 0) {
+                pn.node.left = (uint32_t)pre[b.left];
+                pn.node.right_or_first = (uint32_t)pre[b.right];
+                pn.node.count = 0;
+            } else {
+                pn.node.right_or_first = b.first;
+                pn.node.count = b.count;
+                pn.node.left = 0;
+            }
+            puts.push_back(pn);
+        }
+        PutNode *d_puts = m.get<PutNode>(puts.size());
+        Place *d_places = m.get<Place>(places.size());
+        put(d_puts, puts, s);
+        put(d_places, places, s);
+        if (!puts.empty())
+            k_bb_put<<<(uint32_t)((puts.size() + 255) / 256), 256, 0, s>>>(d_puts, (uint32_t)puts.size(), d_nodes,
+                                                                            (uint32_t)total, d_err);
+        if (!places.empty())
+            k_bb_place<<<(uint32_t)places.size(), 256, 0, s>>>(d_places, (uint32_t)places.size(), scratch, d_nodes,
+                                                               (uint32_t)total, d_err);
+        ok(hipGetLastError(), "placement");
+        ok(hipMemcpyAsync(herr, d_err, 8, hipMemcpyDeviceToHost, s), "readback");
+        ok(hipStreamSynchronize(s), "placement");
+        if (herr[0]) return "device BVH build consistency check failed (code " + std::to_string(herr[0]) + ")";
+        *n_nodes = (uint32_t)total;
+    } catch (const std::string &e) {
+        return e;
+    }
+    st.ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (stats) *stats = st;
+    return "";
+}
+
+}  // namespace dev

@@ -10,6 +10,7 @@
 
 #include "../core/scene_file.hpp"
 #include "../core/tables.h"
+#include "bvh_build.hpp"
 #include "extrude_dev.hpp"
 #include "kernels.hpp"
 #include "specrel.h"
@@ -1200,6 +1201,53 @@ int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t 
             if (attrs && total)
                 check(hipMemcpy(attrs, ctx->ex_attrs.p, total * sizeof(specrel_attr), hipMemcpyDeviceToHost), "attrs");
             if (boxes && total) check(hipMemcpy(boxes, ctx->ex_boxes.p, total * 32, hipMemcpyDeviceToHost), "boxes");
+        }
+        return SPECREL_OK;
+    });
+}
+
+int specrel_build_bvh(specrel_ctx *ctx, const float *boxes, uint64_t n, specrel_node *nodes, uint32_t *order,
+                      uint32_t *n_nodes, specrel_bvh_stats *stats) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!nodes || !order || !n_nodes) throw Fail{SPECREL_EINVAL, "nodes, order or n_nodes is NULL"};
+        if (n > 0x7FFFFFFFull) throw Fail{SPECREL_EINVAL, "too many records"};
+        hipStream_t s = ctx->stream;
+        const float4 *d_boxes;
+        DevBuf in;
+        if (boxes) {
+            dalloc(in, std::max<uint64_t>(1, n) * 32);
+            check(hipMemcpyAsync(in.p, boxes, n * 32, hipMemcpyHostToDevice, s), "boxes upload");
+            d_boxes = (const float4 *)in.p;
+        } else {
+            if (n > ctx->ex_boxes.bytes / 32) throw Fail{SPECREL_EINVAL, "fewer extruded records than n"};
+            d_boxes = (const float4 *)ctx->ex_boxes.p;
+        }
+        DevBuf d_nodes, d_order;
+        dalloc(d_nodes, std::max<uint64_t>(1, 2 * n) * sizeof(specrel_node));
+        dalloc(d_order, std::max<uint64_t>(1, n) * 4);
+        dev::BvhBuildStats st{};
+        uint32_t total = 0;
+        std::string err = dev::build_bvh(d_boxes, (uint32_t)n, (specrel_node *)d_nodes.p, (uint32_t *)d_order.p,
+                                         &total, s, &st);
+        if (!err.empty()) {
+            dfree(in), dfree(d_nodes), dfree(d_order);
+            throw Fail{SPECREL_EDEVICE, err};
+        }
+        if (total) check(hipMemcpy(nodes, d_nodes.p, (size_t)total * sizeof(specrel_node), hipMemcpyDeviceToHost),
+                         "nodes readback");
+        if (n) check(hipMemcpy(order, d_order.p, n * 4, hipMemcpyDeviceToHost), "order readback");
+        dfree(in), dfree(d_nodes), dfree(d_order);
+        *n_nodes = total;
+        if (stats) {
+            std::memset(stats, 0, sizeof *stats);
+            stats->ms = st.ms;
+            stats->levels = st.levels;
+            stats->large_segments = st.large_segments;
+            stats->small_subtrees = st.small_subtrees;
+            stats->fallback_segments = st.fallback_segments;
+            stats->jump_passes = st.jump_passes;
         }
         return SPECREL_OK;
     });

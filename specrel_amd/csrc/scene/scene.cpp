@@ -859,25 +859,25 @@ int guarded(F &&f) {
     }
 }
 
-void finish_into(specrel_builder &b, specrel_host_scene &s) {
-    size_t n = b.prims.size();
+// Bvh4D::build over the records' padded bounds: nodes in preorder and the leaf order.
+void build_tree(const Aabb *boxes, size_t n, std::vector<specrel_node> &out, std::vector<uint32_t> &order) {
     unsigned hw = std::thread::hardware_concurrency();
     g_thread_budget = (int)std::max(1u, std::min(hw ? hw : 1u, 64u));
     std::vector<Node> nodes;
-    std::vector<uint32_t> order(n);
+    order.assign(n, 0);
     if (n > 0) {
         std::vector<BuildRef> refs(n);
         for (size_t i = 0; i < n; ++i) {
-            const Aabb &bx = b.prims.boxes[i];
+            const Aabb &bx = boxes[i];
             refs[i] = BuildRef{bx, (bx.mn + bx.mx) * 0.5f, (uint32_t)i};
         }
         g_threads_live = 0;
         nodes = build_subtree(refs.data(), n, 0, 0);
         for (size_t i = 0; i < n; ++i) order[i] = refs[i].index;
     }
-    s.nodes.resize(nodes.size());
+    out.resize(nodes.size());
     for (size_t i = 0; i < nodes.size(); ++i) {
-        specrel_node &o = s.nodes[i];
+        specrel_node &o = out[i];
         put4(o.min, nodes[i].box.mn);
         put4(o.max, nodes[i].box.mx);
         o.right_or_first = nodes[i].right_or_first;
@@ -885,7 +885,21 @@ void finish_into(specrel_builder &b, specrel_host_scene &s) {
         o.left = nodes[i].left;
         o.pad = 0;
     }
+}
+
+void finish_rest(specrel_builder &b, specrel_host_scene &s);
+
+void finish_into(specrel_builder &b, specrel_host_scene &s) {
+    std::vector<uint32_t> order;
+    build_tree(b.prims.boxes.data(), b.prims.size(), s.nodes, order);
     s.order = order;
+    finish_rest(b, s);
+}
+
+// Everything after the tree: leaf-order records, materials, objects, packed stars, light.
+void finish_rest(specrel_builder &b, specrel_host_scene &s) {
+    size_t n = b.prims.size();
+    const std::vector<uint32_t> &order = s.order;
     s.geoms.resize(n);
     s.attrs.resize(n);
     for (size_t i = 0; i < n; ++i) {
@@ -1699,3 +1713,40 @@ const specrel_object *specrel_scene_file_objects(const specrel_scene_file *f, ui
 }
 
 void specrel_scene_file_close(specrel_scene_file *f) { delete f; }
+
+int specrel_build_bvh_host(const float *boxes, uint64_t n, specrel_node *nodes, uint32_t *order, uint32_t *n_nodes) {
+    return guarded([&] {
+        if ((n && !boxes) || !nodes || !order || !n_nodes) throw Error("boxes, nodes, order or n_nodes is NULL");
+        std::vector<Aabb> bx(n);
+        for (uint64_t i = 0; i < n; ++i)
+            bx[i] = Box4{v4(boxes[8 * i], boxes[8 * i + 1], boxes[8 * i + 2], boxes[8 * i + 3]),
+                         v4(boxes[8 * i + 4], boxes[8 * i + 5], boxes[8 * i + 6], boxes[8 * i + 7])};
+        std::vector<specrel_node> out;
+        std::vector<uint32_t> ord;
+        build_tree(bx.data(), n, out, ord);
+        std::memcpy(nodes, out.data(), out.size() * sizeof(specrel_node));
+        std::memcpy(order, ord.data(), ord.size() * 4);
+        *n_nodes = (uint32_t)out.size();
+        return 0;
+    });
+}
+
+int specrel_builder_finish_with_tree(specrel_builder *b, const specrel_node *nodes, uint32_t n_nodes,
+                                     const uint32_t *order, specrel_host_scene **out) {
+    return guarded([&] {
+        if (!b || !out || (n_nodes && (!nodes || !order))) throw Error("builder, tree or out is NULL");
+        if (!b->pending.empty()) throw Error("the builder has tubes waiting for their records");
+        size_t n = b->prims.size();
+        std::vector<uint8_t> seen(n, 0);
+        for (size_t i = 0; i < n; ++i) {
+            if (order[i] >= n || seen[order[i]]) throw Error("order is not a permutation of the records");
+            seen[order[i]] = 1;
+        }
+        auto s = std::make_unique<specrel_host_scene>();
+        s->nodes.assign(nodes, nodes + n_nodes);
+        s->order.assign(order, order + n);
+        finish_rest(*b, *s);
+        *out = s.release();
+        return 0;
+    });
+}

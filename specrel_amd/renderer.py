@@ -61,6 +61,15 @@ class Tuning(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
 
 
+class BvhStats(C.Structure):
+    _fields_ = [("ms", C.c_float)] + [(n, C.c_uint32) for n in ("levels", "large_segments", "small_subtrees",
+                                                                 "fallback_segments", "jump_passes")] + \
+        [("reserved", C.c_uint32 * 2)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
 class SpecrelError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"specrel error {code}: {msg}")
@@ -81,6 +90,7 @@ SIGNATURES = {
     "specrel_set_option": (C.c_int, [_P, C.c_int, C.c_int]),
     "specrel_upload_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays)]),
     "specrel_upload_scene_file": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_int]),
+    "specrel_build_bvh": (C.c_int, [_P, _FP, C.c_uint64, C.POINTER(F.Node), _U32P, _U32P, C.POINTER(BvhStats)]),
     "specrel_extrude_tubes": (C.c_int, [_P, C.POINTER(F.Tube), C.c_uint32, C.POINTER(F.Geom), C.POINTER(F.Attr),
                                         _FP, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
@@ -188,6 +198,23 @@ class Context:
             self._h, arr, len(tubes), g.ctypes.data_as(C.POINTER(F.Geom)), a.ctypes.data_as(C.POINTER(F.Attr)),
             b.ctypes.data_as(_FP), total, C.byref(n), cptr))
         return g, a, b, counts[:len(tubes)]
+
+    def build_bvh(self, boxes=None, n=None):
+        """specrel_build_bvh: (nodes (m,12) float32 view of GpuNode rows, order (n,) uint32, stats) for
+        boxes (n,8) float32, or for the last extrusion's records when boxes is None (n given)."""
+        if boxes is not None:
+            b = np.ascontiguousarray(boxes, np.float32)
+            n = len(b)
+            bp = b.ctypes.data_as(_FP)
+        else:
+            bp = None
+        nodes = np.zeros((max(1, 2 * n), 12), np.float32)
+        order = np.zeros(max(1, n), np.uint32)
+        m = C.c_uint32(0)
+        st = BvhStats()
+        self._check(self.lib.specrel_build_bvh(self._h, bp, n, nodes.ctypes.data_as(C.POINTER(F.Node)),
+                                               order.ctypes.data_as(_U32P), C.byref(m), C.byref(st)))
+        return nodes[:m.value], order[:n], st.as_dict()
 
     def patch(self, idx, scene, node_ranges=(), prim_ranges=()):
         """specrel_patch_scene: returns True when the ranges were applied, False when the library

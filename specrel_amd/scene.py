@@ -380,6 +380,17 @@ class DeferredScene:
             self._b, g.ctypes.data_as(C.POINTER(F.Geom)), a.ctypes.data_as(C.POINTER(F.Attr)), _fptr(b), len(g),
             c.ctypes.data_as(C.POINTER(C.c_uint64))))
 
+    def finish_with_tree(self, nodes, order) -> Scene:
+        """specrel_builder_finish_with_tree: the scene over a tree built elsewhere."""
+        nd = np.ascontiguousarray(nodes, np.float32)
+        od = np.ascontiguousarray(order, np.uint32)
+        h = C.c_void_p()
+        _check(F.scene_lib().specrel_builder_finish_with_tree(self._b, nd.ctypes.data_as(C.POINTER(F.Node)), len(nd),
+                                                              od.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(h)))
+        s = Scene(h.value, self.name)
+        s.params = self.params
+        return s
+
     def extrude_on_host(self):
         _check(F.scene_lib().specrel_builder_extrude_pending(self._b))
 
@@ -406,6 +417,18 @@ def extrude_tubes_host(tubes):
     _check(lib.specrel_extrude_tubes_host(arr, len(tubes), g.ctypes.data_as(C.POINTER(F.Geom)),
                                           a.ctypes.data_as(C.POINTER(F.Attr)), _fptr(b), total, C.byref(n), cptr))
     return g, a, b, counts[:len(tubes)]
+
+
+def build_bvh_host(boxes):
+    """specrel_build_bvh_host: (nodes (m,12) float32 rows of GpuNode, order (n,) uint32)."""
+    b = np.ascontiguousarray(boxes, np.float32)
+    n = len(b)
+    nodes = np.zeros((max(1, 2 * n), 12), np.float32)
+    order = np.zeros(max(1, n), np.uint32)
+    m = C.c_uint32(0)
+    _check(F.scene_lib().specrel_build_bvh_host(_fptr(b), n, nodes.ctypes.data_as(C.POINTER(F.Node)),
+                                                order.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(m)))
+    return nodes[:m.value], order[:n]
 
 
 def worldline(kind, params, cap=4096):

@@ -118,3 +118,62 @@ def test_rindler_horizon_and_bad_indices_are_refused():
                    S.worldline(F.WL_INERTIAL, [0, 0, 0, 0, 0, 0, 0, 1, 2]))
     with pytest.raises(RuntimeError, match="out of range"):
         S.extrude_tubes_host([bad.t])
+
+
+@pytest.mark.parametrize("name,over", SCENES)
+def test_host_tree_api_and_finish_with_tree(name, over):
+    # specrel_build_bvh_host over the records' bounds gives the scene's own tree, and a scene
+    # finished with that tree equals the generated one
+    p = params(name, over)
+    direct = S.Scene.generate(name, p)
+    d = S.DeferredScene(name, p)
+    g, a, b, counts = S.extrude_tubes_host(d.tubes())
+    nodes, order = S.build_bvh_host(b)
+    assert nodes.tobytes() == direct.nodes().tobytes() and order.tobytes() == direct.order().tobytes()
+    d.set_records(g, a, b, counts)
+    assert arrays(d.finish_with_tree(nodes, order)) == arrays(direct)
+
+
+def test_finish_with_tree_checks_the_order():
+    d = S.DeferredScene("cube")
+    g, a, b, counts = S.extrude_tubes_host(d.tubes())
+    nodes, order = S.build_bvh_host(b)
+    d.set_records(g, a, b, counts)
+    bad = order.copy()
+    bad[0] = bad[1]
+    with pytest.raises(RuntimeError, match="permutation"):
+        d.finish_with_tree(nodes, bad)
+
+
+def synthetic_boxes(seed, n, kind):
+    """Bounds that stress the build's corner cases: clustered centroids with tiny extents
+    (midpoint splits and large leaves of equal centroids), or uniform random boxes."""
+    rng = np.random.default_rng(seed)
+    if kind == "uniform":
+        c = rng.uniform(-50, 50, (n, 4)).astype(np.float32)
+        h = rng.uniform(0.01, 2, (n, 4)).astype(np.float32)
+    else:
+        base = rng.uniform(-5, 5, (8, 4)).astype(np.float32)
+        c = base[rng.integers(0, 8, n)]
+        jitter = rng.choice([0.0, 2e-5, 6e-5], size=(n, 4)).astype(np.float32)
+        c = (c + jitter).astype(np.float32)
+        h = rng.uniform(0.01, 0.5, (n, 4)).astype(np.float32)
+    return np.concatenate([c - h, c + h], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", ["uniform", "clustered"])
+def test_host_tree_of_synthetic_bounds_is_a_valid_bvh(kind):
+    b = synthetic_boxes(3, 5000, kind)
+    nodes, order = S.build_bvh_host(b)
+    assert sorted(order.tolist()) == list(range(len(b)))
+    u = nodes.view(np.uint32)
+    # every record lies in its leaf's bounds; leaves tile the leaf slots in order
+    covered = 0
+    for k in range(len(nodes)):
+        if u[k, 9] > 0:
+            first, cnt = int(u[k, 8]), int(u[k, 9])
+            assert first == covered
+            covered += cnt
+            bx = b[order[first:first + cnt]]
+            assert (nodes[k, 0:4] <= bx[:, 0:4]).all() and (bx[:, 4:8] <= nodes[k, 4:8]).all()
+    assert covered == len(b)
