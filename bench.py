@@ -230,6 +230,19 @@ def bench_extrude(ctx, p, steps):
         np.array_equal(got[3], ref[3])
     n = int(counts.sum())
     out_bytes = n * (80 + 192 + 32)
+    del got
+    # BVH over the extruded records: on the device (the records never leave it) and on the host.
+    ctx.extrude_tubes(tubes, download=False)
+    t0 = time.perf_counter()
+    dn, do, dst = ctx.build_bvh(None, n)
+    dev_build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hn, ho = S.build_bvh_host(ref[2])
+    host_build_s = time.perf_counter() - t0
+    tree_same = np.array_equal(dn.view(np.uint32), hn.view(np.uint32)) and np.array_equal(do, ho)
+    bvh = {"device_ms": round(dev_build_s * 1e3, 1), "device_stats": dst, "host_ms": round(host_build_s * 1e3, 1),
+           "host_cores": cores, "speedup": round(host_build_s / dev_build_s, 1), "nodes": int(len(dn)),
+           "bit_identical_to_host": bool(tree_same)}
     return {"tubes": len(tubes), "tets": n, "kernel_ms": round(kernel_ms, 3),
             "Mtets_per_s": round(n / kernel_ms / 1e3, 1),
             "achieved_GBs": round(out_bytes / (kernel_ms * 1e-3) / 1e9, 1),
@@ -237,7 +250,7 @@ def bench_extrude(ctx, p, steps):
             "e2e_ms_with_download": round(e2e_s * 1e3, 1),
             "host_ms": round(host_s * 1e3, 1), "host_cores": cores,
             "speedup_kernel_vs_host": round(host_s * 1e3 / kernel_ms, 1),
-            "bit_identical_to_host": bool(same),
+            "bit_identical_to_host": bool(same), "bvh_build": bvh,
             "workload": "the large scene's worldtubes (icospheres, 320 triangles, inertial and accelerated), "
                         "every prism of every slice; outputs GpuGeom + GpuAttr + padded bounds"}
 
