@@ -333,6 +333,8 @@ def main():
     ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
     ap.add_argument("--prefetch", type=int, default=-1, help="1/0: child prefetch in the primary trace (-1: default)")
     ap.add_argument("--star-overlap", type=int, default=-1, help="1/0: star pass beside the primary rays (-1: default)")
+    ap.add_argument("--wave-shape", type=int, default=-1,
+                    help="pixels a primary-ray wave takes: 0 8x8, 1 16x4, 2 4x16 (-1: default)")
     ap.add_argument("--trace-waves", type=int, default=0, help="1, 2, 4: waves per primary-ray block (0: default)")
     ap.add_argument("--shadow-trace", type=int, default=-1,
                     help="1/0: shadow queries traced by the primary-ray kernel (-1: default)")
@@ -350,7 +352,8 @@ def main():
     import numpy as np
     from specrel_amd import scene as S
     from specrel_amd.renderer import (OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
-                                      OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, SHADOWS_IN_TRACE,
+                                      OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE,
+                                      SHADOWS_IN_TRACE,
                                       Context)
 
     w, h = args.width, args.height
@@ -378,6 +381,8 @@ def main():
         ctx.set_option(OPT_TRACE_WAVES, args.trace_waves)
     if args.star_overlap >= 0:
         ctx.set_option(OPT_STAR_OVERLAP, args.star_overlap)
+    if args.wave_shape >= 0:
+        ctx.set_option(OPT_WAVE_SHAPE, args.wave_shape)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")

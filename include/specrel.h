@@ -73,8 +73,10 @@ enum {
                                        0: by the shading pass */
     SPECREL_OPT_TRACE_WAVES = 9, /* 1, 2 or 4 (default): waves per block of the primary-ray kernel with
                                     shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
-    SPECREL_OPT_STAR_OVERLAP = 10 /* 1: the star pass runs on a second stream beside the primary
+    SPECREL_OPT_STAR_OVERLAP = 10, /* 1: the star pass runs on a second stream beside the primary
                                      rays; 0 (default): before them on the same stream */
+    SPECREL_OPT_WAVE_SHAPE = 11   /* pixels of a 16x16 tile one wave of the tiled primary-ray kernel
+                                     takes: 0 (default) an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are
@@ -96,7 +98,8 @@ typedef struct specrel_tuning {
     float best_ms;        /* median wall time per render of the batch with the choice */
     float default_ms;     /* the same with the options the context had before */
     int32_t shadow_in_trace; /* SPECREL_OPT_SHADOW_IN_TRACE */
-    uint32_t reserved[4];
+    int32_t wave_shape;      /* SPECREL_OPT_WAVE_SHAPE */
+    uint32_t reserved[3];
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */

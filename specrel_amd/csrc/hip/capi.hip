@@ -75,6 +75,7 @@ struct specrel_ctx {
     int opt_prefetch = 0;
     int opt_shadow_trace = 1;
     int opt_trace_waves = 4;
+    int opt_wave_shape = 0;
     int opt_star_overlap = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
@@ -614,6 +615,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.mask = in_trace ? (uint32_t *)ctx->mask.p : nullptr;
     ta.queue = counters + 0;
     ta.stats = (unsigned long long *)ctx->stats.p;
+    ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(2);
@@ -697,7 +699,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
                                      (uint64_t)ctx->opt_prefetch,       (uint64_t)ctx->opt_trace_waves,
-                                     (uint64_t)ctx->opt_star_overlap,
+                                     (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -1001,6 +1003,12 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
             return SPECREL_EINVAL;
         }
         ctx->opt_trace_waves = value;
+    } else if (option == SPECREL_OPT_WAVE_SHAPE) {
+        if (value < 0 || value > 2) {
+            ctx->err = "wave shape must be 0 (8x8), 1 (16x4) or 2 (4x16)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_wave_shape = value;
     }
     else if (option == SPECREL_OPT_GRAPH) {
         if (value < 0 || value > 2) {
@@ -1392,21 +1400,24 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph, in_trace;
+            int stack, persistent, queue, graph, in_trace, shape;
         };
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
                 for (int sh : {0, 1, 2})  // shadows in the shading pass, queued, in the trace kernel
-                    for (int g : {0, 2}) cands.push_back({st, pe, sh == 1, g, sh == 2});
-        const Cand saved{ctx->opt_stack, ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
-                         ctx->opt_shadow_trace};
+                    for (int g : {0, 2})
+                        for (int ws : {0, 1, 2})  // a wave's pixels: 8x8, 16x4, 4x16 (tiled primary rays only)
+                            if (ws == 0 || !pe) cands.push_back({st, pe, sh == 1, g, sh == 2, ws});
+        const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
+                         ctx->opt_shadow_trace, ctx->opt_wave_shape};
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
             ctx->opt_shadow_queue = c.queue;
             ctx->opt_graph = c.graph;
             ctx->opt_shadow_trace = c.in_trace;
+            ctx->opt_wave_shape = c.shape;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -1447,6 +1458,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->shadow_queue = best.queue;
             out->graph = best.graph;
             out->shadow_in_trace = best.in_trace;
+            out->wave_shape = best.shape;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

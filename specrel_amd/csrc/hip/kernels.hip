@@ -158,13 +158,23 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
     using SS = StackSel<SK, 64 * W>;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint32_t quarter = blockIdx.x * W + (tid >> 6);  // 8x8 quarters, four per tile
+    const uint32_t quarter = blockIdx.x * W + (tid >> 6);  // a wave's quarter of a tile, four per tile
     const uint32_t t = quarter >> 2, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
     const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE, q = quarter & 3;
-    const uint32_t px = (lt % tiles_x) * TILE + (q & 1) * 8 + (lane & 7);
-    const uint32_t py = (lt / tiles_x) * TILE + (q >> 1) * 8 + (lane >> 3);
+    const uint32_t tx = (lt % tiles_x) * TILE, ty = (lt / tiles_x) * TILE;
+    uint32_t px, py;
+    if (a.wave_shape == 1) {  // 16x4 bands
+        px = tx + (lane & 15);
+        py = ty + q * 4 + (lane >> 4);
+    } else if (a.wave_shape == 2) {  // 4x16 bands
+        px = tx + q * 4 + (lane & 3);
+        py = ty + (lane >> 2);
+    } else {  // 8x8 quarters
+        px = tx + (q & 1) * 8 + (lane & 7);
+        py = ty + (q >> 1) * 8 + (lane >> 3);
+    }
     __shared__ uint2 s_stack[SS::lds_entries];
     typename SS::type stk;
     typename SS::Private stk_priv;

@@ -122,6 +122,7 @@ struct TraceArgs {
     uint32_t *mask;   // with shadows traced here: per-pixel occlusion bits (bit 0 light, 1 + i flash i)
     uint32_t *queue;  // persistent variant: ray-queue counter (zeroed per frame)
     unsigned long long *stats;
+    uint32_t wave_shape;  // pixels of a tile one wave takes: 0 an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band
 };
 
 struct OccludeArgs {

@@ -196,3 +196,30 @@ def test_counters_with_shadows_in_the_shading_pass(gpu_ctx, twin):
     oc = O.render(twin, fr, counts=True)["counts"]
     for k in ("shadow_rays", "shadow_node_pops", "shadow_box_tests", "shadow_prim_tests"):
         assert st[k] == oc[k], (k, st[k], oc[k])
+
+
+@pytest.mark.parametrize("shape,waves", [(1, 4), (2, 4), (1, 1), (2, 2)])
+def test_wave_shapes_are_exact(gpu_ctx, twin, shape, waves):
+    # A wave takes a 16x4 or 4x16 band of its tile instead of an 8x8 quarter: only which pixels
+    # share a wave changes, so images and counters are the oracle's.
+    from specrel_amd.renderer import OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.1, -0.2, 0.3))
+    p = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((p[0] - 6.0, p[1], p[2], p[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    fr = S.build_frame(cam, S.RenderConfig(200, 124), dyn)
+    gpu_ctx.set_option(OPT_WAVE_SHAPE, shape)
+    gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+    try:
+        g, o = render_both(gpu_ctx, twin, fr)
+        gpu_ctx.set_option(OPT_STATS, 1)
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 4)
+    assert_same(g, o, f"wave shape {shape}, {waves} waves per block")
+    oc = O.render(twin, fr, counts=True)["counts"]
+    for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
+        assert st[k] == oc[k], (k, st[k], oc[k])
