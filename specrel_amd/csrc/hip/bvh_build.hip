@@ -31,6 +31,7 @@ namespace {
 
 constexpr uint32_t CHUNK = 4096;  // references per chunk of a large segment
 constexpr uint32_t SMALL = 256;   // subtrees at or below this size are built by one thread
+constexpr uint32_t MERGE = 8;     // segments of more chunks have their bins merged by a block first
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 enum { K_SAH = sr::SPLIT_SAH, K_MID = sr::SPLIT_MIDPOINT, K_LEAF = sr::SPLIT_LEAF, K_FALLBACK = 3 };
 enum { ERR_E = 1, ERR_JUMP = 2, ERR_SCATTER = 4, ERR_SMALL_STACK = 8, ERR_SMALL_NODES = 16, ERR_PLACE = 32 };
@@ -204,6 +205,39 @@ __global__ __launch_bounds__(256) void k_bb_bins(const Chunk *chunks, Refs r, co
     }
 }
 
+// Bins of a segment of more than MERGE chunks, merged into its first chunk's partial by one block:
+// lane q of each 32-lane group takes bin q of every 8th chunk, then the 8 groups are merged.
+// Unions are min / max and counts sums, so the order changes nothing.
+__global__ __launch_bounds__(256) void k_bb_merge(const uint32_t *big, const Seg *segs, BinPart *bp) {
+    const Seg sg = segs[big[blockIdx.x]];
+    const uint32_t q = threadIdx.x & 31, g = threadIdx.x >> 5;
+    const float inf = __builtin_inff();
+    float4 mn = make_float4(inf, inf, inf, inf), mx = make_float4(-inf, -inf, -inf, -inf);
+    uint32_t cnt = 0;
+    for (uint32_t c = sg.chunk_first + g; c < sg.chunk_first + sg.n_chunks; c += 8) {
+        mn = f4(sr::vmin(v(mn), v(bp[c].mn[q])));
+        mx = f4(sr::vmax(v(mx), v(bp[c].mx[q])));
+        cnt += bp[c].n[q];
+    }
+    __shared__ float4 smn[8][32], smx[8][32];
+    __shared__ uint32_t sn[8][32];
+    smn[g][q] = mn;
+    smx[g][q] = mx;
+    sn[g][q] = cnt;
+    __syncthreads();
+    if (g == 0) {
+        for (int k = 1; k < 8; ++k) {
+            mn = f4(sr::vmin(v(mn), v(smn[k][q])));
+            mx = f4(sr::vmax(v(mx), v(smx[k][q])));
+            cnt += sn[k][q];
+        }
+        BinPart &o = bp[sg.chunk_first];
+        o.mn[q] = mn;
+        o.mx[q] = mx;
+        o.n[q] = cnt;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_bb_split(const Seg *segs, uint32_t nsegs, const BinPart *bp, SegInfo *info) {
     const uint32_t si = blockIdx.x * 64 + threadIdx.x;
     if (si >= nsegs) return;
@@ -211,7 +245,8 @@ __global__ __launch_bounds__(64) void k_bb_split(const Seg *segs, uint32_t nsegs
     SegInfo &o = info[si];
     sr::Bins B;
     sr::bins_clear(B);
-    for (uint32_t c = sg.chunk_first; c < sg.chunk_first + sg.n_chunks; ++c)
+    const uint32_t nch = sg.n_chunks > MERGE ? 1 : sg.n_chunks;  // merged by k_bb_merge
+    for (uint32_t c = sg.chunk_first; c < sg.chunk_first + nch; ++c)
         for (int a = 0; a < 4; ++a)
             for (int q = 0; q < 8; ++q) {
                 B.b[a][q] = sr::box_union(B.b[a][q], sr::Box4{v(bp[c].mn[a * 8 + q]), v(bp[c].mx[a * 8 + q])});
@@ -422,7 +457,7 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
     struct Ent {
         uint32_t b, c, parent;
     };
-    Ent stack[SMALL];
+    Ent stack[SMALL + 2];  // pending right halves along a path: at most count + 1
     int sp = 0;
     stack[sp++] = Ent{task.begin, task.count, NONE};
     uint32_t nn = 0;
@@ -510,7 +545,7 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
         nd.count = 0;
         nd.left = me + 1;
         out[me] = nd;
-        if (sp + 2 > (int)SMALL) {
+        if (sp + 2 > (int)SMALL + 2) {
             atomicOr(err, ERR_SMALL_STACK);
             return;
         }
@@ -629,17 +664,21 @@ std::string build_bvh(const float4 *d_boxes, uint32_t n, specrel_node *d_nodes, 
         BinPart *d_bins = m.get<BinPart>(max_chunks);
         SegInfo *d_info = m.get<SegInfo>(max_segs);
         uint32_t *d_perm = m.get<uint32_t>(n);
+        uint32_t *d_big = m.get<uint32_t>(max_segs);
         std::vector<Seg> hs;
         std::vector<Chunk> hc;
         std::vector<SegInfo> hi;
+        std::vector<uint32_t> hbig;
         while (!segs.empty()) {
             st.levels += 1;
             st.large_segments += (uint32_t)segs.size();
             hs.clear();
             hc.clear();
+            hbig.clear();
             for (uint32_t k = 0; k < segs.size(); ++k) {
                 const Pending &p = segs[k];
                 Seg sg{p.begin, p.count, (uint32_t)hc.size(), (p.count + CHUNK - 1) / CHUNK};
+                if (sg.n_chunks > MERGE) hbig.push_back(k);
                 for (uint32_t c = 0; c < sg.n_chunks; ++c)
                     hc.push_back(Chunk{k, p.begin + c * CHUNK, std::min(p.begin + p.count, p.begin + (c + 1) * CHUNK), 0});
                 hs.push_back(sg);
@@ -650,6 +689,10 @@ std::string build_bvh(const float4 *d_boxes, uint32_t n, specrel_node *d_nodes, 
             k_bb_bounds<<<nc, 256, 0, s>>>(d_chunks, cur, d_part);
             k_bb_frame<<<(ns + 255) / 256, 256, 0, s>>>(d_segs, ns, d_part, d_info);
             k_bb_bins<<<nc, 256, 0, s>>>(d_chunks, cur, d_info, d_bins);
+            if (!hbig.empty()) {
+                put(d_big, hbig, s);
+                k_bb_merge<<<(uint32_t)hbig.size(), 256, 0, s>>>(d_big, d_segs, d_bins);
+            }
             k_bb_split<<<(ns + 63) / 64, 64, 0, s>>>(d_segs, ns, d_bins, d_info);
             k_bb_flags<<<nc, 256, 0, s>>>(d_chunks, d_segs, cur, d_info, flag);
             k_bb_scan_local<<<nsb, 256, 0, s>>>(flag, n, scanv, bsum);
@@ -733,8 +776,10 @@ std::string build_bvh(const float4 *d_boxes, uint32_t n, specrel_node *d_nodes, 
         SmallTask *d_tasks = m.get<SmallTask>(nt);
         uint32_t *d_task_nodes = m.get<uint32_t>(nt);
         put(d_tasks, small, s);
-        k_bb_small<<<(nt + 63) / 64, 64, 0, s>>>(d_tasks, nt, Fn, scratch, d_task_nodes, d_err);
-        ok(hipGetLastError(), "k_bb_small");
+        if (nt) {  // every segment may have ended as a leaf of the level phase
+            k_bb_small<<<(nt + 63) / 64, 64, 0, s>>>(d_tasks, nt, Fn, scratch, d_task_nodes, d_err);
+            ok(hipGetLastError(), "k_bb_small");
+        }
         std::vector<uint32_t> tn(nt);
         ok(hipMemcpyAsync(tn.data(), d_task_nodes, nt * 4, hipMemcpyDeviceToHost, s), "readback");
         uint32_t herr[2] = {0, 0};

@@ -49,7 +49,7 @@ def test_synthetic_bounds(gpu_ctx, kind, n):
     assert_same_tree(gpu_ctx.build_bvh(b), S.build_bvh_host(b), f"{kind} {n}")
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 4, 255, 256, 257, 5000])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 31, 32, 33, 255, 256, 257, 5000, 40000])
 def test_small_sizes(gpu_ctx, n):
     b = synthetic_boxes(5, n, "uniform")
     assert_same_tree(gpu_ctx.build_bvh(b), S.build_bvh_host(b), f"n={n}")
