@@ -90,30 +90,52 @@ __device__ __forceinline__ float safe_inv(float v) {
     return v >= 0.0f ? 1e12f : -1e12f;
 }
 
-// SimplexGeom4D::intersect_fast (mesh4d.rs:175-211)
+// SimplexGeom4D::intersect_fast (mesh4d.rs:175-211), in two parts: the values (everything but the
+// t range test), and the reference's early-out conditions on them, combined without short-circuit.
+// A record's five vectors are loaded at once and every value is computed before the conditions are
+// tested, so a test costs one round trip to memory instead of one per vector; the arithmetic, and
+// with it every accepted hit, is the reference's.
+struct GeomRec {
+    float4 v0, n, b1, b2, b3;
+};
+__device__ __forceinline__ GeomRec load_geom(const float4 *g) { return GeomRec{g[0], g[1], g[2], g[3], g[4]}; }
+struct Isect {
+    float denom, lambda, u, v, w;
+    V4 event;
+};
+__device__ __forceinline__ Isect isect_values(const GeomRec &g, V4 o, V4 d) {
+    Isect x;
+    V4 n = f4v(g.n);
+    x.denom = edot(n, d);
+    V4 v0 = f4v(g.v0);
+    x.lambda = edot(n, v0 - o) / x.denom;
+    x.event = o + d * x.lambda;
+    V4 q = x.event - v0;
+    x.u = edot(f4v(g.b1), q);
+    x.v = edot(f4v(g.b2), q);
+    x.w = edot(f4v(g.b3), q);
+    return x;
+}
+__device__ __forceinline__ bool isect_accept(const Isect &x, float tmin, float tmax) {
+    const float eps = -1e-5f;
+    const bool reject = (fabsf(x.denom) < 1e-9f) | (x.lambda <= tmin) | (x.lambda >= tmax) | (x.u < eps) |
+                        (x.u > 1.0f - eps) | (x.v < eps) | (x.u + x.v > 1.0f - eps) | (x.w < eps) |
+                        (x.u + x.v + x.w > 1.0f - eps);
+    return !reject;
+}
+__device__ __forceinline__ void isect_bary(const Isect &x, float bary[4]) {
+    bary[0] = 1.0f - x.u - x.v - x.w;
+    bary[1] = x.u;
+    bary[2] = x.v;
+    bary[3] = x.w;
+}
 __device__ __forceinline__ bool intersect(const float4 *g, V4 o, V4 d, float tmin, float tmax, float &lam,
                                           V4 &ev, float bary[4]) {
-    V4 n = f4v(g[1]);
-    float denom = edot(n, d);
-    if (fabsf(denom) < 1e-9f) return false;
-    V4 v0 = f4v(g[0]);
-    float lambda = edot(n, v0 - o) / denom;
-    if (lambda <= tmin || lambda >= tmax) return false;
-    V4 event = o + d * lambda;
-    V4 q = event - v0;
-    const float eps = -1e-5f;
-    float u = edot(f4v(g[2]), q);
-    if (u < eps || u > 1.0f - eps) return false;
-    float v = edot(f4v(g[3]), q);
-    if (v < eps || u + v > 1.0f - eps) return false;
-    float w = edot(f4v(g[4]), q);
-    if (w < eps || u + v + w > 1.0f - eps) return false;
-    lam = lambda;
-    ev = event;
-    bary[0] = 1.0f - u - v - w;
-    bary[1] = u;
-    bary[2] = v;
-    bary[3] = w;
+    const Isect x = isect_values(load_geom(g), o, d);
+    if (!isect_accept(x, tmin, tmax)) return false;
+    lam = x.lambda;
+    ev = x.event;
+    isect_bary(x, bary);
     return true;
 }
 
@@ -260,19 +282,26 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
         if (e & LEAF) {
             uint32_t first, count;
             leaf_range(S, e, first, count);
-            for (uint32_t i = first; i < first + count; ++i) {
-                float lam, b[4];
-                V4 ev;
-                if (STATS) C.prim_tests++;
-                if (intersect(S.geoms + 5 * (size_t)i, o, d, tmin, tmax, lam, ev, b)) {
-                    tmax = lam;
-                    hit.slot = (int32_t)i;
-                    hit.lambda = lam;
-                    hit.event = ev;
-                    hit.bary[0] = b[0];
-                    hit.bary[1] = b[1];
-                    hit.bary[2] = b[2];
-                    hit.bary[3] = b[3];
+            // Up to three primitives per round of loads; they are tested in slot order, each with
+            // the t_max the previous ones left (slots past the leaf repeat its last one, untested).
+            const uint32_t end = first + count;
+            for (uint32_t i = first; i < end; i += 3) {
+                const uint32_t last = end - 1, i1 = min(i + 1, last), i2 = min(i + 2, last);
+                const Isect x0 = isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d);
+                const Isect x1 = isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d);
+                const Isect x2 = isect_values(load_geom(S.geoms + 5 * (size_t)i2), o, d);
+                const Isect *xs[3] = {&x0, &x1, &x2};
+#pragma unroll
+                for (uint32_t k = 0; k < 3; ++k) {
+                    if (i + k >= end) break;
+                    if (STATS) C.prim_tests++;
+                    if (isect_accept(*xs[k], tmin, tmax)) {
+                        tmax = xs[k]->lambda;
+                        hit.slot = (int32_t)(i + k);
+                        hit.lambda = xs[k]->lambda;
+                        hit.event = xs[k]->event;
+                        isect_bary(*xs[k], hit.bary);
+                    }
                 }
             }
         } else {
@@ -341,11 +370,20 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
         if (e & LEAF) {
             uint32_t first, count;
             leaf_range(S, e, first, count);
-            for (uint32_t i = first; i < first + count; ++i) {
-                float lam, b[4];
-                V4 ev;
-                if (STATS) C.shadow_prim_tests++;
-                if (intersect(S.geoms + 5 * (size_t)i, o, d, tmin, tmax, lam, ev, b)) return true;
+            // Up to three primitives per round of loads, tested in slot order (as in trace()).
+            const uint32_t end = first + count;
+            for (uint32_t i = first; i < end; i += 3) {
+                const uint32_t last = end - 1, i1 = min(i + 1, last), i2 = min(i + 2, last);
+                const bool h0 = isect_accept(isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d), tmin, tmax);
+                const bool h1 = isect_accept(isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d), tmin, tmax);
+                const bool h2 = isect_accept(isect_values(load_geom(S.geoms + 5 * (size_t)i2), o, d), tmin, tmax);
+                const bool hs[3] = {h0, h1, h2};
+#pragma unroll
+                for (uint32_t k = 0; k < 3; ++k) {
+                    if (i + k >= end) break;
+                    if (STATS) C.shadow_prim_tests++;
+                    if (hs[k]) return true;
+                }
             }
         } else {
             const float4 *bx = S.nbox + 4 * (size_t)e;
