@@ -331,7 +331,8 @@ def main():
                     help="traversal stack: K>0 positions in LDS, -N top entries in registers (0: library default)")
     ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent primary-ray kernel (-1: default)")
     ap.add_argument("--shadow-queue", type=int, default=-1, help="1/0: queued shadow rays (-1: default)")
-    ap.add_argument("--prefetch", type=int, default=-1, help="1/0: child prefetch in the primary trace (-1: default)")
+    ap.add_argument("--merged", type=int, default=-1,
+                    help="1/0: merged load rounds of node and leaf lanes in the primary trace (-1: default)")
     ap.add_argument("--star-overlap", type=int, default=-1, help="1/0: star pass beside the primary rays (-1: default)")
     ap.add_argument("--wave-shape", type=int, default=-1,
                     help="pixels a primary-ray wave takes: 0 8x8, 1 16x4, 2 4x16 (-1: default)")
@@ -351,7 +352,7 @@ def main():
 
     import numpy as np
     from specrel_amd import scene as S
-    from specrel_amd.renderer import (OPT_PERSISTENT, OPT_PREFETCH, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
+    from specrel_amd.renderer import (OPT_PERSISTENT, OPT_MERGED_LOADS, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
                                       OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE,
                                       SHADOWS_IN_TRACE,
                                       Context)
@@ -373,8 +374,8 @@ def main():
         ctx.set_option(OPT_PERSISTENT, args.persistent)
     if args.shadow_queue >= 0:
         ctx.set_option(OPT_SHADOW_QUEUE, args.shadow_queue)
-    if args.prefetch >= 0:
-        ctx.set_option(OPT_PREFETCH, args.prefetch)
+    if args.merged >= 0:
+        ctx.set_option(OPT_MERGED_LOADS, args.merged)
     if args.shadow_trace >= 0:
         ctx.set_option(OPT_SHADOW_IN_TRACE, args.shadow_trace)
     if args.trace_waves:

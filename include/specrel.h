@@ -66,8 +66,10 @@ enum {
     SPECREL_OPT_SHADOW_QUEUE = 5, /* 1: queue shadow rays and trace them densely; 0 (default): trace them in the shading lane */
     SPECREL_OPT_GRAPH = 6,        /* replay a captured HIP graph of the launches: 0 never, 1 (default) for
                                      batches, 2 for every render. Graph renders report kernel_ms only. */
-    SPECREL_OPT_PREFETCH = 7,     /* 1: primary rays load both interior children of a node while its boxes
-                                     are tested (LDS stacks only); 0 (default): load a node when visited */
+    SPECREL_OPT_MERGED_LOADS = 7, /* 1: a step of the primary-ray kernel (LDS stacks) loads the node
+                                     pairs of a wave's lanes at interior nodes and the first three
+                                     primitives of its lanes at leaves in one round, waited for once;
+                                     0 (default): each kind waits for its own loads. Same visiting order. */
     SPECREL_OPT_SHADOW_IN_TRACE = 8, /* 1 (default): a hit's shadow queries are traced by the primary-ray
                                        kernel right after its ray (LDS stacks, tiled rays, no queue);
                                        0: by the shading pass */

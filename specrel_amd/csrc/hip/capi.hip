@@ -72,7 +72,7 @@ struct specrel_ctx {
     specrel_stats stats_host{};
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
     int opt_graph = 1;
-    int opt_prefetch = 0;
+    int opt_merged = 0;
     int opt_shadow_trace = 1;
     int opt_trace_waves = 4;
     int opt_wave_shape = 0;
@@ -440,31 +440,36 @@ dim3 resident_grid(specrel_ctx *ctx, Kern kern) {
     return dim3((unsigned)(per_cu * ctx->cus));
 }
 
+// Primary rays with an LDS stack; M: merged load rounds.
+template <int K, bool M>
+void launch_lds(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
+    if (ta.mask && (K == 8 || K == 16) && ctx->opt_trace_waves != 4) {  // shadow queries traced here
+        constexpr int KK = K == 8 ? 8 : 16;
+        const dim3 g(tiles.x * 4 / ctx->opt_trace_waves);
+        if (ctx->opt_trace_waves == 1) {
+            if (stats) k_trace<true, KK, M, true, 1><<<g, 64, 0, s>>>(ta);
+            else k_trace<false, KK, M, true, 1><<<g, 64, 0, s>>>(ta);
+        } else {
+            if (stats) k_trace<true, KK, M, true, 2><<<g, 128, 0, s>>>(ta);
+            else k_trace<false, KK, M, true, 2><<<g, 128, 0, s>>>(ta);
+        }
+    } else if (ta.mask) {  // shadow queries traced here
+        if (stats) k_trace<true, K, M, true><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, M, true><<<tiles, 256, 0, s>>>(ta);
+    } else {
+        if (stats) k_trace<true, K, M, false><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, M, false><<<tiles, 256, 0, s>>>(ta);
+    }
+}
+
 template <int K>
 void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
     if (ctx->opt_persistent) {
         if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
         else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
     } else if constexpr (K > 0) {
-        if (ta.mask && (K == 8 || K == 16) && ctx->opt_trace_waves != 4) {  // shadow queries traced here
-            const dim3 g(tiles.x * 4 / ctx->opt_trace_waves);
-            if (ctx->opt_trace_waves == 1) {
-                if (stats) k_trace<true, K == 8 ? 8 : 16, false, true, 1><<<g, 64, 0, s>>>(ta);
-                else k_trace<false, K == 8 ? 8 : 16, false, true, 1><<<g, 64, 0, s>>>(ta);
-            } else {
-                if (stats) k_trace<true, K == 8 ? 8 : 16, false, true, 2><<<g, 128, 0, s>>>(ta);
-                else k_trace<false, K == 8 ? 8 : 16, false, true, 2><<<g, 128, 0, s>>>(ta);
-            }
-        } else if (ta.mask) {  // shadow queries traced here
-            if (stats) k_trace<true, K, false, true><<<tiles, 256, 0, s>>>(ta);
-            else k_trace<false, K, false, true><<<tiles, 256, 0, s>>>(ta);
-        } else if (ctx->opt_prefetch) {
-            if (stats) k_trace<true, K, true, false><<<tiles, 256, 0, s>>>(ta);
-            else k_trace<false, K, true, false><<<tiles, 256, 0, s>>>(ta);
-        } else {
-            if (stats) k_trace<true, K, false, false><<<tiles, 256, 0, s>>>(ta);
-            else k_trace<false, K, false, false><<<tiles, 256, 0, s>>>(ta);
-        }
+        if (ctx->opt_merged) launch_lds<K, true>(ctx, stats, tiles, s, ta);
+        else launch_lds<K, false>(ctx, stats, tiles, s, ta);
     } else {
         if (stats) k_trace<true, K, false, false><<<tiles, 256, 0, s>>>(ta);
         else k_trace<false, K, false, false><<<tiles, 256, 0, s>>>(ta);
@@ -698,7 +703,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
         std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
-                                     (uint64_t)ctx->opt_prefetch,       (uint64_t)ctx->opt_trace_waves,
+                                     (uint64_t)ctx->opt_merged,         (uint64_t)ctx->opt_trace_waves,
                                      (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
@@ -994,7 +999,7 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         ctx->opt_stack = value;
     } else if (option == SPECREL_OPT_PERSISTENT) ctx->opt_persistent = value != 0;
     else if (option == SPECREL_OPT_SHADOW_QUEUE) ctx->opt_shadow_queue = value != 0;
-    else if (option == SPECREL_OPT_PREFETCH) ctx->opt_prefetch = value != 0;
+    else if (option == SPECREL_OPT_MERGED_LOADS) ctx->opt_merged = value != 0;
     else if (option == SPECREL_OPT_SHADOW_IN_TRACE) ctx->opt_shadow_trace = value != 0;
     else if (option == SPECREL_OPT_STAR_OVERLAP) ctx->opt_star_overlap = value != 0;
     else if (option == SPECREL_OPT_TRACE_WAVES) {

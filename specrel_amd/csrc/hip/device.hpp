@@ -257,10 +257,47 @@ __device__ __forceinline__ NodeData load_node(const Scene &S, uint32_t e) {
     return NodeData{bx[0], bx[1], bx[2], bx[3], S.ninfo[e]};
 }
 
-// Bvh4D::trace (bvh4d.rs:443-520). With PF, an interior node's two interior children are loaded
-// while its boxes are tested, so the step into the near child finds its data already on the way;
-// the visiting order, and with it every result and counter, is the same either way.
-template <bool STATS, class ST, bool PF = false>
+// Marks a loaded value as needed here, so that the loads of one step are all issued before any
+// of them is waited for (an empty asm statement: no instruction).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void needed(float4 a) {
+    const f32x4 v = {a.x, a.y, a.z, a.w};
+    asm volatile("" ::"v"(v));
+}
+__device__ __forceinline__ void needed(uint2 a) { asm volatile("" ::"v"(a.x), "v"(a.y)); }
+__device__ __forceinline__ void needed(const GeomRec &g) {
+    needed(g.v0), needed(g.n), needed(g.b1), needed(g.b2), needed(g.b3);
+}
+
+// One leaf's primitives first .. end-1 from slot i on, tested in slot order, each with the t_max
+// the previous ones left (x0..x2: slots i, i+1, i+2, those past the leaf untested).
+template <bool STATS>
+__device__ __forceinline__ void leaf_tests(const Isect &x0, const Isect &x1, const Isect &x2, uint32_t i,
+                                           uint32_t end, float tmin, float &tmax, Hit &hit, Counters &C) {
+    const Isect *xs[3] = {&x0, &x1, &x2};
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+        if (i + k >= end) break;
+        if (STATS) C.prim_tests++;
+        if (isect_accept(*xs[k], tmin, tmax)) {
+            tmax = xs[k]->lambda;
+            hit.slot = (int32_t)(i + k);
+            hit.lambda = xs[k]->lambda;
+            hit.event = xs[k]->event;
+            isect_bary(*xs[k], hit.bary);
+        }
+    }
+}
+
+// Bvh4D::trace (bvh4d.rs:443-520). A leaf's primitives are loaded three at a time, all five vectors
+// of a record in one round. With M, a step also merges the rounds of a wave's two kinds of lanes:
+// the lanes at an interior node load its pair and the lanes at a leaf its first three primitives
+// together (each kind only when some lane of the wave needs it; a lane of the other kind reads a
+// valid record it ignores), and all of them are waited for at once, instead of the leaf lanes and
+// the node lanes each waiting for their own loads in turn (measured slower on the large scene:
+// the extra loads cost more than the round they save). The visiting order, and with it every
+// result and counter, is the same either way.
+template <bool STATS, class ST, bool M = false>
 __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk) {
     hit.slot = -1;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
@@ -268,52 +305,57 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
     uint32_t e = S.root_entry;
     stk.reset();
-    bool have = true, have_data = false;
-    NodeData D;
+    bool have = true;
     while (true) {
         if (!have) {
             if (stk.size() == 0) break;
             stk.pop(e, t_enter);
-            have_data = false;
         }
         have = false;
         if (t_enter >= tmax) continue;
         if (STATS) C.node_pops++;
-        if (e & LEAF) {
-            uint32_t first, count;
-            leaf_range(S, e, first, count);
-            // Up to three primitives per round of loads; they are tested in slot order, each with
-            // the t_max the previous ones left (slots past the leaf repeat its last one, untested).
+        const bool leaf = (e & LEAF) != 0;
+        uint32_t first = 0, count = 0;
+        if (leaf) leaf_range(S, e, first, count);
+        NodeData D;
+        Isect x0, x1, x2;
+        uint32_t done = 0;  // primitives of the leaf tested in the merged round
+        if (M) {
+            const bool prims = leaf && count > 0;
+            const uint32_t last = first + count - 1;
+            const uint32_t j0 = prims ? first : 0u, j1 = prims ? min(first + 1, last) : 0u,
+                           j2 = prims ? min(first + 2, last) : 0u;
+            const bool any_prims = __builtin_amdgcn_ballot_w64(prims) != 0;
+            const bool any_nodes = __builtin_amdgcn_ballot_w64(!leaf) != 0;
+            GeomRec g0{}, g1{}, g2{};
+            D = NodeData{};
+            if (any_nodes) D = load_node(S, leaf ? 0u : e);  // record 0 exists whenever a node does
+            if (any_prims) {  // slot 0 exists whenever a leaf of the scene has a primitive
+                g0 = load_geom(S.geoms + 5 * (size_t)j0);
+                g1 = load_geom(S.geoms + 5 * (size_t)j1);
+                g2 = load_geom(S.geoms + 5 * (size_t)j2);
+            }
+            needed(D.lmn), needed(D.lmx), needed(D.rmn), needed(D.rmx), needed(D.info);
+            needed(g0), needed(g1), needed(g2);
+            if (prims) {
+                x0 = isect_values(g0, o, d);
+                x1 = isect_values(g1, o, d);
+                x2 = isect_values(g2, o, d);
+                leaf_tests<STATS>(x0, x1, x2, first, first + count, tmin, tmax, hit, C);
+                done = min(count, 3u);
+            }
+        }
+        if (leaf) {
             const uint32_t end = first + count;
-            for (uint32_t i = first; i < end; i += 3) {
+            for (uint32_t i = first + done; i < end; i += 3) {
                 const uint32_t last = end - 1, i1 = min(i + 1, last), i2 = min(i + 2, last);
-                const Isect x0 = isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d);
-                const Isect x1 = isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d);
-                const Isect x2 = isect_values(load_geom(S.geoms + 5 * (size_t)i2), o, d);
-                const Isect *xs[3] = {&x0, &x1, &x2};
-#pragma unroll
-                for (uint32_t k = 0; k < 3; ++k) {
-                    if (i + k >= end) break;
-                    if (STATS) C.prim_tests++;
-                    if (isect_accept(*xs[k], tmin, tmax)) {
-                        tmax = xs[k]->lambda;
-                        hit.slot = (int32_t)(i + k);
-                        hit.lambda = xs[k]->lambda;
-                        hit.event = xs[k]->event;
-                        isect_bary(*xs[k], hit.bary);
-                    }
-                }
+                x0 = isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d);
+                x1 = isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d);
+                x2 = isect_values(load_geom(S.geoms + 5 * (size_t)i2), o, d);
+                leaf_tests<STATS>(x0, x1, x2, i, end, tmin, tmax, hit, C);
             }
         } else {
-            if (!PF || !have_data) D = load_node(S, e);
-            NodeData DL, DR;
-            bool l_int = false, r_int = false;
-            if (PF) {
-                l_int = (D.info.x & LEAF) == 0;
-                r_int = (D.info.y & LEAF) == 0;
-                if (l_int) DL = load_node(S, D.info.x);
-                if (r_int) DR = load_node(S, D.info.y);
-            }
+            if (!M) D = load_node(S, e);
             const uint2 info = D.info;
             float tl, tr;
             if (STATS) C.box_tests += 2;
@@ -328,22 +370,10 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
             if (both) stk.push(near_left ? info.y : info.x, near_left ? tr : tl);
             const bool go_left = both ? near_left : (hl && !hr && sz < STACK);
             const bool go_right = both ? !near_left : (hr && !hl && sz < STACK);
-            if (go_left) {
-                e = info.x;
-                t_enter = tl;
+            if (go_left || go_right) {
+                e = go_left ? info.x : info.y;
+                t_enter = go_left ? tl : tr;
                 have = true;
-                if (PF) {
-                    D = DL;
-                    have_data = l_int;
-                }
-            } else if (go_right) {
-                e = info.y;
-                t_enter = tr;
-                have = true;
-                if (PF) {
-                    D = DR;
-                    have_data = r_int;
-                }
             }
         }
     }

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
 // view's hit also traces its shadow queries at once (the ones compute_illumination and
 // compute_flash_illumination make, renderer.rs:585-688, from the same record the shading pass
 // rebuilds) and stores their answers for the shading pass (k_shade PASS 2).
-template <bool STATS, int SK, bool PF, bool SH, int W>
+template <bool STATS, int SK, bool M, bool SH, int W>
 __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
     using SS = StackSel<SK, 64 * W>;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     if (px < v.w && py < v.h) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
-        bool found = !a.S.empty && trace<STATS, typename SS::type, PF>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+        bool found = !a.S.empty && trace<STATS, typename SS::type, M>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
         a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
         if constexpr (SH) {
@@ -495,9 +495,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     }
 }
 
-#define SPECREL_INSTANTIATE(K)                                           \
-    template __global__ void k_trace<false, K, false, false>(TraceArgs); \
-    template __global__ void k_trace<true, K, false, false>(TraceArgs);  \
+#define SPECREL_INSTANTIATE(K)                                   \
     template __global__ void k_trace_p<false, K>(TraceArgs);     \
     template __global__ void k_trace_p<true, K>(TraceArgs);      \
     template __global__ void k_occlude_p<false, K>(OccludeArgs); \
@@ -508,38 +506,37 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     template __global__ void k_shade<true, K, 1>(ShadeArgs);     \
     template __global__ void k_shade<false, K, 2>(ShadeArgs);    \
     template __global__ void k_shade<true, K, 2>(ShadeArgs);
+#define SPECREL_TRACE(K, M, SH, W)                                    \
+    template __global__ void k_trace<false, K, M, SH, W>(TraceArgs); \
+    template __global__ void k_trace<true, K, M, SH, W>(TraceArgs);
+// LDS stacks: merged or separate load rounds, shadow queries traced here or not; blocks of 1 or 2
+// waves for the depths 8 and 16 with shadows traced here. Register stacks: separate rounds.
+#define SPECREL_TRACE_LDS(K) \
+    SPECREL_TRACE(K, false, false, 4) SPECREL_TRACE(K, true, false, 4) SPECREL_TRACE(K, false, true, 4) SPECREL_TRACE(K, true, true, 4)
 SPECREL_INSTANTIATE(8)
 SPECREL_INSTANTIATE(10)
 SPECREL_INSTANTIATE(12)
 SPECREL_INSTANTIATE(16)
-template __global__ void k_trace<false, 10, true, false>(TraceArgs);
-template __global__ void k_trace<true, 10, true, false>(TraceArgs);
-template __global__ void k_trace<false, 12, true, false>(TraceArgs);
-template __global__ void k_trace<true, 12, true, false>(TraceArgs);
-template __global__ void k_trace<false, 16, false, true, 1>(TraceArgs);
-template __global__ void k_trace<true, 16, false, true, 1>(TraceArgs);
-template __global__ void k_trace<false, 16, false, true, 2>(TraceArgs);
-template __global__ void k_trace<true, 16, false, true, 2>(TraceArgs);
-template __global__ void k_trace<false, 8, false, true, 1>(TraceArgs);
-template __global__ void k_trace<true, 8, false, true, 1>(TraceArgs);
-template __global__ void k_trace<false, 8, false, true, 2>(TraceArgs);
-template __global__ void k_trace<true, 8, false, true, 2>(TraceArgs);
-template __global__ void k_trace<false, 10, false, true>(TraceArgs);
-template __global__ void k_trace<true, 10, false, true>(TraceArgs);
-template __global__ void k_trace<false, 12, false, true>(TraceArgs);
-template __global__ void k_trace<true, 12, false, true>(TraceArgs);
-template __global__ void k_trace<false, 8, true, false>(TraceArgs);
-template __global__ void k_trace<true, 8, true, false>(TraceArgs);
-template __global__ void k_trace<false, 16, true, false>(TraceArgs);
-template __global__ void k_trace<true, 16, true, false>(TraceArgs);
-template __global__ void k_trace<false, 8, false, true>(TraceArgs);
-template __global__ void k_trace<true, 8, false, true>(TraceArgs);
-template __global__ void k_trace<false, 16, false, true>(TraceArgs);
-template __global__ void k_trace<true, 16, false, true>(TraceArgs);
+SPECREL_TRACE_LDS(8)
+SPECREL_TRACE_LDS(10)
+SPECREL_TRACE_LDS(12)
+SPECREL_TRACE_LDS(16)
+SPECREL_TRACE(8, false, true, 1)
+SPECREL_TRACE(8, true, true, 1)
+SPECREL_TRACE(8, false, true, 2)
+SPECREL_TRACE(8, true, true, 2)
+SPECREL_TRACE(16, false, true, 1)
+SPECREL_TRACE(16, true, true, 1)
+SPECREL_TRACE(16, false, true, 2)
+SPECREL_TRACE(16, true, true, 2)
 SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)
 SPECREL_INSTANTIATE(-16)
+SPECREL_TRACE(-4, false, false, 4)
+SPECREL_TRACE(-8, false, false, 4)
+SPECREL_TRACE(-12, false, false, 4)
+SPECREL_TRACE(-16, false, false, 4)
 
 // ---------------------------------------------------------------------------------------
 // Bloom (bloom.rs:22-125): the pyramid lives after the full-resolution HDR image.

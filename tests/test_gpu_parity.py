@@ -223,3 +223,34 @@ def test_wave_shapes_are_exact(gpu_ctx, twin, shape, waves):
     oc = O.render(twin, fr, counts=True)["counts"]
     for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
         assert st[k] == oc[k], (k, st[k], oc[k])
+
+
+@pytest.mark.parametrize("merged", [0, 1])
+def test_load_round_variants_are_exact(gpu_ctx, twin, merged):
+    # A wave's node lanes and leaf lanes waiting for their loads in one merged round or each in its
+    # own: the same tests in the same order, so images and every counter match the oracle.
+    from specrel_amd.renderer import OPT_MERGED_LOADS, OPT_STATS
+    p = S.gen_params("large", n_objects=12, slices=32, n_stars=300)
+    big = S.Scene.generate("large", p)
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.2, 0.1, 0.4))
+    q = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((q[0] - 6.0, q[1], q[2], q[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    cases = [(twin, S.build_frame(cam, S.RenderConfig(192, 120), dyn), "twin"),
+             (big, S.build_frame(S.default_camera("large", p), S.RenderConfig(256, 160)), "large 12x32")]
+    gpu_ctx.set_option(OPT_MERGED_LOADS, merged)
+    try:
+        for sc, fr, name in cases:
+            g, o = render_both(gpu_ctx, sc, fr)
+            assert_same(g, o, f"{name}, merged rounds {merged}")
+            gpu_ctx.set_option(OPT_STATS, 1)
+            gpu_ctx.render_frame(0, fr)
+            st = gpu_ctx.stats()
+            gpu_ctx.set_option(OPT_STATS, 0)
+            oc = O.render(sc, fr, counts=True)["counts"]
+            for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops",
+                      "shadow_box_tests", "shadow_prim_tests"):
+                assert st[k] == oc[k], (name, k, st[k], oc[k])
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_MERGED_LOADS, 0)
