@@ -119,6 +119,43 @@ __device__ uint32_t ordered_compact(uint32_t n, Keep keep, uint32_t *list, uint3
     return *s_total;
 }
 
+// The stars of a cone in index order (ordered_compact for the star test, K candidates per thread
+// and round): in a round, thread t of the 256-thread block tests stars base + 256 j + t for
+// j < K, their direction records loaded together (coalesced, the index clamped to the last star
+// so every load is in bounds); one barrier per round shares the waves' counts per j, and the kept
+// stars are placed j-major, wave by wave, lane by lane: increasing index, the same list as
+// ordered_compact's in n / (256 K) rounds instead of n / 256.
+template <int K>
+__device__ uint32_t compact_stars(const StarDir *dirs, uint32_t n, float cx, float cy, float cz, float cl,
+                                  uint32_t *list, uint32_t cap, uint32_t (*s_cnt)[4]) {
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < n; base += 256 * K) {
+        StarDir d[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) d[j] = dirs[min(base + 256 * j + tid, n - 1)];
+        uint64_t mask[K];
+        bool keep[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            keep[j] = base + 256 * j + tid < n && star_in_cone(d[j], cx, cy, cz, cl);
+            mask[j] = __ballot(keep[j]);
+            if (lane == 0) s_cnt[j][wave] = (uint32_t)__popcll(mask[j]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            uint32_t pos = total + (uint32_t)__popcll(mask[j] & below);
+            for (uint32_t w = 0; w < wave; ++w) pos += s_cnt[j][w];
+            if (keep[j] && pos < cap) list[pos] = base + 256 * j + tid;
+            total += s_cnt[j][0] + s_cnt[j][1] + s_cnt[j][2] + s_cnt[j][3];
+        }
+        __syncthreads();
+    }
+    return total;
+}
+
 // First culling level: one block per 64x64-pixel super tile keeps, in index order, the stars that
 // can reach any of its pixels.
 // Grid: every super tile of every view.
@@ -132,20 +169,14 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
     const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
     const uint32_t st = gst - V.super_start[vi], x0 = (st % nsx) * SUPER, y0 = (st / nsx) * SUPER;
     __shared__ float s_cone[4];
-    __shared__ uint32_t s_wave[4], s_total;
+    __shared__ uint32_t s_cnt[8][4];
     if (threadIdx.x == 0)
         tile_cone(v, x0, min(x0 + SUPER, v.w) - 1, y0, min(y0 + SUPER, v.h) - 1, s_cone[0], s_cone[1], s_cone[2],
                   s_cone[3]);
     __syncthreads();
     const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
     uint32_t *list = lists + (size_t)gst * SUPER_CAP;
-    uint32_t n = ordered_compact(
-        num_stars,
-        [&](uint32_t i, uint32_t &item) {
-            item = i;
-            return star_in_cone(dirs[i], cx, cy, cz, cl);
-        },
-        list, SUPER_CAP, s_wave, &s_total);
+    uint32_t n = num_stars ? compact_stars<8>(dirs, num_stars, cx, cy, cz, cl, list, SUPER_CAP, s_cnt) : 0u;
     if (threadIdx.x == 0) counts[gst] = n;
 }
 
