@@ -53,6 +53,7 @@ struct specrel_ctx {
     uint32_t n_views = 0, total_tiles = 0, total_supers = 0, total_pix = 0, total_pyr = 0;
     uint32_t stars_for = 0;  // num_stars the star buffers are sized for (per view)
     DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
+    DevBuf tile_cost, tile_order;  // per tile quarter: last traversal cost; tiles by decreasing cost
     DevBuf counters, mask, squeue;
     // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
     DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
@@ -76,6 +77,7 @@ struct specrel_ctx {
     int opt_shadow_trace = 1;
     int opt_trace_waves = 4;
     int opt_wave_shape = 0;
+    int opt_tile_order = 1;
     int opt_star_overlap = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
@@ -404,6 +406,13 @@ void ensure_views(specrel_ctx *ctx, const specrel_frame *frames, uint32_t n, uin
         galloc(ctx, ctx->super_lists, (size_t)ctx->total_supers * SUPER_CAP * 4);
         galloc(ctx, ctx->frames, (size_t)n * sizeof(specrel_frame));
         galloc(ctx, ctx->tab, tab.size() * 4);
+        // Tile order of the primary rays: row order until a frame has measured the tiles' costs.
+        galloc(ctx, ctx->tile_cost, (size_t)ctx->total_tiles * 16);
+        galloc(ctx, ctx->tile_order, (size_t)ctx->total_tiles * 4);
+        k_tile_init<<<(ctx->total_tiles + 255) / 256, 256, 0, ctx->stream>>>((uint32_t *)ctx->tile_order.p,
+                                                                            (uint32_t *)ctx->tile_cost.p,
+                                                                            ctx->total_tiles);
+        check(hipGetLastError(), "k_tile_init launch");
         ctx->tab_host.swap(tab);
         check(hipMemcpyAsync(ctx->tab.p, ctx->tab_host.data(), ctx->tab_host.size() * 4, hipMemcpyHostToDevice,
                              ctx->stream),
@@ -621,8 +630,16 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.queue = counters + 0;
     ta.stats = (unsigned long long *)ctx->stats.p;
     ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
+    const bool order = ctx->opt_tile_order && !ctx->opt_persistent;
+    ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
+    ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
+    if (order) {  // the next frame's tile order, from this frame's costs
+        k_tile_order<<<1, 1024, 0, s>>>((const uint32_t *)ctx->tile_cost.p, ctx->total_tiles,
+                                        (uint32_t *)ctx->tile_order.p);
+        check(hipGetLastError(), "k_tile_order launch");
+    }
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
@@ -705,6 +722,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
                                      (uint64_t)ctx->opt_merged,         (uint64_t)ctx->opt_trace_waves,
                                      (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
+                                     (uint64_t)ctx->opt_tile_order,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -1008,7 +1026,8 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
             return SPECREL_EINVAL;
         }
         ctx->opt_trace_waves = value;
-    } else if (option == SPECREL_OPT_WAVE_SHAPE) {
+    } else if (option == SPECREL_OPT_TILE_ORDER) ctx->opt_tile_order = value != 0;
+    else if (option == SPECREL_OPT_WAVE_SHAPE) {
         if (value < 0 || value > 2) {
             ctx->err = "wave shape must be 0 (8x8), 1 (16x4) or 2 (4x16)";
             return SPECREL_EINVAL;

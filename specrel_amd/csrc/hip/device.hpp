@@ -70,6 +70,7 @@ SR_HD V4 lv(const float *p) { return V4{p[0], p[1], p[2], p[3]}; }
 struct Counters {
     uint64_t node_pops, box_tests, prim_tests, hits, shadow_rays, shadow_node_pops, shadow_box_tests,
         shadow_prim_tests, star_tests;
+    uint32_t steps;  // traversal loop iterations of the lane (always counted: the tile's cost estimate)
 };
 
 // Aabb4D::intersect_ray (mesh4d.rs:113-146)
@@ -312,6 +313,7 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
             stk.pop(e, t_enter);
         }
         have = false;
+        C.steps++;
         if (t_enter >= tmax) continue;
         if (STATS) C.node_pops++;
         const bool leaf = (e & LEAF) != 0;
@@ -396,6 +398,7 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
             stk.pop(e, unused);
         }
         have = false;
+        C.steps++;
         if (STATS) C.shadow_node_pops++;
         if (e & LEAF) {
             uint32_t first, count;

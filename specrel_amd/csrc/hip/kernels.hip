@@ -189,11 +189,13 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
     using SS = StackSel<SK, 64 * W>;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint32_t quarter = blockIdx.x * W + (tid >> 6);  // a wave's quarter of a tile, four per tile
-    const uint32_t t = quarter >> 2, vi = find_view(a.V.level_start[0], a.V.n, t);
+    // The wave's scheduling slot: blocks start in slot order, and the order table puts the tiles
+    // that were slowest in the last frame first (each pixel's work does not depend on it).
+    const uint32_t slot_q = blockIdx.x * W + (tid >> 6), q = slot_q & 3;
+    const uint32_t t = a.order ? a.order[slot_q >> 2] : slot_q >> 2, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
-    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE, q = quarter & 3;
+    const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
     const uint32_t tx = (lt % tiles_x) * TILE, ty = (lt / tiles_x) * TILE;
     uint32_t px, py;
     if (a.wave_shape == 1) {  // 16x4 bands
@@ -230,6 +232,12 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
             }
             a.mask[pi] = bits;
         }
+    }
+    if (a.cost) {  // the quarter's cost: its lanes' longest traversal (primary and shadow steps)
+        uint32_t c = C.steps;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) c = max(c, (uint32_t)__shfl_xor((int)c, off));
+        if (lane == 0) a.cost[4 * (size_t)t + q] = c;
     }
     if (STATS) {
         __shared__ unsigned long long s_c[4];
@@ -382,6 +390,39 @@ __global__ __launch_bounds__(256) void k_occlude_p(OccludeArgs a) {
         atomicAdd(&a.stats[7], (unsigned long long)C.shadow_box_tests);
         atomicAdd(&a.stats[8], (unsigned long long)C.shadow_prim_tests);
     }
+}
+
+// Order of the next frame's primary-ray tiles: by decreasing cost of their slowest quarter (64
+// buckets of 8 steps, a counting sort in one block; the order inside a bucket is whatever the
+// atomics give, which changes nothing but the schedule). order becomes a permutation of the tiles.
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order) {
+    __shared__ uint32_t hist[64], base[64];
+    const uint32_t tid = threadIdx.x;
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+    auto bucket = [&](uint32_t t) {
+        const uint4 c = reinterpret_cast<const uint4 *>(cost)[t];
+        return min(63u, max(max(c.x, c.y), max(c.z, c.w)) >> 3);
+    };
+    for (uint32_t t = tid; t < n_tiles; t += 1024) atomicAdd(&hist[bucket(t)], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t acc = 0;
+        for (int b = 63; b >= 0; --b) {
+            base[b] = acc;
+            acc += hist[b];
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = tid; t < n_tiles; t += 1024) order[atomicAdd(&base[bucket(t)], 1u)] = t;
+}
+
+// Row order and zero costs, after a change of the view sizes.
+__global__ void k_tile_init(uint32_t *order, uint32_t *cost, uint32_t n_tiles) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_tiles) return;
+    order[t] = t;
+    reinterpret_cast<uint4 *>(cost)[t] = make_uint4(0, 0, 0, 0);
 }
 
 // specrel_patch_scene: node i's new bounds go to the child-box pair its parent holds

@@ -123,6 +123,8 @@ struct TraceArgs {
     uint32_t *queue;  // persistent variant: ray-queue counter (zeroed per frame)
     unsigned long long *stats;
     uint32_t wave_shape;  // pixels of a tile one wave takes: 0 an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band
+    const uint32_t *order;  // tile of each scheduling slot (nullptr: slot = tile), heaviest first
+    uint32_t *cost;         // per tile quarter: the longest traversal of its lanes, for the next order
 };
 
 struct OccludeArgs {
@@ -202,6 +204,8 @@ __global__ void k_occlude_p(OccludeArgs a);
 template <bool STATS, int SK, int PASS>
 __global__ void k_shade(ShadeArgs a);
 __global__ void k_reset(uint32_t *counters, uint32_t n);
+__global__ void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order);
+__global__ void k_tile_init(uint32_t *order, uint32_t *cost, uint32_t n_tiles);
 __global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint32_t n, float4 *nbox);
 __global__ void k_bloom_down(Views V, int level, float4 *pyr);
 __global__ void k_bloom_up(Views V, int level, float4 *pyr);

@@ -254,3 +254,31 @@ def test_load_round_variants_are_exact(gpu_ctx, twin, merged):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_MERGED_LOADS, 0)
+
+
+@pytest.mark.parametrize("waves", [4, 2, 1])
+def test_tile_order_changes_nothing(gpu_ctx, waves):
+    # From the second render of the same views on, the primary-ray tiles start slowest first (the
+    # costs the previous render measured): every render stays the oracle's, counters included.
+    from specrel_amd.renderer import OPT_STATS, OPT_TILE_ORDER, OPT_TRACE_WAVES
+    p = S.gen_params("large", n_objects=16, slices=32, n_stars=400)
+    sc = S.Scene.generate("large", p)
+    fr = S.build_frame(S.default_camera("large", p), S.RenderConfig(336, 216))
+    o = O.render(sc, fr, counts=True)
+    gpu_ctx.upload(0, sc)
+    gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+    try:
+        for order in (1, 0, 1):
+            gpu_ctx.set_option(OPT_TILE_ORDER, order)
+            for i in range(3):
+                assert_same(gpu_ctx.render_frame(0, fr, want_hits=True), o, f"order {order}, render {i}")
+            gpu_ctx.set_option(OPT_STATS, 1)
+            gpu_ctx.render_frame(0, fr)
+            st = gpu_ctx.stats()
+            gpu_ctx.set_option(OPT_STATS, 0)
+            for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
+                assert st[k] == o["counts"][k], (order, k, st[k], o["counts"][k])
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_TILE_ORDER, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 4)
