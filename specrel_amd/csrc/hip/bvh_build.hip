@@ -470,6 +470,7 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
         }
         if (e.parent != NONE) out[e.parent].right_or_first = me;
         sr::Box4 bounds = sr::box_empty(), cb = sr::box_empty();
+#pragma unroll 4
         for (uint32_t i = e.b; i < e.b + e.c; ++i) {
             bounds = sr::box_union(bounds, sr::Box4{v(f.mn[i]), v(f.mx[i])});
             sr::box_add_point(cb, v(f.c[i]));
@@ -484,14 +485,35 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
             const sr::BinFrame fr = sr::bin_frame(cb);
             sr::Bins B;
             sr::bins_clear(B);
-            for (uint32_t i = e.b; i < e.b + e.c; ++i) {
-                const float4 c = f.c[i];
-                const sr::Box4 bx{v(f.mn[i]), v(f.mx[i])};
-                for (int a = 0; a < 4; ++a) {
-                    if (!fr.live[a]) continue;
-                    const uint32_t k = sr::bin_of(fr, a, comp(c, a));
-                    B.b[a][k] = sr::box_union(B.b[a][k], bx);
-                    B.n[a][k] += 1;
+            // One pass per live axis with that axis' 8 bins in registers (unions and counts, so the
+            // order of the references changes nothing), stored into B once per axis.
+            for (int a = 0; a < 4; ++a) {
+                if (!fr.live[a]) continue;
+                const float inf = __builtin_inff();
+                float4 bmn[8], bmx[8];
+                uint32_t bn[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    bmn[q] = make_float4(inf, inf, inf, inf);
+                    bmx[q] = make_float4(-inf, -inf, -inf, -inf);
+                    bn[q] = 0;
+                }
+#pragma unroll 4
+                for (uint32_t i = e.b; i < e.b + e.c; ++i) {
+                    const uint32_t k = sr::bin_of(fr, a, comp(f.c[i], a));
+                    const float4 mn = f.mn[i], mx = f.mx[i];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (k == (uint32_t)q) {
+                            bmn[q] = f4(sr::vmin(v(bmn[q]), v(mn)));
+                            bmx[q] = f4(sr::vmax(v(bmx[q]), v(mx)));
+                            bn[q] += 1;
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    B.b[a][q] = sr::Box4{v(bmn[q]), v(bmx[q])};
+                    B.n[a][q] = bn[q];
                 }
             }
             kind = sr::choose_split(cb, fr, B, axis, pos);
