@@ -79,10 +79,10 @@ enum {
                                      rays; 0 (default): before them on the same stream */
     SPECREL_OPT_WAVE_SHAPE = 11,  /* pixels of a 16x16 tile one wave of the tiled primary-ray kernel
                                      takes: 0 (default) an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band */
-    SPECREL_OPT_TILE_ORDER = 12   /* 1 (default): the tiled primary-ray kernel starts the tiles that
-                                     took longest in the previous render of the same views first
-                                     (row order after a change of view sizes); 0: row order.
-                                     Results do not depend on it. */
+    SPECREL_OPT_TILE_ORDER = 12   /* 1 (default): when a render has more tiles than four per CU, the
+                                     tiled primary-ray kernel starts the tiles that took longest in the
+                                     previous render of the same view sizes first (row order after a
+                                     change of sizes); 0: row order. Results do not depend on it. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are

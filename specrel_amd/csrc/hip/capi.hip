@@ -630,7 +630,9 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.queue = counters + 0;
     ta.stats = (unsigned long long *)ctx->stats.p;
     ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
-    const bool order = ctx->opt_tile_order && !ctx->opt_persistent;
+    // Slowest tiles first only when they cannot all start at once (more than four per CU): then
+    // the order decides when the slowest ones start; otherwise it only costs.
+    const bool order = ctx->opt_tile_order && !ctx->opt_persistent && ctx->total_tiles > 4u * (uint32_t)ctx->cus;
     ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
     ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
     launch_any(ctx, stats, tiles, s, ta);

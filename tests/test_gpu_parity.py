@@ -259,11 +259,12 @@ def test_load_round_variants_are_exact(gpu_ctx, twin, merged):
 @pytest.mark.parametrize("waves", [4, 2, 1])
 def test_tile_order_changes_nothing(gpu_ctx, waves):
     # From the second render of the same views on, the primary-ray tiles start slowest first (the
-    # costs the previous render measured): every render stays the oracle's, counters included.
+    # costs the previous render measured; 1,350 tiles, more than four per CU of a 256-CU device):
+    # every render stays the oracle's, counters included.
     from specrel_amd.renderer import OPT_STATS, OPT_TILE_ORDER, OPT_TRACE_WAVES
-    p = S.gen_params("large", n_objects=16, slices=32, n_stars=400)
+    p = S.gen_params("large", n_objects=16, slices=32, n_stars=100)
     sc = S.Scene.generate("large", p)
-    fr = S.build_frame(S.default_camera("large", p), S.RenderConfig(336, 216))
+    fr = S.build_frame(S.default_camera("large", p), S.RenderConfig(720, 480))
     o = O.render(sc, fr, counts=True)
     gpu_ctx.upload(0, sc)
     gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
