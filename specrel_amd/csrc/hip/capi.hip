@@ -635,6 +635,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const bool order = ctx->opt_tile_order && !ctx->opt_persistent && ctx->total_tiles > 4u * (uint32_t)ctx->cus;
     ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
     ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
+    ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     if (order) {  // the next frame's tile order, from this frame's costs

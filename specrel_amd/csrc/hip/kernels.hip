@@ -193,6 +193,8 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     // that were slowest in the last frame first (each pixel's work does not depend on it).
     const uint32_t slot_q = blockIdx.x * W + (tid >> 6), q = slot_q & 3;
     const uint32_t t = a.order ? a.order[slot_q >> 2] : slot_q >> 2, vi = find_view(a.V.level_start[0], a.V.n, t);
+    // The slowest tiles of the last render issue first on their SIMD while light waves share it.
+    if ((slot_q >> 2) < a.prio_slots) __builtin_amdgcn_s_setprio(3);
     const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, 0);
     const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;

@@ -125,6 +125,7 @@ struct TraceArgs {
     uint32_t wave_shape;  // pixels of a tile one wave takes: 0 an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band
     const uint32_t *order;  // tile of each scheduling slot (nullptr: slot = tile), heaviest first
     uint32_t *cost;         // per tile quarter: the longest traversal of its lanes, for the next order
+    uint32_t prio_slots;    // tiles in the first slots of the order run at a raised wave priority
 };
 
 struct OccludeArgs {
