@@ -314,7 +314,7 @@ def copy_probe():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=800)
@@ -410,8 +410,8 @@ def main():
             parts[k] += t[k]
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
-    parts = {k: v / args.steps for k, v in parts.items()}
     value = w * h * args.steps / wall / 1e6
+    parts = {k: v / args.steps for k, v in parts.items()}
     log(f"{args.steps} steps in {wall:.3f} s: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s; kernels {parts}")
 
     # End to end through the host boundary (the reference's render() includes the readback).

@@ -55,6 +55,17 @@ struct specrel_ctx {
     DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
     DevBuf tile_cost, tile_order;  // per tile quarter: last traversal cost; tiles by decreasing cost
     DevBuf counters, mask, squeue;
+    // Pinned staging of the frame uploads, a ring of slots so that renders can be enqueued ahead of
+    // the device: a slot is refilled once its previous copy has completed.
+    struct Stage {
+        void *host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool used = false;
+    };
+    static constexpr uint32_t STAGES = 4;
+    Stage stage[STAGES];
+    uint32_t stage_next = 0;
     // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
     DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
     DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
@@ -68,6 +79,8 @@ struct specrel_ctx {
     // fork / join order it against the main stream (also inside a graph capture).
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // The next frame's tile order runs on the side stream beside the shading and bloom passes.
+    hipEvent_t order_fork = nullptr, order_join = nullptr;
     bool timed_stages = true, stars_overlapped = false;
     specrel_timings timings{};
     specrel_stats stats_host{};
@@ -571,9 +584,20 @@ Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, 
         if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "views too large for the shadow queue"};
         galloc(ctx, ctx->squeue, (size_t)cap * sizeof(ShadowQuery));
     }
-    check(hipMemcpyAsync(ctx->frames.p, frames, (size_t)n * sizeof(specrel_frame), hipMemcpyHostToDevice,
-                         ctx->stream),
-          "frame upload");
+    specrel_ctx::Stage &st = ctx->stage[ctx->stage_next++ % specrel_ctx::STAGES];
+    const size_t bytes = (size_t)n * sizeof(specrel_frame);
+    if (st.used) check(hipEventSynchronize(st.done), "frame staging");
+    if (st.cap < bytes) {
+        if (st.host) check(hipHostFree(st.host), "hipHostFree");
+        st.host = nullptr;
+        st.cap = 0;
+        check(hipHostMalloc(&st.host, bytes, hipHostMallocDefault), "hipHostMalloc");
+        st.cap = bytes;
+    }
+    std::memcpy(st.host, frames, bytes);
+    check(hipMemcpyAsync(ctx->frames.p, st.host, bytes, hipMemcpyHostToDevice, ctx->stream), "frame upload");
+    check(hipEventRecord(st.done, ctx->stream), "frame staging");
+    st.used = true;
     return pl;
 }
 
@@ -588,7 +612,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const Views V = device_views(ctx);
     const uint32_t nv = ctx->n_views;
     uint32_t *counters = (uint32_t *)ctx->counters.p;
-    k_reset<<<1, 64, 0, s>>>(counters, 16);
+    k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
     check(hipGetLastError(), "k_reset launch");
     mark(0);
     // The star pass runs beside the primary rays on the side stream when overlapped; the shading
@@ -638,10 +662,21 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
-    if (order) {  // the next frame's tile order, from this frame's costs
-        k_tile_order<<<1, 1024, 0, s>>>((const uint32_t *)ctx->tile_cost.p, ctx->total_tiles,
-                                        (uint32_t *)ctx->tile_order.p);
+    if (order) {  // the next frame's tile order, from this frame's costs, beside the later passes
+        check(hipEventRecord(ctx->order_fork, s), "fork");
+        check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
+        const uint32_t *cost = (const uint32_t *)ctx->tile_cost.p, n = ctx->total_tiles;
+        uint32_t *torder = (uint32_t *)ctx->tile_order.p;
+        if (n <= TILE_ORDER_ONE_BLOCK) {
+            k_tile_order<<<1, 1024, 0, ctx->side>>>(cost, n, torder);
+        } else {
+            const uint32_t nb = (n + TILE_ORDER_CHUNK - 1) / TILE_ORDER_CHUNK;
+            k_tile_hist<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL);
+            k_tile_scatter<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL,
+                                                      counters + COUNTER_TILE_CURSOR, torder);
+        }
         check(hipGetLastError(), "k_tile_order launch");
+        check(hipEventRecord(ctx->order_join, ctx->side), "join");
     }
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
@@ -699,6 +734,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     mark(4);
     k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, (uint32_t *)ctx->rgb.p);
     check(hipGetLastError(), "k_composite launch");
+    if (order) check(hipStreamWaitEvent(s, ctx->order_join, 0), "join wait");
     mark(5);
 }
 
@@ -956,13 +992,16 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming), "hipEventCreate");
         check(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ctx->order_fork, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ctx->order_join, hipEventDisableTiming), "hipEventCreate");
+        for (auto &st : ctx->stage) check(hipEventCreateWithFlags(&st.done, hipEventDisableTiming), "hipEventCreate");
         std::vector<uint32_t> words(SPECREL_LUT_SIZE + 14 * 7);
         sr::build_tables(words.data());
         std::vector<uint8_t> bytes(words.size());
         for (size_t i = 0; i < words.size(); ++i) bytes[i] = (uint8_t)words[i];
         upload(ctx->tables, bytes.data(), bytes.size(), ctx->stream);
         dalloc(ctx->stats, 16 * 8);
-        dalloc(ctx->counters, 16 * 4);
+        dalloc(ctx->counters, COUNTERS * 4);
         ctx->cus = prop.multiProcessorCount;
         check(hipStreamSynchronize(ctx->stream), "create");
         return SPECREL_OK;
@@ -1001,6 +1040,12 @@ void specrel_destroy(specrel_ctx *ctx) {
     if (ctx->side) (void)hipStreamSynchronize(ctx->side), (void)hipStreamDestroy(ctx->side);
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
+    if (ctx->order_fork) (void)hipEventDestroy(ctx->order_fork);
+    if (ctx->order_join) (void)hipEventDestroy(ctx->order_join);
+    for (auto &st : ctx->stage) {
+        if (st.done) (void)hipEventDestroy(st.done);
+        if (st.host) (void)hipHostFree(st.host);
+    }
     delete ctx;
 }
 

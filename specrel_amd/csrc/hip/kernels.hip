@@ -395,28 +395,72 @@ __global__ __launch_bounds__(256) void k_occlude_p(OccludeArgs a) {
 }
 
 // Order of the next frame's primary-ray tiles: by decreasing cost of their slowest quarter (64
-// buckets of 8 steps, a counting sort in one block; the order inside a bucket is whatever the
-// atomics give, which changes nothing but the schedule). order becomes a permutation of the tiles.
+// buckets of 8 steps, a counting sort; the order inside a bucket is whatever the atomics give,
+// which changes nothing but the schedule). order becomes a permutation of the tiles.
+__device__ inline uint32_t tile_bucket(const uint32_t *cost, uint32_t t) {
+    const uint4 c = reinterpret_cast<const uint4 *>(cost)[t];
+    return min(63u, max(max(c.x, c.y), max(c.z, c.w)) >> 3);
+}
+
+// Lane b of a full wave: the sum of v over lanes b+1..63 (the buckets of higher cost).
+__device__ inline uint32_t sum_above(uint32_t v, uint32_t lane) {
+    uint32_t s = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_down(s, off, 64);
+        s += lane + off < 64 ? o : 0u;
+    }
+    return s - v;
+}
+
+// Up to four tiles per thread: one block.
 __global__ __launch_bounds__(1024) void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order) {
-    __shared__ uint32_t hist[64], base[64];
+    __shared__ uint32_t hist[64];
     const uint32_t tid = threadIdx.x;
     if (tid < 64) hist[tid] = 0;
     __syncthreads();
-    auto bucket = [&](uint32_t t) {
-        const uint4 c = reinterpret_cast<const uint4 *>(cost)[t];
-        return min(63u, max(max(c.x, c.y), max(c.z, c.w)) >> 3);
-    };
-    for (uint32_t t = tid; t < n_tiles; t += 1024) atomicAdd(&hist[bucket(t)], 1u);
+    for (uint32_t t = tid; t < n_tiles; t += 1024) atomicAdd(&hist[tile_bucket(cost, t)], 1u);
     __syncthreads();
-    if (tid == 0) {
-        uint32_t acc = 0;
-        for (int b = 63; b >= 0; --b) {
-            base[b] = acc;
-            acc += hist[b];
-        }
+    if (tid < 64) {  // wave 0
+        const uint32_t h = hist[tid];
+        hist[tid] = sum_above(h, tid);
     }
     __syncthreads();
-    for (uint32_t t = tid; t < n_tiles; t += 1024) order[atomicAdd(&base[bucket(t)], 1u)] = t;
+    for (uint32_t t = tid; t < n_tiles; t += 1024) order[atomicAdd(&hist[tile_bucket(cost, t)], 1u)] = t;
+}
+
+// Larger grids: TILE_ORDER_CHUNK tiles per 256-thread block in two launches. k_tile_hist adds
+// each block's bucket counts into total[64]; k_tile_scatter turns total into bucket starts
+// (every block the same), reserves its block's range in each bucket with cursor[64] and writes
+// its tiles there. total and cursor start at zero (the frame's k_reset clears them).
+__device__ inline void chunk_hist(const uint32_t *cost, uint32_t n_tiles, uint32_t *hist) {
+    const uint32_t tid = threadIdx.x, t0 = blockIdx.x * TILE_ORDER_CHUNK;
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+    for (uint32_t t = t0 + tid; t < min(n_tiles, t0 + TILE_ORDER_CHUNK); t += 256)
+        atomicAdd(&hist[tile_bucket(cost, t)], 1u);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_tile_hist(const uint32_t *cost, uint32_t n_tiles, uint32_t *total) {
+    __shared__ uint32_t hist[64];
+    chunk_hist(cost, n_tiles, hist);
+    if (threadIdx.x < 64 && hist[threadIdx.x]) atomicAdd(&total[threadIdx.x], hist[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_tile_scatter(const uint32_t *cost, uint32_t n_tiles, const uint32_t *total,
+                                                      uint32_t *cursor, uint32_t *order) {
+    __shared__ uint32_t hist[64];
+    chunk_hist(cost, n_tiles, hist);
+    const uint32_t tid = threadIdx.x, t0 = blockIdx.x * TILE_ORDER_CHUNK;
+    if (tid < 64) {  // wave 0
+        const uint32_t h = hist[tid];
+        const uint32_t start = sum_above(total[tid], tid);
+        hist[tid] = start + (h ? atomicAdd(&cursor[tid], h) : 0u);
+    }
+    __syncthreads();
+    for (uint32_t t = t0 + tid; t < min(n_tiles, t0 + TILE_ORDER_CHUNK); t += 256)
+        order[atomicAdd(&hist[tile_bucket(cost, t)], 1u)] = t;
 }
 
 // Row order and zero costs, after a change of the view sizes.

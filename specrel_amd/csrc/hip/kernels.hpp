@@ -8,6 +8,13 @@ constexpr int TILE = 16;          // pixels per side of a block's tile (4 wave64
 constexpr int SUPER = 64;         // pixels per side of a first-level star-culling tile
 constexpr uint32_t SUPER_CAP = 4096;
 constexpr uint32_t TILE_CAP = 1024;
+// Tile ordering: one block sorts up to TILE_ORDER_ONE_BLOCK tiles; larger grids use blocks of
+// TILE_ORDER_CHUNK tiles and the per-frame counters below.
+constexpr uint32_t TILE_ORDER_ONE_BLOCK = 4096;
+constexpr uint32_t TILE_ORDER_CHUNK = 1024;
+// Per-frame counters (cleared by k_reset): 0 primary-ray queue, 1 shadow-query count, 2 next
+// shadow query, 16..79 tile-order bucket totals, 80..143 tile-order bucket cursors.
+constexpr uint32_t COUNTER_TILE_TOTAL = 16, COUNTER_TILE_CURSOR = 80, COUNTERS = 144;
 constexpr int STACK_DEFAULT = 16; // traversal stack: > 0 positions in LDS, < 0 entries in registers
 
 // The views of one launch (a single render is a batch of one view). Per-view ranges are prefix
@@ -206,6 +213,9 @@ template <bool STATS, int SK, int PASS>
 __global__ void k_shade(ShadeArgs a);
 __global__ void k_reset(uint32_t *counters, uint32_t n);
 __global__ void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order);
+__global__ void k_tile_hist(const uint32_t *cost, uint32_t n_tiles, uint32_t *total);
+__global__ void k_tile_scatter(const uint32_t *cost, uint32_t n_tiles, const uint32_t *total, uint32_t *cursor,
+                               uint32_t *order);
 __global__ void k_tile_init(uint32_t *order, uint32_t *cost, uint32_t n_tiles);
 __global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint32_t n, float4 *nbox);
 __global__ void k_bloom_down(Views V, int level, float4 *pyr);

@@ -135,6 +135,40 @@ def test_batch_stats_are_the_sum_of_single_views(gpu_ctx, twin):
         gpu_ctx.set_option(OPT_STATS, 0)
 
 
+def test_tile_order_over_many_tiles(gpu_ctx, twin):
+    # 16,676 tiles: past one block's share (4,096 tiles), the next render's tile order is sorted
+    # by blocks of 1,024 tiles in two launches. Renders in that order, and in row order, are the same views
+    # with the same counters.
+    from specrel_amd.renderer import OPT_TILE_ORDER
+    gpu_ctx.upload(0, twin)
+    rng = np.random.default_rng(51)
+    base = twin.default_camera()
+    frames = []
+    for w, h in [(1280, 800), (1280, 800), (1280, 800), (1300, 900), (17, 9)]:
+        cam = twin.default_camera()
+        cam.position = tuple(float(p + d) for p, d in zip(base.position, rng.normal(0.0, 0.4, 4)))
+        cam.tetrad.boost_local(rng.uniform(-0.5, 0.5, 3).astype(np.float32))
+        frames.append(S.build_frame(cam, S.RenderConfig(w, h)))
+    try:
+        gpu_ctx.set_option(OPT_TILE_ORDER, 0)
+        ref = gpu_ctx.render_batch(0, frames)
+        assert_views_equal([ref[0], ref[4]], [gpu_ctx.render_frame(0, frames[i], want_hits=True) for i in (0, 4)],
+                           "row order vs single views")
+        stats = {}
+        for order in (0, 1):
+            gpu_ctx.set_option(OPT_TILE_ORDER, order)
+            for i in range(3):
+                assert_views_equal(gpu_ctx.render_batch(0, frames), ref, f"order {order}, render {i}")
+            gpu_ctx.set_option(OPT_STATS, 1)
+            gpu_ctx.render_batch(0, frames, want_hits=False)
+            stats[order] = gpu_ctx.stats()
+            gpu_ctx.set_option(OPT_STATS, 0)
+        assert stats[0] == stats[1] and stats[0]["rays"] == sum(w * h for w, h in (f.size for f in frames))
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_TILE_ORDER, 1)
+
+
 def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
     gpu_ctx.upload(0, twin)
     frames = twin_frames(twin, 41)

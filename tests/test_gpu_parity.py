@@ -283,3 +283,21 @@ def test_tile_order_changes_nothing(gpu_ctx, waves):
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_TILE_ORDER, 1)
         gpu_ctx.set_option(OPT_TRACE_WAVES, 4)
+
+
+def test_renders_enqueued_ahead(gpu_ctx, twin):
+    # Renders enqueued without waiting (more of them than the pinned frame-staging ring has slots)
+    # each upload their own observer: the image left in HBM is the last frame's, bit-exact.
+    rng = np.random.default_rng(3)
+    frames = []
+    for i in range(7):
+        cam = twin.default_camera()
+        cam.tetrad.boost_local(rng.uniform(-0.4, 0.4, 3).astype(np.float32))
+        cam.tetrad.rotate_local(*[float(x) for x in rng.uniform(-0.2, 0.2, 3)])
+        frames.append(S.build_frame(cam, S.RenderConfig(192, 120)))
+    gpu_ctx.upload(0, twin)
+    for last in (2, 6):
+        for f in frames[:last + 1]:
+            gpu_ctx.render_resident(0, f, want_hits=True)
+        gpu_ctx.sync()
+        assert_same(gpu_ctx.download(192, 120, want_hits=True), O.render(twin, frames[last]), f"last frame {last}")
