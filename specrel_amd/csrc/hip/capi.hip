@@ -545,6 +545,8 @@ struct Plan {
     bool queries = false;  // some view is shaded with shadows on (renderer.rs:631-641, 680-682)
     uint32_t max_flashes = 0;
     bool want_hits = false;
+    bool frame_in_reset = false;  // one view launched directly: its frame goes to k_reset_frame
+    FrameWords frame;
 };
 
 // Where a render's shadow queries are traced (specrel_timings.shadow_path): queued for a dense
@@ -557,7 +559,8 @@ uint32_t shadow_path(const specrel_ctx *ctx, const Plan &pl) {
 }
 
 // Validates the frames, sizes the buffers, uploads the frames: everything before the launches.
-Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
+Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits,
+             bool frame_in_reset) {
     if (!frames) throw Fail{SPECREL_EINVAL, "frame is NULL"};
     if (n == 0 || n > MAX_VIEWS) throw Fail{SPECREL_EINVAL, "view count must be 1.." + std::to_string(MAX_VIEWS)};
     if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
@@ -583,6 +586,11 @@ Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, 
         uint64_t cap = (uint64_t)ctx->total_pix * (1u + pl.max_flashes);
         if (cap > 0xFFFFFFFFull) throw Fail{SPECREL_EINVAL, "views too large for the shadow queue"};
         galloc(ctx, ctx->squeue, (size_t)cap * sizeof(ShadowQuery));
+    }
+    if (frame_in_reset) {  // k_reset_frame writes the one frame
+        std::memcpy(pl.frame.w, frames, sizeof(specrel_frame));
+        pl.frame_in_reset = true;
+        return pl;
     }
     specrel_ctx::Stage &st = ctx->stage[ctx->stage_next++ % specrel_ctx::STAGES];
     const size_t bytes = (size_t)n * sizeof(specrel_frame);
@@ -612,21 +620,18 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const Views V = device_views(ctx);
     const uint32_t nv = ctx->n_views;
     uint32_t *counters = (uint32_t *)ctx->counters.p;
-    k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
+    if (pl.frame_in_reset)
+        k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, (uint32_t *)ctx->frames.p);
+    else
+        k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
     check(hipGetLastError(), "k_reset launch");
     mark(0);
-    // The star pass runs beside the primary rays on the side stream when overlapped; the shading
-    // pass waits for both.
+    // The primary rays come first (they need only the frame); the star pass follows them, or runs
+    // beside them on the side stream when overlapped. The shading pass waits for both.
     const bool overlap = pl.stars && ctx->opt_star_overlap;
     ctx->stars_overlapped = overlap;
-    if (pl.stars) {
-        hipStream_t ss = s;
-        if (overlap) {
-            ss = ctx->side;
-            check(hipEventRecord(ctx->fork, s), "fork");
-            check(hipStreamWaitEvent(ss, ctx->fork, 0), "fork wait");
-            if (events) check(hipEventRecord(ctx->ev[9], ss), "event");
-        }
+    if (overlap) check(hipEventRecord(ctx->fork, s), "fork");
+    auto star_pass = [&](hipStream_t ss) {
         k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
             (const float4 *)sc.stars.p, sc.num_stars, V, (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
         check(hipGetLastError(), "k_star_prep launch");
@@ -636,12 +641,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
                                                            (uint32_t *)ctx->super_counts.p);
             check(hipGetLastError(), "k_star_bin launch");
         }
-        if (overlap) {
-            if (events) check(hipEventRecord(ctx->ev[10], ss), "event");
-            check(hipEventRecord(ctx->join, ss), "join");
-        }
-    }
-    mark(1);
+    };
     const dim3 tiles(ctx->total_tiles);
     const uint32_t path = shadow_path(ctx, pl);
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
@@ -662,6 +662,14 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
+    mark(1);
+    if (overlap) {
+        check(hipStreamWaitEvent(ctx->side, ctx->fork, 0), "fork wait");
+        if (events) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
+        star_pass(ctx->side);
+        if (events) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
+        check(hipEventRecord(ctx->join, ctx->side), "join");
+    }
     if (order) {  // the next frame's tile order, from this frame's costs, beside the later passes
         check(hipEventRecord(ctx->order_fork, s), "fork");
         check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
@@ -678,6 +686,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         check(hipGetLastError(), "k_tile_order launch");
         check(hipEventRecord(ctx->order_join, ctx->side), "join");
     }
+    if (pl.stars && !overlap) star_pass(s);
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
@@ -726,9 +735,22 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     check(hipGetLastError(), "k_shade launch");
     mark(3);
     float4 *pyr = (float4 *)ctx->hdr.p;
-    for (int k = 1; k <= SPECREL_BLOOM_LEVELS; ++k)
+    // Levels whose every view holds at most BLOOM_TAIL_PIXELS pixels are done by k_bloom_tail.
+    int k0 = SPECREL_BLOOM_LEVELS;
+    while (k0 > 1) {
+        bool small = true;
+        for (uint32_t v = 0; v < nv && small; ++v) {
+            uint32_t lw, lh;
+            level_size(ctx->sizes[2 * v], ctx->sizes[2 * v + 1], k0 - 1, lw, lh);
+            small = (uint64_t)lw * lh <= BLOOM_TAIL_PIXELS;
+        }
+        if (!small) break;
+        --k0;
+    }
+    for (int k = 1; k < k0; ++k)
         k_bloom_down<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
-    for (int k = SPECREL_BLOOM_LEVELS - 1; k >= 1; --k)
+    k_bloom_tail<<<nv, 1024, 0, s>>>(V, k0, pyr);
+    for (int k = k0 - 1; k >= 1; --k)
         k_bloom_up<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
     check(hipGetLastError(), "bloom launch");
     mark(4);
@@ -749,12 +771,12 @@ void drop_graph(specrel_ctx *ctx) {
 // the view sizes, options or any buffer it references changed.
 int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
     auto t0 = std::chrono::steady_clock::now();
-    const Plan pl = prepare(ctx, scene_idx, frames, n, want_hits);
+    const bool stats = ctx->opt_stats != 0;
+    const bool use_graph = !stats && (ctx->opt_graph == 2 || (ctx->opt_graph == 1 && n > 1));
+    const Plan pl = prepare(ctx, scene_idx, frames, n, want_hits, n == 1 && !use_graph);
     SceneSlot &sc = ctx->scenes[scene_idx];
     hipStream_t s = ctx->stream;
-    const bool stats = ctx->opt_stats != 0;
     if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
-    const bool use_graph = !stats && (ctx->opt_graph == 2 || (ctx->opt_graph == 1 && n > 1));
     if (use_graph) {
         std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
@@ -805,8 +827,8 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
 void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
     if (ctx->timed_stages) {
-        t.star_prep_ms = ctx->stars_overlapped ? ev_ms(ctx->ev[9], ctx->ev[10]) : ev_ms(ctx->ev[0], ctx->ev[1]);
-        t.trace_ms = ev_ms(ctx->ev[1], ctx->ev[2]);
+        t.trace_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
+        t.star_prep_ms = ctx->stars_overlapped ? ev_ms(ctx->ev[9], ctx->ev[10]) : ev_ms(ctx->ev[1], ctx->ev[2]);
         t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
         t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
         t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);

@@ -488,6 +488,14 @@ __global__ void k_reset(uint32_t *counters, uint32_t n) {
     if (i < n) counters[i] = 0u;
 }
 
+// The same for a render of one view launched directly, which also writes its frame (passed by
+// value) to the device frame table: one launch instead of a copy and a launch.
+__global__ void k_reset_frame(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst) {
+    uint32_t i = threadIdx.x;
+    if (i < n) counters[i] = 0u;
+    if (i < FrameWords::N) dst[i] = f.w[i];
+}
+
 // Shading: the winner's record is intersected again (same inputs, same float operations, so the
 // same lambda, event and barycentrics), interpolated and shaded with shadow rays, haze and the
 // tile's stars; writes the HDR pixel and the optional per-pixel outputs.
@@ -683,17 +691,9 @@ __device__ __forceinline__ LevelPix level_pix(const Views &V, int k) {
     return p;
 }
 
-// bloom::downsample (bloom.rs:54-75) from level k-1 into level k of every view.
-__global__ __launch_bounds__(256) void k_bloom_down(Views V, int k, float4 *pyr) {
-    const LevelPix p = level_pix(V, k);
-    uint32_t sw, sh, dw, dh;
-    level_size(p.w, p.h, k - 1, sw, sh);
-    level_size(p.w, p.h, k, dw, dh);
-    const uint32_t x = p.x, y = p.y;
-    if (x >= dw || y >= dh) return;
-    float4 *base = pyr + V.pyr_start[p.vi];
-    const float4 *src = base + level_offset(p.w, p.h, k - 1);
-    float4 *dst = base + level_offset(p.w, p.h, k);
+// bloom::downsample (bloom.rs:54-75): pixel (x, y) of a level from the finer level src.
+__device__ __forceinline__ void downsample_px(const float4 *src, uint32_t sw, uint32_t sh, float4 *dst, uint32_t dw,
+                                              uint32_t x, uint32_t y) {
     const float W[4] = {0.125f, 0.375f, 0.375f, 0.125f};
     V3 acc = V3{0.0f, 0.0f, 0.0f};
     for (int j = 0; j < 4; ++j) {
@@ -722,20 +722,62 @@ __device__ __forceinline__ V3 upsample_at(const float4 *coarse, uint32_t cw, uin
     return r0 * 0.75f + r1 * 0.25f;
 }
 
-// One step of the up pass (bloom.rs:110-114): level k += upsample(level k+1), every view.
+// One step of the up pass (bloom.rs:110-114) at pixel (x, y): fine += upsample(coarse).
+__device__ __forceinline__ void upsample_px(float4 *fine, uint32_t w, const float4 *coarse, uint32_t cw, uint32_t ch,
+                                            uint32_t x, uint32_t y) {
+    size_t i = (size_t)y * w + x;
+    V3 c = ld3(fine, i) + upsample_at(coarse, cw, ch, x, y);
+    fine[i] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
+// Level k of every view from level k-1.
+__global__ __launch_bounds__(256) void k_bloom_down(Views V, int k, float4 *pyr) {
+    const LevelPix p = level_pix(V, k);
+    uint32_t sw, sh, dw, dh;
+    level_size(p.w, p.h, k - 1, sw, sh);
+    level_size(p.w, p.h, k, dw, dh);
+    if (p.x >= dw || p.y >= dh) return;
+    float4 *base = pyr + V.pyr_start[p.vi];
+    downsample_px(base + level_offset(p.w, p.h, k - 1), sw, sh, base + level_offset(p.w, p.h, k), dw, p.x, p.y);
+}
+
+// Level k += upsample(level k+1), every view.
 __global__ __launch_bounds__(256) void k_bloom_up(Views V, int k, float4 *pyr) {
     const LevelPix p = level_pix(V, k);
     uint32_t w, h, cw, ch;
     level_size(p.w, p.h, k, w, h);
     level_size(p.w, p.h, k + 1, cw, ch);
-    const uint32_t x = p.x, y = p.y;
-    if (x >= w || y >= h) return;
+    if (p.x >= w || p.y >= h) return;
     float4 *base = pyr + V.pyr_start[p.vi];
-    float4 *fine = base + level_offset(p.w, p.h, k);
-    const float4 *coarse = base + level_offset(p.w, p.h, k + 1);
-    size_t i = (size_t)y * w + x;
-    V3 c = ld3(fine, i) + upsample_at(coarse, cw, ch, x, y);
-    fine[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    upsample_px(base + level_offset(p.w, p.h, k), w, base + level_offset(p.w, p.h, k + 1), cw, ch, p.x, p.y);
+}
+
+// The small levels of one view per block: down k0..SPECREL_BLOOM_LEVELS, then up
+// SPECREL_BLOOM_LEVELS-1..k0, each level finished by the block before the next reads it (one
+// launch instead of 2*(SPECREL_BLOOM_LEVELS-k0)+1).
+__global__ __launch_bounds__(1024) void k_bloom_tail(Views V, int k0, float4 *pyr) {
+    const uint32_t vi = blockIdx.x;
+    const specrel_frame &f = V.frames[vi];
+    const uint32_t w0 = f.size[0], h0 = f.size[1];
+    float4 *base = pyr + V.pyr_start[vi];
+    for (int k = k0; k <= SPECREL_BLOOM_LEVELS; ++k) {
+        uint32_t sw, sh, dw, dh;
+        level_size(w0, h0, k - 1, sw, sh);
+        level_size(w0, h0, k, dw, dh);
+        const float4 *src = base + level_offset(w0, h0, k - 1);
+        float4 *dst = base + level_offset(w0, h0, k);
+        for (uint32_t i = threadIdx.x; i < dw * dh; i += 1024) downsample_px(src, sw, sh, dst, dw, i % dw, i / dw);
+        __syncthreads();
+    }
+    for (int k = SPECREL_BLOOM_LEVELS - 1; k >= k0; --k) {
+        uint32_t w, h, cw, ch;
+        level_size(w0, h0, k, w, h);
+        level_size(w0, h0, k + 1, cw, ch);
+        float4 *fine = base + level_offset(w0, h0, k);
+        const float4 *coarse = base + level_offset(w0, h0, k + 1);
+        for (uint32_t i = threadIdx.x; i < w * h; i += 1024) upsample_px(fine, w, coarse, cw, ch, i % w, i / w);
+        __syncthreads();
+    }
 }
 
 // Final mix (bloom.rs:116-124) and pack_rgb_with_lut (renderer.rs:1116-1129), every view.

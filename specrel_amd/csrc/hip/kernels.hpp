@@ -212,6 +212,12 @@ __global__ void k_occlude_p(OccludeArgs a);
 template <bool STATS, int SK, int PASS>
 __global__ void k_shade(ShadeArgs a);
 __global__ void k_reset(uint32_t *counters, uint32_t n);
+struct FrameWords {
+    static constexpr uint32_t N = sizeof(specrel_frame) / 4;
+    uint32_t w[N];
+};
+static_assert(sizeof(specrel_frame) % 4 == 0 && FrameWords::N <= 256, "one frame per k_reset_frame thread block");
+__global__ void k_reset_frame(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst);
 __global__ void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order);
 __global__ void k_tile_hist(const uint32_t *cost, uint32_t n_tiles, uint32_t *total);
 __global__ void k_tile_scatter(const uint32_t *cost, uint32_t n_tiles, const uint32_t *total, uint32_t *cursor,
@@ -220,6 +226,10 @@ __global__ void k_tile_init(uint32_t *order, uint32_t *cost, uint32_t n_tiles);
 __global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint32_t n, float4 *nbox);
 __global__ void k_bloom_down(Views V, int level, float4 *pyr);
 __global__ void k_bloom_up(Views V, int level, float4 *pyr);
+__global__ void k_bloom_tail(Views V, int k0, float4 *pyr);
+// The bloom levels from which one block per view finishes the pyramid (k_bloom_tail): every
+// view's level holds at most this many pixels.
+constexpr uint32_t BLOOM_TAIL_PIXELS = 2560;
 __global__ void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);
 
 // Pyramid geometry of bloom.rs:28-49, shared by host and device.
