@@ -183,8 +183,10 @@ struct LdsStack {
     uint2 *deep;      // DEEP entries in private memory
     __device__ __forceinline__ void reset() { n = 0; }
     __device__ __forceinline__ int size() const { return n; }
+    // While no active lane of the wave reaches the private part (the common case), push and pop
+    // take a wave-uniform branch straight to the LDS access.
     __device__ __forceinline__ void push(uint32_t e, float t) {
-        if (n < K) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(n >= K) == 0, 1) || n < K) {
             lds_uint2 *p = lds + (uint32_t)n * stride;
             p->x = e;
             p->y = __float_as_uint(t);
@@ -196,7 +198,7 @@ struct LdsStack {
     __device__ __forceinline__ void pop(uint32_t &e, float &t) {
         n -= 1;
         uint2 v;
-        if (n < K) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(n >= K) == 0, 1) || n < K) {
             const volatile lds_uint2 *p = lds + (uint32_t)n * stride;
             v.x = p->x;
             v.y = p->y;
