@@ -108,7 +108,7 @@ def bench_batch(ctx, scene, n_views, steps, seed):
     rays = sum(f.size[0] * f.size[1] for f in frames)
     arr = ctx._frames(frames)
     log(f"batch: {n_views} views, {rays} rays; autotuning")
-    tuning = ctx.autotune(0, arr, iters=2)
+    tuning = ctx.autotune(0, arr, iters=5)
     log(f"autotune: {tuning}")
     ctx.render_batch_resident(0, arr)
     ctx.sync()

@@ -76,13 +76,20 @@ enum {
     SPECREL_OPT_TRACE_WAVES = 9, /* 1, 2 or 4 (default): waves per block of the primary-ray kernel with
                                     shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
     SPECREL_OPT_STAR_OVERLAP = 10, /* 1: the star pass runs on a second stream beside the primary
-                                     rays; 0 (default): before them on the same stream */
+                                     rays; 0 (default): after them on the same stream */
     SPECREL_OPT_WAVE_SHAPE = 11,  /* pixels of a 16x16 tile one wave of the tiled primary-ray kernel
                                      takes: 0 (default) an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band */
-    SPECREL_OPT_TILE_ORDER = 12   /* 1 (default): when a render has more tiles than four per CU, the
+    SPECREL_OPT_TILE_ORDER = 12,  /* 1 (default): when a render has more tiles than four per CU, the
                                      tiled primary-ray kernel starts the tiles that took longest in the
                                      previous render of the same view sizes first (row order after a
                                      change of sizes); 0: row order. Results do not depend on it. */
+    SPECREL_OPT_LEAF_POSTPONE = 13 /* 1 (default): a closest-hit ray of the primary-ray kernel (LDS
+                                     stacks, not merged loads) that reaches a leaf keeps it pending
+                                     and traverses on to its next leaf; the pending leaves are tested
+                                     in order once 3/8 of the wave's tracing lanes hold one. 0: every
+                                     leaf is tested when reached. Results do not depend on it;
+                                     instrumented renders (SPECREL_OPT_STATS) test every leaf when
+                                     reached, so their counters are the reference's. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are
@@ -105,7 +112,8 @@ typedef struct specrel_tuning {
     float default_ms;     /* the same with the options the context had before */
     int32_t shadow_in_trace; /* SPECREL_OPT_SHADOW_IN_TRACE */
     int32_t wave_shape;      /* SPECREL_OPT_WAVE_SHAPE */
-    uint32_t reserved[3];
+    int32_t leaf_postpone;   /* SPECREL_OPT_LEAF_POSTPONE */
+    uint32_t reserved[2];
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
@@ -206,7 +214,8 @@ int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, ui
                           float *out_tau);
 /* Times every launch configuration (stack depth 16/8 in LDS, persistent or tiled primary rays,
  * shadow rays in the shading pass, queued or in the primary-ray kernel, graph replay or direct
- * launches) on this batch, `iters` renders each after one warm-up, and keeps the fastest by
+ * launches, the wave shape of tiled primary rays), then the fastest with leaf postponement
+ * switched, on this batch, `iters` renders each after one warm-up, and keeps the fastest by
  * median wall time. */
 int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                      uint32_t iters, specrel_tuning *out);

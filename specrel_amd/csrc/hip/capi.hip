@@ -91,6 +91,7 @@ struct specrel_ctx {
     int opt_trace_waves = 4;
     int opt_wave_shape = 0;
     int opt_tile_order = 1;
+    int opt_postpone = 1;
     int opt_star_overlap = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
@@ -660,6 +661,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
     ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
+    ta.postpone = (uint32_t)ctx->opt_postpone;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(1);
@@ -783,7 +785,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)ctx->opt_shadow_queue,   (uint64_t)ctx->opt_shadow_trace,
                                      (uint64_t)ctx->opt_merged,         (uint64_t)ctx->opt_trace_waves,
                                      (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
-                                     (uint64_t)ctx->opt_tile_order,
+                                     (uint64_t)ctx->opt_tile_order,   (uint64_t)ctx->opt_postpone,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -1097,6 +1099,7 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         }
         ctx->opt_trace_waves = value;
     } else if (option == SPECREL_OPT_TILE_ORDER) ctx->opt_tile_order = value != 0;
+    else if (option == SPECREL_OPT_LEAF_POSTPONE) ctx->opt_postpone = value != 0;
     else if (option == SPECREL_OPT_WAVE_SHAPE) {
         if (value < 0 || value > 2) {
             ctx->err = "wave shape must be 0 (8x8), 1 (16x4) or 2 (4x16)";
@@ -1494,17 +1497,17 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph, in_trace, shape;
+            int stack, persistent, queue, graph, in_trace, shape, postpone;
         };
+        const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
+                         ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone};
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
                 for (int sh : {0, 1, 2})  // shadows in the shading pass, queued, in the trace kernel
                     for (int g : {0, 2})
                         for (int ws : {0, 1, 2})  // a wave's pixels: 8x8, 16x4, 4x16 (tiled primary rays only)
-                            if (ws == 0 || !pe) cands.push_back({st, pe, sh == 1, g, sh == 2, ws});
-        const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
-                         ctx->opt_shadow_trace, ctx->opt_wave_shape};
+                            if (ws == 0 || !pe) cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone});
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
@@ -1512,6 +1515,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             ctx->opt_graph = c.graph;
             ctx->opt_shadow_trace = c.in_trace;
             ctx->opt_wave_shape = c.shape;
+            ctx->opt_postpone = c.postpone;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -1540,6 +1544,15 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                     best = c;
                 }
             }
+            // Then the best one with leaf postponement switched.
+            Cand flip = best;
+            flip.postpone = !best.postpone;
+            cands.push_back(flip);
+            float t = time_cand(flip);
+            if (t < best_ms) {
+                best_ms = t;
+                best = flip;
+            }
         } catch (...) {
             apply(saved);
             throw;
@@ -1553,6 +1566,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->graph = best.graph;
             out->shadow_in_trace = best.in_trace;
             out->wave_shape = best.shape;
+            out->leaf_postpone = best.postpone;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

@@ -218,7 +218,12 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     if (px < v.w && py < v.h) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
-        bool found = !a.S.empty && trace<STATS, typename SS::type, M>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+        bool found = false;
+        if (!a.S.empty) {
+            bool redo = true;
+            if (!STATS && !M && a.postpone) found = trace_pp(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk, redo);
+            if (redo) found = trace<STATS, typename SS::type, M>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+        }
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
         a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
         if constexpr (SH) {

@@ -21,6 +21,7 @@ from .scene import build_frame
 OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
 OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH, OPT_MERGED_LOADS = 1, 2, 3, 4, 5, 6, 7
 OPT_SHADOW_IN_TRACE, OPT_TRACE_WAVES, OPT_STAR_OVERLAP, OPT_WAVE_SHAPE, OPT_TILE_ORDER = 8, 9, 10, 11, 12
+OPT_LEAF_POSTPONE = 13
 # specrel_timings.shadow_path: where the last render traced its shadow queries
 SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 
@@ -55,7 +56,8 @@ class Range(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
                 ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
-                ("shadow_in_trace", C.c_int32), ("wave_shape", C.c_int32), ("reserved", C.c_uint32 * 3)]
+                ("shadow_in_trace", C.c_int32), ("wave_shape", C.c_int32), ("leaf_postpone", C.c_int32),
+                ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

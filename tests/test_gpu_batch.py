@@ -6,8 +6,8 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
-from specrel_amd.renderer import (OPT_GRAPH, OPT_PERSISTENT, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS,
-                                  OPT_WAVE_SHAPE)
+from specrel_amd.renderer import (OPT_GRAPH, OPT_LEAF_POSTPONE, OPT_PERSISTENT, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
+                                  OPT_STACK, OPT_STATS, OPT_WAVE_SHAPE)
 from oracle import pyoracle as O
 from tests.helpers import assert_same
 
@@ -177,7 +177,8 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         t = gpu_ctx.autotune(0, frames, iters=2)
         assert t["stack"] in (8, 16) and t["persistent"] in (0, 1) and t["shadow_queue"] in (0, 1)
         assert t["wave_shape"] in (0, 1, 2) and not (t["persistent"] and t["wave_shape"])
-        assert t["candidates"] == 49 and 0 < t["best_ms"] <= t["default_ms"]
+        assert t["leaf_postpone"] in (0, 1)
+        assert t["candidates"] == 50 and 0 < t["best_ms"] <= t["default_ms"]
         assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "autotuned")
     finally:
         gpu_ctx.set_option(OPT_STACK, 16)
@@ -186,6 +187,7 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         gpu_ctx.set_option(OPT_GRAPH, 1)
         gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
         gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
+        gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
 
 
 def test_large_scene_random_observers(gpu_ctx):

@@ -301,3 +301,32 @@ def test_renders_enqueued_ahead(gpu_ctx, twin):
             gpu_ctx.render_resident(0, f, want_hits=True)
         gpu_ctx.sync()
         assert_same(gpu_ctx.download(192, 120, want_hits=True), O.render(twin, frames[last]), f"last frame {last}")
+
+
+@pytest.mark.parametrize("stack,waves", [(16, 4), (8, 1), (-8, 4)])
+def test_postponed_leaves_are_exact(gpu_ctx, stack, waves):
+    # Closest-hit rays that keep a reached leaf pending and traverse on (the default) and rays that
+    # test every leaf when reached give the oracle's image; instrumented renders test every leaf
+    # when reached, so their counters stay the oracle's with the option on.
+    from specrel_amd.renderer import OPT_LEAF_POSTPONE, OPT_STACK, OPT_STATS, OPT_TRACE_WAVES
+    p = S.gen_params("large", n_objects=16, slices=32, n_stars=100)
+    sc = S.Scene.generate("large", p)
+    fr = S.build_frame(S.default_camera("large", p), S.RenderConfig(480, 320))
+    o = O.render(sc, fr, counts=True)
+    gpu_ctx.upload(0, sc)
+    try:
+        gpu_ctx.set_option(OPT_STACK, stack)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+        for post in (1, 0, 1):
+            gpu_ctx.set_option(OPT_LEAF_POSTPONE, post)
+            assert_same(gpu_ctx.render_frame(0, fr, want_hits=True), o, f"postpone {post}")
+        gpu_ctx.set_option(OPT_STATS, 1)
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+        for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
+            assert st[k] == o["counts"][k], (k, st[k], o["counts"][k])
+    finally:
+        gpu_ctx.set_option(OPT_STATS, 0)
+        gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
+        gpu_ctx.set_option(OPT_STACK, 16)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 4)
