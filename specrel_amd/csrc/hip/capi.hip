@@ -54,6 +54,7 @@ struct specrel_ctx {
     uint32_t stars_for = 0;  // num_stars the star buffers are sized for (per view)
     DevBuf tab, frames, hdr, rgb, hit, lam, tau, slot, prep, dirs, super_lists, super_counts, tables, stats;
     DevBuf tile_cost, tile_order;  // per tile quarter: last traversal cost; tiles by decreasing cost
+    DevBuf tile_cones;             // per tile: the star cone of the shading pass's culling
     DevBuf counters, mask, squeue;
     // Pinned staging of the frame uploads, a ring of slots so that renders can be enqueued ahead of
     // the device: a slot is refilled once its previous copy has completed.
@@ -422,6 +423,7 @@ void ensure_views(specrel_ctx *ctx, const specrel_frame *frames, uint32_t n, uin
         galloc(ctx, ctx->tab, tab.size() * 4);
         // Tile order of the primary rays: row order until a frame has measured the tiles' costs.
         galloc(ctx, ctx->tile_cost, (size_t)ctx->total_tiles * 16);
+        galloc(ctx, ctx->tile_cones, (size_t)ctx->total_tiles * 16);
         galloc(ctx, ctx->tile_order, (size_t)ctx->total_tiles * 4);
         k_tile_init<<<(ctx->total_tiles + 255) / 256, 256, 0, ctx->stream>>>((uint32_t *)ctx->tile_order.p,
                                                                             (uint32_t *)ctx->tile_cost.p,
@@ -639,7 +641,8 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         if (ctx->opt_cull) {
             k_star_bin<<<ctx->total_supers, 256, 0, ss>>>(V, (const StarDir *)ctx->dirs.p, sc.num_stars,
                                                            (uint32_t *)ctx->super_lists.p,
-                                                           (uint32_t *)ctx->super_counts.p);
+                                                           (uint32_t *)ctx->super_counts.p,
+                                                           (float4 *)ctx->tile_cones.p);
             check(hipGetLastError(), "k_star_bin launch");
         }
     };
@@ -700,6 +703,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ra.dirs = (const StarDir *)ctx->dirs.p;
     ra.super_lists = (const uint32_t *)ctx->super_lists.p;
     ra.super_counts = (const uint32_t *)ctx->super_counts.p;
+    ra.tile_cones = (const float4 *)ctx->tile_cones.p;
     ra.num_stars = sc.num_stars;
     ra.cull = ctx->opt_cull;
     ra.want_hits = pl.want_hits ? 1 : 0;
@@ -1050,6 +1054,7 @@ void specrel_destroy(specrel_ctx *ctx) {
     dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
     dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
+    dfree(ctx->tile_cost), dfree(ctx->tile_order), dfree(ctx->tile_cones);
     dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
     dfree(ctx->ex_tubes), dfree(ctx->ex_tris), dfree(ctx->ex_samples), dfree(ctx->ex_decided);
     dfree(ctx->ex_counts), dfree(ctx->ex_offsets), dfree(ctx->ex_first), dfree(ctx->ex_geoms);

@@ -157,10 +157,11 @@ __device__ uint32_t compact_stars(const StarDir *dirs, uint32_t n, float cx, flo
 }
 
 // First culling level: one block per 64x64-pixel super tile keeps, in index order, the stars that
-// can reach any of its pixels.
+// can reach any of its pixels. Its second wave meanwhile computes the cones of its 16x16 tiles for
+// the shading pass's second level.
 // Grid: every super tile of every view.
 __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists,
-                                                  uint32_t *counts) {
+                                                  uint32_t *counts, float4 *cones) {
     const uint32_t gst = blockIdx.x, vi = find_view(V.super_start, V.n, gst);
     const specrel_frame &f = V.frames[vi];
     if (f.mode != SPECREL_MODE_SHADED) return;
@@ -170,9 +171,19 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
     const uint32_t st = gst - V.super_start[vi], x0 = (st % nsx) * SUPER, y0 = (st / nsx) * SUPER;
     __shared__ float s_cone[4];
     __shared__ uint32_t s_cnt[8][4];
-    if (threadIdx.x == 0)
+    constexpr uint32_t PER_SIDE = SUPER / TILE;
+    if (threadIdx.x == 0) {
         tile_cone(v, x0, min(x0 + SUPER, v.w) - 1, y0, min(y0 + SUPER, v.h) - 1, s_cone[0], s_cone[1], s_cone[2],
                   s_cone[3]);
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + PER_SIDE * PER_SIDE) {
+        const uint32_t k = threadIdx.x - 64, tx0 = x0 + (k % PER_SIDE) * TILE, ty0 = y0 + (k / PER_SIDE) * TILE;
+        if (tx0 < v.w && ty0 < v.h) {
+            float c[4];
+            tile_cone(v, tx0, min(tx0 + TILE, v.w) - 1, ty0, min(ty0 + TILE, v.h) - 1, c[0], c[1], c[2], c[3]);
+            const uint32_t tiles_x = (v.w + TILE - 1) / TILE;
+            cones[V.level_start[0][vi] + (ty0 / TILE) * tiles_x + tx0 / TILE] = make_float4(c[0], c[1], c[2], c[3]);
+        }
+    }
     __syncthreads();
     const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
     uint32_t *list = lists + (size_t)gst * SUPER_CAP;
@@ -539,12 +550,8 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         bool all = sc > SUPER_CAP;
         src = all ? nullptr : a.super_lists + (size_t)st * SUPER_CAP;
         n_src = all ? a.num_stars : sc;
-        __shared__ float s_cone[4];
-        if (tid == 0)
-            tile_cone(v, x0, min(x0 + TILE, v.w) - 1, y0, min(y0 + TILE, v.h) - 1, s_cone[0], s_cone[1], s_cone[2],
-                      s_cone[3]);
-        __syncthreads();
-        const float cx = s_cone[0], cy = s_cone[1], cz = s_cone[2], cl = s_cone[3];
+        const float4 cone = a.tile_cones[t];  // tile_cone of this tile, from k_star_bin
+        const float cx = cone.x, cy = cone.y, cz = cone.z, cl = cone.w;
         n_tile = ordered_compact(
             n_src,
             [&](uint32_t k, uint32_t &item) {
