@@ -634,15 +634,16 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const bool overlap = pl.stars && ctx->opt_star_overlap;
     ctx->stars_overlapped = overlap;
     if (overlap) check(hipEventRecord(ctx->fork, s), "fork");
-    auto star_pass = [&](hipStream_t ss) {
+    // order_tiles: k_star_bin also sorts the next render's tile order (one block's share of tiles).
+    auto star_pass = [&](hipStream_t ss, uint32_t order_tiles) {
         k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
             (const float4 *)sc.stars.p, sc.num_stars, V, (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
         check(hipGetLastError(), "k_star_prep launch");
         if (ctx->opt_cull) {
-            k_star_bin<<<ctx->total_supers, 256, 0, ss>>>(V, (const StarDir *)ctx->dirs.p, sc.num_stars,
-                                                           (uint32_t *)ctx->super_lists.p,
-                                                           (uint32_t *)ctx->super_counts.p,
-                                                           (float4 *)ctx->tile_cones.p);
+            k_star_bin<<<ctx->total_supers + (order_tiles ? 1u : 0u), 256, 0, ss>>>(
+                V, (const StarDir *)ctx->dirs.p, sc.num_stars, (uint32_t *)ctx->super_lists.p,
+                (uint32_t *)ctx->super_counts.p, (float4 *)ctx->tile_cones.p, (const uint32_t *)ctx->tile_cost.p,
+                order_tiles, (uint32_t *)ctx->tile_order.p);
             check(hipGetLastError(), "k_star_bin launch");
         }
     };
@@ -671,11 +672,17 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     if (overlap) {
         check(hipStreamWaitEvent(ctx->side, ctx->fork, 0), "fork wait");
         if (events) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
-        star_pass(ctx->side);
+        star_pass(ctx->side, 0u);
         if (events) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
         check(hipEventRecord(ctx->join, ctx->side), "join");
     }
-    if (order) {  // the next frame's tile order, from this frame's costs, beside the later passes
+    // The next frame's tile order, from this frame's costs: by the star binning pass when it runs
+    // after the primary rays on this stream and one block sorts the tiles, else on the side stream
+    // beside the later passes.
+    const bool order_in_bin =
+        order && pl.stars && !overlap && ctx->opt_cull && ctx->total_tiles <= TILE_ORDER_ONE_BLOCK;
+    const bool order_side = order && !order_in_bin;
+    if (order_side) {
         check(hipEventRecord(ctx->order_fork, s), "fork");
         check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
         const uint32_t *cost = (const uint32_t *)ctx->tile_cost.p, n = ctx->total_tiles;
@@ -691,7 +698,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         check(hipGetLastError(), "k_tile_order launch");
         check(hipEventRecord(ctx->order_join, ctx->side), "join");
     }
-    if (pl.stars && !overlap) star_pass(s);
+    if (pl.stars && !overlap) star_pass(s, order_in_bin ? ctx->total_tiles : 0u);
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
@@ -762,7 +769,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     mark(4);
     k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, (uint32_t *)ctx->rgb.p);
     check(hipGetLastError(), "k_composite launch");
-    if (order) check(hipStreamWaitEvent(s, ctx->order_join, 0), "join wait");
+    if (order_side) check(hipStreamWaitEvent(s, ctx->order_join, 0), "join wait");
     mark(5);
 }
 

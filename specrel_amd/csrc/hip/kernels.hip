@@ -156,13 +156,21 @@ __device__ uint32_t compact_stars(const StarDir *dirs, uint32_t n, float cx, flo
     return total;
 }
 
+__device__ void tile_order_block(const uint32_t *cost, uint32_t n_tiles, uint32_t *order);
+
 // First culling level: one block per 64x64-pixel super tile keeps, in index order, the stars that
 // can reach any of its pixels. Its second wave meanwhile computes the cones of its 16x16 tiles for
-// the shading pass's second level.
-// Grid: every super tile of every view.
+// the shading pass's second level. With order_tiles, a leading block writes the next render's
+// tile order (the sort of k_tile_order) beside the binning.
+// Grid: every super tile of every view (+ 1 with order_tiles).
 __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists,
-                                                  uint32_t *counts, float4 *cones) {
-    const uint32_t gst = blockIdx.x, vi = find_view(V.super_start, V.n, gst);
+                                                  uint32_t *counts, float4 *cones, const uint32_t *order_cost,
+                                                  uint32_t order_tiles, uint32_t *order) {
+    if (order_tiles && blockIdx.x == 0) {
+        tile_order_block(order_cost, order_tiles, order);
+        return;
+    }
+    const uint32_t gst = blockIdx.x - (order_tiles ? 1u : 0u), vi = find_view(V.super_start, V.n, gst);
     const specrel_frame &f = V.frames[vi];
     if (f.mode != SPECREL_MODE_SHADED) return;
     const ViewDerived v = derive(f, num_stars);
@@ -429,20 +437,24 @@ __device__ inline uint32_t sum_above(uint32_t v, uint32_t lane) {
     return s - v;
 }
 
-// Up to four tiles per thread: one block.
-__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order) {
+// One block (any size) sorts every tile: up to 4,096 tiles.
+__device__ void tile_order_block(const uint32_t *cost, uint32_t n_tiles, uint32_t *order) {
     __shared__ uint32_t hist[64];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, nt = blockDim.x;
     if (tid < 64) hist[tid] = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < n_tiles; t += 1024) atomicAdd(&hist[tile_bucket(cost, t)], 1u);
+    for (uint32_t t = tid; t < n_tiles; t += nt) atomicAdd(&hist[tile_bucket(cost, t)], 1u);
     __syncthreads();
     if (tid < 64) {  // wave 0
         const uint32_t h = hist[tid];
         hist[tid] = sum_above(h, tid);
     }
     __syncthreads();
-    for (uint32_t t = tid; t < n_tiles; t += 1024) order[atomicAdd(&hist[tile_bucket(cost, t)], 1u)] = t;
+    for (uint32_t t = tid; t < n_tiles; t += nt) order[atomicAdd(&hist[tile_bucket(cost, t)], 1u)] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order) {
+    tile_order_block(cost, n_tiles, order);
 }
 
 // Larger grids: TILE_ORDER_CHUNK tiles per 256-thread block in two launches. k_tile_hist adds

@@ -172,7 +172,7 @@ struct ShadeArgs {
 
 __global__ void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out, StarDir *dirs);
 __global__ void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists, uint32_t *counts,
-                           float4 *cones);
+                           float4 *cones, const uint32_t *order_cost, uint32_t order_tiles, uint32_t *order);
 // Adjoint of the shaded image (adjoint.hip): per-pixel floats, camera sums, pixel classes.
 constexpr int ADJ_PIX = 12;     // d origin[4], d direction[4], d object boost[3], loss term
 constexpr int ADJ_CAM = 21;     // loss, d event[4], d e0..e3[16]
