@@ -86,10 +86,13 @@ enum {
     SPECREL_OPT_LEAF_POSTPONE = 13 /* 1 (default): a closest-hit ray of the primary-ray kernel (LDS
                                      stacks, not merged loads) that reaches a leaf keeps it pending
                                      and traverses on to its next leaf; the pending leaves are tested
-                                     in order once 3/8 of the wave's tracing lanes hold one. 0: every
-                                     leaf is tested when reached. Results do not depend on it;
-                                     instrumented renders (SPECREL_OPT_STATS) test every leaf when
-                                     reached, so their counters are the reference's. */
+                                     in order once 3/8 of the wave's tracing lanes hold one; a ray
+                                     whose stack would pass 62 entries is traced again testing every
+                                     leaf when reached. 2..62: the same with that many entries as the
+                                     limit (to exercise the second trace). 0: every leaf is tested
+                                     when reached. Results do not depend on it; instrumented renders
+                                     (SPECREL_OPT_STATS) test every leaf when reached, so their
+                                     counters are the reference's. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are

@@ -665,7 +665,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
     ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
-    ta.postpone = (uint32_t)ctx->opt_postpone;
+    ta.postpone = ctx->opt_postpone == 1 ? (uint32_t)(STACK - 2) : (uint32_t)ctx->opt_postpone;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(1);
@@ -1111,7 +1111,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         }
         ctx->opt_trace_waves = value;
     } else if (option == SPECREL_OPT_TILE_ORDER) ctx->opt_tile_order = value != 0;
-    else if (option == SPECREL_OPT_LEAF_POSTPONE) ctx->opt_postpone = value != 0;
+    else if (option == SPECREL_OPT_LEAF_POSTPONE) {
+        if (value < 0 || value > STACK - 2) {
+            ctx->err = "leaf postponement must be 0, 1 or a stack limit of 2..62 entries";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_postpone = value;
+    }
     else if (option == SPECREL_OPT_WAVE_SHAPE) {
         if (value < 0 || value > 2) {
             ctx->err = "wave shape must be 0 (8x8), 1 (16x4) or 2 (4x16)";
@@ -1558,7 +1564,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             }
             // Then the best one with leaf postponement switched.
             Cand flip = best;
-            flip.postpone = !best.postpone;
+            flip.postpone = best.postpone ? 0 : 1;
             cands.push_back(flip);
             float t = time_cand(flip);
             if (t < best_ms) {
@@ -1578,7 +1584,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->graph = best.graph;
             out->shadow_in_trace = best.in_trace;
             out->wave_shape = best.shape;
-            out->leaf_postpone = best.postpone;
+            out->leaf_postpone = best.postpone != 0;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

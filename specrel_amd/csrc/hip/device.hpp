@@ -395,13 +395,14 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
 // prunes; every leaf under such a node fails that check when its turn comes (its t_enter is at
 // least that of the pruned node, which reached the t_max the reference had there), so the tested
 // leaves, their order and the hit are the reference's. The extra entries can deepen the stack:
-// when a push would come near the 64-entry limit, whose drop rule depends on the exact depth,
-// `redo` is set and the caller traces the ray with trace(). Counters are not the reference's
-// (more nodes are visited); instrumented renders use trace().
+// when a step would leave more than `limit` entries (at most 62, two below the 64-entry limit,
+// whose drop rule depends on the exact depth), `redo` is set and the caller traces the ray with
+// trace(). Counters are not the reference's (more nodes are visited); instrumented renders use
+// trace().
 constexpr int POSTPONE_EIGHTHS = 3;  // leaf rounds start once 3/8 of a wave's tracing lanes hold a leaf
 template <class ST>
 __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk,
-                         bool &redo) {
+                         int limit, bool &redo) {
     hit.slot = -1;
     redo = false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
@@ -447,7 +448,7 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
             have = false;
             if (!(t_enter >= tmax)) {
                 const int sz = stk.size();
-                if (sz + 2 > STACK - 2) {
+                if (sz + 2 > limit) {
                     redo = true;
                     return false;
                 }

@@ -240,7 +240,8 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
         bool found = false;
         if (!a.S.empty) {
             bool redo = true;
-            if (!STATS && !M && a.postpone) found = trace_pp(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk, redo);
+            if (!STATS && !M && a.postpone)
+                found = trace_pp(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk, (int)a.postpone, redo);
             if (redo) found = trace<STATS, typename SS::type, M>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
         }
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;

@@ -133,7 +133,8 @@ struct TraceArgs {
     const uint32_t *order;  // tile of each scheduling slot (nullptr: slot = tile), heaviest first
     uint32_t *cost;         // per tile quarter: the longest traversal of its lanes, for the next order
     uint32_t prio_slots;    // tiles in the first slots of the order run at a raised wave priority
-    uint32_t postpone;      // uninstrumented closest-hit rays postpone their leaves (trace_pp)
+    uint32_t postpone;      // 0, or uninstrumented closest-hit rays postpone their leaves (trace_pp) and
+                            // are traced again by trace() past this many stack entries
 };
 
 struct OccludeArgs {

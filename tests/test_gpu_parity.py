@@ -307,8 +307,10 @@ def test_renders_enqueued_ahead(gpu_ctx, twin):
 @pytest.mark.parametrize("stack,waves", [(16, 4), (8, 1), (-8, 4)])
 def test_postponed_leaves_are_exact(gpu_ctx, stack, waves):
     # Closest-hit rays that keep a reached leaf pending and traverse on (the default) and rays that
-    # test every leaf when reached give the oracle's image; instrumented renders test every leaf
-    # when reached, so their counters stay the oracle's with the option on.
+    # test every leaf when reached give the oracle's image; so do rays traced again without
+    # postponement once their stack passes a small limit (2 or 5 entries instead of 62, so that
+    # many rays take that path). Instrumented renders test every leaf when reached, so their
+    # counters stay the oracle's with the option on.
     from specrel_amd.renderer import OPT_LEAF_POSTPONE, OPT_STACK, OPT_STATS, OPT_TRACE_WAVES
     p = S.gen_params("large", n_objects=16, slices=32, n_stars=100)
     sc = S.Scene.generate("large", p)
@@ -318,7 +320,7 @@ def test_postponed_leaves_are_exact(gpu_ctx, stack, waves):
     try:
         gpu_ctx.set_option(OPT_STACK, stack)
         gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
-        for post in (1, 0, 1):
+        for post in (1, 0, 2, 5, 1):
             gpu_ctx.set_option(OPT_LEAF_POSTPONE, post)
             assert_same(gpu_ctx.render_frame(0, fr, want_hits=True), o, f"postpone {post}")
         gpu_ctx.set_option(OPT_STATS, 1)
