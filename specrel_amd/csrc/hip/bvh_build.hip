@@ -485,35 +485,50 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
             const sr::BinFrame fr = sr::bin_frame(cb);
             sr::Bins B;
             sr::bins_clear(B);
-            // One pass per live axis with that axis' 8 bins in registers (unions and counts, so the
-            // order of the references changes nothing), stored into B once per axis.
-            for (int a = 0; a < 4; ++a) {
-                if (!fr.live[a]) continue;
+            // One pass per pair of axes with those axes' 8 bins each in registers (unions and
+            // counts, so the order of the references changes nothing), stored into B once per pass.
+            for (int a0 = 0; a0 < 4; a0 += 2) {
+                const bool live0 = fr.live[a0], live1 = fr.live[a0 + 1];
+                if (!live0 && !live1) continue;
                 const float inf = __builtin_inff();
-                float4 bmn[8], bmx[8];
-                uint32_t bn[8];
+                float4 bmn[2][8], bmx[2][8];
+                uint32_t bn[2][8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    bmn[q] = make_float4(inf, inf, inf, inf);
-                    bmx[q] = make_float4(-inf, -inf, -inf, -inf);
-                    bn[q] = 0;
-                }
-#pragma unroll 4
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        bmn[h][q] = make_float4(inf, inf, inf, inf);
+                        bmx[h][q] = make_float4(-inf, -inf, -inf, -inf);
+                        bn[h][q] = 0;
+                    }
+#pragma unroll 2
                 for (uint32_t i = e.b; i < e.b + e.c; ++i) {
-                    const uint32_t k = sr::bin_of(fr, a, comp(f.c[i], a));
-                    const float4 mn = f.mn[i], mx = f.mx[i];
+                    const float4 c = f.c[i], mn = f.mn[i], mx = f.mx[i];
+                    // an axis that is not live puts every reference in a bin no one reads
+                    const uint32_t k0 = live0 ? sr::bin_of(fr, a0, comp(c, a0)) : 0u;
+                    const uint32_t k1 = live1 ? sr::bin_of(fr, a0 + 1, comp(c, a0 + 1)) : 0u;
 #pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (k == (uint32_t)q) {
-                            bmn[q] = f4(sr::vmin(v(bmn[q]), v(mn)));
-                            bmx[q] = f4(sr::vmax(v(bmx[q]), v(mx)));
-                            bn[q] += 1;
+                    for (int q = 0; q < 8; ++q) {
+                        if (k0 == (uint32_t)q) {
+                            bmn[0][q] = f4(sr::vmin(v(bmn[0][q]), v(mn)));
+                            bmx[0][q] = f4(sr::vmax(v(bmx[0][q]), v(mx)));
+                            bn[0][q] += 1;
                         }
+                        if (k1 == (uint32_t)q) {
+                            bmn[1][q] = f4(sr::vmin(v(bmn[1][q]), v(mn)));
+                            bmx[1][q] = f4(sr::vmax(v(bmx[1][q]), v(mx)));
+                            bn[1][q] += 1;
+                        }
+                    }
                 }
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    B.b[a][q] = sr::Box4{v(bmn[q]), v(bmx[q])};
-                    B.n[a][q] = bn[q];
+                for (int h = 0; h < 2; ++h) {
+                    if (!(h ? live1 : live0)) continue;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        B.b[a0 + h][q] = sr::Box4{v(bmn[h][q]), v(bmx[h][q])};
+                        B.n[a0 + h][q] = bn[h][q];
+                    }
                 }
             }
             kind = sr::choose_split(cb, fr, B, axis, pos);
