@@ -192,7 +192,10 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
 
 /* The same render, leaving every output in device memory (no readback) and returning once
  * the work is enqueued; with want_hits the hit/lambda/tau buffers are written too.
- * specrel_sync waits; specrel_download copies the resident outputs of the last render. */
+ * specrel_sync waits; specrel_download copies the resident outputs of the last render. The
+ * frame is copied before the call returns (to pinned staging memory, or into the launch's
+ * arguments), so the caller may change it at once and enqueue further renders; each render's
+ * outputs replace the previous one's. */
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
 int specrel_sync(specrel_ctx *ctx);
 int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau);
