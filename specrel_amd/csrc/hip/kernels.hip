@@ -550,6 +550,12 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     typename StackSel<SK>::Private stk_priv;
     if constexpr (PASS == 0) StackSel<SK>::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
+    // The pixel's hit slot and occlusion bits, loaded before the star culling so that their
+    // latency passes during it.
+    const bool inside = px < v.w && py < v.h;
+    const size_t li = (size_t)py * v.w + px, pi = a.V.pix_start[vi] + li;
+    const uint32_t slot0 = inside ? a.slot[pi] : SPECREL_NO_HIT;
+    const uint32_t mask0 = PASS == 2 && inside ? a.mask[pi] : 0u;
 
     // Star list of this tile: culled from its super tile's list (or from every star when that
     // list overflowed); when this tile's own list overflows, the super-tile list is used as is.
@@ -575,11 +581,10 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
         use_tile_list = n_tile <= TILE_CAP;
     }
 
-    if (px < v.w && py < v.h) {
-        const size_t li = (size_t)py * v.w + px, pi = a.V.pix_start[vi] + li;
+    if (inside) {
         if (PASS == 1) a.mask[pi] = 0u;
         V4 dir = ray_dir(v, f, px, py);
-        uint32_t slot = a.slot[pi];
+        const uint32_t slot = slot0;
         bool found = slot != SPECREL_NO_HIT;
         V3 color = V3{0.0f, 0.0f, 0.0f};
         Rec rec;
@@ -597,7 +602,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
                 EnqueueShadow sh{a.queue, a.qcount, a.qcap, (uint32_t)pi};
                 shade_hit(a.S, a.T, f, dir, rec, sh);
             } else {
-                LookupShadow sh{a.mask[pi]};
+                LookupShadow sh{mask0};
                 color = shade_hit(a.S, a.T, f, dir, rec, sh);
             }
             if (PASS != 1 && f.num_haze > 0) color = haze(color, rec.event, f);
