@@ -73,7 +73,7 @@ enum {
     SPECREL_OPT_SHADOW_IN_TRACE = 8, /* 1 (default): a hit's shadow queries are traced by the primary-ray
                                        kernel right after its ray (LDS stacks, tiled rays, no queue);
                                        0: by the shading pass */
-    SPECREL_OPT_TRACE_WAVES = 9, /* 1, 2 (default) or 4: waves per block of the primary-ray kernel with
+    SPECREL_OPT_TRACE_WAVES = 9, /* 1 (default), 2 or 4: waves per block of the primary-ray kernel with
                                     shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
     SPECREL_OPT_STAR_OVERLAP = 10, /* 1: the star pass runs on a second stream beside the primary
                                      rays; 0 (default): after them on the same stream */

@@ -89,7 +89,7 @@ struct specrel_ctx {
     int opt_graph = 1;
     int opt_merged = 0;
     int opt_shadow_trace = 1;
-    int opt_trace_waves = 2;
+    int opt_trace_waves = 1;
     int opt_wave_shape = 0;
     int opt_tile_order = 1;
     int opt_postpone = 1;

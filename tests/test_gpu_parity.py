@@ -172,7 +172,7 @@ def test_primary_blocks_of_fewer_waves_are_exact(gpu_ctx, twin, waves, depth):
         st = gpu_ctx.stats()
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 2)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
         gpu_ctx.set_option(OPT_STACK, 16)
     assert_same(g, o, f"{waves} waves per block, stack {depth}")
     oc = O.render(twin, fr, counts=True)["counts"]
@@ -218,7 +218,7 @@ def test_wave_shapes_are_exact(gpu_ctx, twin, shape, waves):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 2)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
     assert_same(g, o, f"wave shape {shape}, {waves} waves per block")
     oc = O.render(twin, fr, counts=True)["counts"]
     for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
@@ -283,7 +283,7 @@ def test_tile_order_changes_nothing(gpu_ctx, waves, stars):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_TILE_ORDER, 1)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 2)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
 
 
 def test_renders_enqueued_ahead(gpu_ctx, twin):
@@ -332,4 +332,4 @@ def test_postponed_leaves_are_exact(gpu_ctx, stack, waves):
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
         gpu_ctx.set_option(OPT_STACK, 16)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 2)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
