@@ -47,6 +47,7 @@ typedef struct specrel_timings {
     float adjoint_ms;     /* the last specrel_adjoint's kernels */
     uint32_t shadow_path; /* where the last render traced its shadow queries: SPECREL_SHADOWS_* */
     float extrude_ms;     /* the last specrel_extrude_tubes' kernels */
+    uint32_t trace_block_waves; /* waves per block the last render's primary-ray kernel ran with */
 } specrel_timings;
 
 enum {
@@ -73,8 +74,10 @@ enum {
     SPECREL_OPT_SHADOW_IN_TRACE = 8, /* 1 (default): a hit's shadow queries are traced by the primary-ray
                                        kernel right after its ray (LDS stacks, tiled rays, no queue);
                                        0: by the shading pass */
-    SPECREL_OPT_TRACE_WAVES = 9, /* 1 (default), 2 or 4: waves per block of the primary-ray kernel with
-                                    shadows traced there (stack 8 or 16); each wave one 8x8 quarter */
+    SPECREL_OPT_TRACE_WAVES = 9, /* 1 (default), 2 or 4: waves per block of the tiled primary-ray kernel
+                                    with an LDS stack, each wave one quarter of a 16x16 tile; the
+                                    persistent kernel and register stacks run blocks of 4
+                                    (specrel_timings.trace_block_waves reports what ran) */
     SPECREL_OPT_STAR_OVERLAP = 10, /* 1: the star pass runs on a second stream beside the primary
                                      rays; 0 (default): after them on the same stream */
     SPECREL_OPT_WAVE_SHAPE = 11,  /* pixels of a 16x16 tile one wave of the tiled primary-ray kernel
@@ -95,13 +98,15 @@ enum {
                                      counters are the reference's. */
 };
 
-/* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). Sizes are
- * the view's width*height. */
+/* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the
+ * element count of each non-NULL buffer and must be the view's width*height (else
+ * SPECREL_EINVAL before any work, where the reference asserts, gpu.rs:808). */
 typedef struct specrel_view_out {
     uint32_t *rgb;
     uint32_t *hit;
     float *lambda;
     float *tau;
+    uint64_t len;
 } specrel_view_out;
 
 /* Launch configuration chosen by specrel_autotune (it is also applied to the context). */
@@ -116,7 +121,8 @@ typedef struct specrel_tuning {
     int32_t shadow_in_trace; /* SPECREL_OPT_SHADOW_IN_TRACE */
     int32_t wave_shape;      /* SPECREL_OPT_WAVE_SHAPE */
     int32_t leaf_postpone;   /* SPECREL_OPT_LEAF_POSTPONE */
-    uint32_t reserved[2];
+    int32_t trace_waves;     /* SPECREL_OPT_TRACE_WAVES */
+    uint32_t reserved[1];
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
@@ -186,9 +192,13 @@ uint32_t specrel_num_scenes(const specrel_ctx *ctx);
 /* GpuRenderer::render (gpu.rs:798-900): render one view of scenes[scene_idx] and wait.
  * out_rgb receives width*height 0x00RRGGBB pixels (required). The optional per-pixel outputs
  * (NULL to skip) are the hit tetrahedron's input-order index (SPECREL_NO_HIT on a miss), the
- * hit's ray parameter lambda (+inf on a miss) and its interpolated proper time (0 on a miss). */
+ * hit's ray parameter lambda (+inf on a miss) and its interpolated proper time (0 on a miss).
+ * len is the element count of each output buffer: it must equal width*height of the frame, or
+ * the call returns SPECREL_EINVAL before any work (the reference's
+ * `assert_eq!(framebuffer.len(), w * h)`, gpu.rs:808). The outputs are copied behind the render
+ * on its stream into pinned memory and from there into the caller's buffers. */
 int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *out_rgb,
-                   uint32_t *out_hit, float *out_lambda, float *out_tau);
+                   uint32_t *out_hit, float *out_lambda, float *out_tau, uint64_t len);
 
 /* The same render, leaving every output in device memory (no readback) and returning once
  * the work is enqueued; with want_hits the hit/lambda/tau buffers are written too.
@@ -198,11 +208,14 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
  * outputs replace the previous one's. */
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
 int specrel_sync(specrel_ctx *ctx);
-int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau);
+/* len: element count of each non-NULL buffer, which must be the view's width*height. */
+int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau,
+                     uint64_t len);
 
 /* The linear HDR radiance of view `view` of the last render before bloom and tone mapping (the
- * image the adjoint's loss is defined on): width*height*3 floats, RGB per pixel. */
-int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3);
+ * image the adjoint's loss is defined on): width*height*3 floats, RGB per pixel; len (floats)
+ * must be width*height*3. */
+int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3, uint64_t len);
 
 /* Batched observers: n_views (1..4096) frames of any sizes and modes rendered over one scene
  * with one launch per stage (each kernel's grid spans the tiles of every view), replayed from a
@@ -217,12 +230,12 @@ int specrel_render_batch(specrel_ctx *ctx, uint32_t scene_idx, const specrel_fra
 int specrel_render_batch_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames,
                                   uint32_t n_views, int want_hits);
 int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda,
-                          float *out_tau);
+                          float *out_tau, uint64_t len);
 /* Times every launch configuration (stack depth 16/8 in LDS, persistent or tiled primary rays,
  * shadow rays in the shading pass, queued or in the primary-ray kernel, graph replay or direct
  * launches, the wave shape of tiled primary rays), then the fastest with leaf postponement
- * switched, on this batch, `iters` renders each after one warm-up, and keeps the fastest by
- * median wall time. */
+ * switched, then the fastest with each other block size of tiled primary rays, on this batch,
+ * `iters` renders each after one warm-up, and keeps the fastest by median wall time. */
 int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                      uint32_t iters, specrel_tuning *out);
 
@@ -243,7 +256,9 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
  * tetrahedron); the others (misses, silhouettes, texture, shadow and facet edges) are counted per
  * class and left out. mask_in (width*height bytes, nonzero =
  * valid; may be NULL) replaces that test, so a loss can be evaluated on a fixed pixel set;
- * mask_out (may be NULL) receives the mask used. d_objects receives 3 doubles per object. */
+ * mask_out (may be NULL) receives the mask used. n_pixels is the pixel count the target and the
+ * masks hold and must equal the last render's (SPECREL_EINVAL otherwise). d_objects receives 3
+ * doubles per object. */
 typedef struct specrel_adj_object {
     uint32_t prim_first, prim_count; /* input-order primitive range; objects sorted and disjoint */
     float center[4];                 /* event the boost is about (t, x, y, z) */
@@ -257,16 +272,16 @@ typedef struct specrel_adjoint_out {
 } specrel_adjoint_out;
 
 int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
-                    const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out,
-                    double *d_objects);
+                    uint64_t n_pixels, const specrel_adj_object *objects, uint32_t n_objects,
+                    specrel_adjoint_out *out, double *d_objects);
 /* The same for every view of the last render (a batch of uneven views, each with its own
  * observer): one launch per pass over all views' tiles, one gradient per view. targets, masks_in
  * and masks_out hold the views' pixels one after another (view 0 first, width*height each, 3
- * floats per pixel for targets); outs receives n_views results and d_objects n_views * 3 *
- * n_objects doubles (view-major). */
+ * floats per pixel for targets), n_pixels of them in all; outs receives n_views results and
+ * d_objects n_views * 3 * n_objects doubles (view-major). */
 int specrel_adjoint_batch(specrel_ctx *ctx, const float *targets, const uint8_t *masks_in, uint8_t *masks_out,
-                          const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *outs,
-                          double *d_objects);
+                          uint64_t n_pixels, const specrel_adj_object *objects, uint32_t n_objects,
+                          specrel_adjoint_out *outs, double *d_objects);
 
 int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
 /* Counters of the last render made with SPECREL_OPT_STATS on. */

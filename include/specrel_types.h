@@ -149,7 +149,10 @@ typedef struct specrel_stats {
     uint64_t shadow_box_tests;
     uint64_t shadow_prim_tests;
     uint64_t star_tests;         /* per-pixel star kernel tests after tile culling */
-    uint64_t reserved[6];
+    uint64_t distinct_nodes;     /* interior nodes whose child boxes some ray tested (each once) */
+    uint64_t distinct_prims;     /* primitive records some ray tested (each once) */
+    uint64_t distinct_hits;      /* primitives whose attributes some hit read (each once) */
+    uint64_t reserved[3];
 } specrel_stats;
 
 #ifdef __cplusplus

@@ -173,6 +173,15 @@ typedef struct {
     float bary[4];
 } o_hit;
 
+/* Distinct records a frame touches (oracle_render with counts): one bit per interior node whose
+ * children's boxes were tested, per primitive record tested and per hit primitive whose attributes
+ * were interpolated. Set for the worker threads of one render. */
+typedef struct {
+    _Atomic uint32_t *nodes, *prims, *hits;
+} o_touch;
+static _Thread_local const o_touch *tl_touch;
+static void touch_bit(_Atomic uint32_t *bits, uint32_t i) { atomic_fetch_or(&bits[i >> 5], 1u << (i & 31u)); }
+
 /* Bvh4D::trace (bvh4d.rs:445-520), up to the leaf slot of the winner. */
 static int trace(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float tmax, o_hit *hit,
                  oracle_counts *c) {
@@ -200,6 +209,7 @@ static int trace(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float tm
                 float lam, b[4];
                 o4 ev;
                 if (c) c->prim_tests++;
+                if (tl_touch) touch_bit(tl_touch->prims, i);
                 if (intersect_fast(&s->geoms[i], o, d, tmin, tmax, &lam, &ev, b)) {
                     tmax = lam;
                     hit->slot = i;
@@ -212,6 +222,7 @@ static int trace(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float tm
             uint32_t li = node->left, ri = node->right_or_first;
             float tl, tr;
             if (c) c->box_tests += 2;
+            if (tl_touch) touch_bit(tl_touch->nodes, ni);
             int hl = aabb_ray(N[li].min, N[li].max, o, inv, tmin, tmax, &tl);
             int hr = aabb_ray(N[ri].min, N[ri].max, o, inv, tmin, tmax, &tr);
             if (hl && hr) {
@@ -247,7 +258,8 @@ static int occluded(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float
     st[0] = 0;
     while (sp > 0) {
         sp -= 1;
-        const specrel_node *node = &N[st[sp]];
+        const uint32_t ni = st[sp];
+        const specrel_node *node = &N[ni];
         if (c) c->shadow_node_pops++;
         if (node->count > 0) {
             uint32_t first = node->right_or_first, end = first + node->count;
@@ -255,12 +267,14 @@ static int occluded(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float
                 float lam, b[4];
                 o4 ev;
                 if (c) c->shadow_prim_tests++;
+                if (tl_touch) touch_bit(tl_touch->prims, i);
                 if (intersect_fast(&s->geoms[i], o, d, tmin, tmax, &lam, &ev, b)) return 1;
             }
         } else {
             uint32_t li = node->left, ri = node->right_or_first;
             float t;
             if (c) c->shadow_box_tests += 2;
+            if (tl_touch) touch_bit(tl_touch->nodes, ni);
             int hl = aabb_ray(N[li].min, N[li].max, o, inv, tmin, tmax, &t);
             int hr = aabb_ray(N[ri].min, N[ri].max, o, inv, tmin, tmax, &t);
             if (hr && sp < 64) st[sp++] = ri;
@@ -1027,9 +1041,12 @@ typedef struct {
     atomic_uint next_row;
     pthread_mutex_t lock;
     oracle_counts *counts;
+    const o_touch *touch;
 } o_job;
 
-static void shade_row(o_job *J, uint32_t py, float *hdr_row, oracle_counts *c) {
+/* One image row: HDR into hdr_row, hit / lambda / tau (when asked) at row `out_row` of those
+ * arrays (the frame row, or the row's place in a sample). */
+static void shade_row(o_job *J, uint32_t py, uint32_t out_row, float *hdr_row, oracle_counts *c) {
     const specrel_frame *f = J->base.f;
     o_ctx x = J->base;
     x.c = c;
@@ -1057,6 +1074,7 @@ static void shade_row(o_job *J, uint32_t py, float *hdr_row, oracle_counts *c) {
         o3 color;
         if (found) {
             if (c) c->hits++;
+            if (tl_touch) touch_bit(tl_touch->hits, (uint32_t)h.slot);
             interpolate_hit(&J->base.s->attrs[h.slot], &h, &rec);
             color = shade_hit(&x, dir, &rec);
             if (f->num_haze > 0) color = haze(color, rec.event, (const float (*)[4])f->haze_origin, f->num_haze);
@@ -1072,7 +1090,7 @@ static void shade_row(o_job *J, uint32_t py, float *hdr_row, oracle_counts *c) {
         hdr_row[3 * px] = color.x;
         hdr_row[3 * px + 1] = color.y;
         hdr_row[3 * px + 2] = color.z;
-        size_t pi = (size_t)py * J->w + px;
+        size_t pi = (size_t)out_row * J->w + px;
         if (J->hit_index) J->hit_index[pi] = found ? J->base.s->order[h.slot] : SPECREL_NO_HIT;
         if (J->lambda) J->lambda[pi] = found ? h.lambda : INFINITY;
         if (J->tau) J->tau[pi] = found ? rec.proper_time : 0.0f;
@@ -1081,14 +1099,15 @@ static void shade_row(o_job *J, uint32_t py, float *hdr_row, oracle_counts *c) {
 
 static void *worker(void *arg) {
     o_job *J = (o_job *)arg;
+    tl_touch = J->touch;
     oracle_counts local;
     memset(&local, 0, sizeof local);
     for (;;) {
         uint32_t k = atomic_fetch_add(&J->next_row, 1);
         if (k >= J->nrows) break;
         uint32_t py = J->rows ? J->rows[k] : k;
-        float *dst = J->rows ? J->hdr + (size_t)k * J->w * 3 : J->hdr + (size_t)py * J->w * 3;
-        shade_row(J, py, dst, J->counts ? &local : NULL);
+        const uint32_t out_row = J->rows ? k : py;
+        shade_row(J, py, out_row, J->hdr + (size_t)out_row * J->w * 3, J->counts ? &local : NULL);
     }
     if (J->counts) {
         pthread_mutex_lock(&J->lock);
@@ -1096,6 +1115,7 @@ static void *worker(void *arg) {
         for (size_t i = 0; i < sizeof(oracle_counts) / 8; ++i) a[i] += b[i];
         pthread_mutex_unlock(&J->lock);
     }
+    tl_touch = NULL;
     return NULL;
 }
 
@@ -1150,8 +1170,23 @@ int oracle_render(const specrel_scene_arrays *s, const specrel_frame *f, int nth
     J.tau = tau;
     J.nrows = h;
     J.counts = counts;
+    o_touch touch = {NULL, NULL, NULL};
+    const size_t wn = ((size_t)s->num_nodes + 31) / 32, wp = ((size_t)s->num_prims + 31) / 32;
+    if (counts) {
+        touch.nodes = (_Atomic uint32_t *)calloc(wn + 1, 4);
+        touch.prims = (_Atomic uint32_t *)calloc(wp + 1, 4);
+        touch.hits = (_Atomic uint32_t *)calloc(wp + 1, 4);
+        J.touch = &touch;
+    }
     atomic_init(&J.next_row, 0);
     run_workers(&J, nthreads);
+    if (counts) {
+        counts->distinct_nodes = counts->distinct_prims = counts->distinct_hits = 0;
+        for (size_t i = 0; i < wn; ++i) counts->distinct_nodes += (uint64_t)__builtin_popcount(touch.nodes[i]);
+        for (size_t i = 0; i < wp; ++i) counts->distinct_prims += (uint64_t)__builtin_popcount(touch.prims[i]);
+        for (size_t i = 0; i < wp; ++i) counts->distinct_hits += (uint64_t)__builtin_popcount(touch.hits[i]);
+        free((void *)touch.nodes), free((void *)touch.prims), free((void *)touch.hits);
+    }
     if (hdr_out) memcpy(hdr_out, J.hdr, sizeof(float) * 3 * (size_t)w * h);
     oracle_bloom(J.hdr, w, h);
     if (rgb)
@@ -1163,7 +1198,8 @@ int oracle_render(const specrel_scene_arrays *s, const specrel_frame *f, int nth
 }
 
 int oracle_render_rows(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, uint32_t row0,
-                       uint32_t step, uint32_t nrows, float *hdr_rows, oracle_counts *counts) {
+                       uint32_t step, uint32_t nrows, float *hdr_rows, uint32_t *hit_rows, float *lambda_rows,
+                       float *tau_rows, oracle_counts *counts) {
     uint8_t lut[SPECREL_LUT_SIZE];
     oracle_gamma_lut(lut);
     o_job J;
@@ -1173,6 +1209,9 @@ int oracle_render_rows(const specrel_scene_arrays *s, const specrel_frame *f, in
     J.rows = rows;
     J.nrows = nrows;
     J.hdr = hdr_rows;
+    J.hit_index = hit_rows;
+    J.lambda = lambda_rows;
+    J.tau = tau_rows;
     J.counts = counts;
     atomic_init(&J.next_row, 0);
     run_workers(&J, nthreads);

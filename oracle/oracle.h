@@ -21,16 +21,21 @@ typedef struct oracle_counts {
     uint64_t rays, node_pops, box_tests, prim_tests, hits;
     uint64_t shadow_rays, shadow_node_pops, shadow_box_tests, shadow_prim_tests;
     uint64_t star_pairs; /* star x pixel pairs evaluated (all stars, every pixel) */
+    /* oracle_render only: distinct interior nodes whose children were tested, distinct primitive
+     * records tested and distinct hit primitives shaded (each counted once per frame) */
+    uint64_t distinct_nodes, distinct_prims, distinct_hits;
 } oracle_counts;
 
 /* Full frame: HDR shading, bloom, tone map (render_frame, renderer.rs:327-412).
  * Optional outputs may be NULL. hdr_out receives the pre-bloom HDR image (3 floats/pixel). */
 int oracle_render(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, uint32_t *rgb,
                   uint32_t *hit_index, float *lambda, float *tau, float *hdr_out, oracle_counts *counts);
-/* Shade only rows row0, row0+step, ... (nrows of them) into hdr_rows (3 floats/pixel);
- * the CPU-baseline sample. */
+/* Shade only rows row0, row0+step, ... (nrows of them) into hdr_rows (3 floats/pixel, the
+ * pre-bloom image) and, when not NULL, their hit index / lambda / tau (w per row); the
+ * CPU-baseline sample. */
 int oracle_render_rows(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, uint32_t row0,
-                       uint32_t step, uint32_t nrows, float *hdr_rows, oracle_counts *counts);
+                       uint32_t step, uint32_t nrows, float *hdr_rows, uint32_t *hit_rows, float *lambda_rows,
+                       float *tau_rows, oracle_counts *counts);
 void oracle_bloom(float *hdr3, uint32_t w, uint32_t h);
 void oracle_pack(const float *hdr3, uint32_t n, uint32_t *out);
 void oracle_gamma_lut(uint8_t out[SPECREL_LUT_SIZE]);

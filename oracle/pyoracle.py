@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 class Counts(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays",
                                           "shadow_node_pops", "shadow_box_tests", "shadow_prim_tests",
-                                          "star_pairs")]
+                                          "star_pairs", "distinct_nodes", "distinct_prims", "distinct_hits")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -36,7 +36,7 @@ def lib():
             "oracle_render": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.c_int, _U32P, _U32P, _FP,
                                         _FP, _FP, C.POINTER(Counts)]),
             "oracle_render_rows": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.c_int, C.c_uint32,
-                                             C.c_uint32, C.c_uint32, _FP, C.POINTER(Counts)]),
+                                             C.c_uint32, C.c_uint32, _FP, _U32P, _FP, _FP, C.POINTER(Counts)]),
             "oracle_bloom": (None, [_FP, C.c_uint32, C.c_uint32]),
             "oracle_pack": (None, [_FP, C.c_uint32, _U32P]),
             "oracle_gamma_lut": (None, [C.POINTER(C.c_uint8)]),
@@ -75,6 +75,16 @@ def lib():
     return _lib
 
 
+def threads():
+    """Worker threads of an oracle call: the cores this process may run on, at most 16 (the host
+    share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def f32(a, n=None):
     a = np.ascontiguousarray(np.asarray(a, np.float32))
     return a, a.ctypes.data_as(_FP)
@@ -90,7 +100,7 @@ def render(scene, frame, nthreads=None, want_hdr=False, counts=False):
     tau = np.zeros(n, np.float32)
     hdr = np.zeros(n * 3, np.float32) if want_hdr else None
     cnt = Counts() if counts else None
-    nthreads = nthreads or os.cpu_count() or 1
+    nthreads = nthreads or threads()
     lib().oracle_render(C.byref(scene.arrays), C.byref(frame), nthreads, rgb.ctypes.data_as(_U32P),
                         hit.ctypes.data_as(_U32P), lam.ctypes.data_as(_FP), tau.ctypes.data_as(_FP),
                         None if hdr is None else hdr.ctypes.data_as(_FP), None if cnt is None else C.byref(cnt))
@@ -102,13 +112,23 @@ def render(scene, frame, nthreads=None, want_hdr=False, counts=False):
     return out
 
 
-def render_rows(scene, frame, row0, step, nrows, nthreads=None, counts=False):
+def render_rows(scene, frame, row0, step, nrows, nthreads=None, counts=False, hits=False):
+    """Rows row0, row0 + step, ... of the frame: the pre-bloom HDR (nrows, w, 3), the counts when
+    asked, and with hits a dict of the rows' hit index / lambda / tau (nrows, w)."""
     w = frame.size[0]
     hdr = np.zeros(nrows * w * 3, np.float32)
     cnt = Counts() if counts else None
-    lib().oracle_render_rows(C.byref(scene.arrays), C.byref(frame), nthreads or os.cpu_count() or 1, row0, step,
-                             nrows, hdr.ctypes.data_as(_FP), None if cnt is None else C.byref(cnt))
-    return hdr.reshape(nrows, w, 3), (cnt.as_dict() if cnt is not None else None)
+    hit = np.zeros(nrows * w, np.uint32) if hits else None
+    lam = np.zeros(nrows * w, np.float32) if hits else None
+    tau = np.zeros(nrows * w, np.float32) if hits else None
+    lib().oracle_render_rows(C.byref(scene.arrays), C.byref(frame), nthreads or threads(), row0, step,
+                             nrows, hdr.ctypes.data_as(_FP), None if hit is None else hit.ctypes.data_as(_U32P),
+                             None if lam is None else lam.ctypes.data_as(_FP),
+                             None if tau is None else tau.ctypes.data_as(_FP), None if cnt is None else C.byref(cnt))
+    out = (hdr.reshape(nrows, w, 3), (cnt.as_dict() if cnt is not None else None))
+    if hits:
+        out = out + ({"hit": hit.reshape(nrows, w), "lam": lam.reshape(nrows, w), "tau": tau.reshape(nrows, w)},)
+    return out
 
 
 def bloom(hdr):
@@ -201,7 +221,7 @@ def adjoint_ref(scene, frame, theta, target, objects, mask_in=None, nthreads=Non
     rc = lib().oracle_adjoint_ref(C.byref(scene.arrays), C.byref(frame), th.ctypes.data_as(C.POINTER(C.c_double)),
                                   tgt.ctypes.data_as(_FP), None if mi is None else mi.ctypes.data_as(u8),
                                   first.ctypes.data_as(_U32P), count.ctypes.data_as(_U32P),
-                                  center.ctypes.data_as(_FP), n, int(frame_values), nthreads or os.cpu_count() or 1,
+                                  center.ctypes.data_as(_FP), n, int(frame_values), nthreads or threads(),
                                   C.byref(loss),
                                   grad.ctypes.data_as(C.POINTER(C.c_double)),
                                   cnt.ctypes.data_as(C.POINTER(C.c_uint64)), mo.ctypes.data_as(u8), C.byref(err))
@@ -223,7 +243,7 @@ def pixel_states(scene, frame, nthreads=None):
     w, h = frame.size[0], frame.size[1]
     out = np.zeros(w * h, PIXEL_STATE)
     assert out.itemsize == 80
-    if lib().oracle_pixel_states(C.byref(scene.arrays), C.byref(frame), nthreads or os.cpu_count() or 1,
+    if lib().oracle_pixel_states(C.byref(scene.arrays), C.byref(frame), nthreads or threads(),
                                  out.ctypes.data) != 0:
         raise RuntimeError("oracle_pixel_states: unsupported view")
     return out.reshape(h, w)

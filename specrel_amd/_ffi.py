@@ -50,7 +50,8 @@ class Frame(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays",
                                           "shadow_node_pops", "shadow_box_tests", "shadow_prim_tests",
-                                          "star_tests")] + [("reserved", C.c_uint64 * 6)]
+                                          "star_tests", "distinct_nodes", "distinct_prims", "distinct_hits")] + \
+        [("reserved", C.c_uint64 * 3)]
 
 
 class WlSample(C.Structure):

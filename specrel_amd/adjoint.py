@@ -311,7 +311,7 @@ def batch_targets(ctx, slot, hidden, sizes):
     frames = [o.frame(w, h) for o, (w, h) in zip(hidden, sizes)]
     ctx.render_batch_resident(slot, frames)
     ctx.sync()
-    return [ctx.download_linear(v, w, h) for v, (w, h) in enumerate(sizes)]
+    return [ctx.download_linear(v) for v in range(len(sizes))]
 
 
 def batch_gradients(ctx, slot, obs, sizes, targets, objects, masks_in=None, want_mask=False):

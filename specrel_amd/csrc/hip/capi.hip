@@ -19,9 +19,26 @@ using namespace dev;
 
 namespace {
 
+// Device allocation owned by one holder: freed when the holder goes (also on the throw paths of
+// the entry points), moved but never copied.
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr, o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) {
+            if (p) (void)hipFree(p);
+            p = o.p, bytes = o.bytes;
+            o.p = nullptr, o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
 };
 
 struct SceneSlot {
@@ -56,6 +73,8 @@ struct specrel_ctx {
     DevBuf tile_cost, tile_order;  // per tile quarter: last traversal cost; tiles by decreasing cost
     DevBuf tile_cones;             // per tile: the star cone of the shading pass's culling
     DevBuf counters, mask, squeue;
+    DevBuf touch;  // instrumented renders: bitmaps of the distinct nodes, primitives and hits touched
+    uint32_t touch_words[3] = {};  // words of each bitmap of the last instrumented render
     // Pinned staging of the frame uploads, a ring of slots so that renders can be enqueued ahead of
     // the device: a slot is refilled once its previous copy has completed.
     struct Stage {
@@ -67,6 +86,12 @@ struct specrel_ctx {
     static constexpr uint32_t STAGES = 4;
     Stage stage[STAGES];
     uint32_t stage_next = 0;
+    // Pinned host memory the readbacks land in (grow-only): the device-to-host copies are enqueued
+    // behind the render on its stream and the caller's buffers filled from here by memcpy, as
+    // GpuRenderer::render copies its mapped readback buffer (gpu.rs:853-886).
+    void *readback = nullptr;
+    size_t readback_cap = 0;
+    uint64_t ex_records = 0;  // records the last specrel_extrude_tubes left on the device
     // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
     DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
     DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
@@ -465,26 +490,27 @@ dim3 resident_grid(specrel_ctx *ctx, Kern kern) {
     return dim3((unsigned)(per_cu * ctx->cus));
 }
 
-// Primary rays with an LDS stack; M: merged load rounds.
+// Primary rays with an LDS stack in blocks of 1, 2 or 4 waves (SPECREL_OPT_TRACE_WAVES), each wave
+// one quarter of a 16x16 tile; M: merged load rounds; SH: shadow queries traced here.
+template <int K, bool M, bool SH>
+void launch_lds_w(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
+    const int w = ctx->opt_trace_waves;
+    const dim3 g(tiles.x * 4 / w);
+    if (w == 1) {
+        if (stats) k_trace<true, K, M, SH, 1><<<g, 64, 0, s>>>(ta);
+        else k_trace<false, K, M, SH, 1><<<g, 64, 0, s>>>(ta);
+    } else if (w == 2) {
+        if (stats) k_trace<true, K, M, SH, 2><<<g, 128, 0, s>>>(ta);
+        else k_trace<false, K, M, SH, 2><<<g, 128, 0, s>>>(ta);
+    } else {
+        if (stats) k_trace<true, K, M, SH, 4><<<g, 256, 0, s>>>(ta);
+        else k_trace<false, K, M, SH, 4><<<g, 256, 0, s>>>(ta);
+    }
+}
 template <int K, bool M>
 void launch_lds(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
-    if (ta.mask && (K == 8 || K == 16) && ctx->opt_trace_waves != 4) {  // shadow queries traced here
-        constexpr int KK = K == 8 ? 8 : 16;
-        const dim3 g(tiles.x * 4 / ctx->opt_trace_waves);
-        if (ctx->opt_trace_waves == 1) {
-            if (stats) k_trace<true, KK, M, true, 1><<<g, 64, 0, s>>>(ta);
-            else k_trace<false, KK, M, true, 1><<<g, 64, 0, s>>>(ta);
-        } else {
-            if (stats) k_trace<true, KK, M, true, 2><<<g, 128, 0, s>>>(ta);
-            else k_trace<false, KK, M, true, 2><<<g, 128, 0, s>>>(ta);
-        }
-    } else if (ta.mask) {  // shadow queries traced here
-        if (stats) k_trace<true, K, M, true><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K, M, true><<<tiles, 256, 0, s>>>(ta);
-    } else {
-        if (stats) k_trace<true, K, M, false><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K, M, false><<<tiles, 256, 0, s>>>(ta);
-    }
+    if (ta.mask) launch_lds_w<K, M, true>(ctx, stats, tiles, s, ta);
+    else launch_lds_w<K, M, false>(ctx, stats, tiles, s, ta);
 }
 
 template <int K>
@@ -496,8 +522,8 @@ void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const Tra
         if (ctx->opt_merged) launch_lds<K, true>(ctx, stats, tiles, s, ta);
         else launch_lds<K, false>(ctx, stats, tiles, s, ta);
     } else {
-        if (stats) k_trace<true, K, false, false><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K, false, false><<<tiles, 256, 0, s>>>(ta);
+        if (stats) k_trace<true, K, false, false, 4><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, false, false, 4><<<tiles, 256, 0, s>>>(ta);
     }
 }
 template <int K>
@@ -551,6 +577,12 @@ struct Plan {
     bool frame_in_reset = false;  // one view launched directly: its frame goes to k_reset_frame
     FrameWords frame;
 };
+
+// Waves per block of the primary-ray kernel as launched: SPECREL_OPT_TRACE_WAVES for tiled rays
+// with LDS stacks, else 4 (persistent rays, register stacks).
+uint32_t trace_block_waves(const specrel_ctx *ctx) {
+    return (ctx->opt_stack > 0 && !ctx->opt_persistent) ? (uint32_t)ctx->opt_trace_waves : 4u;
+}
 
 // Where a render's shadow queries are traced (specrel_timings.shadow_path): queued for a dense
 // pass, in the primary-ray kernel (LDS stacks, tiled rays), or by the shading pass itself.
@@ -623,6 +655,13 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const Views V = device_views(ctx);
     const uint32_t nv = ctx->n_views;
     uint32_t *counters = (uint32_t *)ctx->counters.p;
+    Scene S = sc.S;
+    if (stats) {  // distinct records touched (cleared in enqueue_views)
+        uint32_t *t = (uint32_t *)ctx->touch.p;
+        S.touch_nodes = t;
+        S.touch_prims = t + ctx->touch_words[0];
+        S.touch_hits = t + ctx->touch_words[0] + ctx->touch_words[1];
+    }
     if (pl.frame_in_reset)
         k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, (uint32_t *)ctx->frames.p);
     else
@@ -651,7 +690,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const uint32_t path = shadow_path(ctx, pl);
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
     TraceArgs ta;
-    ta.S = sc.S;
+    ta.S = S;
     ta.V = V;
     ta.num_tiles = ctx->total_tiles;
     ta.slot = (uint32_t *)ctx->slot.p;
@@ -702,7 +741,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
-    ra.S = sc.S;
+    ra.S = S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;
     ra.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
     ra.V = V;
@@ -730,7 +769,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
         check(hipGetLastError(), "k_shade setup launch");
         OccludeArgs oa;
-        oa.S = sc.S;
+        oa.S = S;
         oa.queue = ra.queue;
         oa.count = counters + 1;
         oa.next = counters + 2;
@@ -789,7 +828,14 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     const Plan pl = prepare(ctx, scene_idx, frames, n, want_hits, n == 1 && !use_graph);
     SceneSlot &sc = ctx->scenes[scene_idx];
     hipStream_t s = ctx->stream;
-    if (stats) check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
+    if (stats) {
+        check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
+        ctx->touch_words[0] = (sc.num_nodes + 31) / 32;
+        ctx->touch_words[1] = ctx->touch_words[2] = (sc.num_prims + 31) / 32;
+        const size_t words = (size_t)ctx->touch_words[0] + ctx->touch_words[1] + ctx->touch_words[2];
+        dalloc(ctx->touch, words * 4);
+        check(hipMemsetAsync(ctx->touch.p, 0, words * 4, s), "touch clear");
+    }
     if (use_graph) {
         std::vector<uint64_t> key = {ctx->generation,   scene_idx,        (uint64_t)ctx->opt_cull,
                                      (uint64_t)(int64_t)ctx->opt_stack, (uint64_t)ctx->opt_persistent,
@@ -833,8 +879,54 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     ctx->timings = specrel_timings{};
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     ctx->timings.shadow_path = shadow_path(ctx, pl);
+    ctx->timings.trace_block_waves = trace_block_waves(ctx);
     ctx->have_hits = pl.want_hits;
     return SPECREL_OK;
+}
+
+// The pinned readback buffer, grown to at least `bytes` (the stream is idle when it moves).
+char *readback_buf(specrel_ctx *ctx, size_t bytes) {
+    if (ctx->readback_cap < bytes) {
+        check(hipStreamSynchronize(ctx->stream), "readback");
+        if (ctx->readback) check(hipHostFree(ctx->readback), "hipHostFree");
+        ctx->readback = nullptr;
+        ctx->readback_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, (size_t)1 << 20);
+        check(hipHostMalloc(&ctx->readback, cap, hipHostMallocDefault), "hipHostMalloc");
+        ctx->readback_cap = cap;
+    }
+    return (char *)ctx->readback;
+}
+
+// One device-to-host copy of a readback: `bytes` from device `src` into host `dst`.
+struct Fetch {
+    void *dst;
+    const void *src;
+    size_t bytes;
+};
+
+// Enqueues the copies into the pinned readback buffer behind whatever the stream holds, waits
+// once, then fills the callers' buffers. Returns the wait and the host copy in milliseconds.
+void fetch(specrel_ctx *ctx, const std::vector<Fetch> &f, float *wait_ms, float *copy_ms) {
+    size_t total = 0;
+    for (const Fetch &x : f) total += (x.bytes + 255) & ~(size_t)255;
+    char *stage = readback_buf(ctx, total);
+    size_t off = 0;
+    for (const Fetch &x : f) {
+        if (x.bytes) check(hipMemcpyAsync(stage + off, x.src, x.bytes, hipMemcpyDeviceToHost, ctx->stream), "readback");
+        off += (x.bytes + 255) & ~(size_t)255;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    check(hipStreamSynchronize(ctx->stream), "readback");
+    auto t1 = std::chrono::steady_clock::now();
+    off = 0;
+    for (const Fetch &x : f) {
+        if (x.bytes) std::memcpy(x.dst, stage + off, x.bytes);
+        off += (x.bytes + 255) & ~(size_t)255;
+    }
+    auto t2 = std::chrono::steady_clock::now();
+    if (wait_ms) *wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    if (copy_ms) *copy_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
 }
 
 void collect(specrel_ctx *ctx) {
@@ -848,8 +940,16 @@ void collect(specrel_ctx *ctx) {
     }
     t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[5]);
     if (ctx->opt_stats) {
+        unsigned long long *dst = (unsigned long long *)ctx->stats.p;
+        const uint32_t *bits = (const uint32_t *)ctx->touch.p;
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t n = ctx->touch_words[k];
+            if (n) k_popcount<<<std::min<uint32_t>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(bits, n, dst + 10 + k);
+            check(hipGetLastError(), "k_popcount launch");
+            bits += n;
+        }
         unsigned long long c[16];
-        check(hipMemcpy(c, ctx->stats.p, sizeof c, hipMemcpyDeviceToHost), "stats readback");
+        fetch(ctx, {{c, ctx->stats.p, sizeof c}}, nullptr, nullptr);
         specrel_stats &st = ctx->stats_host;
         std::memset(&st, 0, sizeof st);
         st.rays = c[0];
@@ -862,28 +962,53 @@ void collect(specrel_ctx *ctx) {
         st.shadow_box_tests = c[7];
         st.shadow_prim_tests = c[8];
         st.star_tests = c[9];
+        st.distinct_nodes = c[10];
+        st.distinct_prims = c[11];
+        st.distinct_hits = c[12];
     }
 }
 
-// Copies view `view` of the last render to the host (NULL pointers are skipped).
-void download_view(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit, float *lam, float *tau) {
+void check_len(uint64_t len, uint64_t want, const char *what, uint32_t view) {
+    if (len != want)
+        throw Fail{SPECREL_EINVAL, std::string(what) + " size mismatch (view " + std::to_string(view) + "): " +
+                                       std::to_string(len) + " elements given, " + std::to_string(want) +
+                                       " needed"};
+}
+
+// The readback copies of view `view` of the last render (NULL pointers are skipped); every
+// non-NULL buffer holds `len` elements, which must be the view's width * height.
+void view_fetches(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit, float *lam, float *tau, uint64_t len,
+                  std::vector<Fetch> &f) {
     if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
     if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
     if ((hit || lam || tau) && !ctx->have_hits) throw Fail{SPECREL_EINVAL, "the last render did not write hit buffers"};
     const size_t off = host_start(ctx, ROW_PIX, view);
     const size_t n = (size_t)ctx->sizes[2 * view] * ctx->sizes[2 * view + 1];
-    if (rgb) check(hipMemcpy(rgb, (uint32_t *)ctx->rgb.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
-    if (hit) check(hipMemcpy(hit, (uint32_t *)ctx->hit.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
-    if (lam) check(hipMemcpy(lam, (float *)ctx->lam.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
-    if (tau) check(hipMemcpy(tau, (float *)ctx->tau.p + off, n * 4, hipMemcpyDeviceToHost), "readback");
+    if (rgb || hit || lam || tau) check_len(len, n, "framebuffer", view);
+    if (rgb) f.push_back({rgb, (const uint32_t *)ctx->rgb.p + off, n * 4});
+    if (hit) f.push_back({hit, (const uint32_t *)ctx->hit.p + off, n * 4});
+    if (lam) f.push_back({lam, (const float *)ctx->lam.p + off, n * 4});
+    if (tau) f.push_back({tau, (const float *)ctx->tau.p + off, n * 4});
+}
+
+// Copies view `view` of the last render to the host.
+void download_view(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit, float *lam, float *tau,
+                   uint64_t len) {
+    std::vector<Fetch> f;
+    view_fetches(ctx, view, rgb, hit, lam, tau, len, f);
+    fetch(ctx, f, nullptr, nullptr);
 }
 
 // specrel_adjoint / specrel_adjoint_batch on the views of the last render: per-view gradients,
 // one launch per pass over every view's tiles.
-int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
+int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out, uint64_t n_pixels,
                   const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out, double *d_objects) {
     check(hipSetDevice(ctx->device), "hipSetDevice");
     if (!target) throw Fail{SPECREL_EINVAL, "target is NULL"};
+    if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+    if (n_pixels != ctx->total_pix)
+        throw Fail{SPECREL_EINVAL, "target size mismatch: " + std::to_string(n_pixels) + " pixels given, the last "
+                                   "render has " + std::to_string(ctx->total_pix)};
     if (n_objects && (!objects || !d_objects)) throw Fail{SPECREL_EINVAL, "objects or d_objects is NULL"};
     if (n_objects > 1024) throw Fail{SPECREL_EINVAL, "at most 1024 objects"};
     const uint32_t nv = ctx->n_views;
@@ -962,12 +1087,12 @@ int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in,
     k_adj_final<<<dim3(nv, (stride + ADJ_CLASSES + 15) / 16), 256, 0, s>>>(a);
     check(hipGetLastError(), "k_adj_final launch");
     check(hipEventRecord(ctx->ev[8], s), "event");
-    check(hipStreamSynchronize(s), "adjoint");
     std::vector<double> res((size_t)nv * stride);
     std::vector<uint64_t> cnt((size_t)nv * ADJ_CLASSES);
-    check(hipMemcpy(res.data(), a.result, res.size() * sizeof(double), hipMemcpyDeviceToHost), "adjoint readback");
-    check(hipMemcpy(cnt.data(), a.result_counts, cnt.size() * 8, hipMemcpyDeviceToHost), "adjoint readback");
-    if (mask_out) check(hipMemcpy(mask_out, ctx->adj_mask_out.p, np, hipMemcpyDeviceToHost), "mask readback");
+    std::vector<Fetch> f = {{res.data(), a.result, res.size() * sizeof(double)},
+                            {cnt.data(), a.result_counts, cnt.size() * 8}};
+    if (mask_out) f.push_back({mask_out, ctx->adj_mask_out.p, np});
+    fetch(ctx, f, nullptr, nullptr);
     ctx->timings.adjoint_ms = ev_ms(ctx->ev[7], ctx->ev[8]);
     for (uint32_t v = 0; v < nv; ++v) {
         const double *r = res.data() + (size_t)v * stride;
@@ -1060,7 +1185,7 @@ void specrel_destroy(specrel_ctx *ctx) {
     }
     dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
-    dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue);
+    dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue), dfree(ctx->touch);
     dfree(ctx->tile_cost), dfree(ctx->tile_order), dfree(ctx->tile_cones);
     dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
     dfree(ctx->ex_tubes), dfree(ctx->ex_tris), dfree(ctx->ex_samples), dfree(ctx->ex_decided);
@@ -1082,6 +1207,7 @@ void specrel_destroy(specrel_ctx *ctx) {
         if (st.done) (void)hipEventDestroy(st.done);
         if (st.host) (void)hipHostFree(st.host);
     }
+    if (ctx->readback) (void)hipHostFree(ctx->readback);
     delete ctx;
 }
 
@@ -1144,13 +1270,11 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
         check(hipSetDevice(ctx->device), "hipSetDevice");
         if (scene_idx > 4096) throw Fail{SPECREL_EINVAL, "scene index too large"};
         if (ctx->scenes.size() <= scene_idx) ctx->scenes.resize(scene_idx + 1);
-        SceneSlot fresh;
         // Keep the old device buffers for reuse; rebuild the slot's contents.
         SceneSlot &slot = ctx->scenes[scene_idx];
         slot.used = false;
         ctx->generation++;  // a captured graph holds the old scene pointers
         build_device_scene(ctx, slot, scene);
-        (void)fresh;
         return SPECREL_OK;
     });
 }
@@ -1182,6 +1306,7 @@ int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t 
     return guarded(ctx, [&] {
         check(hipSetDevice(ctx->device), "hipSetDevice");
         if ((n_tubes && !tubes) || !n_out) throw Fail{SPECREL_EINVAL, "tubes or n_out is NULL"};
+        ctx->ex_records = 0;  // until this extrusion has completed
         // Host tables: the tubes' prepared triangles (TubeTriangle::new), samples, decided_at.
         std::vector<dev::ExTube> desc(n_tubes);
         std::vector<sr::TubeTri> tris;
@@ -1300,6 +1425,7 @@ int specrel_extrude_tubes(specrel_ctx *ctx, const specrel_tube *tubes, uint32_t 
             }
         }
         ctx->timings.extrude_ms = ms;
+        ctx->ex_records = total;
         *n_out = total;
         std::vector<uint64_t> first(n_tubes, 0);
         if (direct)
@@ -1344,7 +1470,9 @@ int specrel_build_bvh(specrel_ctx *ctx, const float *boxes, uint64_t n, specrel_
             check(hipMemcpyAsync(in.p, boxes, n * 32, hipMemcpyHostToDevice, s), "boxes upload");
             d_boxes = (const float4 *)in.p;
         } else {
-            if (n > ctx->ex_boxes.bytes / 32) throw Fail{SPECREL_EINVAL, "fewer extruded records than n"};
+            if (n != ctx->ex_records)
+                throw Fail{SPECREL_EINVAL, "n is " + std::to_string(n) + " but the last extrusion left " +
+                                               std::to_string(ctx->ex_records) + " records on the device"};
             d_boxes = (const float4 *)ctx->ex_boxes.p;
         }
         DevBuf d_nodes, d_order;
@@ -1354,10 +1482,7 @@ int specrel_build_bvh(specrel_ctx *ctx, const float *boxes, uint64_t n, specrel_
         uint32_t total = 0;
         std::string err = dev::build_bvh(d_boxes, (uint32_t)n, (specrel_node *)d_nodes.p, (uint32_t *)d_order.p,
                                          &total, s, &st);
-        if (!err.empty()) {
-            dfree(in), dfree(d_nodes), dfree(d_order);
-            throw Fail{SPECREL_EDEVICE, err};
-        }
+        if (!err.empty()) throw Fail{SPECREL_EDEVICE, err};
         if (total) check(hipMemcpy(nodes, d_nodes.p, (size_t)total * sizeof(specrel_node), hipMemcpyDeviceToHost),
                          "nodes readback");
         if (n) check(hipMemcpy(order, d_order.p, n * 4, hipMemcpyDeviceToHost), "order readback");
@@ -1400,21 +1525,23 @@ int specrel_patch_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scen
 uint32_t specrel_num_scenes(const specrel_ctx *ctx) { return ctx ? (uint32_t)ctx->scenes.size() : 0; }
 
 int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *out_rgb,
-                   uint32_t *out_hit, float *out_lambda, float *out_tau) {
+                   uint32_t *out_hit, float *out_lambda, float *out_tau, uint64_t len) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
         if (!out_rgb) throw Fail{SPECREL_EINVAL, "out_rgb is NULL"};
+        if (!frame) throw Fail{SPECREL_EINVAL, "frame is NULL"};
+        // The framebuffer must hold width * height pixels (gpu.rs:808), checked before any work.
+        check_len(len, (uint64_t)frame->size[0] * frame->size[1], "framebuffer", 0);
         check(hipSetDevice(ctx->device), "hipSetDevice");
         bool hits = out_hit || out_lambda || out_tau;
         enqueue_views(ctx, scene_idx, frame, 1, hits);
-        auto t0 = std::chrono::steady_clock::now();
-        check(hipStreamSynchronize(ctx->stream), "render");
-        auto t1 = std::chrono::steady_clock::now();
-        download_view(ctx, 0, out_rgb, out_hit, out_lambda, out_tau);
-        auto t2 = std::chrono::steady_clock::now();
+        std::vector<Fetch> f;
+        view_fetches(ctx, 0, out_rgb, out_hit, out_lambda, out_tau, len, f);
+        float wait_ms = 0.0f, copy_ms = 0.0f;
+        fetch(ctx, f, &wait_ms, &copy_ms);
         collect(ctx);
-        ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
-        ctx->timings.readback_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
+        ctx->timings.wait_ms = wait_ms;
+        ctx->timings.readback_ms = copy_ms;
         return SPECREL_OK;
     });
 }
@@ -1433,19 +1560,23 @@ int specrel_render_batch(specrel_ctx *ctx, uint32_t scene_idx, const specrel_fra
     return guarded(ctx, [&] {
         check(hipSetDevice(ctx->device), "hipSetDevice");
         bool hits = false;
-        if (outs)
-            for (uint32_t i = 0; i < n_views; ++i) hits = hits || outs[i].hit || outs[i].lambda || outs[i].tau;
+        if (outs && frames)
+            for (uint32_t i = 0; i < n_views; ++i) {
+                const specrel_view_out &o = outs[i];
+                hits = hits || o.hit || o.lambda || o.tau;
+                if (o.rgb || o.hit || o.lambda || o.tau)
+                    check_len(o.len, (uint64_t)frames[i].size[0] * frames[i].size[1], "view output", i);
+            }
         enqueue_views(ctx, scene_idx, frames, n_views, hits);
-        auto t0 = std::chrono::steady_clock::now();
-        check(hipStreamSynchronize(ctx->stream), "render");
-        auto t1 = std::chrono::steady_clock::now();
+        std::vector<Fetch> f;
         if (outs)
             for (uint32_t i = 0; i < n_views; ++i)
-                download_view(ctx, i, outs[i].rgb, outs[i].hit, outs[i].lambda, outs[i].tau);
-        auto t2 = std::chrono::steady_clock::now();
+                view_fetches(ctx, i, outs[i].rgb, outs[i].hit, outs[i].lambda, outs[i].tau, outs[i].len, f);
+        float wait_ms = 0.0f, copy_ms = 0.0f;
+        fetch(ctx, f, &wait_ms, &copy_ms);
         collect(ctx);
-        ctx->timings.wait_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
-        ctx->timings.readback_ms = std::chrono::duration<float, std::milli>(t2 - t1).count();
+        ctx->timings.wait_ms = wait_ms;
+        ctx->timings.readback_ms = copy_ms;
         return SPECREL_OK;
     });
 }
@@ -1471,33 +1602,33 @@ int specrel_sync(specrel_ctx *ctx) {
     });
 }
 
-int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau) {
-    return specrel_download_view(ctx, 0, out_rgb, out_hit, out_lambda, out_tau);
+int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau,
+                     uint64_t len) {
+    return specrel_download_view(ctx, 0, out_rgb, out_hit, out_lambda, out_tau, len);
 }
 
 int specrel_download_view(specrel_ctx *ctx, uint32_t view, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda,
-                          float *out_tau) {
+                          float *out_tau, uint64_t len) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
-        check(hipStreamSynchronize(ctx->stream), "download");
-        download_view(ctx, view, out_rgb, out_hit, out_lambda, out_tau);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        download_view(ctx, view, out_rgb, out_hit, out_lambda, out_tau, len);
         return SPECREL_OK;
     });
 }
 
-int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3) {
+int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3, uint64_t len) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
         if (!out_rgb3) throw Fail{SPECREL_EINVAL, "output is NULL"};
         if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
         if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
-        check(hipSetDevice(ctx->device), "hipSetDevice");
-        check(hipStreamSynchronize(ctx->stream), "download");
         const size_t n = (size_t)ctx->sizes[2 * view] * ctx->sizes[2 * view + 1];
+        check_len(len, 3 * (uint64_t)n, "linear image", view);
+        check(hipSetDevice(ctx->device), "hipSetDevice");
         std::vector<float4> tmp(n);
-        check(hipMemcpy(tmp.data(), (float4 *)ctx->hdr.p + host_start(ctx, ROW_PYR, view), n * sizeof(float4),
-                        hipMemcpyDeviceToHost),
-              "readback");
+        fetch(ctx, {{tmp.data(), (const float4 *)ctx->hdr.p + host_start(ctx, ROW_PYR, view), n * sizeof(float4)}},
+              nullptr, nullptr);
         for (size_t i = 0; i < n; ++i) {
             out_rgb3[3 * i] = tmp[i].x;
             out_rgb3[3 * i + 1] = tmp[i].y;
@@ -1515,17 +1646,18 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph, in_trace, shape, postpone;
+            int stack, persistent, queue, graph, in_trace, shape, postpone, waves;
         };
         const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
-                         ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone};
+                         ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone,   ctx->opt_trace_waves};
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
                 for (int sh : {0, 1, 2})  // shadows in the shading pass, queued, in the trace kernel
                     for (int g : {0, 2})
                         for (int ws : {0, 1, 2})  // a wave's pixels: 8x8, 16x4, 4x16 (tiled primary rays only)
-                            if (ws == 0 || !pe) cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone});
+                            if (ws == 0 || !pe)
+                                cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone, saved.waves});
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
@@ -1534,6 +1666,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             ctx->opt_shadow_trace = c.in_trace;
             ctx->opt_wave_shape = c.shape;
             ctx->opt_postpone = c.postpone;
+            ctx->opt_trace_waves = c.waves;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -1571,6 +1704,21 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                 best_ms = t;
                 best = flip;
             }
+            // Then the best one with the other block sizes of tiled LDS-stack primary rays.
+            if (best.stack > 0 && !best.persistent) {
+                const Cand base = best;
+                for (int w : {1, 2, 4}) {
+                    if (w == base.waves) continue;
+                    Cand c = base;
+                    c.waves = w;
+                    cands.push_back(c);
+                    float tw = time_cand(c);
+                    if (tw < best_ms) {
+                        best_ms = tw;
+                        best = c;
+                    }
+                }
+            }
         } catch (...) {
             apply(saved);
             throw;
@@ -1585,6 +1733,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->shadow_in_trace = best.in_trace;
             out->wave_shape = best.shape;
             out->leaf_postpone = best.postpone != 0;
+            out->trace_waves = (best.stack > 0 && !best.persistent) ? best.waves : 4;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;
@@ -1594,21 +1743,21 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
 }
 
 int specrel_adjoint(specrel_ctx *ctx, const float *target, const uint8_t *mask_in, uint8_t *mask_out,
-                    const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *out,
-                    double *d_objects) {
+                    uint64_t n_pixels, const specrel_adj_object *objects, uint32_t n_objects,
+                    specrel_adjoint_out *out, double *d_objects) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
         if (ctx->n_views != 1) throw Fail{SPECREL_EINVAL, "the last render is not a single view"};
-        return adjoint_views(ctx, target, mask_in, mask_out, objects, n_objects, out, d_objects);
+        return adjoint_views(ctx, target, mask_in, mask_out, n_pixels, objects, n_objects, out, d_objects);
     });
 }
 
 int specrel_adjoint_batch(specrel_ctx *ctx, const float *targets, const uint8_t *masks_in, uint8_t *masks_out,
-                          const specrel_adj_object *objects, uint32_t n_objects, specrel_adjoint_out *outs,
-                          double *d_objects) {
+                          uint64_t n_pixels, const specrel_adj_object *objects, uint32_t n_objects,
+                          specrel_adjoint_out *outs, double *d_objects) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {
-        return adjoint_views(ctx, targets, masks_in, masks_out, objects, n_objects, outs, d_objects);
+        return adjoint_views(ctx, targets, masks_in, masks_out, n_pixels, objects, n_objects, outs, d_objects);
     });
 }
 

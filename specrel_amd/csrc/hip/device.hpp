@@ -42,7 +42,15 @@ struct Scene {
     uint32_t empty;  // no nodes: every ray misses
     float4 root_min, root_max;
     float4 light;  // (0, x, y, z)
+    // Instrumented renders only (NULL otherwise): one bit per interior node whose box pair was
+    // loaded, per primitive record tested and per primitive whose attributes a hit read, so the
+    // distinct records a frame touches can be counted.
+    uint32_t *touch_nodes, *touch_prims, *touch_hits;
 };
+
+__device__ __forceinline__ void touch(uint32_t *bits, uint32_t i) {
+    if (bits) atomicOr(bits + (i >> 5), 1u << (i & 31u));
+}
 
 struct Tables {
     const uint8_t *lut;     // 4096
@@ -351,6 +359,8 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
         }
         if (leaf) {
             const uint32_t end = first + count;
+            if (STATS)
+                for (uint32_t j = first; j < end; ++j) touch(S.touch_prims, j);
             for (uint32_t i = first + done; i < end; i += 3) {
                 const uint32_t last = end - 1, i1 = min(i + 1, last), i2 = min(i + 2, last);
                 x0 = isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d);
@@ -363,6 +373,7 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
             const uint2 info = D.info;
             float tl, tr;
             if (STATS) C.box_tests += 2;
+            if (STATS) touch(S.touch_nodes, e);
             bool hl = box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl);
             bool hr = box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr);
             // Both hit: push the far one, continue with the near one (ties to the left), if two
@@ -505,6 +516,7 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
                 for (uint32_t k = 0; k < 3; ++k) {
                     if (i + k >= end) break;
                     if (STATS) C.shadow_prim_tests++;
+                    if (STATS) touch(S.touch_prims, i + k);
                     if (hs[k]) return true;
                 }
             }
@@ -513,6 +525,7 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
             uint2 info = S.ninfo[e];
             float t;
             if (STATS) C.shadow_box_tests += 2;
+            if (STATS) touch(S.touch_nodes, e);
             bool hl = box_hit(bx[0], bx[1], o, inv, tmin, tmax, t);
             bool hr = box_hit(bx[2], bx[3], o, inv, tmin, tmax, t);
             if (hr && stk.size() < STACK) stk.push(info.y, 0.0f);
@@ -1012,6 +1025,7 @@ __device__ __forceinline__ bool trace_step(const Scene &S, RayState &r, float tm
             float lam, b[4];
             V4 ev;
             if (STATS) C.prim_tests++;
+            if (STATS) touch(S.touch_prims, i);
             if (intersect(S.geoms + 5 * (size_t)i, r.o, r.d, tmin, r.tmax, lam, ev, b)) {
                 r.tmax = lam;
                 r.best = (int32_t)i;
@@ -1023,6 +1037,7 @@ __device__ __forceinline__ bool trace_step(const Scene &S, RayState &r, float tm
         uint2 info = S.ninfo[r.e];
         float tl, tr;
         if (STATS) C.box_tests += 2;
+        if (STATS) touch(S.touch_nodes, r.e);
         bool hl = box_hit(lmn, lmx, r.o, r.inv, tmin, r.tmax, tl);
         bool hr = box_hit(rmn, rmx, r.o, r.inv, tmin, r.tmax, tr);
         if (hl && hr) {
@@ -1072,6 +1087,7 @@ __device__ __forceinline__ int occlude_step(const Scene &S, RayState &r, float t
             float lam, b[4];
             V4 ev;
             if (STATS) C.shadow_prim_tests++;
+            if (STATS) touch(S.touch_prims, i);
             if (intersect(S.geoms + 5 * (size_t)i, r.o, r.d, tmin, r.tmax, lam, ev, b)) return 1;
         }
     } else {
@@ -1079,6 +1095,7 @@ __device__ __forceinline__ int occlude_step(const Scene &S, RayState &r, float t
         uint2 info = S.ninfo[r.e];
         float t;
         if (STATS) C.shadow_box_tests += 2;
+        if (STATS) touch(S.touch_nodes, r.e);
         bool hl = box_hit(bx[0], bx[1], r.o, r.inv, tmin, r.tmax, t);
         bool hr = box_hit(bx[2], bx[3], r.o, r.inv, tmin, r.tmax, t);
         if (hr && stk.size() < STACK) stk.push(info.y, 0.0f);

@@ -251,6 +251,7 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
             if (found && f.mode == SPECREL_MODE_SHADED && f.shadow_mode != SPECREL_SHADOW_OFF) {
                 Rec rec;
                 interpolate(a.S.attrs + 12 * (size_t)hit.slot, hit, rec);
+                if (STATS) touch(a.S.touch_hits, (uint32_t)hit.slot);
                 const Mat m = load_mat(a.S, rec.material);
                 RecordShadow<STATS, typename SS::type> sh{a.S, C, stk, 0u};
                 illumination(a.S, f, dir, rec, m, sh);
@@ -512,6 +513,14 @@ __global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint3
 
 // Zeroes the per-frame queue counters (a kernel rather than a memset, so the frame's work can
 // be captured in a graph).
+// Set bits of n words, added to *out (the distinct records of an instrumented render).
+__global__ void k_popcount(const uint32_t *bits, uint64_t n, unsigned long long *out) {
+    uint32_t c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        c += __popc(bits[i]);
+    if (c) atomicAdd(out, (unsigned long long)c);
+}
+
 __global__ void k_reset(uint32_t *counters, uint32_t n) {
     uint32_t i = threadIdx.x;
     if (i < n) counters[i] = 0u;
@@ -595,6 +604,7 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
             intersect(a.S.geoms + 5 * (size_t)slot, v.origin, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
             lambda = hit.lambda;
             interpolate(a.S.attrs + 12 * (size_t)slot, hit, rec);
+            if (STATS) touch(a.S.touch_hits, slot);
             if constexpr (PASS == 0) {
                 TraceShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk};
                 color = shade_hit(a.S, a.T, f, dir, rec, sh);
@@ -665,10 +675,11 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
 #define SPECREL_TRACE(K, M, SH, W)                                    \
     template __global__ void k_trace<false, K, M, SH, W>(TraceArgs); \
     template __global__ void k_trace<true, K, M, SH, W>(TraceArgs);
-// LDS stacks: merged or separate load rounds, shadow queries traced here or not; blocks of 1 or 2
-// waves for the depths 8 and 16 with shadows traced here. Register stacks: separate rounds.
-#define SPECREL_TRACE_LDS(K) \
-    SPECREL_TRACE(K, false, false, 4) SPECREL_TRACE(K, true, false, 4) SPECREL_TRACE(K, false, true, 4) SPECREL_TRACE(K, true, true, 4)
+// LDS stacks: merged or separate load rounds, shadow queries traced here or not, blocks of 1, 2 or
+// 4 waves. Register stacks: separate rounds, blocks of 4 waves.
+#define SPECREL_TRACE_LDS_W(K, W) \
+    SPECREL_TRACE(K, false, false, W) SPECREL_TRACE(K, true, false, W) SPECREL_TRACE(K, false, true, W) SPECREL_TRACE(K, true, true, W)
+#define SPECREL_TRACE_LDS(K) SPECREL_TRACE_LDS_W(K, 1) SPECREL_TRACE_LDS_W(K, 2) SPECREL_TRACE_LDS_W(K, 4)
 SPECREL_INSTANTIATE(8)
 SPECREL_INSTANTIATE(10)
 SPECREL_INSTANTIATE(12)
@@ -677,14 +688,6 @@ SPECREL_TRACE_LDS(8)
 SPECREL_TRACE_LDS(10)
 SPECREL_TRACE_LDS(12)
 SPECREL_TRACE_LDS(16)
-SPECREL_TRACE(8, false, true, 1)
-SPECREL_TRACE(8, true, true, 1)
-SPECREL_TRACE(8, false, true, 2)
-SPECREL_TRACE(8, true, true, 2)
-SPECREL_TRACE(16, false, true, 1)
-SPECREL_TRACE(16, true, true, 1)
-SPECREL_TRACE(16, false, true, 2)
-SPECREL_TRACE(16, true, true, 2)
 SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)

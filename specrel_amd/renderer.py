@@ -29,7 +29,7 @@ SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
                                          "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "adjoint_ms")] + \
-        [("shadow_path", C.c_uint32), ("extrude_ms", C.c_float)]
+        [("shadow_path", C.c_uint32), ("extrude_ms", C.c_float), ("trace_block_waves", C.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -37,7 +37,7 @@ class Timings(C.Structure):
 
 class ViewOut(C.Structure):
     _fields_ = [("rgb", C.POINTER(C.c_uint32)), ("hit", C.POINTER(C.c_uint32)), ("lambda_", C.POINTER(C.c_float)),
-                ("tau", C.POINTER(C.c_float))]
+                ("tau", C.POINTER(C.c_float)), ("len", C.c_uint64)]
 
 
 class AdjObject(C.Structure):
@@ -57,7 +57,7 @@ class Tuning(C.Structure):
     _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
                 ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
                 ("shadow_in_trace", C.c_int32), ("wave_shape", C.c_int32), ("leaf_postpone", C.c_int32),
-                ("reserved", C.c_uint32 * 2)]
+                ("trace_waves", C.c_int32), ("reserved", C.c_uint32 * 1)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -98,19 +98,20 @@ SIGNATURES = {
     "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
                                      C.POINTER(Range), C.c_uint32, C.POINTER(C.c_int)]),
     "specrel_num_scenes": (C.c_uint32, [_P]),
-    "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP]),
+    "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP, C.c_uint64]),
     "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
     "specrel_sync": (C.c_int, [_P]),
-    "specrel_download": (C.c_int, [_P, _U32P, _U32P, _FP, _FP]),
+    "specrel_download": (C.c_int, [_P, _U32P, _U32P, _FP, _FP, C.c_uint64]),
     "specrel_render_batch": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.POINTER(ViewOut)]),
     "specrel_render_batch_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_int]),
-    "specrel_download_view": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, _FP, _FP]),
-    "specrel_download_linear": (C.c_int, [_P, C.c_uint32, _FP]),
+    "specrel_download_view": (C.c_int, [_P, C.c_uint32, _U32P, _U32P, _FP, _FP, C.c_uint64]),
+    "specrel_download_linear": (C.c_int, [_P, C.c_uint32, _FP, C.c_uint64]),
     "specrel_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.c_uint32, C.POINTER(Tuning)]),
-    "specrel_adjoint": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(AdjObject),
-                                  C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
-    "specrel_adjoint_batch": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.POINTER(AdjObject),
-                                        C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
+    "specrel_adjoint": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.c_uint64,
+                                  C.POINTER(AdjObject), C.c_uint32, C.POINTER(AdjointOut), C.POINTER(C.c_double)]),
+    "specrel_adjoint_batch": (C.c_int, [_P, _FP, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), C.c_uint64,
+                                        C.POINTER(AdjObject), C.c_uint32, C.POINTER(AdjointOut),
+                                        C.POINTER(C.c_double)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
 }
@@ -151,6 +152,7 @@ class Context:
             raise SpecrelError(rc, lib.specrel_last_error(None).decode())
         self._h = h
         self.lib = lib
+        self._sizes = []  # (w, h) of every view of the last render, in view order
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -228,38 +230,47 @@ class Context:
                                                  len(prim_ranges), C.byref(full)))
         return full.value == 0
 
-    def render_frame(self, idx, frame, want_hits=True):
-        w, h = frame.size[0], frame.size[1]
+    def _outputs(self, w, h, want_hits, rgb=None):
         n = w * h
-        rgb = np.zeros(n, np.uint32)
+        rgb = np.zeros(n, np.uint32) if rgb is None else rgb
         hit = np.zeros(n, np.uint32) if want_hits else None
         lam = np.zeros(n, np.float32) if want_hits else None
         tau = np.zeros(n, np.float32) if want_hits else None
-        self._check(self.lib.specrel_render(self._h, idx, C.byref(frame), _ptr(rgb, _U32P), _ptr(hit, _U32P),
-                                            _ptr(lam, _FP), _ptr(tau, _FP)))
+        return rgb, hit, lam, tau
+
+    @staticmethod
+    def _result(w, h, rgb, hit, lam, tau):
         out = {"rgb": rgb.reshape(h, w)}
-        if want_hits:
+        if hit is not None:
             out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
         return out
 
+    def render_frame(self, idx, frame, want_hits=True, rgb=None):
+        """specrel_render of one view. rgb: optional caller-owned uint32 buffer to render into (the
+        framebuffer of GpuRenderer::render); its element count is passed along and checked."""
+        w, h = frame.size[0], frame.size[1]
+        rgb, hit, lam, tau = self._outputs(w, h, want_hits, rgb)
+        self._check(self.lib.specrel_render(self._h, idx, C.byref(frame), _ptr(rgb, _U32P), _ptr(hit, _U32P),
+                                            _ptr(lam, _FP), _ptr(tau, _FP), rgb.size))
+        self._sizes = [(w, h)]
+        return self._result(w, h, rgb, hit, lam, tau)
+
     def render_resident(self, idx, frame, want_hits=False):
         self._check(self.lib.specrel_render_resident(self._h, idx, C.byref(frame), int(want_hits)))
+        self._sizes = [(frame.size[0], frame.size[1])]
 
     def sync(self):
         self._check(self.lib.specrel_sync(self._h))
 
-    def download(self, w, h, want_hits=False):
-        n = w * h
-        rgb = np.zeros(n, np.uint32)
-        hit = np.zeros(n, np.uint32) if want_hits else None
-        lam = np.zeros(n, np.float32) if want_hits else None
-        tau = np.zeros(n, np.float32) if want_hits else None
-        self._check(self.lib.specrel_download(self._h, _ptr(rgb, _U32P), _ptr(hit, _U32P), _ptr(lam, _FP),
-                                              _ptr(tau, _FP)))
-        out = {"rgb": rgb.reshape(h, w)}
-        if want_hits:
-            out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
-        return out
+    def view_size(self, view=0):
+        """(w, h) of view `view` of the last render made through this object."""
+        if view >= len(self._sizes):
+            raise SpecrelError(EINVAL, "view index out of range")
+        return self._sizes[view]
+
+    def download(self, want_hits=False):
+        """Outputs of the last single-view render (its size is the context's, not the caller's)."""
+        return self.download_view(0, want_hits)
 
     @staticmethod
     def _frames(frames):
@@ -275,40 +286,34 @@ class Context:
         res = []
         for i, f in enumerate(frames):
             w, h = f.size[0], f.size[1]
-            o = {"rgb": np.zeros((h, w), np.uint32)}
-            if want_hits:
-                o.update(hit=np.zeros((h, w), np.uint32), lam=np.zeros((h, w), np.float32),
-                         tau=np.zeros((h, w), np.float32))
-            outs[i].rgb = _ptr(o["rgb"], _U32P)
-            if want_hits:
-                outs[i].hit = _ptr(o["hit"], _U32P)
-                outs[i].lambda_ = _ptr(o["lam"], _FP)
-                outs[i].tau = _ptr(o["tau"], _FP)
-            res.append(o)
+            rgb, hit, lam, tau = self._outputs(w, h, want_hits)
+            outs[i].rgb = _ptr(rgb, _U32P)
+            outs[i].hit = _ptr(hit, _U32P)
+            outs[i].lambda_ = _ptr(lam, _FP)
+            outs[i].tau = _ptr(tau, _FP)
+            outs[i].len = w * h
+            res.append(self._result(w, h, rgb, hit, lam, tau))
         self._check(self.lib.specrel_render_batch(self._h, idx, arr, len(frames), outs))
+        self._sizes = [(f.size[0], f.size[1]) for f in frames]
         return res
 
     def render_batch_resident(self, idx, frames, want_hits=False):
         arr = frames if isinstance(frames, C.Array) else self._frames(frames)
         self._check(self.lib.specrel_render_batch_resident(self._h, idx, arr, len(arr), int(want_hits)))
+        self._sizes = [(f.size[0], f.size[1]) for f in arr]
 
-    def download_view(self, view, w, h, want_hits=False):
-        n = w * h
-        rgb = np.zeros(n, np.uint32)
-        hit = np.zeros(n, np.uint32) if want_hits else None
-        lam = np.zeros(n, np.float32) if want_hits else None
-        tau = np.zeros(n, np.float32) if want_hits else None
+    def download_view(self, view, want_hits=False):
+        w, h = self.view_size(view)
+        rgb, hit, lam, tau = self._outputs(w, h, want_hits)
         self._check(self.lib.specrel_download_view(self._h, view, _ptr(rgb, _U32P), _ptr(hit, _U32P),
-                                                   _ptr(lam, _FP), _ptr(tau, _FP)))
-        out = {"rgb": rgb.reshape(h, w)}
-        if want_hits:
-            out.update(hit=hit.reshape(h, w), lam=lam.reshape(h, w), tau=tau.reshape(h, w))
-        return out
+                                                   _ptr(lam, _FP), _ptr(tau, _FP), w * h))
+        return self._result(w, h, rgb, hit, lam, tau)
 
-    def download_linear(self, view, w, h):
+    def download_linear(self, view=0):
         """The view's pre-bloom linear HDR image of the last render, (h, w, 3) float32."""
+        w, h = self.view_size(view)
         out = np.zeros((h, w, 3), np.float32)
-        self._check(self.lib.specrel_download_linear(self._h, view, out.ctypes.data_as(_FP)))
+        self._check(self.lib.specrel_download_linear(self._h, view, out.ctypes.data_as(_FP), out.size))
         return out
 
     def autotune(self, idx, frames, iters=3):
@@ -333,8 +338,8 @@ class Context:
         out = AdjointOut()
         dobj = np.zeros((max(1, len(objects)), 3), np.float64)
         u8 = C.POINTER(C.c_uint8)
-        self._check(self.lib.specrel_adjoint(self._h, tgt.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), objs,
-                                             len(objects), C.byref(out),
+        self._check(self.lib.specrel_adjoint(self._h, tgt.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), h * w,
+                                             objs, len(objects), C.byref(out),
                                              dobj.ctypes.data_as(C.POINTER(C.c_double))))
         res = {"loss": out.loss, "d_event": np.array(out.d_event[:]),
                "d_tetrad": np.array([[out.d_tetrad[a][k] for k in range(4)] for a in range(4)]),
@@ -361,8 +366,8 @@ class Context:
         outs = (AdjointOut * len(targets))()
         dobj = np.zeros((len(targets), max(1, len(objects)), 3), np.float64)
         u8 = C.POINTER(C.c_uint8)
-        self._check(self.lib.specrel_adjoint_batch(self._h, tg.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), objs,
-                                                   len(objects), outs,
+        self._check(self.lib.specrel_adjoint_batch(self._h, tg.ctypes.data_as(_FP), _ptr(mi, u8), _ptr(mo, u8), npx,
+                                                   objs, len(objects), outs,
                                                    dobj.ctypes.data_as(C.POINTER(C.c_double))))
         res, off = [], 0
         for v, (h, w) in enumerate(shapes):
@@ -420,10 +425,13 @@ class GpuRenderer:
             self.ctx.patch(scene_idx, scene, node_ranges or (), prim_ranges or ())
 
     def render(self, scene_idx, camera, cfg, dynamics, framebuffer):
-        if framebuffer.size != cfg.width * cfg.height:
-            raise ValueError("framebuffer size mismatch")
+        """Renders into `framebuffer` (a contiguous uint32 array of cfg.width * cfg.height pixels,
+        written in place); a size mismatch raises SpecrelError(EINVAL) from the library, as the
+        reference asserts (gpu.rs:808)."""
+        if not (isinstance(framebuffer, np.ndarray) and framebuffer.dtype == np.uint32
+                and framebuffer.flags.c_contiguous):
+            raise TypeError("framebuffer must be a contiguous uint32 numpy array")
         frame = build_frame(camera, cfg, dynamics)
-        out = self.ctx.render_frame(scene_idx, frame, want_hits=False)
-        framebuffer.reshape(-1)[:] = out["rgb"].reshape(-1)
+        self.ctx.render_frame(scene_idx, frame, want_hits=False, rgb=framebuffer.reshape(-1))
         t = self.ctx.timings()
         self.last_timings = GpuTimings(t["submit_ms"], t["wait_ms"], t["kernel_ms"], t["readback_ms"])

@@ -38,26 +38,28 @@ def test_layouts_match_the_header():
     assert F.Attr.misc.offset == 160 and F.Attr.decided_at.offset == 176
     assert C.sizeof(F.SceneArrays) == 88 and F.SceneArrays.light_pos.offset == 72
     assert C.sizeof(F.Stats) == 128
-    assert C.sizeof(R.Timings) == 48
-    assert C.sizeof(R.ViewOut) == 32 and C.sizeof(R.Tuning) == 48
+    assert C.sizeof(R.Timings) == 52 and R.Timings.trace_block_waves.offset == 48
+    assert C.sizeof(R.ViewOut) == 40 and R.ViewOut.len.offset == 32
+    assert C.sizeof(R.Tuning) == 48 and R.Tuning.trace_waves.offset == 40
     assert C.sizeof(R.AdjObject) == 24 and C.sizeof(R.AdjointOut) == 216 and C.sizeof(R.Range) == 8
     assert C.sizeof(F.Object) == 32
 
 
 def test_argument_errors_are_codes_not_aborts():
     lib = R.hip_lib()
-    assert lib.specrel_render(None, 0, None, None, None, None, None) == R.EINVAL
+    assert lib.specrel_render(None, 0, None, None, None, None, None, 0) == R.EINVAL
     assert lib.specrel_upload_scene(None, 0, None) == R.EINVAL
     assert lib.specrel_sync(None) == R.EINVAL
     assert lib.specrel_set_option(None, 1, 1) == R.EINVAL
     assert lib.specrel_render_batch(None, 0, None, 1, None) == R.EINVAL
     assert lib.specrel_render_batch_resident(None, 0, None, 1, 0) == R.EINVAL
-    assert lib.specrel_download_view(None, 0, None, None, None, None) == R.EINVAL
+    assert lib.specrel_download_view(None, 0, None, None, None, None, 0) == R.EINVAL
+    assert lib.specrel_download(None, None, None, None, None, 0) == R.EINVAL
     assert lib.specrel_autotune(None, 0, None, 1, 1, None) == R.EINVAL
-    assert lib.specrel_adjoint(None, None, None, None, None, 0, None, None) == R.EINVAL
-    assert lib.specrel_adjoint_batch(None, None, None, None, None, 0, None, None) == R.EINVAL
+    assert lib.specrel_adjoint(None, None, None, None, 0, None, 0, None, None) == R.EINVAL
+    assert lib.specrel_adjoint_batch(None, None, None, None, 0, None, 0, None, None) == R.EINVAL
     assert lib.specrel_patch_scene(None, 0, None, None, 0, None, 0, None) == R.EINVAL
-    assert lib.specrel_download_linear(None, 0, None) == R.EINVAL
+    assert lib.specrel_download_linear(None, 0, None, 0) == R.EINVAL
 
 
 def test_create_reports_missing_device():

@@ -25,7 +25,7 @@ def make_problem(ctx, scene, w, h, seed, obs=None):
     tgt = A.Observer(obs.event + rng.normal(size=4) * 0.3, obs.phi + rng.normal(size=3) * 0.1, obs.rot)
     ctx.upload(0, scene)
     ctx.render_resident(0, tgt.frame(w, h))
-    target = ctx.download_linear(0, w, h)
+    target = ctx.download_linear(0)
     ctx.render_resident(0, obs.frame(w, h))
     ctx.sync()
     return obs, target
@@ -177,7 +177,7 @@ def test_adjoint_full_workload_against_fp64(gpu_ctx):
     w, h = 512, 320
     gpu_ctx.upload(0, sc)
     gpu_ctx.render_resident(0, hidden.frame(w, h))
-    target = gpu_ctx.download_linear(0, w, h)
+    target = gpu_ctx.download_linear(0)
     gpu_ctx.render_resident(0, obs.frame(w, h))
     assert len(objs) == 32
     g, res = A.gradient(gpu_ctx, obs, target, objs)

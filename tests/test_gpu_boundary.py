@@ -1,0 +1,184 @@
+"""The drop-in boundary on the GPU: every output entry point checks the caller's element count
+before any work (the reference asserts the framebuffer length, gpu.rs:808), GpuRenderer.render
+writes the caller's framebuffer in place, the primary-ray block size the options ask for is the
+one that runs, and the instrumented counters of distinct records touched equal the oracle's."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from specrel_amd import renderer as R
+from specrel_amd import scene as S
+from specrel_amd.renderer import SpecrelError
+from oracle import pyoracle as O
+from tests.helpers import assert_same
+
+pytestmark = pytest.mark.gpu
+
+U32P = C.POINTER(C.c_uint32)
+FP = C.POINTER(C.c_float)
+
+
+@pytest.fixture(scope="module")
+def twin():
+    return S.Scene.generate("twin")
+
+
+def frames(scene):
+    a = S.build_frame(scene.default_camera(), S.RenderConfig(160, 100))
+    cam = scene.default_camera()
+    cam.tetrad.boost_local((0.3, 0.0, 0.2))
+    b = S.build_frame(cam, S.RenderConfig(160, 100))
+    return a, b
+
+
+def test_render_rejects_a_wrong_length_before_any_work(gpu_ctx, twin):
+    a, b = frames(twin)
+    gpu_ctx.upload(0, twin)
+    first = gpu_ctx.render_frame(0, a, want_hits=True)
+    lib = gpu_ctx.lib
+    for n in (160 * 100 - 1, 160 * 100 + 1, 0):
+        buf = np.full(160 * 100 + 1, 7, np.uint32)
+        rc = lib.specrel_render(gpu_ctx.handle, 0, C.byref(b), buf.ctypes.data_as(U32P), None, None, None, n)
+        assert rc == R.EINVAL and "size mismatch" in lib.specrel_last_error(gpu_ctx.handle).decode()
+        assert (buf == 7).all()
+    # nothing was rendered: the outputs are still the first frame's
+    again = gpu_ctx.download(want_hits=True)
+    for k in ("rgb", "hit", "lam", "tau"):
+        assert np.array_equal(again[k], first[k]), k
+
+
+def test_downloads_check_their_lengths(gpu_ctx, twin):
+    a, _ = frames(twin)
+    gpu_ctx.upload(0, twin)
+    gpu_ctx.render_resident(0, a, want_hits=True)
+    gpu_ctx.sync()
+    lib, h, n = gpu_ctx.lib, gpu_ctx.handle, 160 * 100
+    rgb = np.zeros(n + 8, np.uint32)
+    lam = np.zeros(n + 8, np.float32)
+    assert lib.specrel_download(h, rgb.ctypes.data_as(U32P), None, None, None, n - 1) == R.EINVAL
+    assert lib.specrel_download_view(h, 0, None, None, lam.ctypes.data_as(FP), None, n + 1) == R.EINVAL
+    assert not rgb.any() and not lam.any()
+    assert lib.specrel_download(h, rgb.ctypes.data_as(U32P), None, lam.ctypes.data_as(FP), None, n) == R.OK
+    lin = np.zeros(3 * n + 3, np.float32)
+    assert lib.specrel_download_linear(h, 0, lin.ctypes.data_as(FP), 3 * n - 3) == R.EINVAL
+    assert lib.specrel_download_linear(h, 0, lin.ctypes.data_as(FP), n) == R.EINVAL
+    assert not lin.any()
+    assert lib.specrel_download_linear(h, 0, lin.ctypes.data_as(FP), 3 * n) == R.OK
+    o = O.render(twin, a, want_hdr=True)
+    assert np.array_equal(rgb[:n].reshape(100, 160), o["rgb"])
+    assert np.array_equal(lin[:3 * n].reshape(100, 160, 3), o["hdr"])
+
+
+def test_batch_outputs_check_their_lengths(gpu_ctx, twin):
+    a, b = frames(twin)
+    gpu_ctx.upload(0, twin)
+    first = gpu_ctx.render_frame(0, a)
+    arr = gpu_ctx._frames([a, b])
+    outs = (R.ViewOut * 2)()
+    bufs = [np.zeros(160 * 100, np.uint32) for _ in range(2)]
+    for i in range(2):
+        outs[i].rgb = bufs[i].ctypes.data_as(U32P)
+        outs[i].len = 160 * 100
+    outs[1].len = 160 * 99
+    assert gpu_ctx.lib.specrel_render_batch(gpu_ctx.handle, 0, arr, 2, outs) == R.EINVAL
+    assert not bufs[0].any() and not bufs[1].any()
+    assert np.array_equal(gpu_ctx.download()["rgb"], first["rgb"])  # no work was done
+    outs[1].len = 160 * 100
+    assert gpu_ctx.lib.specrel_render_batch(gpu_ctx.handle, 0, arr, 2, outs) == R.OK
+    assert np.array_equal(bufs[0].reshape(100, 160), first["rgb"])
+
+
+def test_adjoint_checks_the_pixel_count(gpu_ctx, twin):
+    from specrel_amd import adjoint as A
+    gpu_ctx.upload(0, twin)
+    obs = A.Observer(np.array(twin.default_camera().position, np.float64), np.zeros(3))
+    gpu_ctx.render_resident(0, obs.frame(64, 40))
+    tgt = np.zeros(64 * 40 * 3, np.float32)
+    out = R.AdjointOut()
+    rc = gpu_ctx.lib.specrel_adjoint(gpu_ctx.handle, tgt.ctypes.data_as(FP), None, None, 64 * 39, None, 0,
+                                     C.byref(out), None)
+    assert rc == R.EINVAL
+    rc = gpu_ctx.lib.specrel_adjoint(gpu_ctx.handle, tgt.ctypes.data_as(FP), None, None, 64 * 40, None, 0,
+                                     C.byref(out), None)
+    assert rc == R.OK and out.valid + out.miss + out.starlight + out.border + out.silhouette + out.edge == 64 * 40
+
+
+def test_gpu_renderer_writes_the_framebuffer_in_place(twin):
+    gpu = R.GpuRenderer([twin])
+    try:
+        cam = twin.default_camera()
+        cfg = S.RenderConfig(160, 100)
+        fb = np.zeros(160 * 100, np.uint32)
+        gpu.render(0, cam, cfg, None, fb)
+        o = O.render(twin, S.build_frame(cam, cfg))
+        assert np.array_equal(fb.reshape(100, 160), o["rgb"])
+        short = np.zeros(160 * 100 - 1, np.uint32)
+        with pytest.raises(SpecrelError):
+            gpu.render(0, cam, cfg, None, short)
+        assert not short.any()
+        assert gpu.last_timings.kernel_ms > 0.0
+    finally:
+        gpu.ctx.close()
+
+
+@pytest.mark.parametrize("stack", [16, 12, 10, 8])
+def test_primary_ray_block_sizes_run_as_asked(gpu_ctx, twin, stack):
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.0, 0.2, 0.5))
+    gpu_ctx.upload(0, twin)
+    ref = None
+    try:
+        gpu_ctx.set_option(R.OPT_STACK, stack)
+        for shadows in (S.ShadowLorentzInvariant, S.ShadowOff):
+            fr = S.build_frame(cam, S.RenderConfig(160, 100, S.RelativisticShaded, shadows, S.PastLightCone))
+            o = O.render(twin, fr)
+            for in_trace in (1, 0):
+                gpu_ctx.set_option(R.OPT_SHADOW_IN_TRACE, in_trace)
+                for waves in (1, 2, 4):
+                    gpu_ctx.set_option(R.OPT_TRACE_WAVES, waves)
+                    g = gpu_ctx.render_frame(0, fr, want_hits=True)
+                    assert gpu_ctx.timings()["trace_block_waves"] == waves
+                    assert_same(g, o, f"stack {stack} shadows {shadows} in-trace {in_trace} waves {waves}")
+        ref = True
+    finally:
+        gpu_ctx.set_option(R.OPT_STACK, 16)
+        gpu_ctx.set_option(R.OPT_SHADOW_IN_TRACE, 1)
+        gpu_ctx.set_option(R.OPT_TRACE_WAVES, 1)
+    assert ref
+
+
+def test_block_size_of_other_kernels_is_reported(gpu_ctx, twin):
+    fr = S.build_frame(twin.default_camera(), S.RenderConfig(96, 64))
+    gpu_ctx.upload(0, twin)
+    try:
+        gpu_ctx.set_option(R.OPT_TRACE_WAVES, 2)
+        gpu_ctx.set_option(R.OPT_STACK, -8)
+        gpu_ctx.render_frame(0, fr)
+        assert gpu_ctx.timings()["trace_block_waves"] == 4
+        gpu_ctx.set_option(R.OPT_STACK, 16)
+        gpu_ctx.set_option(R.OPT_PERSISTENT, 1)
+        gpu_ctx.render_frame(0, fr)
+        assert gpu_ctx.timings()["trace_block_waves"] == 4
+    finally:
+        gpu_ctx.set_option(R.OPT_STACK, 16)
+        gpu_ctx.set_option(R.OPT_PERSISTENT, 0)
+        gpu_ctx.set_option(R.OPT_TRACE_WAVES, 1)
+
+
+def test_distinct_records_touched_match_the_oracle(gpu_ctx, twin):
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.1, -0.2, 0.4))
+    fr = S.build_frame(cam, S.RenderConfig(200, 125))
+    gpu_ctx.upload(0, twin)
+    gpu_ctx.set_option(R.OPT_STATS, 1)
+    try:
+        gpu_ctx.render_frame(0, fr)
+        st = gpu_ctx.stats()
+    finally:
+        gpu_ctx.set_option(R.OPT_STATS, 0)
+    oc = O.render(twin, fr, counts=True)["counts"]
+    print({k: st[k] for k in ("distinct_nodes", "distinct_prims", "distinct_hits")})
+    for k in ("distinct_nodes", "distinct_prims", "distinct_hits", "node_pops", "prim_tests", "shadow_prim_tests"):
+        assert st[k] == oc[k], (k, st[k], oc[k])
+    assert st["distinct_nodes"] >= 30 and st["distinct_prims"] >= 30 and st["distinct_hits"] >= 30

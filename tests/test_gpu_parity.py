@@ -301,7 +301,7 @@ def test_renders_enqueued_ahead(gpu_ctx, twin):
         for f in frames[:last + 1]:
             gpu_ctx.render_resident(0, f, want_hits=True)
         gpu_ctx.sync()
-        assert_same(gpu_ctx.download(192, 120, want_hits=True), O.render(twin, frames[last]), f"last frame {last}")
+        assert_same(gpu_ctx.download(want_hits=True), O.render(twin, frames[last]), f"last frame {last}")
 
 
 @pytest.mark.parametrize("stack,waves", [(16, 4), (8, 1), (-8, 4)])
