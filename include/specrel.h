@@ -207,6 +207,23 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
  * arguments), so the caller may change it at once and enqueue further renders; each render's
  * outputs replace the previous one's. */
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
+
+/* No counterpart in the reference. For callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc -- the same one-view
+ * render as specrel_render_resident, writing the caller's device buffers (d_rgb: width*height
+ * 0x00RRGGBB pixels, required; d_hit / d_lambda / d_tau: all three or none, as specrel_render's
+ * optional outputs; len: their element count, which must be width*height) and allocating nothing:
+ * every intermediate buffer lives in a fixed scratch area of SPECREL_NOALLOC_SCRATCH_BYTES that the
+ * library holds on each GPU until the process ends (a 1280x800 view of a scene with 10,000 stars
+ * needs about 35 MB). A view needing more returns SPECREL_EINVAL, as do SPECREL_OPT_STATS and
+ * SPECREL_OPT_SHADOW_QUEUE (they need buffers of their own). Tiles are traced in row order.
+ * Returns once enqueued on the context's stream; specrel_sync waits. Every call on one GPU, from
+ * any context, uses that GPU's one scratch area: calls that overlap on a GPU race. The downloads
+ * and the adjoint refuse to run after it (its outputs are the caller's). Images are bit-identical
+ * to specrel_render's. */
+#define SPECREL_NOALLOC_SCRATCH_BYTES (40u << 20)
+int specrel_render_noalloc(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *d_rgb,
+                           uint32_t *d_hit, float *d_lambda, float *d_tau, uint64_t len);
 int specrel_sync(specrel_ctx *ctx);
 /* len: element count of each non-NULL buffer, which must be the view's width*height. */
 int specrel_download(specrel_ctx *ctx, uint32_t *out_rgb, uint32_t *out_hit, float *out_lambda, float *out_tau,

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -17,7 +18,18 @@
 
 using namespace dev;
 
+// The fixed scratch area of specrel_render_noalloc: every intermediate buffer of a one-view render
+// (frame and view table, per-frame counters, hit slots, occlusion bits, the HDR pyramid, star
+// lists and cones, the stars' per-view values) is carved from it, so that render allocates
+// nothing. Each GPU holds its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns for that GPU (g_scratch below).
+static __device__ __align__(256) unsigned char scratch[SPECREL_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_lock;
+void *g_scratch[SCRATCH_DEVICES] = {};  // device address of `scratch` on each GPU, looked up once
 
 // Device allocation owned by one holder: freed when the holder goes (also on the throw paths of
 // the entry points), moved but never copied.
@@ -92,6 +104,8 @@ struct specrel_ctx {
     void *readback = nullptr;
     size_t readback_cap = 0;
     uint64_t ex_records = 0;  // records the last specrel_extrude_tubes left on the device
+    void *scratch = nullptr;  // this device's scratch area (specrel_render_noalloc)
+    bool last_noalloc = false;  // the last render wrote the caller's device buffers
     // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
     DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
     DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
@@ -131,6 +145,19 @@ std::string g_create_err;
 
 void check(hipError_t e, const char *what) {
     if (e != hipSuccess) throw Fail{SPECREL_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+// The scratch area's address on `device`, which must be current; looked up on the first call for
+// that device (specrel_create).
+void *scratch_address(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_lock);
+    if (!g_scratch[device]) {
+        void *ptr = nullptr;
+        check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch[device] = ptr;
+    }
+    return g_scratch[device];
 }
 
 void dfree(DevBuf &b) {
@@ -388,10 +415,20 @@ bool patch_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
 
 uint32_t num_super(uint32_t w, uint32_t h) { return ((w + SUPER - 1) / SUPER) * ((h + SUPER - 1) / SUPER); }
 
+constexpr const char *NOALLOC_LAST =
+    "the last render was specrel_render_noalloc: its outputs are in the caller's device buffers";
 constexpr int NLEV = SPECREL_BLOOM_LEVELS + 1;
 // Rows of the view table: NLEV tile levels, then super tiles, pixels, pyramid float4s.
 constexpr int ROW_SUPER = NLEV, ROW_PIX = NLEV + 1, ROW_PYR = NLEV + 2, ROWS = NLEV + 3;
+static_assert(ROWS == TABLE_ROWS, "the view table of k_reset_view has the rows of the context's table");
 constexpr uint32_t MAX_VIEWS = 4096;
+
+void check_len(uint64_t len, uint64_t want, const char *what, uint32_t view) {
+    if (len != want)
+        throw Fail{SPECREL_EINVAL, std::string(what) + " size mismatch (view " + std::to_string(view) + "): " +
+                                       std::to_string(len) + " elements given, " + std::to_string(want) +
+                                       " needed"};
+}
 
 void check_frame(const specrel_frame *f, uint32_t i) {
     const uint32_t w = f->size[0], h = f->size[1];
@@ -568,6 +605,58 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// The device buffers and view-table facts of one launch sequence: the context's own buffers
+// (sized by ensure_views), or for specrel_render_noalloc the fixed scratch area.
+struct Bufs {
+    Views V;
+    uint32_t n_views = 0, total_tiles = 0, total_supers = 0, total_pix = 0;
+    const uint32_t *tab_host = nullptr;  // host copy of the view table (ROWS x (n_views + 1))
+    const uint32_t *sizes = nullptr;     // w, h per view
+    uint32_t *frames = nullptr;          // where k_reset_frame / k_reset_view put the frame
+    const TableWords *table = nullptr;   // k_reset_view writes this view table ...
+    uint32_t *table_dst = nullptr;       // ... here
+    uint32_t *counters = nullptr, *slot = nullptr, *mask = nullptr, *rgb = nullptr, *hit = nullptr;
+    float *lam = nullptr, *tau = nullptr;
+    float4 *hdr = nullptr, *tile_cones = nullptr;
+    uint32_t *super_lists = nullptr, *super_counts = nullptr;
+    uint32_t *tile_cost = nullptr, *tile_order = nullptr;  // NULL: tiles in row order
+    StarPrep *prep = nullptr;
+    StarDir *dirs = nullptr;
+    ShadowQuery *squeue = nullptr;
+    unsigned long long *stats = nullptr;
+    uint32_t start(int row, uint32_t view) const { return tab_host[(size_t)row * (n_views + 1) + view]; }
+};
+
+Bufs ctx_bufs(specrel_ctx *ctx) {
+    Bufs B;
+    B.V = device_views(ctx);
+    B.n_views = ctx->n_views;
+    B.total_tiles = ctx->total_tiles;
+    B.total_supers = ctx->total_supers;
+    B.total_pix = ctx->total_pix;
+    B.tab_host = ctx->tab_host.data();
+    B.sizes = ctx->sizes.data();
+    B.frames = (uint32_t *)ctx->frames.p;
+    B.counters = (uint32_t *)ctx->counters.p;
+    B.slot = (uint32_t *)ctx->slot.p;
+    B.mask = (uint32_t *)ctx->mask.p;
+    B.rgb = (uint32_t *)ctx->rgb.p;
+    B.hit = (uint32_t *)ctx->hit.p;
+    B.lam = (float *)ctx->lam.p;
+    B.tau = (float *)ctx->tau.p;
+    B.hdr = (float4 *)ctx->hdr.p;
+    B.tile_cones = (float4 *)ctx->tile_cones.p;
+    B.super_lists = (uint32_t *)ctx->super_lists.p;
+    B.super_counts = (uint32_t *)ctx->super_counts.p;
+    B.tile_cost = (uint32_t *)ctx->tile_cost.p;
+    B.tile_order = (uint32_t *)ctx->tile_order.p;
+    B.prep = (StarPrep *)ctx->prep.p;
+    B.dirs = (StarDir *)ctx->dirs.p;
+    B.squeue = (ShadowQuery *)ctx->squeue.p;
+    B.stats = (unsigned long long *)ctx->stats.p;
+    return B;
+}
+
 // What a launch over the current views needs, decided on the host from the frames.
 struct Plan {
     bool stars = false;    // some view is shaded and the scene has stars (renderer.rs:357)
@@ -646,15 +735,15 @@ Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, 
 
 // The launch sequence of one render of the uploaded views; stage events only when `events`
 // (never inside a graph capture).
-void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool events) {
+void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool events, const Bufs &B) {
     hipStream_t s = ctx->stream;
     const bool stats = ctx->opt_stats != 0;
     auto mark = [&](int i) {
         if (events) check(hipEventRecord(ctx->ev[i], s), "event");
     };
-    const Views V = device_views(ctx);
-    const uint32_t nv = ctx->n_views;
-    uint32_t *counters = (uint32_t *)ctx->counters.p;
+    const Views V = B.V;
+    const uint32_t nv = B.n_views;
+    uint32_t *counters = B.counters;
     Scene S = sc.S;
     if (stats) {  // distinct records touched (cleared in enqueue_views)
         uint32_t *t = (uint32_t *)ctx->touch.p;
@@ -662,8 +751,10 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         S.touch_prims = t + ctx->touch_words[0];
         S.touch_hits = t + ctx->touch_words[0] + ctx->touch_words[1];
     }
-    if (pl.frame_in_reset)
-        k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, (uint32_t *)ctx->frames.p);
+    if (B.table)  // one view without staging: frame and view table from the kernel arguments
+        k_reset_view<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames, *B.table, B.table_dst);
+    else if (pl.frame_in_reset)
+        k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames);
     else
         k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
     check(hipGetLastError(), "k_reset launch");
@@ -676,33 +767,33 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     // order_tiles: k_star_bin also sorts the next render's tile order (one block's share of tiles).
     auto star_pass = [&](hipStream_t ss, uint32_t order_tiles) {
         k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
-            (const float4 *)sc.stars.p, sc.num_stars, V, (StarPrep *)ctx->prep.p, (StarDir *)ctx->dirs.p);
+            (const float4 *)sc.stars.p, sc.num_stars, V, B.prep, B.dirs);
         check(hipGetLastError(), "k_star_prep launch");
         if (ctx->opt_cull) {
-            k_star_bin<<<ctx->total_supers + (order_tiles ? 1u : 0u), 256, 0, ss>>>(
-                V, (const StarDir *)ctx->dirs.p, sc.num_stars, (uint32_t *)ctx->super_lists.p,
-                (uint32_t *)ctx->super_counts.p, (float4 *)ctx->tile_cones.p, (const uint32_t *)ctx->tile_cost.p,
-                order_tiles, (uint32_t *)ctx->tile_order.p);
+            k_star_bin<<<B.total_supers + (order_tiles ? 1u : 0u), 256, 0, ss>>>(
+                V, B.dirs, sc.num_stars, B.super_lists, B.super_counts, B.tile_cones, B.tile_cost, order_tiles,
+                B.tile_order);
             check(hipGetLastError(), "k_star_bin launch");
         }
     };
-    const dim3 tiles(ctx->total_tiles);
+    const dim3 tiles(B.total_tiles);
     const uint32_t path = shadow_path(ctx, pl);
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
     TraceArgs ta;
     ta.S = S;
     ta.V = V;
-    ta.num_tiles = ctx->total_tiles;
-    ta.slot = (uint32_t *)ctx->slot.p;
-    ta.mask = in_trace ? (uint32_t *)ctx->mask.p : nullptr;
+    ta.num_tiles = B.total_tiles;
+    ta.slot = B.slot;
+    ta.mask = in_trace ? B.mask : nullptr;
     ta.queue = counters + 0;
-    ta.stats = (unsigned long long *)ctx->stats.p;
+    ta.stats = B.stats;
     ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
     // Slowest tiles first only when they cannot all start at once (more than four per CU): then
     // the order decides when the slowest ones start; otherwise it only costs.
-    const bool order = ctx->opt_tile_order && !ctx->opt_persistent && ctx->total_tiles > 4u * (uint32_t)ctx->cus;
-    ta.order = order ? (const uint32_t *)ctx->tile_order.p : nullptr;
-    ta.cost = order ? (uint32_t *)ctx->tile_cost.p : nullptr;
+    const bool order = B.tile_order && ctx->opt_tile_order && !ctx->opt_persistent &&
+                       B.total_tiles > 4u * (uint32_t)ctx->cus;
+    ta.order = order ? B.tile_order : nullptr;
+    ta.cost = order ? B.tile_cost : nullptr;
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
     ta.postpone = ctx->opt_postpone == 1 ? (uint32_t)(STACK - 2) : (uint32_t)ctx->opt_postpone;
     launch_any(ctx, stats, tiles, s, ta);
@@ -719,13 +810,13 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     // after the primary rays on this stream and one block sorts the tiles, else on the side stream
     // beside the later passes.
     const bool order_in_bin =
-        order && pl.stars && !overlap && ctx->opt_cull && ctx->total_tiles <= TILE_ORDER_ONE_BLOCK;
+        order && pl.stars && !overlap && ctx->opt_cull && B.total_tiles <= TILE_ORDER_ONE_BLOCK;
     const bool order_side = order && !order_in_bin;
     if (order_side) {
         check(hipEventRecord(ctx->order_fork, s), "fork");
         check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
-        const uint32_t *cost = (const uint32_t *)ctx->tile_cost.p, n = ctx->total_tiles;
-        uint32_t *torder = (uint32_t *)ctx->tile_order.p;
+        const uint32_t *cost = B.tile_cost, n = B.total_tiles;
+        uint32_t *torder = B.tile_order;
         if (n <= TILE_ORDER_ONE_BLOCK) {
             k_tile_order<<<1, 1024, 0, ctx->side>>>(cost, n, torder);
         } else {
@@ -737,7 +828,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         check(hipGetLastError(), "k_tile_order launch");
         check(hipEventRecord(ctx->order_join, ctx->side), "join");
     }
-    if (pl.stars && !overlap) star_pass(s, order_in_bin ? ctx->total_tiles : 0u);
+    if (pl.stars && !overlap) star_pass(s, order_in_bin ? B.total_tiles : 0u);
     mark(2);
     if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
     ShadeArgs ra;
@@ -745,27 +836,27 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ra.T.lut = (const uint8_t *)ctx->tables.p;
     ra.T.glyphs = (const uint8_t *)ctx->tables.p + SPECREL_LUT_SIZE;
     ra.V = V;
-    ra.stars = (const StarPrep *)ctx->prep.p;
-    ra.dirs = (const StarDir *)ctx->dirs.p;
-    ra.super_lists = (const uint32_t *)ctx->super_lists.p;
-    ra.super_counts = (const uint32_t *)ctx->super_counts.p;
-    ra.tile_cones = (const float4 *)ctx->tile_cones.p;
+    ra.stars = B.prep;
+    ra.dirs = B.dirs;
+    ra.super_lists = B.super_lists;
+    ra.super_counts = B.super_counts;
+    ra.tile_cones = B.tile_cones;
     ra.num_stars = sc.num_stars;
     ra.cull = ctx->opt_cull;
     ra.want_hits = pl.want_hits ? 1 : 0;
-    ra.slot = (const uint32_t *)ctx->slot.p;
-    ra.hdr = (float4 *)ctx->hdr.p;
-    ra.hit = (uint32_t *)ctx->hit.p;
-    ra.lambda = (float *)ctx->lam.p;
-    ra.tau = (float *)ctx->tau.p;
-    ra.mask = (uint32_t *)ctx->mask.p;
+    ra.slot = B.slot;
+    ra.hdr = B.hdr;
+    ra.hit = B.hit;
+    ra.lambda = B.lam;
+    ra.tau = B.tau;
+    ra.mask = B.mask;
     ra.queue = nullptr;
     ra.qcount = counters + 1;
     ra.qcap = 0;
-    ra.stats = (unsigned long long *)ctx->stats.p;
+    ra.stats = B.stats;
     if (queued) {
-        ra.queue = (ShadowQuery *)ctx->squeue.p;
-        ra.qcap = (uint32_t)((uint64_t)ctx->total_pix * (1u + pl.max_flashes));
+        ra.queue = B.squeue;
+        ra.qcap = (uint32_t)((uint64_t)B.total_pix * (1u + pl.max_flashes));
         launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
         check(hipGetLastError(), "k_shade setup launch");
         OccludeArgs oa;
@@ -786,27 +877,27 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     }
     check(hipGetLastError(), "k_shade launch");
     mark(3);
-    float4 *pyr = (float4 *)ctx->hdr.p;
+    float4 *pyr = B.hdr;
     // Levels whose every view holds at most BLOOM_TAIL_PIXELS pixels are done by k_bloom_tail.
     int k0 = SPECREL_BLOOM_LEVELS;
     while (k0 > 1) {
         bool small = true;
         for (uint32_t v = 0; v < nv && small; ++v) {
             uint32_t lw, lh;
-            level_size(ctx->sizes[2 * v], ctx->sizes[2 * v + 1], k0 - 1, lw, lh);
+            level_size(B.sizes[2 * v], B.sizes[2 * v + 1], k0 - 1, lw, lh);
             small = (uint64_t)lw * lh <= BLOOM_TAIL_PIXELS;
         }
         if (!small) break;
         --k0;
     }
     for (int k = 1; k < k0; ++k)
-        k_bloom_down<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
+        k_bloom_down<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
     k_bloom_tail<<<nv, 1024, 0, s>>>(V, k0, pyr);
     for (int k = k0 - 1; k >= 1; --k)
-        k_bloom_up<<<host_start(ctx, k, nv), 256, 0, s>>>(V, k, pyr);
+        k_bloom_up<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
     check(hipGetLastError(), "bloom launch");
     mark(4);
-    k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, (uint32_t *)ctx->rgb.p);
+    k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, B.rgb);
     check(hipGetLastError(), "k_composite launch");
     if (order_side) check(hipStreamWaitEvent(s, ctx->order_join, 0), "join wait");
     mark(5);
@@ -850,7 +941,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
             drop_graph(ctx);
             check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             try {
-                record_launches(ctx, sc, pl, false);
+                record_launches(ctx, sc, pl, false, ctx_bufs(ctx));
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(s, &g);
@@ -872,7 +963,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
         check(hipEventRecord(ctx->ev[5], s), "event");
         ctx->timed_stages = false;
     } else {
-        record_launches(ctx, sc, pl, true);
+        record_launches(ctx, sc, pl, true, ctx_bufs(ctx));
         ctx->timed_stages = true;
     }
     auto t1 = std::chrono::steady_clock::now();
@@ -881,7 +972,130 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     ctx->timings.shadow_path = shadow_path(ctx, pl);
     ctx->timings.trace_block_waves = trace_block_waves(ctx);
     ctx->have_hits = pl.want_hits;
+    ctx->last_noalloc = false;
     return SPECREL_OK;
+}
+
+// Byte offsets of one view's intermediate buffers in the scratch area (256-B aligned pieces).
+struct ScratchLayout {
+    size_t frame, table, counters, slot, mask, hdr, super_counts, super_lists, tile_cones, prep, dirs, total;
+};
+
+ScratchLayout scratch_layout(uint64_t pixels, uint64_t pyr, uint64_t tiles, uint64_t supers, uint64_t stars) {
+    ScratchLayout L;
+    size_t at = 0;
+    auto take = [&](uint64_t bytes) {
+        const size_t here = at;
+        at += (size_t)((bytes + 255) & ~(uint64_t)255);
+        return here;
+    };
+    L.frame = take(sizeof(specrel_frame));
+    L.table = take(sizeof(TableWords));
+    L.counters = take(COUNTERS * 4);
+    L.slot = take(pixels * 4);
+    L.mask = take(pixels * 4);
+    L.hdr = take(pyr * sizeof(float4));
+    L.super_counts = take(supers * 4);
+    L.super_lists = take(supers * SUPER_CAP * 4);
+    L.tile_cones = take(tiles * 16);
+    L.prep = take(std::max<uint64_t>(stars, 1) * sizeof(StarPrep));
+    L.dirs = take(std::max<uint64_t>(stars, 1) * sizeof(StarDir));
+    L.total = at;
+    return L;
+}
+
+// specrel_render_noalloc: one view, enqueued, every intermediate buffer in this device's scratch
+// area and the outputs in the caller's device buffers. No allocation, no host-to-device copy (the
+// frame and the view table travel as kernel arguments), tiles in row order (a tile order would
+// have to survive between calls in memory every call shares).
+void enqueue_noalloc(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *d_rgb,
+                     uint32_t *d_hit, float *d_lambda, float *d_tau, uint64_t len) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!frame) throw Fail{SPECREL_EINVAL, "frame is NULL"};
+    if (!d_rgb) throw Fail{SPECREL_EINVAL, "d_rgb is NULL"};
+    const bool hits = d_hit || d_lambda || d_tau;
+    if (hits && !(d_hit && d_lambda && d_tau)) throw Fail{SPECREL_EINVAL, "d_hit, d_lambda and d_tau: all or none"};
+    check_frame(frame, 0);
+    const uint32_t w = frame->size[0], h = frame->size[1];
+    check_len(len, (uint64_t)w * h, "framebuffer", 0);
+    if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
+        throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
+    if (ctx->opt_stats || ctx->opt_shadow_queue)
+        throw Fail{SPECREL_EINVAL, "SPECREL_OPT_STATS and SPECREL_OPT_SHADOW_QUEUE need buffers the "
+                                   "allocation-free render does not have"};
+    if (!ctx->scratch) throw Fail{SPECREL_EDEVICE, "no scratch area on this device"};
+    SceneSlot &sc = ctx->scenes[scene_idx];
+    // The view table of this one view: rows of (0, total).
+    TableWords tw{};
+    uint64_t tiles = 0, pyr = pyramid_len(w, h);
+    for (int k = 0; k < NLEV; ++k) {
+        uint32_t lw, lh;
+        level_size(w, h, k, lw, lh);
+        const uint64_t t = (uint64_t)((lw + 15) / 16) * ((lh + 15) / 16);
+        if (k == 0) tiles = t;
+        tw.w[2 * k + 1] = (uint32_t)t;
+    }
+    const uint64_t supers = num_super(w, h), pixels = (uint64_t)w * h;
+    tw.w[2 * ROW_SUPER + 1] = (uint32_t)supers;
+    tw.w[2 * ROW_PIX + 1] = (uint32_t)pixels;
+    tw.w[2 * ROW_PYR + 1] = (uint32_t)pyr;
+    const ScratchLayout L = scratch_layout(pixels, pyr, tiles, supers, sc.num_stars);
+    if (L.total > sizeof(scratch))
+        throw Fail{SPECREL_EINVAL, "a " + std::to_string(w) + "x" + std::to_string(h) + " view with " +
+                                       std::to_string(sc.num_stars) + " stars needs " + std::to_string(L.total) +
+                                       " bytes of scratch; the allocation-free render has " +
+                                       std::to_string(sizeof(scratch))};
+    Plan pl;
+    if (frame->mode == SPECREL_MODE_SHADED) {
+        pl.stars = sc.num_stars > 0;
+        if (frame->shadow_mode != SPECREL_SHADOW_OFF) {
+            pl.queries = true;
+            pl.max_flashes = frame->num_flashes;
+        }
+    }
+    pl.want_hits = hits;
+    std::memcpy(pl.frame.w, frame, sizeof(specrel_frame));
+    pl.frame_in_reset = true;
+    char *base = (char *)ctx->scratch;
+    uint32_t *table = (uint32_t *)(base + L.table);
+    const uint32_t sizes[2] = {w, h};
+    Bufs B;
+    B.V.frames = (const specrel_frame *)(base + L.frame);
+    for (int k = 0; k < NLEV; ++k) B.V.level_start[k] = table + 2 * k;
+    B.V.super_start = table + 2 * ROW_SUPER;
+    B.V.pix_start = table + 2 * ROW_PIX;
+    B.V.pyr_start = table + 2 * ROW_PYR;
+    B.V.n = 1;
+    B.n_views = 1;
+    B.total_tiles = (uint32_t)tiles;
+    B.total_supers = (uint32_t)supers;
+    B.total_pix = (uint32_t)pixels;
+    B.tab_host = tw.w;
+    B.sizes = sizes;
+    B.frames = (uint32_t *)(base + L.frame);
+    B.table = &tw;
+    B.table_dst = table;
+    B.counters = (uint32_t *)(base + L.counters);
+    B.slot = (uint32_t *)(base + L.slot);
+    B.mask = (uint32_t *)(base + L.mask);
+    B.hdr = (float4 *)(base + L.hdr);
+    B.super_counts = (uint32_t *)(base + L.super_counts);
+    B.super_lists = (uint32_t *)(base + L.super_lists);
+    B.tile_cones = (float4 *)(base + L.tile_cones);
+    B.prep = (StarPrep *)(base + L.prep);
+    B.dirs = (StarDir *)(base + L.dirs);
+    B.rgb = d_rgb;
+    B.hit = d_hit;
+    B.lam = d_lambda;
+    B.tau = d_tau;
+    record_launches(ctx, sc, pl, true, B);
+    ctx->timed_stages = true;
+    auto t1 = std::chrono::steady_clock::now();
+    ctx->timings = specrel_timings{};
+    ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    ctx->timings.shadow_path = shadow_path(ctx, pl);
+    ctx->timings.trace_block_waves = trace_block_waves(ctx);
+    ctx->last_noalloc = true;
 }
 
 // The pinned readback buffer, grown to at least `bytes` (the stream is idle when it moves).
@@ -968,18 +1182,13 @@ void collect(specrel_ctx *ctx) {
     }
 }
 
-void check_len(uint64_t len, uint64_t want, const char *what, uint32_t view) {
-    if (len != want)
-        throw Fail{SPECREL_EINVAL, std::string(what) + " size mismatch (view " + std::to_string(view) + "): " +
-                                       std::to_string(len) + " elements given, " + std::to_string(want) +
-                                       " needed"};
-}
 
 // The readback copies of view `view` of the last render (NULL pointers are skipped); every
 // non-NULL buffer holds `len` elements, which must be the view's width * height.
 void view_fetches(specrel_ctx *ctx, uint32_t view, uint32_t *rgb, uint32_t *hit, float *lam, float *tau, uint64_t len,
                   std::vector<Fetch> &f) {
     if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+    if (ctx->last_noalloc) throw Fail{SPECREL_EINVAL, NOALLOC_LAST};
     if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
     if ((hit || lam || tau) && !ctx->have_hits) throw Fail{SPECREL_EINVAL, "the last render did not write hit buffers"};
     const size_t off = host_start(ctx, ROW_PIX, view);
@@ -1013,6 +1222,7 @@ int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in,
     if (n_objects > 1024) throw Fail{SPECREL_EINVAL, "at most 1024 objects"};
     const uint32_t nv = ctx->n_views;
     if (nv == 0 || ctx->last_frames.size() != nv) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+    if (ctx->last_noalloc) throw Fail{SPECREL_EINVAL, NOALLOC_LAST};
     for (uint32_t v = 0; v < nv; ++v) {
         const specrel_frame &f = ctx->last_frames[v];
         if (f.mode != SPECREL_MODE_SHADED || f.slice_mode != SPECREL_VIEW_PAST_CONE ||
@@ -1147,6 +1357,7 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         ctx->device = device_ordinal;
         ctx->name = std::string(prop.name) + " (" + prop.gcnArchName + ", HIP)";
         check(hipSetDevice(device_ordinal), "hipSetDevice");
+        ctx->scratch = scratch_address(device_ordinal);
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto &e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
         check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
@@ -1554,6 +1765,16 @@ int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_
     });
 }
 
+int specrel_render_noalloc(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, uint32_t *d_rgb,
+                           uint32_t *d_hit, float *d_lambda, float *d_tau, uint64_t len) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        enqueue_noalloc(ctx, scene_idx, frame, d_rgb, d_hit, d_lambda, d_tau, len);
+        return SPECREL_OK;
+    });
+}
+
 int specrel_render_batch(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n_views,
                          const specrel_view_out *outs) {
     if (!ctx) return SPECREL_EINVAL;
@@ -1622,6 +1843,7 @@ int specrel_download_linear(specrel_ctx *ctx, uint32_t view, float *out_rgb3, ui
     return guarded(ctx, [&] {
         if (!out_rgb3) throw Fail{SPECREL_EINVAL, "output is NULL"};
         if (ctx->n_views == 0) throw Fail{SPECREL_EINVAL, "nothing rendered yet"};
+        if (ctx->last_noalloc) throw Fail{SPECREL_EINVAL, NOALLOC_LAST};
         if (view >= ctx->n_views) throw Fail{SPECREL_EINVAL, "view index out of range"};
         const size_t n = (size_t)ctx->sizes[2 * view] * ctx->sizes[2 * view + 1];
         check_len(len, 3 * (uint64_t)n, "linear image", view);

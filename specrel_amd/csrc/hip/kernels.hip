@@ -534,6 +534,16 @@ __global__ void k_reset_frame(uint32_t *counters, uint32_t n, FrameWords f, uint
     if (i < FrameWords::N) dst[i] = f.w[i];
 }
 
+// The same for a view rendered without any copy from the host: the view table comes as an
+// argument too (specrel_render_noalloc).
+__global__ void k_reset_view(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst_frame, TableWords t,
+                             uint32_t *dst_table) {
+    uint32_t i = threadIdx.x;
+    if (i < n) counters[i] = 0u;
+    if (i < FrameWords::N) dst_frame[i] = f.w[i];
+    if (i < 2 * TABLE_ROWS) dst_table[i] = t.w[i];
+}
+
 // Shading: the winner's record is intersected again (same inputs, same float operations, so the
 // same lambda, event and barycentrics), interpolated and shaded with shadow rays, haze and the
 // tile's stars; writes the HDR pixel and the optional per-pixel outputs.

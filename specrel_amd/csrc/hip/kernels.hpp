@@ -223,6 +223,14 @@ struct FrameWords {
 };
 static_assert(sizeof(specrel_frame) % 4 == 0 && FrameWords::N <= 256, "one frame per k_reset_frame thread block");
 __global__ void k_reset_frame(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst);
+// The view table of one view (level tile prefix sums, super tiles, pixels, pyramid floats: each
+// row's two entries), passed by value to k_reset_view.
+constexpr int TABLE_ROWS = SPECREL_BLOOM_LEVELS + 4;
+struct TableWords {
+    uint32_t w[2 * TABLE_ROWS];
+};
+__global__ void k_reset_view(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst_frame, TableWords t,
+                             uint32_t *dst_table);
 __global__ void k_tile_order(const uint32_t *cost, uint32_t n_tiles, uint32_t *order);
 __global__ void k_tile_hist(const uint32_t *cost, uint32_t n_tiles, uint32_t *total);
 __global__ void k_tile_scatter(const uint32_t *cost, uint32_t n_tiles, const uint32_t *total, uint32_t *cursor,

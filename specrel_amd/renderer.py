@@ -100,6 +100,8 @@ SIGNATURES = {
     "specrel_num_scenes": (C.c_uint32, [_P]),
     "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP, C.c_uint64]),
     "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
+    "specrel_render_noalloc": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_uint64]),
     "specrel_sync": (C.c_int, [_P]),
     "specrel_download": (C.c_int, [_P, _U32P, _U32P, _FP, _FP, C.c_uint64]),
     "specrel_render_batch": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_uint32, C.POINTER(ViewOut)]),
@@ -258,6 +260,12 @@ class Context:
     def render_resident(self, idx, frame, want_hits=False):
         self._check(self.lib.specrel_render_resident(self._h, idx, C.byref(frame), int(want_hits)))
         self._sizes = [(frame.size[0], frame.size[1])]
+
+    def render_noalloc(self, idx, frame, d_rgb, d_hit=None, d_lambda=None, d_tau=None):
+        """specrel_render_noalloc: one view into caller-owned device buffers (device addresses as
+        ints), every intermediate in the library's fixed per-GPU scratch area; enqueued only."""
+        n = frame.size[0] * frame.size[1]
+        self._check(self.lib.specrel_render_noalloc(self._h, idx, C.byref(frame), d_rgb, d_hit, d_lambda, d_tau, n))
 
     def sync(self):
         self._check(self.lib.specrel_sync(self._h))
