@@ -86,7 +86,7 @@ enum {
                                      tiled primary-ray kernel starts the tiles that took longest in the
                                      previous render of the same view sizes first (row order after a
                                      change of sizes); 0: row order. Results do not depend on it. */
-    SPECREL_OPT_LEAF_POSTPONE = 13 /* 1 (default): a closest-hit ray of the primary-ray kernel (LDS
+    SPECREL_OPT_LEAF_POSTPONE = 13, /* 1 (default): a closest-hit ray of the primary-ray kernel (LDS
                                      stacks, not merged loads) that reaches a leaf keeps it pending
                                      and traverses on to its next leaf; the pending leaves are tested
                                      in order once 3/8 of the wave's tracing lanes hold one; a ray
@@ -96,6 +96,10 @@ enum {
                                      when reached. Results do not depend on it; instrumented renders
                                      (SPECREL_OPT_STATS) test every leaf when reached, so their
                                      counters are the reference's. */
+    SPECREL_OPT_ADJOINT_MASK = 14  /* which pixels specrel_adjoint[_batch] counts as valid: 0 (default)
+                                     every hit inside its tetrahedron whose neighbours are on the same
+                                     object (silhouettes masked); 1 also no facet, texture or shadow
+                                     edge among the neighbours (the set finite differences resolve) */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the
@@ -267,11 +271,14 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
  * is the derivative with respect to the object's velocity there.
  *
  * Each pixel is recomputed from the render's per-pixel hit (the checkpoint): intersection,
- * interpolation, shading and its shadow ray, then differentiated by hand. Valid pixels hit an
- * object, carry no starlight and have 8 neighbours inside the image with the same object, albedo
- * (without the wireframe blend), shadow state and facet (normal and four-velocity, constant per
- * tetrahedron); the others (misses, silhouettes, texture, shadow and facet edges) are counted per
- * class and left out. mask_in (width*height bytes, nonzero =
+ * interpolation, shading and its shadow ray, then differentiated by hand. Valid pixels hit inside
+ * their tetrahedron (every barycentric > 0), carry no starlight and have 8 neighbours inside the
+ * image on the same object, none of them a miss or starlight: their shading is differentiable in
+ * every parameter, and the gradient is its derivative. The others are counted per class and left
+ * out: misses, starlight, the image border, silhouettes (a neighbour off the object) and edges (a
+ * hit on its tetrahedron's boundary). With SPECREL_OPT_ADJOINT_MASK = 1, a neighbour with another
+ * albedo (without the wireframe blend), shadow state or facet (normal and four-velocity, constant
+ * per tetrahedron) makes an edge too: the set a finite step can cross nothing in. mask_in (width*height bytes, nonzero =
  * valid; may be NULL) replaces that test, so a loss can be evaluated on a fixed pixel set;
  * mask_out (may be NULL) receives the mask used. n_pixels is the pixel count the target and the
  * masks hold and must equal the last render's (SPECREL_EINVAL otherwise). d_objects receives 3

@@ -354,7 +354,7 @@ void pixel_color(const Ctx &c, const oracle_pixel_state &st, uint32_t obj, D out
 extern "C" int oracle_adjoint_ref(const specrel_scene_arrays *s, const specrel_frame *f, const double theta[10],
                                   const float *target, const uint8_t *mask_in, const uint32_t *obj_first,
                                   const uint32_t *obj_count, const float *obj_center, uint32_t n_obj,
-                                  int frame_values, int nthreads, double *loss_out, double *grad_out,
+                                  int frame_values, int strict, int nthreads, double *loss_out, double *grad_out,
                                   uint64_t counts_out[6], uint8_t *mask_out, double *max_rel_color_err) {
     const uint32_t w = f->size[0], h = f->size[1];
     const size_t np = (size_t)w * h;
@@ -396,8 +396,9 @@ extern "C" int oracle_adjoint_ref(const specrel_scene_arrays *s, const specrel_f
                         size_t ni = (size_t)(py + dy) * w + (px + dx);
                         if (sig[ni] == sig[i]) continue;
                         if (sig[ni] == MISS || sig[ni] == STAR || obj[ni] != obj[i]) cls = 4;
-                        else if (cls == 0) cls = 5;
+                        else if (strict && cls == 0) cls = 5;
                     }
+                if (cls == 0 && !(st[i].flags & ORACLE_PX_INSIDE)) cls = 5;
             }
             cnt[cls]++;
             valid[i] = cls == 0;

@@ -659,8 +659,9 @@ static o3 surface_albedo(const o_ctx *x, const o_rec *h, const o_mat *m) { /* re
     return albedo;
 }
 
-/* compute_illumination + light_emission_event (renderer.rs:585-653) */
-static float illumination(const o_ctx *x, o4 dir, const o_rec *h, const o_mat *m) {
+/* compute_illumination + light_emission_event (renderer.rs:585-653); `parts` (may be NULL)
+ * receives the intermediate values for the tests. */
+static float illumination(const o_ctx *x, o4 dir, const o_rec *h, const o_mat *m, oracle_shade_parts *parts) {
     const specrel_frame *f = x->f;
     o3 light = V3(x->s->light_pos[1], x->s->light_pos[2], x->s->light_pos[3]);
     o3 s2l = sub3(light, spat(h->event));
@@ -677,12 +678,20 @@ static float illumination(const o_ctx *x, o4 dir, const o_rec *h, const o_mat *m
     o4 to_light = mul4(sub4(emit, h->event), 1.0f / dist);
     float cl = fmx(rest_frame_cos(to_light, h->normal, h->four_vel), 0.0f);
     float cv = fmx(-rest_frame_cos(dir, h->normal, h->four_vel), 0.0f);
+    if (parts) {
+        parts->t_emit = t_emit;
+        parts->dist = dist;
+        parts->cos_light = cl;
+        parts->cos_view = cv;
+        parts->light_shadowed = 0;
+    }
     if (m->pattern == SPECREL_PATTERN_CHECKER) {
         int in_shadow = 0;
         if (cl > 0.0f && f->shadow_mode != SPECREL_SHADOW_OFF) {
             if (x->c) x->c->shadow_rays++;
             in_shadow = occluded(x->s, h->event, to_light, 2e-3f, dist - 2e-3f, x->c);
         }
+        if (parts) parts->light_shadowed = in_shadow;
         float sf = in_shadow ? 0.32f : 1.0f;
         float atten = 1.0f / (1.0f + 0.006f * dist * dist);
         return m->emission + 0.30f * cv + 0.80f * sf * cl * atten;
@@ -720,19 +729,26 @@ static o3 flash_illumination(const o_ctx *x, const o_rec *h) {
     return total;
 }
 
-/* shade_hit (renderer.rs:414-489) */
-static o3 shade_hit(const o_ctx *x, o4 dir, const o_rec *h) {
+/* shade_hit (renderer.rs:414-489); `parts` (may be NULL) receives the intermediate values. */
+static o3 shade_hit(const o_ctx *x, o4 dir, const o_rec *h, oracle_shade_parts *parts) {
     const specrel_frame *f = x->f;
     o_mat m = load_mat(x->s, h->material);
     switch (f->mode) {
         case SPECREL_MODE_SHADED: {
             o3 albedo = surface_albedo(x, h, &m);
-            float lighting = illumination(x, dir, h, &m);
+            float lighting = illumination(x, dir, h, &m, parts);
             o3 lit = mul3(albedo, lighting);
+            o3 fl = V3(0.0f, 0.0f, 0.0f);
             if (f->num_flashes > 0) {
-                o3 fl = flash_illumination(x, h);
+                fl = flash_illumination(x, h);
                 o3 a = add3(mul3(albedo, 0.65f), splat3(0.35f));
                 lit = add3(lit, V3(a.x * fl.x, a.y * fl.y, a.z * fl.z));
+            }
+            if (parts) {
+                parts->albedo[0] = albedo.x, parts->albedo[1] = albedo.y, parts->albedo[2] = albedo.z;
+                parts->lighting = lighting;
+                parts->flash[0] = fl.x, parts->flash[1] = fl.y, parts->flash[2] = fl.z;
+                parts->lit[0] = lit.x, parts->lit[1] = lit.y, parts->lit[2] = lit.z;
             }
             return distance_fog(lit, dir, h->lambda);
         }
@@ -1044,30 +1060,44 @@ typedef struct {
     const o_touch *touch;
 } o_job;
 
+/* The ray of pixel (px, py) (renderer.rs:334-384): normalised device coordinates scaled by the
+ * frame's half extents, the local direction normalised (twice on the light cones, as
+ * past_light_ray_direction / future_light_ray_direction normalise again, minkowski.rs:564-575),
+ * then in the tetrad's frame. *sx, *sy receive the scaled coordinates. */
+static o4 pixel_ray(const specrel_frame *f, uint32_t w, uint32_t h, float half_w, float half_h, uint32_t px,
+                    uint32_t py, float *sx_out, float *sy_out) {
+    float ndc_y = 1.0f - 2.0f * (((float)py + 0.5f) / (float)h);
+    float sy = ndc_y * half_h;
+    float ndc_x = 2.0f * (((float)px + 0.5f) / (float)w) - 1.0f;
+    float sx = ndc_x * half_w;
+    o3 local_dir = norm3(V3(sx, sy, 1.0f));
+    o4 e0 = ld4(f->e[0]), e1 = ld4(f->e[1]), e2 = ld4(f->e[2]), e3 = ld4(f->e[3]);
+    o4 dir;
+    if (f->slice_mode == SPECREL_VIEW_PAST_CONE) { /* past_light_ray_direction */
+        o3 n = norm3(local_dir);
+        dir = add4(add4(add4(neg4(e0), mul4(e1, n.x)), mul4(e2, n.y)), mul4(e3, n.z));
+    } else if (f->slice_mode == SPECREL_VIEW_FRONTIER) { /* future_light_ray_direction */
+        o3 n = norm3(local_dir);
+        dir = add4(add4(add4(e0, mul4(e1, n.x)), mul4(e2, n.y)), mul4(e3, n.z));
+    } else { /* local_spatial_to_world */
+        dir = add4(add4(mul4(e1, local_dir.x), mul4(e2, local_dir.y)), mul4(e3, local_dir.z));
+    }
+    *sx_out = sx;
+    *sy_out = sy;
+    return dir;
+}
+
 /* One image row: HDR into hdr_row, hit / lambda / tau (when asked) at row `out_row` of those
  * arrays (the frame row, or the row's place in a sample). */
 static void shade_row(o_job *J, uint32_t py, uint32_t out_row, float *hdr_row, oracle_counts *c) {
     const specrel_frame *f = J->base.f;
     o_ctx x = J->base;
     x.c = c;
-    float ndc_y = 1.0f - 2.0f * (((float)py + 0.5f) / (float)J->h);
-    float sy = ndc_y * J->half_h;
     o_rec rec;
+    o4 e0 = ld4(f->e[0]);
     for (uint32_t px = 0; px < J->w; ++px) {
-        float ndc_x = 2.0f * (((float)px + 0.5f) / (float)J->w) - 1.0f;
-        float sx = ndc_x * J->half_w;
-        o3 local_dir = norm3(V3(sx, sy, 1.0f));
-        o4 e0 = ld4(f->e[0]), e1 = ld4(f->e[1]), e2 = ld4(f->e[2]), e3 = ld4(f->e[3]);
-        o4 dir;
-        if (f->slice_mode == SPECREL_VIEW_PAST_CONE) { /* past_light_ray_direction */
-            o3 n = norm3(local_dir);
-            dir = add4(add4(add4(neg4(e0), mul4(e1, n.x)), mul4(e2, n.y)), mul4(e3, n.z));
-        } else if (f->slice_mode == SPECREL_VIEW_FRONTIER) { /* future_light_ray_direction */
-            o3 n = norm3(local_dir);
-            dir = add4(add4(add4(e0, mul4(e1, n.x)), mul4(e2, n.y)), mul4(e3, n.z));
-        } else { /* local_spatial_to_world */
-            dir = add4(add4(mul4(e1, local_dir.x), mul4(e2, local_dir.y)), mul4(e3, local_dir.z));
-        }
+        float sx, sy;
+        o4 dir = pixel_ray(f, J->w, J->h, J->half_w, J->half_h, px, py, &sx, &sy);
         if (c) c->rays++;
         o_hit h;
         int found = trace(J->base.s, J->ray_origin, dir, 1e-3f, 180.0f, &h, c);
@@ -1076,7 +1106,7 @@ static void shade_row(o_job *J, uint32_t py, uint32_t out_row, float *hdr_row, o
             if (c) c->hits++;
             if (tl_touch) touch_bit(tl_touch->hits, (uint32_t)h.slot);
             interpolate_hit(&J->base.s->attrs[h.slot], &h, &rec);
-            color = shade_hit(&x, dir, &rec);
+            color = shade_hit(&x, dir, &rec, NULL);
             if (f->num_haze > 0) color = haze(color, rec.event, (const float (*)[4])f->haze_origin, f->num_haze);
         } else {
             color = shade_sky(dir);
@@ -1245,6 +1275,7 @@ static void pixel_state(o_job *J, uint32_t px, uint32_t py, oracle_pixel_state *
     o_mat m = load_mat(s, rec.material);
     out->slot = h.slot;
     out->material = rec.material;
+    if (h.bary[0] > 0.0f && h.bary[1] > 0.0f && h.bary[2] > 0.0f && h.bary[3] > 0.0f) out->flags |= ORACLE_PX_INSIDE;
     out->pattern = (int32_t)m.pattern;
     o3 nw;
     if (m.pattern == SPECREL_PATTERN_CLOCK || m.pattern == SPECREL_PATTERN_LIGHT_CLOCK) {
@@ -1315,7 +1346,7 @@ static void pixel_state(o_job *J, uint32_t px, uint32_t py, oracle_pixel_state *
         float cl = fmx(rest_frame_cos(to_light, rec.normal, rec.four_vel), 0.0f);
         if (cl > 0.0f) out->in_shadow = occluded(s, rec.event, to_light, 2e-3f, dist - 2e-3f, NULL);
     }
-    o3 color = shade_hit(&J->base, dir, &rec);
+    o3 color = shade_hit(&J->base, dir, &rec, NULL);
     out->color[0] = color.x;
     out->color[1] = color.y;
     out->color[2] = color.z;
@@ -1354,4 +1385,75 @@ int oracle_pixel_states(const specrel_scene_arrays *s, const specrel_frame *f, i
     free(th);
     pthread_mutex_destroy(&J.lock);
     return 0;
+}
+
+/* ---------------------------------------------------------------- test entries (pinning tests) */
+/* shade_hit on a hit record given field by field (renderer.rs:414-489), with the scene's light,
+ * materials and BVH (for the shadow rays) and the frame's mode, shadow mode, camera and flashes.
+ * out: the colour before haze and stars; parts (may be NULL): the intermediate values. */
+int oracle_shade_record(const specrel_scene_arrays *s, const specrel_frame *f, const float dir[4],
+                        const oracle_record *r, float out[3], oracle_shade_parts *parts) {
+    if (r->material >= s->num_materials) return -1;
+    uint8_t lut[SPECREL_LUT_SIZE];
+    oracle_gamma_lut(lut);
+    o_job J;
+    setup_job(&J, s, f, lut);
+    o_rec rec;
+    rec.lambda = r->lambda;
+    rec.event = ld4(r->event);
+    memcpy(rec.tet_bary, r->tet_bary, 16);
+    rec.tri_bary = V3(r->tri_bary[0], r->tri_bary[1], r->tri_bary[2]);
+    rec.uv_u = r->uv[0];
+    rec.uv_v = r->uv[1];
+    rec.proper_time = r->proper_time;
+    rec.grad_u = r->tau_uv_grad[0];
+    rec.grad_v = r->tau_uv_grad[1];
+    rec.uvws_u = r->uv_world_scale[0];
+    rec.uvws_v = r->uv_world_scale[1];
+    rec.normal = ld4(r->normal);
+    rec.four_vel = ld4(r->four_velocity);
+    rec.decided_at = r->decided_at;
+    rec.material = r->material;
+    if (parts) memset(parts, 0, sizeof *parts);
+    o3 c = shade_hit(&J.base, ld4(dir), &rec, parts);
+    out[0] = c.x, out[1] = c.y, out[2] = c.z;
+    pthread_mutex_destroy(&J.lock);
+    return 0;
+}
+
+/* The ray origin and direction of pixel (px, py) of frame f (renderer.rs:334-384). */
+void oracle_pixel_ray(const specrel_frame *f, uint32_t px, uint32_t py, float origin[4], float dir[4]) {
+    o_job J;
+    uint8_t lut[SPECREL_LUT_SIZE] = {0};
+    specrel_scene_arrays empty;
+    memset(&empty, 0, sizeof empty);
+    setup_job(&J, &empty, f, lut);
+    float sx, sy;
+    o4 d = pixel_ray(f, J.w, J.h, J.half_w, J.half_h, px, py, &sx, &sy);
+    origin[0] = J.ray_origin.t, origin[1] = J.ray_origin.x, origin[2] = J.ray_origin.y, origin[3] = J.ray_origin.z;
+    dir[0] = d.t, dir[1] = d.x, dir[2] = d.y, dir[3] = d.z;
+    pthread_mutex_destroy(&J.lock);
+}
+
+void oracle_sky_gradient(const float dir[4], float out[3]) {
+    o3 c = sky_gradient(ld4(dir));
+    out[0] = c.x, out[1] = c.y, out[2] = c.z;
+}
+
+void oracle_distance_fog(const float color[3], const float dir[4], float lambda, float out[3]) {
+    o3 c = distance_fog(V3(color[0], color[1], color[2]), ld4(dir), lambda);
+    out[0] = c.x, out[1] = c.y, out[2] = c.z;
+}
+
+/* The f32 helpers over arrays: fn 0 exp, 1 sin, 2 cos, 3 atan2 (y = a, x = b), 4 acos. */
+void oracle_math_vec(int fn, const float *a, const float *b, float *out, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = ex(a[i]); break;
+            case 1: out[i] = sn(a[i]); break;
+            case 2: out[i] = cs(a[i]); break;
+            case 3: out[i] = at2(a[i], b[i]); break;
+            default: out[i] = ac(a[i]); break;
+        }
+    }
 }

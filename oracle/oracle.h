@@ -86,9 +86,40 @@ enum {
     ORACLE_PX_RL_FREE = 8,    /* light rest-frame cosine neither clamped nor degenerate */
     ORACLE_PX_RV_FREE = 16,   /* view rest-frame cosine neither clamped nor degenerate */
     ORACLE_PX_HZ_FREE = 32,   /* sky horizon blend not clamped */
-    ORACLE_PX_DIST_FREE = 64  /* light distance above its 1e-4 floor */
+    ORACLE_PX_DIST_FREE = 64, /* light distance above its 1e-4 floor */
+    ORACLE_PX_INSIDE = 128    /* the hit lies inside its tetrahedron (every barycentric > 0) */
 };
 int oracle_pixel_states(const specrel_scene_arrays *s, const specrel_frame *f, int nthreads, oracle_pixel_state *out);
+
+/* Test entries: shading of a hand-built hit record and its intermediate values. */
+typedef struct oracle_record {
+    float lambda;
+    float event[4];
+    float tet_bary[4];
+    float tri_bary[3];
+    float uv[2];
+    float proper_time;
+    float tau_uv_grad[2];
+    float uv_world_scale[2];
+    float normal[4];
+    float four_velocity[4];
+    float decided_at;
+    uint32_t material;
+} oracle_record;
+typedef struct oracle_shade_parts {
+    float albedo[3];   /* sample_surface_albedo */
+    float lighting;    /* compute_illumination */
+    float cos_light, cos_view, t_emit, dist;
+    int32_t light_shadowed;
+    float flash[3];    /* compute_flash_illumination */
+    float lit[3];      /* before the distance fog */
+} oracle_shade_parts;
+int oracle_shade_record(const specrel_scene_arrays *s, const specrel_frame *f, const float dir[4],
+                        const oracle_record *r, float out[3], oracle_shade_parts *parts);
+void oracle_pixel_ray(const specrel_frame *f, uint32_t px, uint32_t py, float origin[4], float dir[4]);
+void oracle_sky_gradient(const float dir[4], float out[3]);
+void oracle_distance_fog(const float color[3], const float dir[4], float lambda, float out[3]);
+void oracle_math_vec(int fn, const float *a, const float *b, float *out, uint64_t n);
 
 /* f32 helpers as the oracle defines them (checked against high-precision values). */
 float oracle_expf(float x);

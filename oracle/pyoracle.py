@@ -20,6 +20,24 @@ class Counts(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Record(C.Structure):
+    """oracle_record: a hit record given field by field (HitRecord4D, mesh4d.rs:382-425)."""
+    _fields_ = [("lambda_", C.c_float), ("event", C.c_float * 4), ("tet_bary", C.c_float * 4),
+                ("tri_bary", C.c_float * 3), ("uv", C.c_float * 2), ("proper_time", C.c_float),
+                ("tau_uv_grad", C.c_float * 2), ("uv_world_scale", C.c_float * 2), ("normal", C.c_float * 4),
+                ("four_velocity", C.c_float * 4), ("decided_at", C.c_float), ("material", C.c_uint32)]
+
+
+class ShadeParts(C.Structure):
+    _fields_ = [("albedo", C.c_float * 3), ("lighting", C.c_float), ("cos_light", C.c_float),
+                ("cos_view", C.c_float), ("t_emit", C.c_float), ("dist", C.c_float),
+                ("light_shadowed", C.c_int32), ("flash", C.c_float * 3), ("lit", C.c_float * 3)]
+
+    def as_dict(self):
+        return {n: (np.array(getattr(self, n)[:], np.float32) if n in ("albedo", "flash", "lit") else getattr(self, n))
+                for n, _ in self._fields_}
+
+
 _lib = None
 _FP = C.POINTER(C.c_float)
 _U32P = C.POINTER(C.c_uint32)
@@ -62,8 +80,15 @@ def lib():
             "oracle_acosf": (C.c_float, [C.c_float]),
             "oracle_rem_euclid": (C.c_float, [C.c_float, C.c_float]),
             "oracle_pixel_states": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.c_int, C.c_void_p]),
+            "oracle_shade_record": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), _FP,
+                                              C.POINTER(Record), _FP, C.POINTER(ShadeParts)]),
+            "oracle_pixel_ray": (None, [C.POINTER(F.Frame), C.c_uint32, C.c_uint32, _FP, _FP]),
+            "oracle_sky_gradient": (None, [_FP, _FP]),
+            "oracle_distance_fog": (None, [_FP, _FP, C.c_float, _FP]),
+            "oracle_math_vec": (None, [C.c_int, _FP, _FP, _FP, C.c_uint64]),
             "oracle_adjoint_ref": (C.c_int, [C.POINTER(F.SceneArrays), C.POINTER(F.Frame), C.POINTER(C.c_double),
                                              _FP, C.POINTER(C.c_uint8), _U32P, _U32P, _FP, C.c_uint32, C.c_int, C.c_int,
+                                             C.c_int,
                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
                                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint8), C.POINTER(C.c_double)]),
         }
@@ -199,7 +224,7 @@ def cone_event(words, num_stars, star, origin, e0, cone):
     return (d, u) if ok else None
 
 
-def adjoint_ref(scene, frame, theta, target, objects, mask_in=None, nthreads=None, frame_values=True):
+def adjoint_ref(scene, frame, theta, target, objects, mask_in=None, nthreads=None, frame_values=True, strict=False):
     """FP64 forward-mode gradient of the masked image loss (adjoint_ref.cpp) in the parameters
     [event(4), phi(3), rot(3), d_0(3), ...]; `frame` must be the f32 frame of `theta`. With
     frame_values the model is evaluated at the frame's f32 event and tetrad (the renderer's
@@ -221,7 +246,7 @@ def adjoint_ref(scene, frame, theta, target, objects, mask_in=None, nthreads=Non
     rc = lib().oracle_adjoint_ref(C.byref(scene.arrays), C.byref(frame), th.ctypes.data_as(C.POINTER(C.c_double)),
                                   tgt.ctypes.data_as(_FP), None if mi is None else mi.ctypes.data_as(u8),
                                   first.ctypes.data_as(_U32P), count.ctypes.data_as(_U32P),
-                                  center.ctypes.data_as(_FP), n, int(frame_values), nthreads or threads(),
+                                  center.ctypes.data_as(_FP), n, int(frame_values), int(strict), nthreads or threads(),
                                   C.byref(loss),
                                   grad.ctypes.data_as(C.POINTER(C.c_double)),
                                   cnt.ctypes.data_as(C.POINTER(C.c_uint64)), mo.ctypes.data_as(u8), C.byref(err))
@@ -247,3 +272,65 @@ def pixel_states(scene, frame, nthreads=None):
                                  out.ctypes.data) != 0:
         raise RuntimeError("oracle_pixel_states: unsupported view")
     return out.reshape(h, w)
+
+
+def record(event=(0, 0, 0, 0), lam=1.0, tet_bary=(0.25, 0.25, 0.25, 0.25), tri_bary=(0.3, 0.3, 0.4), uv=(0.5, 0.5),
+           tau=0.0, tau_uv_grad=(0.0, 0.0), uv_world_scale=(1.0, 1.0), normal=(0, 0, 1, 0), four_velocity=(1, 0, 0, 0),
+           decided_at=-1e9, material=0):
+    r = Record()
+    r.lambda_ = lam
+    for name, val in (("event", event), ("tet_bary", tet_bary), ("tri_bary", tri_bary), ("uv", uv),
+                      ("tau_uv_grad", tau_uv_grad), ("uv_world_scale", uv_world_scale), ("normal", normal),
+                      ("four_velocity", four_velocity)):
+        arr = getattr(r, name)
+        for i, v in enumerate(val):
+            arr[i] = v
+    r.proper_time = tau
+    r.decided_at = decided_at
+    r.material = material
+    return r
+
+
+def shade_record(scene, frame, direction, rec):
+    """shade_hit of a hand-built record: (colour before haze and stars, intermediate parts)."""
+    d, dp = f32(direction)
+    out = np.zeros(3, np.float32)
+    parts = ShadeParts()
+    if lib().oracle_shade_record(C.byref(scene.arrays), C.byref(frame), dp, C.byref(rec), out.ctypes.data_as(_FP),
+                                 C.byref(parts)) != 0:
+        raise ValueError("material out of range")
+    return out, parts.as_dict()
+
+
+def pixel_ray(frame, px, py):
+    o = np.zeros(4, np.float32)
+    d = np.zeros(4, np.float32)
+    lib().oracle_pixel_ray(C.byref(frame), px, py, o.ctypes.data_as(_FP), d.ctypes.data_as(_FP))
+    return o, d
+
+
+def sky_gradient(direction):
+    d, dp = f32(direction)
+    out = np.zeros(3, np.float32)
+    lib().oracle_sky_gradient(dp, out.ctypes.data_as(_FP))
+    return out
+
+
+def distance_fog(color, direction, lam):
+    c, cp = f32(color)
+    d, dp = f32(direction)
+    out = np.zeros(3, np.float32)
+    lib().oracle_distance_fog(cp, dp, lam, out.ctypes.data_as(_FP))
+    return out
+
+
+MATH_FNS = {"exp": 0, "sin": 1, "cos": 2, "atan2": 3, "acos": 4}
+
+
+def math_vec(fn, a, b=None):
+    """The oracle's f32 helper `fn` over arrays (atan2: a = y, b = x)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(a if b is None else b, np.float32)
+    out = np.empty_like(a)
+    lib().oracle_math_vec(MATH_FNS[fn], a.ctypes.data_as(_FP), b.ctypes.data_as(_FP), out.ctypes.data_as(_FP), a.size)
+    return out

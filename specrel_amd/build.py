@@ -1,7 +1,9 @@
 """Builds the native libraries in-tree (they travel with the source tree; no JIT cache).
 
-  lib/libspecrel_scene.so  host scene producer (g++)
-  lib/libspecrel_hip.so    gfx950 renderer behind the C ABI of include/specrel.h (hipcc)
+  lib/libspecrel_scene.so      host scene producer (g++)
+  lib/libspecrel_hip.so        gfx950 renderer behind the C ABI of include/specrel.h (hipcc)
+  lib/libspecrel_mathcheck.so  device evaluation of the renderer's f32 transcendental helpers,
+                               for the test that sweeps them against the host definitions (hipcc)
 
 Both are compiled with -ffp-contract=off and without fast-math: the renderer reproduces the
 reference's f32 arithmetic bit for bit. HIP code additionally disables the packed-FP32 target
@@ -45,10 +47,18 @@ def hip_cmd(out=None):
             os.path.join(CSRC, "hip", "capi.hip")]
 
 
+def mathcheck_cmd(out=None):
+    out = out or os.path.join(LIB, "libspecrel_mathcheck.so")
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    return [hipcc, f"--offload-arch={HIP_ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared", *FP_FLAGS,
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-I" + INC,
+            "-o", out, os.path.join(CSRC, "diag", "mathcheck.hip")]
+
+
 def build_all(verbose=False):
     os.makedirs(LIB, exist_ok=True)
-    cmds = [scene_cmd(), hip_cmd()]
-    with ThreadPoolExecutor(2) as ex:
+    cmds = [scene_cmd(), hip_cmd(), mathcheck_cmd()]
+    with ThreadPoolExecutor(3) as ex:
         outs = list(ex.map(_run, cmds))
     if verbose:
         for c, o in zip(cmds, outs):

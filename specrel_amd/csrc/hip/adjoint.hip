@@ -107,6 +107,7 @@ __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
     double out[ADJ_PIX];
     for (int k = 0; k < ADJ_PIX; ++k) out[k] = 0.0;
     uint32_t sig = SIG_MISS, obj = NO_OBJ;
+    bool inside = false;
     const float4 hv = a.hdr[a.V.pyr_start[vi] + lp];
     const V3 hdr = V3{hv.x, hv.y, hv.z};
     const V3 tgt = V3{a.target[3 * li], a.target[3 * li + 1], a.target[3 * li + 2]};
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
         Hit hit;
         hit.slot = (int32_t)slot;
         intersect(g, o, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
+        inside = hit.bary[0] > 0.0f && hit.bary[1] > 0.0f && hit.bary[2] > 0.0f && hit.bary[3] > 0.0f;
         Rec rec;
         const float4 *A = a.S.attrs + 12 * (size_t)slot;
         interpolate(A, hit, rec);
@@ -355,13 +357,17 @@ __global__ __launch_bounds__(256) void k_adj_pixel(AdjArgs a) {
     for (int k = 0; k < ADJ_PIX; ++k) p[k] = out[k];
     a.sig[li] = sig;
     a.obj[li] = obj;
+    a.inside[li] = inside ? 1 : 0;
 }
 
 template __global__ void k_adj_pixel<16>(AdjArgs);
 
-// Pass 2: one block per 16x16 tile of every view. A pixel is valid when it hit, carries no starlight and its
-// 8 neighbours lie in the image with the same signature (or, with a caller's mask, when the mask
-// says so). Valid pixels' terms are summed in double in a fixed order per block.
+// Pass 2: one block per 16x16 tile of every view. A pixel is valid when it hit inside its
+// tetrahedron, carries no starlight and its 8 neighbours lie in the image on the same object and
+// carry no starlight (silhouettes are masked); with `strict` its neighbours must also have its
+// shading signature (no facet, texture or shadow edge nearby: for finite differences); or, with a
+// caller's mask, when the mask says so. Valid pixels' terms are summed in double in a fixed order
+// per block.
 __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
     const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
@@ -382,7 +388,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
     if (px < v.w && py < v.h) {
         const size_t li = base + (size_t)py * v.w + px;
         const uint32_t sig = a.sig[li];
-        int cls;  // 0 valid, 1 miss, 2 starlight, 3 image border, 4 silhouette, 5 shading edge
+        int cls;  // 0 valid, 1 miss, 2 starlight, 3 image border, 4 silhouette, 5 edge (of the tet; strict: shading)
         if (a.mask_in) {
             cls = a.mask_in[li] ? 0 : (sig == SIG_MISS ? 1 : (sig == SIG_STAR ? 2 : 5));
         } else if (sig == SIG_MISS) {
@@ -400,8 +406,9 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjArgs a) {
                     const uint32_t s2 = a.sig[ni];
                     if (s2 == sig) continue;
                     if (s2 == SIG_MISS || s2 == SIG_STAR || a.obj[ni] != mine) cls = 4;
-                    else if (cls == 0) cls = 5;
+                    else if (a.strict && cls == 0) cls = 5;
                 }
+            if (cls == 0 && !a.inside[li]) cls = 5;
         }
         atomicAdd(&s_cnt[cls], 1u);
         if (a.mask_out) a.mask_out[li] = cls == 0 ? 1 : 0;

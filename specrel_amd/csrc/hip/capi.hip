@@ -108,8 +108,8 @@ struct specrel_ctx {
     bool last_noalloc = false;  // the last render wrote the caller's device buffers
     // specrel_extrude_tubes: inputs, per-block counts and offsets, outputs.
     DevBuf ex_tubes, ex_tris, ex_samples, ex_decided, ex_counts, ex_offsets, ex_first, ex_geoms, ex_attrs, ex_boxes;
-    DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_partials, adj_counts,
-        adj_result;
+    DevBuf adj_target, adj_mask_in, adj_mask_out, adj_obj, adj_pix, adj_sig, adj_objid, adj_inside, adj_partials,
+        adj_counts, adj_result;
     std::vector<specrel_frame> last_frames;
     uint32_t last_scene = 0;
     uint64_t generation = 0;  // bumped whenever a buffer a captured graph points to moves
@@ -133,6 +133,7 @@ struct specrel_ctx {
     int opt_tile_order = 1;
     int opt_postpone = 1;
     int opt_star_overlap = 0;
+    int opt_adj_strict = 0;
     bool have_hits = false;
     // The captured launch sequence of the last batch shape (SPECREL_OPT_GRAPH).
     hipGraphExec_t graph = nullptr;
@@ -1252,6 +1253,7 @@ int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in,
     dalloc(ctx->adj_pix, np * ADJ_PIX * sizeof(double));
     dalloc(ctx->adj_sig, np * 4);
     dalloc(ctx->adj_objid, np * 4);
+    dalloc(ctx->adj_inside, np);
     dalloc(ctx->adj_partials, (size_t)tiles * stride * sizeof(double));
     dalloc(ctx->adj_counts, (size_t)tiles * ADJ_CLASSES * 4);
     dalloc(ctx->adj_result, (size_t)nv * (stride * sizeof(double) + ADJ_CLASSES * 8));
@@ -1284,6 +1286,8 @@ int adjoint_views(specrel_ctx *ctx, const float *target, const uint8_t *mask_in,
     a.pix = (double *)ctx->adj_pix.p;
     a.sig = (uint32_t *)ctx->adj_sig.p;
     a.obj = (uint32_t *)ctx->adj_objid.p;
+    a.inside = (uint8_t *)ctx->adj_inside.p;
+    a.strict = ctx->opt_adj_strict ? 1u : 0u;
     a.partials = (double *)ctx->adj_partials.p;
     a.block_counts = (uint32_t *)ctx->adj_counts.p;
     a.stride = stride;
@@ -1403,7 +1407,7 @@ void specrel_destroy(specrel_ctx *ctx) {
     dfree(ctx->ex_counts), dfree(ctx->ex_offsets), dfree(ctx->ex_first), dfree(ctx->ex_geoms);
     dfree(ctx->ex_attrs), dfree(ctx->ex_boxes);
     dfree(ctx->adj_target), dfree(ctx->adj_mask_in), dfree(ctx->adj_mask_out), dfree(ctx->adj_obj);
-    dfree(ctx->adj_pix), dfree(ctx->adj_sig), dfree(ctx->adj_objid), dfree(ctx->adj_partials);
+    dfree(ctx->adj_pix), dfree(ctx->adj_sig), dfree(ctx->adj_objid), dfree(ctx->adj_inside), dfree(ctx->adj_partials);
     dfree(ctx->adj_counts), dfree(ctx->adj_result);
     drop_graph(ctx);
     for (auto &e : ctx->ev)
@@ -1448,6 +1452,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
         }
         ctx->opt_trace_waves = value;
     } else if (option == SPECREL_OPT_TILE_ORDER) ctx->opt_tile_order = value != 0;
+    else if (option == SPECREL_OPT_ADJOINT_MASK) {
+        if (value != 0 && value != 1) {
+            ctx->err = "adjoint mask must be 0 (silhouettes) or 1 (every shading edge)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_adj_strict = value;
+    }
     else if (option == SPECREL_OPT_LEAF_POSTPONE) {
         if (value < 0 || value > STACK - 2) {
             ctx->err = "leaf postponement must be 0, 1 or a stack limit of 2..62 entries";

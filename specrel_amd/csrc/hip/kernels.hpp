@@ -194,6 +194,8 @@ struct AdjArgs {
     double *pix;     // ADJ_PIX per pixel
     uint32_t *sig;   // per pixel
     uint32_t *obj;   // per pixel
+    uint8_t *inside; // per pixel: the hit lies inside its tetrahedron (every barycentric > 0)
+    uint32_t strict; // validity: 0 silhouettes masked; 1 also every shading edge (SPECREL_OPT_ADJOINT_MASK)
     double *partials;       // per tile: ADJ_CAM + 3 n_obj
     uint32_t *block_counts; // per tile: ADJ_CLASSES
     uint32_t stride;        // ADJ_CAM + 3 n_obj

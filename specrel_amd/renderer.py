@@ -22,6 +22,7 @@ OK, ENODEV, EINVAL, ENOMEM, EDEVICE = 0, -1, -2, -3, -4
 OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH, OPT_MERGED_LOADS = 1, 2, 3, 4, 5, 6, 7
 OPT_SHADOW_IN_TRACE, OPT_TRACE_WAVES, OPT_STAR_OVERLAP, OPT_WAVE_SHAPE, OPT_TILE_ORDER = 8, 9, 10, 11, 12
 OPT_LEAF_POSTPONE = 13
+OPT_ADJOINT_MASK = 14
 # specrel_timings.shadow_path: where the last render traced its shadow queries
 SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 

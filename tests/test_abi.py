@@ -29,6 +29,11 @@ def test_library_exports_every_declared_symbol(header, loader):
     assert not missing, missing
 
 
+def test_mathcheck_library_exports_its_entry():
+    lib = C.CDLL(os.path.join(F.LIB_DIR, "libspecrel_mathcheck.so"))
+    assert hasattr(lib, "specrel_mathcheck")
+
+
 def test_python_signatures_cover_the_header():
     assert set(declared("specrel.h")) == set(R.SIGNATURES)
 

@@ -71,7 +71,8 @@ def test_reference_gradient_matches_its_differences_camera(twin):
     w, h = 96, 60
     obs, target = setup_view(twin, w, h, seed=1)
     objs = A.scene_objects(twin)
-    r = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs, frame_values=False)
+    # finite differences over the pixels a small step crosses no edge in (the strict validity rule)
+    r = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs, frame_values=False, strict=True)
     assert r["counts"]["valid"] > 300
     mask = r["mask"]
     step = 1e-6
@@ -90,7 +91,7 @@ def test_reference_gradient_matches_its_differences_objects(twin):
     obs, target = setup_view(twin, w, h, seed=2)
     objs = A.scene_objects(twin)
     frame = obs.frame(w, h)
-    r = O.adjoint_ref(twin, frame, obs.theta, target, objs)
+    r = O.adjoint_ref(twin, frame, obs.theta, target, objs, strict=True)
     mask = r["mask"]
     seen = np.nonzero(np.abs(r["grad"][10:]).reshape(-1, 3).sum(axis=1) > 0)[0]
     assert seen.size >= 1
@@ -136,3 +137,18 @@ def test_object_boost_keeps_the_bvh_valid_and_undo_restores(twin):
     booster.undo()
     assert np.array_equal(twin.geoms(), before[0]) and np.array_equal(twin.attrs(), before[1])
     assert np.array_equal(twin.nodes(), before[2])
+
+
+def test_validity_rules_nest(twin):
+    """The default rule (silhouettes masked) keeps every pixel the strict rule (also no shading edge
+    nearby) keeps, and more; both leave out misses, starlight and the border alike."""
+    w, h = 96, 60
+    obs, target = setup_view(twin, w, h, seed=3)
+    objs = A.scene_objects(twin)
+    d = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs)
+    s = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs, strict=True)
+    assert ((s["mask"] > 0) <= (d["mask"] > 0)).all()
+    assert d["counts"]["valid"] > s["counts"]["valid"] > 0
+    for k in ("miss", "starlight", "border", "silhouette"):
+        assert d["counts"][k] == s["counts"][k], k
+    assert sum(d["counts"].values()) == sum(s["counts"].values()) == w * h

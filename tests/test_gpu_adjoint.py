@@ -7,7 +7,7 @@ import pytest
 
 from specrel_amd import adjoint as A
 from specrel_amd import scene as S
-from specrel_amd.renderer import SpecrelError
+from specrel_amd.renderer import OPT_ADJOINT_MASK, SpecrelError
 from oracle import pyoracle as O
 
 pytestmark = pytest.mark.gpu
@@ -36,20 +36,28 @@ def twin():
     return S.Scene.generate("twin")
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_adjoint_matches_fp64_reference_twin(gpu_ctx, twin, seed):
+@pytest.mark.parametrize("seed,strict", [(1, 0), (2, 0), (1, 1)])
+def test_adjoint_matches_fp64_reference_twin(gpu_ctx, twin, seed, strict):
     w, h = 192, 120
     obs, target = make_problem(gpu_ctx, twin, w, h, seed)
     objs = A.scene_objects(twin)
-    g, res = A.gradient(gpu_ctx, obs, target, objs, want_mask=True)
-    ref = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs)
+    gpu_ctx.set_option(OPT_ADJOINT_MASK, strict)
+    try:
+        g, res = A.gradient(gpu_ctx, obs, target, objs, want_mask=True)
+    finally:
+        gpu_ctx.set_option(OPT_ADJOINT_MASK, 0)
+    ref = O.adjoint_ref(twin, obs.frame(w, h), obs.theta, target, objs, strict=bool(strict))
     for k in ("valid", "miss", "starlight", "border", "silhouette", "edge"):
         assert res[k] == ref["counts"][k], (k, res[k], ref["counts"][k])
     assert np.array_equal(res["mask"], ref["mask"])
     assert res["valid"] > 1000
+    hits = w * h - res["miss"] - res["starlight"]
+    if not strict:  # silhouettes and the border masked, nothing else of note
+        assert res["valid"] >= 0.85 * hits, (res, hits)
     assert abs(res["loss"] - ref["loss"]) <= 1e-5 * ref["loss"]
     err = rel_err(g, ref["grad"])
-    print(f"twin seed {seed}: {res['valid']} valid pixels, gradient relative error {err:.2e}")
+    print(f"twin seed {seed} strict {strict}: {res['valid']} valid of {hits} hit pixels, gradient relative "
+          f"error {err:.2e}")
     assert err <= 1e-4
 
 
@@ -68,6 +76,11 @@ def test_adjoint_matches_fp64_reference_large_subset(gpu_ctx):
     assert err <= 1e-4
 
 
+# Central differences with step 1e-3 of the f32 loss against the adjoint on the measured pixel
+# set: the agreement reached (see the test's printout), with a margin.
+FD_TOL = 1e-4  # measured 5.05e-5 over 16 parameters on 800 pixels
+
+
 def dilate(m, r):
     out = m.copy()
     for dy in range(-r, r + 1):
@@ -77,11 +90,13 @@ def dilate(m, r):
 
 
 def test_adjoint_against_gpu_finite_differences(gpu_ctx):
-    """Central differences of the renderer's own f32 loss over a fixed pixel set, on bodies of the
-    large scene seen from 5 units. Left out of the set: clock faces, whose hands and digits move
+    """Central differences of the renderer's own f32 loss with the measured configuration's step
+    1e-3, over a fixed pixel set, on bodies of the large scene seen from 5 units. The set is the
+    strict validity rule's (no facet, texture or shadow edge among a pixel's neighbours: a finite
+    step must not carry a pixel across one), without clock faces, whose hands and digits move
     with the proper time a pixel sees (a small step can flip a whole digit, a jump no mask in
-    space excludes), and the wireframe blend bands with a pixel around them, which are steeper
-    than the step in parameter space resolves. The FP64 comparison covers those pixels."""
+    space excludes), and without the wireframe blend bands with a pixel around them, which are
+    steeper than the step resolves. The FP64 comparisons cover those pixels."""
     p = S.gen_params("large", n_objects=32, slices=32, n_stars=300)
     sc = S.Scene.generate("large", p)
     objs = A.scene_objects(sc)
@@ -89,7 +104,11 @@ def test_adjoint_against_gpu_finite_differences(gpu_ctx):
     obs = A.framing_observer(sc, objs, k, 5.0, phi=(0.05, -0.1, 0.02))
     w, h = 512, 320
     obs, target = make_problem(gpu_ctx, sc, w, h, 3, obs=obs)
-    _g, res = A.gradient(gpu_ctx, obs, target, objs, want_mask=True)
+    gpu_ctx.set_option(OPT_ADJOINT_MASK, 1)
+    try:
+        _g, res = A.gradient(gpu_ctx, obs, target, objs, want_mask=True)
+    finally:
+        gpu_ctx.set_option(OPT_ADJOINT_MASK, 0)
     st = O.pixel_states(sc, obs.frame(w, h))
     wire = (st["flags"] & 1) != 0
     mask = ((res["mask"] > 0) & (st["pattern"] == 0) & ~dilate(wire, 1)).astype(np.uint8)
@@ -99,16 +118,15 @@ def test_adjoint_against_gpu_finite_differences(gpu_ctx):
     seen = [j for j in range(len(objs)) if np.abs(g[10 + 3 * j:13 + 3 * j]).sum() > 0][:2]
     assert seen
     params = list(range(10)) + [10 + 3 * j + a for j in seen for a in range(3)]
-    fds = [A.fd_gradient(gpu_ctx, sc, obs, w, h, target, objs, mask, params, step=st_) for st_ in (1e-3, 3e-4, 1e-4)]
+    fd = A.fd_gradient(gpu_ctx, sc, obs, w, h, target, objs, mask, params, step=1e-3)
     gs = np.array([g[q] for q in params])
-    # per parameter the step whose difference agrees best: a large step sees curvature, a small
-    # one the f32 rounding of the loss
-    fs = np.array([min((fd[q] for fd in fds), key=lambda x: abs(x - g[q])) for q in params])
+    fs = np.array([fd[q] for q in params])
     err = rel_err(gs, fs)
-    print(f"FD check over {len(params)} parameters on {int(mask.sum())} pixels: relative difference {err:.2e}")
+    print(f"FD check (step 1e-3) over {len(params)} parameters on {int(mask.sum())} pixels: relative "
+          f"difference {err:.2e}")
     for q in params:
-        print(f"   param {q}: adjoint {g[q]:.6g}, differences " + ", ".join(f"{fd[q]:.6g}" for fd in fds))
-    assert err <= 5e-2
+        print(f"   param {q}: adjoint {g[q]:.6g}, difference {fd[q]:.6g}")
+    assert err <= FD_TOL
     # the scene is back as it was: the same loss as before the object differences
     gpu_ctx.render_resident(0, obs.frame(w, h))
     again = gpu_ctx.adjoint(target, objs, mask_in=mask)["loss"]
