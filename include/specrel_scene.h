@@ -36,7 +36,12 @@ enum {
     SPECREL_WL_HYPERBOLIC = 1,   /* d: centre x,y,z, dir x,y,z, accel a, t_turn, max a|t-t_turn|, t_min, t_max, steps */
     SPECREL_WL_TWIN = 2,         /* d: t_offset, y, z, t_min, t_max, steps (scene.rs:124-138, 333-341) */
     SPECREL_WL_CIRCLE = 3,       /* d: radius, beta, t_min, t_max, steps (orbit in the xz plane) */
-    SPECREL_WL_BORN_RAMP = 4     /* d: beta_max, t_min, t_max, steps (mesh4d.rs:1254-1267 profile) */
+    SPECREL_WL_BORN_RAMP = 4,    /* d: beta_max, t_min, t_max, steps (mesh4d.rs:1254-1267 profile) */
+    SPECREL_WL_OSCILLATE = 5,    /* d: cx, amp, omega, y, z, t_min, t_max, steps: x = cx + amp sin(omega t)
+                                    (scene.rs:553-563) */
+    SPECREL_WL_ORBIT = 6         /* d: cx, cy, cz, radius, omega, t_ref, phase, wobble, t_min, t_max, steps:
+                                    angle = omega (t - t_ref) + phase, (cx + r cos, cy + wobble sin 2angle,
+                                    cz + r sin) (scene.rs:567-578, 683-700) */
 };
 
 typedef struct specrel_object {
@@ -97,8 +102,10 @@ const char *specrel_scene_last_error(void);
 /* Defaults for a named generator ("cube", "large", "accel_clocks"). */
 int specrel_gen_defaults(const char *name, specrel_gen_params *out);
 /* Named scenes: "cube" (2304-tet cube), "large" (icosphere field + star sphere),
- * "accel_clocks" (Born-rigid accelerated bodies + light clocks), "twin" (the reference's
- * TwinParadox demo, scene.rs:283-379), "decision_age" (gpu.rs:1156-1185). p may be NULL. */
+ * "accel_clocks" (Born-rigid accelerated bodies + light clocks), the reference's five demo scenes
+ * "twin" (TwinParadox, scene.rs:283-379), "barn" (BarnAndPole, 383-455), "penrose"
+ * (TerrellPenroseStreet, 460-571), "train" (EinsteinTrainSync, 576-643), "carousel"
+ * (RelativisticCarousel, 648-719), and "decision_age" (gpu.rs:1156-1185). p may be NULL. */
 int specrel_scene_generate(const char *name, const specrel_gen_params *p, specrel_host_scene **out);
 /* Suggested observer for a named scene (config camera / DemoScene::initial_camera_pos). */
 int specrel_scene_default_camera(const char *name, const specrel_gen_params *p, specrel_camera *out);

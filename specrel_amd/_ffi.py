@@ -131,7 +131,7 @@ MODE_SHADED, MODE_ISOCHRONES, MODE_INSPECTOR, MODE_DECISION_AGE = 0, 1, 2, 3
 SHADOW_OFF, SHADOW_NULL, SHADOW_CAMERA_SLICE = 0, 1, 2
 VIEW_PAST_CONE, VIEW_SIMULTANEITY, VIEW_FRONTIER, VIEW_PROPHECY = 0, 1, 2, 3
 MESH_CUBE, MESH_UV_SPHERE, MESH_GROUND, MESH_ICOSPHERE = 0, 1, 2, 3
-WL_INERTIAL, WL_HYPERBOLIC, WL_TWIN, WL_CIRCLE, WL_BORN_RAMP = 0, 1, 2, 3, 4
+WL_INERTIAL, WL_HYPERBOLIC, WL_TWIN, WL_CIRCLE, WL_BORN_RAMP, WL_OSCILLATE, WL_ORBIT = 0, 1, 2, 3, 4, 5, 6
 NO_HIT = 0xFFFFFFFF
 
 _P = C.c_void_p

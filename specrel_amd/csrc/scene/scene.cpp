@@ -401,6 +401,29 @@ std::vector<Sample> sample_worldline(uint32_t kind, const double *d) {
             };
             return sample_accelerated(f, (float)d[2], (float)d[3], (size_t)d[4]);
         }
+        case SPECREL_WL_OSCILLATE: {
+            // x = cx + amp sin(omega t) at fixed y, z (scene.rs:553-563)
+            double cx = d[0], amp = d[1], omega = d[2], y = d[3], z = d[4];
+            PosFn f = [=](double t, double *p) {
+                p[0] = cx + amp * std::sin(omega * t);
+                p[1] = y;
+                p[2] = z;
+            };
+            return sample_accelerated(f, (float)d[5], (float)d[6], (size_t)d[7]);
+        }
+        case SPECREL_WL_ORBIT: {
+            // angle = omega (t - t_ref) + phase; (cx + r cos, cy + wobble sin(2 angle), cz + r sin)
+            // (scene.rs:567-578, 683-700)
+            double cx = d[0], cy = d[1], cz = d[2], r = d[3], omega = d[4], t_ref = d[5], phase = d[6], wob = d[7];
+            PosFn f = [=](double t, double *p) {
+                double angle = omega * (t - t_ref) + phase;
+                double sn = std::sin(angle), cs = std::cos(angle);
+                p[0] = cx + r * cs;
+                p[1] = cy + wob * std::sin(2.0 * angle);
+                p[2] = cz + r * sn;
+            };
+            return sample_accelerated(f, (float)d[8], (float)d[9], (size_t)d[10]);
+        }
         case SPECREL_WL_BORN_RAMP: {
             double beta_max = d[0];
             double k = 3.141592653589793 / 20.0;
@@ -1189,6 +1212,133 @@ void gen_twin(specrel_builder &b) {
     b.stars = starfield(v3(0, 0, 0), 90, 60.0f, 170.0f, 7);
 }
 
+// The ground plane of the demo scenes, static over [t0, t1] (one worldline step).
+void demo_ground(specrel_builder &b, float half_x, float half_z, float divs_x, float divs_z, double t0, double t1) {
+    float mp[5] = {half_x, half_z, -2.2f, divs_x, divs_z};
+    double wp[9] = {0, 0, 0, 0, 0, 0, t0, t1, 1};
+    add_tube(b, SPECREL_MESH_GROUND, mp, SPECREL_WL_INERTIAL, wp, false, 0, 0, nullptr, 0);
+}
+
+// A body at rest at p over [t0, t1] in `steps` worldline steps.
+void demo_static(specrel_builder &b, const float *he, float px, float py, float pz, double t0, double t1,
+                 double steps, uint32_t material) {
+    double wp[9] = {(double)px, (double)py, (double)pz, 0, 0, 0, t0, t1, steps};
+    add_tube(b, SPECREL_MESH_CUBE, he, SPECREL_WL_INERTIAL, wp, false, 0, material, nullptr, 0);
+}
+
+// Scene 2 of the reference: the barn and the pole (scene.rs:383-455). A 5 m barn at rest and an
+// 8 m pole at 0.85c along x (contracted to 4.21 m), in loop copies k = -2..1 of 24 s.
+void gen_barn(specrel_builder &b) {
+    const float t_min = -12.0f, t_max = 12.0f, t_loop = t_max - t_min;
+    demo_ground(b, 22.0f, 22.0f, 4, 4, -140.0, 140.0);
+    float pillar[3] = {0.35f, 1.5f, 0.35f};
+    for (float x_door : {-2.85f, 2.85f}) demo_static(b, pillar, x_door, -0.7f, 5.2f, -140.0, 140.0, 1, 1);
+    float beam[3] = {3.2f, 0.8f, 0.6f};
+    demo_static(b, beam, 0.0f, 1.6f, 5.8f, -140.0, 140.0, 1, 1);
+    float wall[3] = {2.5f, 1.5f, 0.25f};
+    demo_static(b, wall, 0.0f, -0.7f, 7.4f, -140.0, 140.0, 1, 6);
+    float pole[3] = {4.0f, 0.55f, 0.45f};
+    const V3 beta = v3(0.85f, 0.0f, 0.0f);
+    for (int k = -2; k <= 1; ++k) {
+        const float t_offset = (float)k * t_loop;
+        const V3 pos = v3(0.0f, -0.85f, 6.0f) - beta * t_offset;
+        double wp[9] = {pos.x, pos.y, pos.z, beta.x, beta.y, beta.z, (double)(t_offset - 55.0f),
+                        (double)(t_offset + 55.0f), 12};
+        add_tube(b, SPECREL_MESH_CUBE, pole, SPECREL_WL_INERTIAL, wp, true, t_offset, 2, nullptr, 0);
+    }
+    b.light = v3(0.0f, 6.0f, 1.5f);
+    b.stars = starfield(v3(0, 0, 0), 90, 60.0f, 170.0f, 7);
+}
+
+// Scene 3 of the reference: the Terrell-Penrose street (scene.rs:460-571). A colonnade of clock
+// cubes and light clocks at rest, loop copies (k = -2..1 of 28 s) of a cube flying at 0.85c along
+// x and a uv sphere at 0.86c along -z, a cube oscillating along x and one orbiting in the xz plane.
+void gen_penrose(specrel_builder &b) {
+    const float t_min = -14.0f, t_max = 14.0f, t_loop = t_max - t_min;
+    demo_ground(b, 20.0f, 28.0f, 4, 6, -140.0, 140.0);
+    float pillar[3] = {0.95f, 0.95f, 0.95f}, lc[3] = {0.65f, 0.625f, 0.65f};
+    for (int iz = -3; iz <= 3; ++iz) {
+        const float z = (float)iz * 5.0f;
+        for (float x : {-4.2f, 4.2f}) {
+            if (iz == 0) demo_static(b, lc, x, -1.575f, z, -140.0, 140.0, 1, 5);
+            else demo_static(b, pillar, x, -1.25f, z, -140.0, 140.0, 1, 1);
+        }
+    }
+    float flyer[3] = {1.05f, 1.05f, 1.05f};
+    float sphere[3] = {1.05f, 10, 16};
+    const V3 beta_f = v3(0.85f, 0.0f, 0.0f), beta_s = v3(0.0f, 0.0f, -0.86f);
+    for (int k = -2; k <= 1; ++k) {
+        const float t_offset = (float)k * t_loop;
+        const V3 pf = v3(0.0f, -0.75f, 4.5f) - beta_f * t_offset;
+        double wf[9] = {pf.x, pf.y, pf.z, beta_f.x, beta_f.y, beta_f.z, (double)(t_offset - 55.0f),
+                        (double)(t_offset + 55.0f), 10};
+        add_tube(b, SPECREL_MESH_CUBE, flyer, SPECREL_WL_INERTIAL, wf, true, t_offset, 2, nullptr, 0);
+        const V3 ps = v3(-1.8f, 0.6f, 5.0f) - beta_s * t_offset;
+        double ws[9] = {ps.x, ps.y, ps.z, beta_s.x, beta_s.y, beta_s.z, (double)(t_offset - 55.0f),
+                        (double)(t_offset + 55.0f), 10};
+        add_tube(b, SPECREL_MESH_UV_SPHERE, sphere, SPECREL_WL_INERTIAL, ws, true, t_offset, 6, nullptr, 0);
+    }
+    const double omega0 = 6.283185307179586 / (double)t_loop;
+    float accel[3] = {0.95f, 0.95f, 0.95f};
+    double wa[8] = {0.0, 3.6, omega0, 1.1, 8.5, -110.0, 70.0, 260};
+    add_tube(b, SPECREL_MESH_CUBE, accel, SPECREL_WL_OSCILLATE, wa, false, 0, 3, nullptr, 0);
+    float orbit[3] = {0.75f, 0.75f, 0.75f};
+    double wo[11] = {1.5, 0.5, 11.5, 3.2, omega0, 0.0, 0.0, 0.0, -110.0, 70.0, 260};
+    add_tube(b, SPECREL_MESH_CUBE, orbit, SPECREL_WL_ORBIT, wo, false, 0, 4, nullptr, 0);
+    b.light = v3(0.0f, 6.5f, 4.0f);
+    b.stars = starfield(v3(0, 0, 0), 90, 60.0f, 170.0f, 7);
+}
+
+// Scene 4 of the reference: Einstein's train (scene.rs:576-643). Seven platform clocks at rest
+// synchronised in the lab, and loop copies (k = -2..1 of 28 s) of a seven-wagon train at 0.8c
+// synchronised in its own rest frame.
+void gen_train(specrel_builder &b) {
+    const float t_min = -14.0f, t_max = 14.0f, t_loop = t_max - t_min;
+    demo_ground(b, 24.0f, 22.0f, 4, 4, -140.0, 140.0);
+    float clock[3] = {0.95f, 0.95f, 0.95f};
+    for (int i = -3; i <= 3; ++i) demo_static(b, clock, (float)i * 2.6f, -1.15f, 6.8f, -140.0, 140.0, 1, 1);
+    float lc[3] = {0.55f, 0.625f, 0.55f};
+    demo_static(b, lc, 0.0f, -1.575f, 5.1f, -140.0, 140.0, 1, 5);
+    const V3 beta = v3(0.80f, 0.0f, 0.0f);
+    const float gamma = 1.0f / sqrtf(1.0f - beta.x * beta.x);
+    for (int k = -2; k <= 1; ++k) {
+        const float t_offset = (float)k * t_loop;
+        for (int i = -3; i <= 3; ++i) {
+            const float x_rest = (float)i * 2.6f;
+            const V3 pos = v3(x_rest / gamma, 1.05f, 6.8f) - beta * t_offset;
+            const float t_sync = t_offset + gamma * beta.x * x_rest;
+            double wp[9] = {pos.x, pos.y, pos.z, beta.x, beta.y, beta.z, (double)(t_offset - 60.0f),
+                            (double)(t_offset + 60.0f), 10};
+            add_tube(b, SPECREL_MESH_CUBE, clock, SPECREL_WL_INERTIAL, wp, true, t_sync, 2, nullptr, 0);
+        }
+    }
+    b.light = v3(0.0f, 6.5f, 2.0f);
+    b.stars = starfield(v3(0, 0, 0), 90, 60.0f, 170.0f, 7);
+}
+
+// Scene 5 of the reference: the relativistic carousel (scene.rs:648-719). A master clock and a
+// light clock at rest, six cubes orbiting them at 0.8c, two turns per 48 s.
+void gen_carousel(specrel_builder &b) {
+    const float t_min = -24.0f, t_max = 24.0f, t_loop = t_max - t_min, t_sync = -20.0f;
+    demo_ground(b, 22.0f, 24.0f, 4, 4, -140.0, 120.0);
+    float master[3] = {1.05f, 1.05f, 1.05f};
+    double wm[9] = {0.0, (double)-0.7f, 8.0, 0, 0, 0, -110.0, 70.0, 2};
+    add_tube(b, SPECREL_MESH_CUBE, master, SPECREL_WL_INERTIAL, wm, true, t_sync, 1, nullptr, 0);
+    float lc[3] = {0.55f, 0.625f, 0.55f};
+    double wl[9] = {0.0, (double)0.975f, 8.0, 0, 0, 0, -110.0, 70.0, 2};
+    add_tube(b, SPECREL_MESH_CUBE, lc, SPECREL_WL_INERTIAL, wl, true, t_sync, 5, nullptr, 0);
+    float orbiter[3] = {0.62f, 0.62f, 0.62f};
+    const double tau_c = 6.283185307179586;
+    const double omega = 2.0 * tau_c / (double)t_loop, radius = 0.80 / omega;
+    for (int k = 0; k < 6; ++k) {
+        const double phase = (double)k * tau_c / 6.0;
+        double wo[11] = {0.0, -0.6, 8.0, radius, omega, (double)t_sync, phase, 0.25, -110.0, 70.0, 320};
+        add_tube(b, SPECREL_MESH_CUBE, orbiter, SPECREL_WL_ORBIT, wo, true, t_sync, k % 2 == 0 ? 2 : 3, nullptr, 0);
+    }
+    b.light = v3(0.0f, 6.8f, 3.0f);
+    b.stars = starfield(v3(0, 0, 0), 90, 60.0f, 170.0f, 7);
+}
+
 // gpu_matches_cpu_decision_age_with_frontier's scene (gpu.rs:1156-1185).
 void gen_decision_age(specrel_builder &b) {
     float gp[5] = {22.0f, 22.0f, -2.2f, 4, 4};
@@ -1319,6 +1469,14 @@ void generate_into(specrel_builder &b, const char *name, const specrel_gen_param
             gen_twin(b);
         } else if (n == "decision_age") {
             gen_decision_age(b);
+        } else if (n == "barn") {
+            gen_barn(b);
+        } else if (n == "penrose") {
+            gen_penrose(b);
+        } else if (n == "train") {
+            gen_train(b);
+        } else if (n == "carousel") {
+            gen_carousel(b);
         } else {
             throw Error("unknown scene name: " + n);
         }
@@ -1468,8 +1626,16 @@ int specrel_scene_default_camera(const char *name, const specrel_gen_params *pp,
             double d[3];
             rng.unit(d);
             boost_local(T, v3((float)d[0], (float)d[1], (float)d[2]));
-        } else if (n == "twin") {
+        } else if (n == "twin") {  // DemoScene::initial_camera_pos (scene.rs:92-101)
             pos = v4(-15.0f, 2.2f, 0.0f, -3.5f);
+        } else if (n == "barn") {
+            pos = v4(-4.5f, 0.0f, 0.1f, -3.2f);
+        } else if (n == "penrose") {
+            pos = v4(0.0f, 0.0f, 0.0f, -7.5f);
+        } else if (n == "train") {
+            pos = v4(-3.0f, 0.0f, 0.0f, -4.5f);
+        } else if (n == "carousel") {
+            pos = v4(-18.0f, 0.0f, 0.8f, -2.5f);
         } else if (n == "decision_age") {
             pos = v4(0.0f, 0.0f, 0.0f, -4.0f);
         }

@@ -1,5 +1,6 @@
-"""Writes tests/golden/oracle_golden.json: SHA-256 digests of the oracle's outputs (RGB8, hit
-index, lambda, tau) for fixed scenes and views. They freeze the oracle and the scene producer
+"""Writes tests/golden/oracle_golden.json and demo_golden.json: SHA-256 digests of the oracle's
+outputs (RGB8, hit index, lambda, tau) for fixed scenes and views (the second: the five demo
+scenes under the reference's 9-configuration harness). They freeze the oracle and the scene producer
 against unintended change; the reference itself cannot be run here (no Rust toolchain), so these
 are regression vectors of the restatement, not reference outputs."""
 import hashlib
@@ -27,7 +28,24 @@ def digest(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def demo_harness():
+    """Digests of the five demo scenes under the reference's 9-configuration harness at 192x120
+    (gpu.rs:1256-1311), written to demo_golden.json."""
+    from tests.helpers import DEMO_SCENES, HARNESS_CONFIGS, harness_frame
+    out = {}
+    for name in DEMO_SCENES:
+        sc = S.Scene.generate(name)
+        for c in range(len(HARNESS_CONFIGS)):
+            r = O.render(sc, harness_frame(name, c))
+            out[f"{name}_{c}"] = {"scene": name, "config": c, **{k: digest(r[k]) for k in ("rgb", "hit", "lam", "tau")}}
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "demo_golden.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("wrote", path)
+
+
 def main():
+    demo_harness()
     out = {}
     for key, spec in CASES.items():
         sc = S.Scene.generate(spec["scene"])

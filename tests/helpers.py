@@ -31,3 +31,28 @@ def frame_for(scene_name, w, h, mode=S.RelativisticShaded, shadow=S.ShadowLorent
               view=S.PastLightCone, camera=None, dynamics=None, params=None):
     cam = camera or S.default_camera(scene_name, params)
     return S.build_frame(cam, S.RenderConfig(w, h, mode, shadow, view), dynamics)
+
+
+# gpu_matches_cpu_reference's 9 (mode, shadow, view) configurations (gpu.rs:1268-1278).
+HARNESS_CONFIGS = [
+    (S.RelativisticShaded, S.ShadowLorentzInvariant, S.PastLightCone),
+    (S.RelativisticShaded, S.ShadowCameraSlice, S.CameraSimultaneity),
+    (S.LookbackIsochrones, S.ShadowOff, S.PastLightCone),
+    (S.Simplex4DInspector, S.ShadowOff, S.PastLightCone),
+    (S.DecisionAge, S.ShadowOff, S.PastLightCone),
+    (S.DecisionAge, S.ShadowOff, S.ProphecySlice),
+    (S.RelativisticShaded, S.ShadowLorentzInvariant, S.Frontier),
+    (S.RelativisticShaded, S.ShadowCameraSlice, S.ProphecySlice),
+    (S.LookbackIsochrones, S.ShadowOff, S.Frontier),
+]
+DEMO_SCENES = ["twin", "barn", "penrose", "train", "carousel"]
+
+
+def harness_frame(scene_name, config, w=192, h=120):
+    """The harness's view of a demo scene (gpu.rs:1283-1297): the scene's initial camera, boosted
+    by (0.2, 0, 0.6) and rotated by (0.15, -0.05, 0) in its own frame."""
+    cam = S.default_camera(scene_name)
+    cam.tetrad.boost_local((0.2, 0.0, 0.6))
+    cam.tetrad.rotate_local(0.15, -0.05, 0.0)
+    mode, shadow, view = HARNESS_CONFIGS[config]
+    return S.build_frame(cam, S.RenderConfig(w, h, mode, shadow, view))

@@ -7,7 +7,7 @@ import pytest
 
 from specrel_amd import scene as S
 from oracle import pyoracle as O
-from tests.helpers import assert_same, frame_for, render_both
+from tests.helpers import DEMO_SCENES, HARNESS_CONFIGS, assert_same, frame_for, harness_frame, render_both
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,25 @@ def test_twin_scene_all_modes(gpu_ctx, twin, mode, shadow, view):
     g, o = render_both(gpu_ctx, twin, fr)
     assert len(np.unique(o["rgb"])) > 50
     assert_same(g, o, f"twin {mode}/{shadow}/{view}")
+
+
+@pytest.fixture(scope="module")
+def demo_scenes():
+    return {n: S.Scene.generate(n) for n in DEMO_SCENES}
+
+
+@pytest.mark.parametrize("name", DEMO_SCENES)
+def test_demo_scenes_under_the_reference_harness(gpu_ctx, demo_scenes, name):
+    # gpu_matches_cpu_reference (gpu.rs:1256-1311) over all five demo scenes and its 9
+    # configurations, boosted and rotated observer, 192x120 -- with zero tolerance
+    sc = demo_scenes[name]
+    gpu_ctx.upload(0, sc)
+    for c in range(len(HARNESS_CONFIGS)):
+        fr = harness_frame(name, c)
+        g = gpu_ctx.render_frame(0, fr, want_hits=True)
+        o = O.render(sc, fr)
+        assert len(np.unique(o["rgb"])) > 20, (name, c)
+        assert_same(g, o, f"{name} config {c} {HARNESS_CONFIGS[c]}")
 
 
 def test_flashes(gpu_ctx, twin):
