@@ -190,6 +190,34 @@ typedef struct specrel_range {
 int specrel_patch_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
                         const specrel_range *node_ranges, uint32_t n_node_ranges, const specrel_range *prim_ranges,
                         uint32_t n_prim_ranges, int *out_full);
+/* BvhVersion (bvh4d.rs:47-61): which layout of a BVH's arrays a copy was made from (numbered from 1;
+ * 0 means none) and how many in-place edits that layout has had since. */
+typedef struct specrel_bvh_version {
+    uint64_t layout;
+    uint64_t edit;
+} specrel_bvh_version;
+
+/* GpuRenderer::upload_scene (gpu.rs:622-699) with the version of the BVH the arrays were packed
+ * from. As the reference does for a scene edited in place (edit > 0, gpu.rs:639-656), the node,
+ * primitive and star arrays get room to grow by half, so later edits can be patched in. */
+int specrel_upload_scene_versioned(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                                   const specrel_bvh_version *version);
+
+/* GpuRenderer::update_scene (gpu.rs:701-731): `scene` is the slot's scene after the BVH edit that
+ * took it to `version` (Bvh4D::begin_edit .. write_part / write_nodes, bvh4d.rs:181-215), and
+ * node_ranges / prim_ranges are the ranges that edit wrote (Bvh4D::dirty). When the slot holds
+ * the version just before (same layout, edit - 1: `follows`, gpu.rs:341-343) and the grown arrays
+ * fit its buffers, only what changed is written: the device rows (child boxes and entries) of the
+ * listed interior nodes and of their parents, and of reached nodes that the previous tree did not
+ * reach or whose links differ from the device's; the primitives' records, attributes and order
+ * entries of prim_ranges; the stars, materials and light whole. The new tree is validated first.
+ * Otherwise everything is uploaded afresh; *out_patched (may be NULL) tells which happened. As in
+ * the reference, a node whose box changed must be listed. A scene file
+ * (specrel_upload_scene_file) carries its version too. */
+int specrel_update_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                         const specrel_bvh_version *version, const specrel_range *node_ranges, uint32_t n_node_ranges,
+                         const specrel_range *prim_ranges, uint32_t n_prim_ranges, int *out_patched);
+
 /* Number of scene slots in use (highest uploaded index + 1). */
 uint32_t specrel_num_scenes(const specrel_ctx *ctx);
 

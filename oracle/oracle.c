@@ -155,6 +155,14 @@ int oracle_intersect(const specrel_geom *g, const float o[4], const float d[4], 
 }
 
 /* ---------------------------------------------------------------- BVH (bvh4d.rs:429-578) */
+/* BvhNode4D::NOTHING (bvh4d.rs:30-31), "an internal node that no ray enters": no primitives and
+   an empty box (Aabb4D::empty, mesh4d.rs:71-76). The slab test (mesh4d.rs:114-140) does not
+   reject an empty box -- every axis gives (-inf, +inf) -- so the reference's traversal of a tree
+   whose root is NOTHING (an arena holding nothing, scene_cache.rs:324) pushes node 0 as its own
+   children and never ends, on the CPU (bvh4d.rs:445-516) and in the shader alike. The oracle
+   follows the documented meaning instead: no ray enters it. */
+static int is_nothing(const specrel_node *n) { return n->count == 0 && !(n->min[0] <= n->max[0]); }
+
 static o4 safe_inv_dir(o4 d) { /* bvh4d.rs:430-441 */
     float c[4] = {d.t, d.x, d.y, d.z}, r[4];
     for (int i = 0; i < 4; ++i) {
@@ -190,7 +198,7 @@ static int trace(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float tm
     o4 inv = safe_inv_dir(d);
     const specrel_node *N = s->nodes;
     float root_enter;
-    if (!aabb_ray(N[0].min, N[0].max, o, inv, tmin, tmax, &root_enter)) return 0;
+    if (is_nothing(&N[0]) || !aabb_ray(N[0].min, N[0].max, o, inv, tmin, tmax, &root_enter)) return 0;
     uint32_t st_n[64];
     float st_t[64];
     size_t sp = 1;
@@ -203,6 +211,7 @@ static int trace(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float tm
         if (t_enter >= tmax) continue;
         const specrel_node *node = &N[ni];
         if (c) c->node_pops++;
+        if (is_nothing(node)) continue; /* entered as a leaf without primitives */
         if (node->count > 0) {
             uint32_t first = node->right_or_first, end = first + node->count;
             for (uint32_t i = first; i < end; ++i) {
@@ -252,7 +261,7 @@ static int occluded(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float
     o4 inv = safe_inv_dir(d);
     const specrel_node *N = s->nodes;
     float e;
-    if (!aabb_ray(N[0].min, N[0].max, o, inv, tmin, tmax, &e)) return 0;
+    if (is_nothing(&N[0]) || !aabb_ray(N[0].min, N[0].max, o, inv, tmin, tmax, &e)) return 0;
     uint32_t st[64];
     size_t sp = 1;
     st[0] = 0;
@@ -261,6 +270,7 @@ static int occluded(const specrel_scene_arrays *s, o4 o, o4 d, float tmin, float
         const uint32_t ni = st[sp];
         const specrel_node *node = &N[ni];
         if (c) c->shadow_node_pops++;
+        if (is_nothing(node)) continue;
         if (node->count > 0) {
             uint32_t first = node->right_or_first, end = first + node->count;
             for (uint32_t i = first; i < end; ++i) {

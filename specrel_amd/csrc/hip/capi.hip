@@ -58,10 +58,14 @@ struct SceneSlot {
     DevBuf nbox, ninfo, big, geoms, attrs, order, mats, stars;
     Scene S{};
     uint32_t num_stars = 0, num_star_words = 0, num_materials = 0, num_nodes = 0, num_prims = 0;
-    // Host copies for specrel_patch_scene: each node's parent slot (parent << 1 | right side;
-    // ~0u for the root) and the encoded child entries and large leaves as uploaded.
+    // Host copies for scene patching: each node's parent slot (parent << 1 | right side; ~0u for
+    // the root and for nodes the tree does not reach) and the encoded child entries and large
+    // leaves as uploaded.
     std::vector<uint32_t> parent;
     std::vector<uint2> ninfo_host, big_host;
+    // The BVH version (bvh4d.rs:47-61) the device arrays hold, when the caller gave one.
+    bool versioned = false;
+    uint64_t layout = 0, edit = 0;
 };
 
 struct Fail {
@@ -187,8 +191,8 @@ void galloc(specrel_ctx *ctx, DevBuf &b, size_t bytes) {
     if (b.p != before) ctx->generation++;
 }
 
-void upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s) {
-    dalloc(b, bytes);
+void upload(DevBuf &b, const void *src, size_t bytes, hipStream_t s, size_t cap = 0) {
+    dalloc(b, std::max(bytes, cap));
     if (bytes) check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
 }
 
@@ -205,96 +209,144 @@ int guarded(specrel_ctx *ctx, F &&f) {
     }
 }
 
-// Re-pack the reference arrays into the device layout, validating the tree on the way.
-void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_arrays *a) {
+// A tree over nothing (BvhNode4D::NOTHING, bvh4d.rs:31: no primitives, an empty box, as the root
+// of an empty arena): no ray enters it, so it is packed as a leaf without primitives -- entry
+// LEAF, large leaf 0, which every scene's large-leaf table holds as (0, 0). (The reference's slab
+// test admits the empty box, and its traversal then pushes node 0 as its own children forever.)
+bool is_nothing(const specrel_node &n) { return n.count == 0 && !(n.min[0] <= n.max[0]); }
+
+// Large leaf 0: no primitives, for NOTHING nodes and the root of a scene without nodes.
+std::vector<uint2> empty_big_table() { return {make_uint2(0, 0)}; }
+
+// The device entry of node j of `a`: a leaf (count and first slot, or an index into the large
+// leaves, appended to `big`) or, for an interior node, j itself.
+uint32_t encode_entry(const specrel_scene_arrays *a, uint32_t j, std::vector<uint2> &big) {
+    const specrel_node &n = a->nodes[j];
+    if (is_nothing(n)) return LEAF;
+    if (n.count > 0) {
+        if (n.count <= COUNT_MAX && n.right_or_first <= FIRST_MASK)
+            return LEAF | (n.count << COUNT_SHIFT) | n.right_or_first;
+        if (big.size() > FIRST_MASK) throw Fail{SPECREL_EINVAL, "too many large leaves"};
+        big.push_back(make_uint2(n.right_or_first, n.count));
+        return LEAF | (uint32_t)(big.size() - 1);
+    }
+    return j;
+}
+
+// Walks the tree of `a` from the root: every node reached once, children in range, leaf ranges
+// within the primitives. parent[j] receives each reached node's parent slot (parent << 1 | right
+// side); the root and the nodes the tree does not reach get ~0u. `interior` (may be NULL) receives
+// the reached interior nodes. Malformed input throws, so no traversal can loop.
+void walk_tree(const specrel_scene_arrays *a, std::vector<uint32_t> &parent, std::vector<uint32_t> *interior) {
+    const uint32_t nn = a->num_nodes, np = a->num_prims;
+    parent.assign(nn, ~0u);
+    if (interior) interior->clear();
+    if (!nn) return;
+    std::vector<uint8_t> seen(nn, 0);
+    std::vector<uint32_t> todo{0};
+    seen[0] = 1;
+    while (!todo.empty()) {
+        const uint32_t i = todo.back();
+        todo.pop_back();
+        const specrel_node &n = a->nodes[i];
+        if (is_nothing(n)) continue;
+        if (n.count > 0) {
+            if ((uint64_t)n.right_or_first + n.count > np) throw Fail{SPECREL_EINVAL, "leaf range out of bounds"};
+            continue;
+        }
+        const uint32_t l = n.left, r = n.right_or_first;
+        if (l >= nn || r >= nn) throw Fail{SPECREL_EINVAL, "child index out of range"};
+        if (seen[l] || seen[r] || l == r) throw Fail{SPECREL_EINVAL, "node graph is not a tree"};
+        seen[l] = seen[r] = 1;
+        parent[l] = i << 1;
+        parent[r] = (i << 1) | 1u;
+        if (interior) interior->push_back(i);
+        todo.push_back(r);
+        todo.push_back(l);
+    }
+}
+
+// The materials every primitive references exist (Scene4D::materials index, renderer.rs:422) and
+// the stars' segments lie within their words (pack_stars layout, gpu.rs:102-119).
+void check_prims_and_stars(const specrel_scene_arrays *a, uint32_t first, uint32_t count) {
+    for (uint32_t i = first; i < first + count; ++i) {
+        uint32_t m;
+        std::memcpy(&m, &a->attrs[i].misc[2], 4);
+        if (m >= a->num_materials) throw Fail{SPECREL_EINVAL, "primitive " + std::to_string(i) + " has material " +
+                                                                  std::to_string(m) + " out of range"};
+        if (a->order[i] >= a->num_prims) throw Fail{SPECREL_EINVAL, "order entry out of range"};
+    }
+}
+void check_stars(const specrel_scene_arrays *a) {
+    if (!a->num_stars) return;
+    if (!a->star_words || a->num_star_words < a->num_stars) throw Fail{SPECREL_EINVAL, "star words too short"};
+    uint64_t nseg = (a->num_star_words - a->num_stars) / 3;
+    for (uint32_t i = 0; i < a->num_stars; ++i) {
+        uint32_t first, count;
+        std::memcpy(&first, &a->star_words[4 * i + 2], 4);
+        std::memcpy(&count, &a->star_words[4 * i + 3], 4);
+        if ((uint64_t)first + count > nseg) throw Fail{SPECREL_EINVAL, "star segments out of range"};
+    }
+}
+
+// Node j's two child boxes (left min, left max, right min, right max) and entries.
+void node_row(const specrel_scene_arrays *a, uint32_t j, float4 *row, uint2 &info, std::vector<uint2> &big) {
+    const specrel_node &n = a->nodes[j];
+    const specrel_node &L = a->nodes[n.left], &R = a->nodes[n.right_or_first];
+    row[0] = make_float4(L.min[0], L.min[1], L.min[2], L.min[3]);
+    row[1] = make_float4(L.max[0], L.max[1], L.max[2], L.max[3]);
+    row[2] = make_float4(R.min[0], R.min[1], R.min[2], R.min[3]);
+    row[3] = make_float4(R.max[0], R.max[1], R.max[2], R.max[3]);
+    info = make_uint2(encode_entry(a, n.left, big), encode_entry(a, n.right_or_first, big));
+}
+
+// Room the device arrays of a scene edited in place get at upload: half as much again, as the
+// reference's GPU buffers of such a scene (gpu.rs:639-656).
+size_t with_room(size_t bytes, bool room) { return room ? bytes + bytes / 2 : bytes; }
+
+// Re-pack the reference arrays into the device layout, validating the tree on the way. With
+// `room`, the node, primitive and star arrays are allocated half again as large, so that later
+// edits can grow in place.
+void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_arrays *a, bool room = false) {
     if (!a) throw Fail{SPECREL_EINVAL, "scene is NULL"};
     const uint32_t nn = a->num_nodes, np = a->num_prims;
     if (nn && !a->nodes) throw Fail{SPECREL_EINVAL, "nodes is NULL"};
     if (np && (!a->geoms || !a->attrs || !a->order)) throw Fail{SPECREL_EINVAL, "prim arrays are NULL"};
     if (a->num_materials == 0 && np) throw Fail{SPECREL_EINVAL, "no materials"};
     if (np >= 0x7FFFFFFFu) throw Fail{SPECREL_EINVAL, "too many primitives"};
-    // Materials referenced by every primitive must exist (Scene4D::materials index, renderer.rs:422).
-    for (uint32_t i = 0; i < np; ++i) {
-        uint32_t m;
-        std::memcpy(&m, &a->attrs[i].misc[2], 4);
-        if (m >= a->num_materials) throw Fail{SPECREL_EINVAL, "primitive " + std::to_string(i) + " has material " +
-                                                                  std::to_string(m) + " out of range"};
-    }
-    // Stars (pack_stars layout).
-    if (a->num_stars) {
-        if (!a->star_words || a->num_star_words < a->num_stars) throw Fail{SPECREL_EINVAL, "star words too short"};
-        uint64_t nseg = (a->num_star_words - a->num_stars) / 3;
-        for (uint32_t i = 0; i < a->num_stars; ++i) {
-            uint32_t first, count;
-            std::memcpy(&first, &a->star_words[4 * i + 2], 4);
-            std::memcpy(&count, &a->star_words[4 * i + 3], 4);
-            if ((uint64_t)first + count > nseg) throw Fail{SPECREL_EINVAL, "star segments out of range"};
-        }
-    }
+    if (nn > 0x7FFFFFFFu / 2) throw Fail{SPECREL_EINVAL, "too many nodes"};
+    check_prims_and_stars(a, 0, np);
+    check_stars(a);
 
     std::vector<float4> nbox;
     std::vector<uint2> ninfo;
-    std::vector<uint2> big;
-    auto encode = [&](uint32_t j) -> uint32_t {
-        const specrel_node &n = a->nodes[j];
-        if (n.count > 0) {
-            if (n.count <= COUNT_MAX && n.right_or_first <= FIRST_MASK)
-                return LEAF | (n.count << COUNT_SHIFT) | n.right_or_first;
-            if (big.size() > FIRST_MASK) throw Fail{SPECREL_EINVAL, "too many large leaves"};
-            big.push_back(make_uint2(n.right_or_first, n.count));
-            return LEAF | (uint32_t)(big.size() - 1);
-        }
-        return j;
-    };
+    std::vector<uint2> big = empty_big_table();
+    std::vector<uint32_t> interior;
+    walk_tree(a, slot.parent, &interior);
     Scene &S = slot.S;
-    S.empty = nn == 0;
+    S.empty = nn == 0 || is_nothing(a->nodes[0]);
+    S.root_entry = LEAF;
+    S.root_min = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+    S.root_max = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
     if (nn) {
-        if (nn > 0x7FFFFFFFu) throw Fail{SPECREL_EINVAL, "too many nodes"};
         nbox.assign((size_t)nn * 4, make_float4(0, 0, 0, 0));
         ninfo.assign(nn, make_uint2(0, 0));
-        // Walk the tree from the root: each node reached once, children in range, leaves in range.
-        std::vector<uint8_t> seen(nn, 0);
-        std::vector<uint32_t> todo{0};
-        seen[0] = 1;
-        slot.parent.assign(nn, ~0u);
-        while (!todo.empty()) {
-            uint32_t i = todo.back();
-            todo.pop_back();
-            const specrel_node &n = a->nodes[i];
-            if (n.count > 0) {
-                if ((uint64_t)n.right_or_first + n.count > np) throw Fail{SPECREL_EINVAL, "leaf range out of bounds"};
-                continue;
-            }
-            uint32_t l = n.left, r = n.right_or_first;
-            if (l >= nn || r >= nn) throw Fail{SPECREL_EINVAL, "child index out of range"};
-            if (seen[l] || seen[r] || l == r) throw Fail{SPECREL_EINVAL, "node graph is not a tree"};
-            seen[l] = seen[r] = 1;
-            slot.parent[l] = i << 1;
-            slot.parent[r] = (i << 1) | 1u;
-            todo.push_back(r);
-            todo.push_back(l);
-            const specrel_node &L = a->nodes[l], &R = a->nodes[r];
-            nbox[4 * (size_t)i + 0] = make_float4(L.min[0], L.min[1], L.min[2], L.min[3]);
-            nbox[4 * (size_t)i + 1] = make_float4(L.max[0], L.max[1], L.max[2], L.max[3]);
-            nbox[4 * (size_t)i + 2] = make_float4(R.min[0], R.min[1], R.min[2], R.min[3]);
-            nbox[4 * (size_t)i + 3] = make_float4(R.max[0], R.max[1], R.max[2], R.max[3]);
-            ninfo[i] = make_uint2(encode(l), encode(r));
-        }
+        for (uint32_t i : interior) node_row(a, i, &nbox[4 * (size_t)i], ninfo[i], big);
         const specrel_node &root = a->nodes[0];
-        S.root_entry = encode(0);
+        S.root_entry = encode_entry(a, 0, big);
         S.root_min = make_float4(root.min[0], root.min[1], root.min[2], root.min[3]);
         S.root_max = make_float4(root.max[0], root.max[1], root.max[2], root.max[3]);
     }
-    if (nn > 0x7FFFFFFFu / 2) throw Fail{SPECREL_EINVAL, "too many nodes"};
     hipStream_t s = ctx->stream;
-    upload(slot.nbox, nbox.data(), nbox.size() * sizeof(float4), s);
-    upload(slot.ninfo, ninfo.data(), ninfo.size() * sizeof(uint2), s);
-    upload(slot.big, big.data(), big.size() * sizeof(uint2), s);
-    upload(slot.geoms, a->geoms, (size_t)np * sizeof(specrel_geom), s);
-    upload(slot.attrs, a->attrs, (size_t)np * sizeof(specrel_attr), s);
-    upload(slot.order, a->order, (size_t)np * 4, s);
+    const size_t sb = sizeof(float4), gb = sizeof(specrel_geom), ab = sizeof(specrel_attr);
+    upload(slot.nbox, nbox.data(), nbox.size() * sb, s, with_room(nbox.size() * sb, room));
+    upload(slot.ninfo, ninfo.data(), ninfo.size() * sizeof(uint2), s, with_room(ninfo.size() * sizeof(uint2), room));
+    upload(slot.big, big.data(), big.size() * sizeof(uint2), s, with_room(big.size() * sizeof(uint2) + 64, room));
+    upload(slot.geoms, a->geoms, (size_t)np * gb, s, with_room((size_t)np * gb, room));
+    upload(slot.attrs, a->attrs, (size_t)np * ab, s, with_room((size_t)np * ab, room));
+    upload(slot.order, a->order, (size_t)np * 4, s, with_room((size_t)np * 4, room));
     upload(slot.mats, a->materials, (size_t)a->num_materials * sizeof(specrel_material), s);
-    upload(slot.stars, a->star_words, (size_t)a->num_star_words * 16, s);
+    upload(slot.stars, a->star_words, (size_t)a->num_star_words * 16, s, with_room((size_t)a->num_star_words * 16, room));
     check(hipStreamSynchronize(s), "upload");
     S.nbox = (const float4 *)slot.nbox.p;
     S.ninfo = (const uint2 *)slot.ninfo.p;
@@ -411,6 +463,144 @@ bool patch_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
               "patch stars");
     slot.S.light = make_float4(a->light_pos[0], a->light_pos[1], a->light_pos[2], a->light_pos[3]);
     check(hipStreamSynchronize(s), "patch");
+    return true;
+}
+
+// The device entry `have` (as uploaded) still describes node j of `a`: the same kind of node,
+// with the same children or primitive range.
+bool entry_matches(const std::vector<uint2> &big, const specrel_scene_arrays *a, uint32_t j, uint32_t have) {
+    const specrel_node &n = a->nodes[j];
+    if (is_nothing(n)) return have == LEAF;
+    if (n.count == 0) return have == j;
+    if ((have & LEAF) == 0) return false;
+    if (n.count <= COUNT_MAX && n.right_or_first <= FIRST_MASK)
+        return have == (LEAF | (n.count << COUNT_SHIFT) | n.right_or_first);
+    const uint32_t c = (have >> COUNT_SHIFT) & COUNT_MAX, idx = have & FIRST_MASK;
+    return c == 0 && idx < big.size() && big[idx].x == n.right_or_first && big[idx].y == n.count;
+}
+
+// GpuRenderer::update_scene's patch (gpu.rs:704-731) for the edit that takes the slot's arrays
+// to `a`: the nodes and primitives of the ranges that edit wrote (bvh4d.rs:158-215), re-encoded
+// into the device layout, plus the stars, materials and light. The arrays may have grown. The
+// device holds, per interior node, the boxes and entries of its two children, and only for the
+// nodes the uploaded tree reached, so the rows written are those of the reached interior nodes
+// that are listed, that hold a listed node's box (its parent), that the previous tree did not
+// reach (a subtree linked in again: its rows were never made or are stale), or whose links
+// differ from the device's (a node rewritten but left out of the ranges). Before any device
+// write the new tree is walked: every node reached once, links and leaf ranges in bounds.
+// Returns false, having written nothing, when the grown arrays do not fit the slot's buffers:
+// the caller then uploads everything.
+bool update_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_arrays *a, const specrel_range *nr,
+                         uint32_t n_nr, const specrel_range *pr, uint32_t n_pr) {
+    const uint32_t nn = a->num_nodes, np = a->num_prims;
+    if (nn && !a->nodes) throw Fail{SPECREL_EINVAL, "nodes is NULL"};
+    if (np && (!a->geoms || !a->attrs || !a->order)) throw Fail{SPECREL_EINVAL, "prim arrays are NULL"};
+    if ((n_nr && !nr) || (n_pr && !pr)) throw Fail{SPECREL_EINVAL, "range list is NULL"};
+    if (a->num_materials == 0 && np) throw Fail{SPECREL_EINVAL, "no materials"};
+    if (np >= 0x7FFFFFFFu || nn > 0x7FFFFFFFu / 2) throw Fail{SPECREL_EINVAL, "too many nodes or primitives"};
+    for (uint32_t r = 0; r < n_nr; ++r)
+        if ((uint64_t)nr[r].first + nr[r].count > nn) throw Fail{SPECREL_EINVAL, "node range out of bounds"};
+    for (uint32_t r = 0; r < n_pr; ++r) {
+        if ((uint64_t)pr[r].first + pr[r].count > np) throw Fail{SPECREL_EINVAL, "primitive range out of bounds"};
+        check_prims_and_stars(a, pr[r].first, pr[r].count);
+    }
+    check_stars(a);
+    if (nn == 0 || slot.num_nodes == 0) return false;
+    const bool fits = (size_t)nn * 64 <= slot.nbox.bytes && (size_t)nn * 8 <= slot.ninfo.bytes &&
+                      (size_t)np * sizeof(specrel_geom) <= slot.geoms.bytes &&
+                      (size_t)np * sizeof(specrel_attr) <= slot.attrs.bytes && (size_t)np * 4 <= slot.order.bytes &&
+                      (size_t)a->num_star_words * 16 <= slot.stars.bytes &&
+                      (size_t)a->num_materials * sizeof(specrel_material) <= slot.mats.bytes;
+    if (!fits) return false;
+    std::vector<uint32_t> parent, interior;
+    walk_tree(a, parent, &interior);
+    std::vector<uint8_t> write(nn, 0);
+    for (uint32_t r = 0; r < n_nr; ++r)
+        for (uint32_t j = nr[r].first; j < nr[r].first + nr[r].count; ++j) {
+            if (j != 0 && parent[j] == ~0u) continue;  // listed but not in the tree: free room
+            const specrel_node &n = a->nodes[j];
+            if (n.count == 0 && !is_nothing(n)) write[j] = 1;
+            if (parent[j] != ~0u) write[parent[j] >> 1] = 1;
+        }
+    const size_t old_nn = slot.ninfo_host.size();
+    for (uint32_t i : interior) {
+        if (write[i]) continue;
+        const bool was_reached = i < old_nn && i < slot.parent.size() && (i == 0 ? !slot.S.empty : slot.parent[i] != ~0u);
+        const specrel_node &n = a->nodes[i];
+        if (!was_reached || !entry_matches(slot.big_host, a, n.left, slot.ninfo_host[i].x) ||
+            !entry_matches(slot.big_host, a, n.right_or_first, slot.ninfo_host[i].y))
+            write[i] = 1;
+    }
+    std::vector<uint2> big = slot.big_host;
+    const size_t big0 = big.size();
+    std::vector<uint32_t> idx;
+    std::vector<float4> rows;
+    std::vector<uint2> info;
+    for (uint32_t i : interior) {
+        if (!write[i]) continue;
+        idx.push_back(i);
+        rows.resize(rows.size() + 4);
+        info.push_back(make_uint2(0, 0));
+        node_row(a, i, &rows[rows.size() - 4], info.back(), big);
+    }
+    Scene S = slot.S;
+    {  // the root's own box and entry live in the scene record: always brought up to date
+        const specrel_node &root = a->nodes[0];
+        if (!entry_matches(big, a, 0, S.root_entry)) S.root_entry = encode_entry(a, 0, big);
+        S.root_min = make_float4(root.min[0], root.min[1], root.min[2], root.min[3]);
+        S.root_max = make_float4(root.max[0], root.max[1], root.max[2], root.max[3]);
+        S.empty = is_nothing(root);
+    }
+    if (big.size() * sizeof(uint2) > slot.big.bytes) return false;
+    // Everything checked: write.
+    hipStream_t s = ctx->stream;
+    DevBuf tmp;
+    if (!idx.empty()) {
+        const size_t n = idx.size();
+        dalloc(tmp, n * (64 + 8 + 4));
+        float4 *d_rows = (float4 *)tmp.p;
+        uint2 *d_info = (uint2 *)(d_rows + 4 * n);
+        uint32_t *d_idx = (uint32_t *)(d_info + n);
+        check(hipMemcpyAsync(d_rows, rows.data(), n * 64, hipMemcpyHostToDevice, s), "update rows");
+        check(hipMemcpyAsync(d_info, info.data(), n * 8, hipMemcpyHostToDevice, s), "update rows");
+        check(hipMemcpyAsync(d_idx, idx.data(), n * 4, hipMemcpyHostToDevice, s), "update rows");
+        k_patch_rows<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(d_idx, d_rows, d_info, (uint32_t)n, (float4 *)slot.nbox.p,
+                                                                  (uint2 *)slot.ninfo.p);
+        check(hipGetLastError(), "k_patch_rows launch");
+    }
+    if (big.size() > big0)
+        check(hipMemcpyAsync((uint2 *)slot.big.p + big0, big.data() + big0, (big.size() - big0) * sizeof(uint2),
+                             hipMemcpyHostToDevice, s),
+              "update large leaves");
+    for (uint32_t r = 0; r < n_pr; ++r) {
+        const size_t f = pr[r].first, c = pr[r].count;
+        if (!c) continue;
+        check(hipMemcpyAsync((specrel_geom *)slot.geoms.p + f, a->geoms + f, c * sizeof(specrel_geom),
+                             hipMemcpyHostToDevice, s), "update geoms");
+        check(hipMemcpyAsync((specrel_attr *)slot.attrs.p + f, a->attrs + f, c * sizeof(specrel_attr),
+                             hipMemcpyHostToDevice, s), "update attrs");
+        check(hipMemcpyAsync((uint32_t *)slot.order.p + f, a->order + f, c * 4, hipMemcpyHostToDevice, s),
+              "update order");
+    }
+    if (a->num_materials)
+        check(hipMemcpyAsync(slot.mats.p, a->materials, (size_t)a->num_materials * sizeof(specrel_material),
+                             hipMemcpyHostToDevice, s), "update materials");
+    if (a->num_star_words)
+        check(hipMemcpyAsync(slot.stars.p, a->star_words, (size_t)a->num_star_words * 16, hipMemcpyHostToDevice, s),
+              "update stars");
+    check(hipStreamSynchronize(s), "update");
+    S.num_prims = np;
+    S.light = make_float4(a->light_pos[0], a->light_pos[1], a->light_pos[2], a->light_pos[3]);
+    slot.S = S;
+    slot.ninfo_host.resize(nn, make_uint2(0, 0));
+    for (size_t k = 0; k < idx.size(); ++k) slot.ninfo_host[idx[k]] = info[k];
+    slot.parent.swap(parent);
+    slot.big_host.swap(big);
+    slot.num_nodes = nn;
+    slot.num_prims = np;
+    slot.num_materials = a->num_materials;
+    slot.num_stars = a->num_stars;
+    slot.num_star_words = a->num_star_words;
     return true;
 }
 
@@ -1495,8 +1685,57 @@ int specrel_upload_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_sce
         // Keep the old device buffers for reuse; rebuild the slot's contents.
         SceneSlot &slot = ctx->scenes[scene_idx];
         slot.used = false;
+        slot.versioned = false;
         ctx->generation++;  // a captured graph holds the old scene pointers
         build_device_scene(ctx, slot, scene);
+        return SPECREL_OK;
+    });
+}
+
+int specrel_upload_scene_versioned(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                                   const specrel_bvh_version *version) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!version) throw Fail{SPECREL_EINVAL, "version is NULL"};
+        if (scene_idx > 4096) throw Fail{SPECREL_EINVAL, "scene index too large"};
+        if (ctx->scenes.size() <= scene_idx) ctx->scenes.resize(scene_idx + 1);
+        SceneSlot &slot = ctx->scenes[scene_idx];
+        slot.used = false;
+        slot.versioned = false;
+        ctx->generation++;
+        build_device_scene(ctx, slot, scene, version->edit > 0);
+        slot.versioned = true;
+        slot.layout = version->layout;
+        slot.edit = version->edit;
+        return SPECREL_OK;
+    });
+}
+
+int specrel_update_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scene_arrays *scene,
+                         const specrel_bvh_version *version, const specrel_range *node_ranges, uint32_t n_node_ranges,
+                         const specrel_range *prim_ranges, uint32_t n_prim_ranges, int *out_patched) {
+    if (!ctx) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!scene || !version) throw Fail{SPECREL_EINVAL, "scene or version is NULL"};
+        if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
+            throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
+        SceneSlot &slot = ctx->scenes[scene_idx];
+        ctx->generation++;  // a captured graph holds the scene by value
+        // follows (gpu.rs:341-343): the device holds the version just before this edit
+        const bool follows = slot.versioned && slot.layout == version->layout && slot.edit + 1 == version->edit;
+        bool patched = follows && update_device_scene(ctx, slot, scene, node_ranges, n_node_ranges, prim_ranges,
+                                                       n_prim_ranges);
+        if (!patched) {
+            slot.used = false;
+            slot.versioned = false;
+            build_device_scene(ctx, slot, scene, version->edit > 0);
+        }
+        slot.versioned = true;
+        slot.layout = version->layout;
+        slot.edit = version->edit;
+        if (out_patched) *out_patched = patched ? 1 : 0;
         return SPECREL_OK;
     });
 }
@@ -1516,8 +1755,14 @@ int specrel_upload_scene_file(specrel_ctx *ctx, uint32_t scene_idx, const char *
         if (ctx->scenes.size() <= scene_idx) ctx->scenes.resize(scene_idx + 1);
         SceneSlot &slot = ctx->scenes[scene_idx];
         slot.used = false;
+        slot.versioned = false;
         ctx->generation++;
-        build_device_scene(ctx, slot, &a);  // copies out of the mapping before it is unmapped
+        // copies out of the mapping before it is unmapped; a file of an edited BVH gets room to grow
+        build_device_scene(ctx, slot, &a, m.h.bvh_layout != 0 && m.h.bvh_edit > 0);
+        // the file's BVH version (layout 0: none; the reference numbers layouts from 1, bvh4d.rs:57-61)
+        slot.versioned = m.h.bvh_layout != 0;
+        slot.layout = m.h.bvh_layout;
+        slot.edit = m.h.bvh_edit;
         return SPECREL_OK;
     });
 }
@@ -1735,6 +1980,7 @@ int specrel_patch_scene(specrel_ctx *ctx, uint32_t scene_idx, const specrel_scen
         SceneSlot &slot = ctx->scenes[scene_idx];
         ctx->generation++;  // a captured graph holds the scene by value
         bool ok = patch_device_scene(ctx, slot, scene, node_ranges, n_node_ranges, prim_ranges, n_prim_ranges);
+        slot.versioned = false;  // an unversioned edit: the next versioned update uploads afresh
         if (!ok) {
             slot.used = false;
             build_device_scene(ctx, slot, scene);

@@ -311,6 +311,7 @@ __device__ __forceinline__ void leaf_tests(const Isect &x0, const Isect &x1, con
 template <bool STATS, class ST, bool M = false>
 __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk) {
     hit.slot = -1;
+    if (S.empty) return false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float t_enter;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
@@ -416,6 +417,7 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
                          int limit, bool &redo) {
     hit.slot = -1;
     redo = false;
+    if (S.empty) return false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float t_enter;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
@@ -486,6 +488,7 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
 // Bvh4D::is_occluded (bvh4d.rs:523-578): right pushed, then left pushed and popped at once.
 template <bool STATS, class ST>
 __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Counters &C, ST &stk) {
+    if (S.empty) return false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float en;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, en)) return false;

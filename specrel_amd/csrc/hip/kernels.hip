@@ -511,8 +511,16 @@ __global__ void k_patch_nodes(const uint32_t *pslot, const float4 *bounds, uint3
     nbox[at + 1] = bounds[2 * (size_t)i + 1];
 }
 
-// Zeroes the per-frame queue counters (a kernel rather than a memset, so the frame's work can
-// be captured in a graph).
+// specrel_update_scene: whole rows (child boxes and entries) of the listed interior nodes.
+__global__ void k_patch_rows(const uint32_t *idx, const float4 *rows, const uint2 *info, uint32_t n, float4 *nbox,
+                             uint2 *ninfo) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t r = idx[i];
+    for (int k = 0; k < 4; ++k) nbox[4 * r + k] = rows[4 * (size_t)i + k];
+    ninfo[r] = info[i];
+}
+
 // Set bits of n words, added to *out (the distinct records of an instrumented render).
 __global__ void k_popcount(const uint32_t *bits, uint64_t n, unsigned long long *out) {
     uint32_t c = 0;
@@ -520,6 +528,9 @@ __global__ void k_popcount(const uint32_t *bits, uint64_t n, unsigned long long 
         c += __popc(bits[i]);
     if (c) atomicAdd(out, (unsigned long long)c);
 }
+
+// Zeroes the per-frame queue counters (a kernel rather than a memset, so the frame's work can
+// be captured in a graph).
 
 __global__ void k_reset(uint32_t *counters, uint32_t n) {
     uint32_t i = threadIdx.x;

@@ -218,6 +218,8 @@ __global__ void k_occlude_p(OccludeArgs a);
 template <bool STATS, int SK, int PASS>
 __global__ void k_shade(ShadeArgs a);
 __global__ void k_popcount(const uint32_t *bits, uint64_t n, unsigned long long *out);
+__global__ void k_patch_rows(const uint32_t *idx, const float4 *rows, const uint2 *info, uint32_t n, float4 *nbox,
+                             uint2 *ninfo);
 __global__ void k_reset(uint32_t *counters, uint32_t n);
 struct FrameWords {
     static constexpr uint32_t N = sizeof(specrel_frame) / 4;

@@ -54,6 +54,10 @@ class Range(C.Structure):
     _fields_ = [("first", C.c_uint32), ("count", C.c_uint32)]
 
 
+class BvhVersion(C.Structure):
+    _fields_ = [("layout", C.c_uint64), ("edit", C.c_uint64)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
                 ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
@@ -99,6 +103,10 @@ SIGNATURES = {
     "specrel_patch_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(Range), C.c_uint32,
                                      C.POINTER(Range), C.c_uint32, C.POINTER(C.c_int)]),
     "specrel_num_scenes": (C.c_uint32, [_P]),
+    "specrel_upload_scene_versioned": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(BvhVersion)]),
+    "specrel_update_scene": (C.c_int, [_P, C.c_uint32, C.POINTER(F.SceneArrays), C.POINTER(BvhVersion),
+                                      C.POINTER(Range), C.c_uint32, C.POINTER(Range), C.c_uint32,
+                                      C.POINTER(C.c_int)]),
     "specrel_render": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), _U32P, _U32P, _FP, _FP, C.c_uint64]),
     "specrel_render_resident": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_int]),
     "specrel_render_noalloc": (C.c_int, [_P, C.c_uint32, C.POINTER(F.Frame), C.c_void_p, C.c_void_p, C.c_void_p,
@@ -182,6 +190,24 @@ class Context:
 
     def upload(self, idx, scene):
         self._check(self.lib.specrel_upload_scene(self._h, idx, C.byref(scene.arrays)))
+
+    def upload_versioned(self, idx, packed):
+        """specrel_upload_scene_versioned of a PackedScene (its arrays and BVH version)."""
+        v = BvhVersion(*packed.version)
+        self._check(self.lib.specrel_upload_scene_versioned(self._h, idx, C.byref(packed.arrays), C.byref(v)))
+
+    def update(self, idx, packed, node_ranges=None, prim_ranges=None):
+        """specrel_update_scene of a PackedScene after an edit (its dirty ranges unless given):
+        True when the ranges were patched in, False when the library uploaded it all."""
+        nrs = packed.dirty_nodes if node_ranges is None else node_ranges
+        prs = packed.dirty_prims if prim_ranges is None else prim_ranges
+        nr = (Range * max(1, len(nrs)))(*[Range(int(a), int(c)) for a, c in nrs])
+        pr = (Range * max(1, len(prs)))(*[Range(int(a), int(c)) for a, c in prs])
+        v = BvhVersion(*packed.version)
+        out = C.c_int(0)
+        self._check(self.lib.specrel_update_scene(self._h, idx, C.byref(packed.arrays), C.byref(v), nr, len(nrs), pr,
+                                                  len(prs), C.byref(out)))
+        return out.value == 1
 
     def upload_file(self, idx, path, verify=True):
         """specrel_upload_scene_file: a scene file written by any producer (scene.Scene.save)."""
@@ -418,20 +444,29 @@ class GpuRenderer:
             self.ctx = Context(device)
         except SpecrelError as e:
             raise RuntimeError(f"no GPU adapter: {e}") from e
+        from .scene import PackedScene
         for i, s in enumerate(scenes):
-            self.ctx.upload(i, s)
+            if isinstance(s, PackedScene):
+                self.ctx.upload_versioned(i, s)
+            else:
+                self.ctx.upload(i, s)
         self.last_timings = GpuTimings()
 
     def adapter_name(self):
         return self.ctx.device_name()
 
     def update_scene(self, scene_idx, scene, node_ranges=None, prim_ranges=None):
-        # Dirty ranges are patched in place when given (gpu.rs:704-731); otherwise, or when they
-        # no longer fit the uploaded tree, the whole scene is uploaded again.
+        """gpu.rs:704-731. A PackedScene (versioned, edited in place) is patched with its latest
+        edit's dirty ranges when the device holds the version before it and the arrays fit, else
+        uploaded afresh; a producer's scene with explicit ranges goes through specrel_patch_scene
+        (same shape), without them it is uploaded again. Returns whether ranges were patched."""
+        from .scene import PackedScene
+        if isinstance(scene, PackedScene):
+            return self.ctx.update(scene_idx, scene, node_ranges, prim_ranges)
         if node_ranges is None and prim_ranges is None:
             self.ctx.upload(scene_idx, scene)
-        else:
-            self.ctx.patch(scene_idx, scene, node_ranges or (), prim_ranges or ())
+            return False
+        return self.ctx.patch(scene_idx, scene, node_ranges or (), prim_ranges or ())
 
     def render(self, scene_idx, camera, cfg, dynamics, framebuffer):
         """Renders into `framebuffer` (a contiguous uint32 array of cfg.width * cfg.height pixels,

@@ -297,6 +297,152 @@ def default_camera(name, params=None) -> Camera4D:
     return Camera4D(tuple(c.position[i] for i in range(4)), t, c.proper_time, c.fov_y_rad)
 
 
+_NEXT_LAYOUT = [1]
+
+
+def new_layout():
+    """A fresh BVH layout number (bvh4d.rs:57-61): numbered from 1; 0 stands for none."""
+    v = _NEXT_LAYOUT[0]
+    _NEXT_LAYOUT[0] += 1
+    return v
+
+
+class PackedScene:
+    """The packed arrays of a scene (PackedSceneGpuData, gpu.rs:232-289) in numpy, with the version
+    of the BVH they come from and the node and primitive ranges its latest edit wrote (BvhVersion
+    and Bvh4D::dirty, bvh4d.rs:47-61, 150-215): what GpuRenderer::update_scene consumes. As an
+    arena, it is edited in place the way Bvh4D is -- begin_edit, then write_part (a whole subtree
+    and its primitives into free ranges or past the ends, links moved with it) and write_nodes --
+    so that a copy of the arrays (the device's) can be patched with the dirty ranges."""
+
+    def __init__(self, nodes, geoms, attrs, order, materials, star_words, num_stars, light, layout=None, edit=0):
+        self.nodes = np.ascontiguousarray(nodes, np.float32).reshape(-1, 12)
+        self.geoms = np.ascontiguousarray(geoms, np.float32).reshape(-1, 20)
+        self.attrs = np.ascontiguousarray(attrs, np.float32).reshape(-1, 48)
+        self.order = np.ascontiguousarray(order, np.uint32).reshape(-1)
+        self.materials = np.ascontiguousarray(materials, np.float32).reshape(-1, 12)
+        self.star_words = np.ascontiguousarray(star_words, np.float32).reshape(-1, 4)
+        self.num_stars = int(num_stars)
+        self.light = tuple(float(x) for x in light)
+        self.layout = new_layout() if layout is None else int(layout)
+        self.edit = int(edit)
+        self.dirty_nodes, self.dirty_prims = [], []
+
+    @classmethod
+    def from_scene(cls, scene, layout=None):
+        """A copy of a producer's scene, as a fresh layout (edit 0)."""
+        a = scene.arrays
+        mats = np.frombuffer((C.c_char * (a.num_materials * 48)).from_address(a.materials), np.float32).copy()
+        return cls(scene.nodes().copy(), scene.geoms().copy(), scene.attrs().copy(), scene.order().copy(), mats,
+                   scene.star_words().copy(), a.num_stars, tuple(a.light_pos[i] for i in range(4)), layout)
+
+    def copy(self):
+        """The same arrays and version (a second, independent copy)."""
+        return PackedScene(self.nodes.copy(), self.geoms.copy(), self.attrs.copy(), self.order.copy(),
+                           self.materials.copy(), self.star_words.copy(), self.num_stars, self.light, self.layout,
+                           self.edit)
+
+    @classmethod
+    def arena(cls, scene):
+        """An empty arena with `scene`'s materials, stars and light (Bvh4D::reset_arena,
+        bvh4d.rs:169-179): a new layout holding only the root, a tree over nothing."""
+        p = scene.copy() if isinstance(scene, PackedScene) else cls.from_scene(scene)
+        p.reset_arena()
+        return p
+
+    def reset_arena(self):
+        self.nodes = NOTHING.reshape(1, 12).copy()
+        self.geoms = np.zeros((0, 20), np.float32)
+        self.attrs = np.zeros((0, 48), np.float32)
+        self.order = np.zeros(0, np.uint32)
+        self.layout, self.edit = new_layout(), 0
+        self.dirty_nodes, self.dirty_prims = [], []
+
+    @property
+    def num_nodes(self):
+        return len(self.nodes)
+
+    @property
+    def num_prims(self):
+        return len(self.geoms)
+
+    @property
+    def version(self):
+        return (self.layout, self.edit)
+
+    @property
+    def arrays(self) -> F.SceneArrays:
+        """specrel_scene_arrays over the current numpy arrays (valid while they are not replaced)."""
+        a = F.SceneArrays()
+        a.nodes = self.nodes.ctypes.data
+        a.num_nodes = self.num_nodes
+        a.num_prims = self.num_prims
+        a.geoms = self.geoms.ctypes.data
+        a.attrs = self.attrs.ctypes.data
+        a.order = self.order.ctypes.data
+        a.materials = self.materials.ctypes.data
+        a.num_materials = len(self.materials)
+        a.num_stars = self.num_stars
+        a.star_words = self.star_words.ctypes.data
+        a.num_star_words = len(self.star_words)
+        for i in range(4):
+            a.light_pos[i] = self.light[i]
+        self._keep = a
+        return a
+
+    def begin_edit(self):
+        """Bvh4D::begin_edit (bvh4d.rs:181-186): the next edit in place."""
+        self.edit += 1
+        self.dirty_nodes, self.dirty_prims = [], []
+
+    @staticmethod
+    def _put(arr, at, rows):
+        n = at + len(rows)
+        if n > len(arr):
+            grown = np.zeros((n,) + arr.shape[1:], arr.dtype)
+            grown[:len(arr)] = arr
+            arr = grown
+        arr[at:n] = rows
+        return arr
+
+    def write_part(self, src, nodes, prims, node_at, prim_at):
+        """Bvh4D::write_part (bvh4d.rs:188-210): src's nodes `nodes` and primitives `prims` (ranges,
+        a whole subtree and its primitives) to nodes from node_at and primitives from prim_at,
+        links moved with them; the primitives in leaf order."""
+        dn = np.uint32((node_at - nodes.start) & 0xFFFFFFFF)
+        dp = np.uint32((prim_at - prims.start) & 0xFFFFFFFF)
+        moved = src.nodes[nodes.start:nodes.stop].copy()
+        u = moved.view(np.uint32)
+        leaf = u[:, 9] > 0
+        u[leaf, 8] += dp
+        u[~leaf, 8] += dn
+        u[~leaf, 10] += dn
+        self.nodes = self._put(self.nodes, node_at, moved)
+        self.geoms = self._put(self.geoms, prim_at, src.geoms[prims.start:prims.stop])
+        self.attrs = self._put(self.attrs, prim_at, src.attrs[prims.start:prims.stop])
+        self.order = self._put(self.order, prim_at, np.arange(prim_at, prim_at + len(prims), dtype=np.uint32))
+        self.dirty_nodes.append((node_at, len(nodes)))
+        self.dirty_prims.append((prim_at, len(prims)))
+
+    def write_nodes(self, at, rows):
+        """Bvh4D::write_nodes (bvh4d.rs:212-215): nodes (12 floats each, links as they are) from at."""
+        rows = np.asarray(rows, np.float32).reshape(-1, 12)
+        self.nodes = self._put(self.nodes, at, rows)
+        self.dirty_nodes.append((at, len(rows)))
+
+
+def node_row(lo, hi, left=0, right_or_first=0, count=0):
+    """One GpuNode (gpu.rs:44-51) as 12 floats: min[4], max[4], right_or_first, count, left, pad."""
+    r = np.zeros(12, np.float32)
+    r[0:4], r[4:8] = lo, hi
+    r.view(np.uint32)[8:12] = (right_or_first, count, left, 0)
+    return r
+
+
+# BvhNode4D::NOTHING (bvh4d.rs:31): no primitives, an empty box (Aabb4D::empty, mesh4d.rs:71-76).
+NOTHING = node_row([np.inf] * 4, [-np.inf] * 4)
+
+
 class Builder:
     """Incremental scene construction (worldtubes, raw simplices, stars)."""
 

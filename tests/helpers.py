@@ -56,3 +56,114 @@ def harness_frame(scene_name, config, w=192, h=120):
     cam.tetrad.rotate_local(0.15, -0.05, 0.0)
     mode, shadow, view = HARNESS_CONFIGS[config]
     return S.build_frame(cam, S.RenderConfig(w, h, mode, shadow, view))
+
+
+# Arenas edited in place (the scene cache of Dynamic Mode, scene_cache.rs:280-336), built from
+# the subtrees of a producer's tree.
+
+def links(p, i):
+    u = p.nodes.view(np.uint32)[i]
+    return int(u[10]), int(u[8]), int(u[9])  # left, right_or_first, count
+
+
+def subtree(p, i):
+    """The node and primitive ranges of the subtree at node i of a built tree (Bvh4D::build lays
+    a subtree out contiguously, left child after its parent, primitives in leaf order)."""
+    stack, nodes, lo, hi = [i], 0, None, None
+    while stack:
+        j = stack.pop()
+        nodes += 1
+        left, right, count = links(p, j)
+        if count:
+            lo = right if lo is None else min(lo, right)
+            hi = right + count if hi is None else max(hi, right + count)
+        else:
+            stack += [left, right]
+    return range(i, i + nodes), range(lo, hi)
+
+
+def cut(p, depth):
+    """The subtrees `depth` levels below the root (leaves above that depth stand alone), left to
+    right, and the interior nodes above them in preorder."""
+    parts, tops = [], []
+
+    def go(j, d):
+        left, right, count = links(p, j)
+        if count or d == depth:
+            parts.append(j)
+            return
+        tops.append(j)
+        go(left, d + 1)
+        go(right, d + 1)
+
+    go(0, 0)
+    return parts, tops
+
+
+def box(p, i):
+    return p.nodes[i, 0:4].copy(), p.nodes[i, 4:8].copy()
+
+
+def top_tree(p, roots, base):
+    """Rows of a tree over the subtrees at nodes `roots` of p (split in halves, in the given
+    order): row 0 is the root, for node 0, the rest for nodes base, base + 1, ... (top_tree's
+    numbering, bvh4d.rs:598-644)."""
+    rows = []
+
+    def go(items):
+        if len(items) == 1:
+            return (items[0],) + box(p, items[0])
+        k = len(rows)
+        rows.append(None)
+        mid = len(items) // 2
+        (left, llo, lhi), (right, rlo, rhi) = go(items[:mid]), go(items[mid:])
+        lo, hi = np.minimum(llo, rlo), np.maximum(lhi, rhi)
+        rows[k] = S.node_row(lo, hi, left=left, right_or_first=right)
+        return (0 if k == 0 else base + k - 1), lo, hi
+
+    go(list(roots))
+    return rows
+
+
+def place_parts(arena, src, parts, node_at, prim_at=None):
+    """write_part each of src's subtrees `parts` into arena: nodes from node_at on, primitives
+    from prim_at on (or at their places in src when None). Returns the arena nodes of their roots
+    and the next free node."""
+    roots = []
+    for j in parts:
+        nr, pr = subtree(src, j)
+        pa = pr.start if prim_at is None else prim_at
+        arena.write_part(src, nr, pr, node_at, pa)
+        roots.append(node_at)
+        node_at += len(nr)
+        if prim_at is not None:
+            prim_at += len(pr)
+    return roots, node_at
+
+
+def relaid(src, depth, seed):
+    """src's tree in an arena, laid out afresh: the subtrees `depth` levels down written as parts
+    in shuffled node places (primitives where src has them), the nodes above rewritten over node
+    0 and the free range after the parts, with src's own boxes and shape. Rays meet exactly what
+    they meet in src, in the same order."""
+    parts, tops = cut(src, depth)
+    arena = S.PackedScene.arena(src)
+    arena.begin_edit()
+    perm = np.random.default_rng(seed).permutation(len(parts))
+    at = {}
+    node_at = 1
+    for k in perm:
+        roots, node_at = place_parts(arena, src, [parts[k]], node_at)
+        at[parts[k]] = roots[0]
+    base = node_at
+    top_at = {j: (0 if n == 0 else base + n - 1) for n, j in enumerate(tops)}
+    rows = []
+    for j in tops:
+        left, right, _ = links(src, j)
+        where = lambda c: top_at[c] if c in top_at else at[c]
+        lo, hi = box(src, j)
+        rows.append(S.node_row(lo, hi, left=where(left), right_or_first=where(right)))
+    arena.write_nodes(0, rows[:1])
+    if len(rows) > 1:
+        arena.write_nodes(base, rows[1:])
+    return arena

@@ -65,6 +65,8 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_adjoint_batch(None, None, None, None, 0, None, 0, None, None) == R.EINVAL
     assert lib.specrel_patch_scene(None, 0, None, None, 0, None, 0, None) == R.EINVAL
     assert lib.specrel_download_linear(None, 0, None, 0) == R.EINVAL
+    assert lib.specrel_upload_scene_versioned(None, 0, None, None) == R.EINVAL
+    assert lib.specrel_update_scene(None, 0, None, None, None, 0, None, 0, None) == R.EINVAL
 
 
 def test_create_reports_missing_device():
