@@ -8,7 +8,8 @@ import numpy as np
 from specrel_amd import _ffi as F
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SPECREL_ORACLE_LIB names another build of the same source (tests/run_sanitized.sh's).
+LIB_PATH = os.environ.get("SPECREL_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class Counts(C.Structure):

@@ -202,10 +202,11 @@ _scene_lib = None
 
 
 def scene_lib():
-    """The scene producer library (built by specrel_amd.build)."""
+    """The scene producer library (built by specrel_amd.build; SPECREL_SCENE_LIB names another
+    build of the same source, e.g. an address-sanitized one)."""
     global _scene_lib
     if _scene_lib is None:
-        path = os.path.join(LIB_DIR, "libspecrel_scene.so")
+        path = os.environ.get("SPECREL_SCENE_LIB") or os.path.join(LIB_DIR, "libspecrel_scene.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is missing: run specrel_amd.build.build_all() first")
         _scene_lib = _scene_protos(C.CDLL(path))

@@ -1599,8 +1599,8 @@ int specrel_extrude_tubes_host(const specrel_tube *tubes, uint32_t n, specrel_ge
         uint64_t at = 0;
         for (uint32_t i = 0; i < n; ++i) {
             const Prims &p = parts[i];
-            if (geoms) std::memcpy(geoms + at, p.geoms.data(), p.size() * sizeof(specrel_geom));
-            if (attrs) std::memcpy(attrs + at, p.attrs.data(), p.size() * sizeof(specrel_attr));
+            if (geoms && p.size()) std::memcpy(geoms + at, p.geoms.data(), p.size() * sizeof(specrel_geom));
+            if (attrs && p.size()) std::memcpy(attrs + at, p.attrs.data(), p.size() * sizeof(specrel_attr));
             if (boxes)
                 for (size_t k = 0; k < p.size(); ++k) {
                     put4(boxes + 8 * (at + k), p.boxes[k].mn);

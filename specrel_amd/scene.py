@@ -208,6 +208,7 @@ class Scene:
         if n == 0:
             return np.zeros((0, width), dtype)
         buf = (C.c_char * (n * width * np.dtype(dtype).itemsize)).from_address(ptr)
+        buf._owner = self  # the view keeps the scene (and the memory it points into) alive
         return np.frombuffer(buf, dtype).reshape(n, width)
 
     def nodes(self):
