@@ -96,10 +96,14 @@ enum {
                                      when reached. Results do not depend on it; instrumented renders
                                      (SPECREL_OPT_STATS) test every leaf when reached, so their
                                      counters are the reference's. */
-    SPECREL_OPT_ADJOINT_MASK = 14  /* which pixels specrel_adjoint[_batch] counts as valid: 0 (default)
+    SPECREL_OPT_ADJOINT_MASK = 14, /* which pixels specrel_adjoint[_batch] counts as valid: 0 (default)
                                      every hit inside its tetrahedron whose neighbours are on the same
                                      object (silhouettes masked); 1 also no facet, texture or shadow
                                      edge among the neighbours (the set finite differences resolve) */
+    SPECREL_OPT_WAVE_CLOCK = 15   /* 1: each wave of the tiled primary-ray kernel records when it
+                                     started and ended (the device's constant-rate clock) and where it
+                                     ran; read with specrel_last_wave_clock. Renders do not replay
+                                     graphs meanwhile. Results never depend on it */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the
@@ -338,6 +342,13 @@ int specrel_adjoint_batch(specrel_ctx *ctx, const float *targets, const uint8_t 
 int specrel_last_timings(const specrel_ctx *ctx, specrel_timings *out);
 /* Counters of the last render made with SPECREL_OPT_STATS on. */
 int specrel_last_stats(const specrel_ctx *ctx, specrel_stats *out);
+
+/* No counterpart in the reference: a profiling aid. The records of the last render's waves, when
+ * it ran with SPECREL_OPT_WAVE_CLOCK and the tiled primary-ray kernel: copies min(cap, waves) of
+ * them to out (may be NULL to ask the count), *n_waves = waves (0 when none were recorded),
+ * *clock_khz = the clock's rate. Waits for the render. */
+int specrel_last_wave_clock(specrel_ctx *ctx, specrel_wave_clock *out, uint64_t cap, uint64_t *n_waves,
+                            uint32_t *clock_khz);
 
 #ifdef __cplusplus
 }

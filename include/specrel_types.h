@@ -137,6 +137,15 @@ typedef struct specrel_frame {     /* per-view inputs; replaces GpuUniforms (gpu
     float flash_intensity[SPECREL_MAX_FLASHES];
 } specrel_frame;
 
+/* One wave of a render's tiled primary-ray kernel (SPECREL_OPT_WAVE_CLOCK, specrel_last_wave_clock). */
+typedef struct specrel_wave_clock {
+    uint64_t start, end;  /* device clock ticks at the wave's first and last instruction */
+    uint32_t tile;        /* the 16x16 tile (index over all views' tiles) */
+    uint32_t quarter;     /* which quarter of it */
+    uint32_t hw_id;       /* the wave's hardware location (HW_ID register: wave, SIMD, CU, shader array, engine) */
+    uint32_t xcc;         /* the accelerator complex die it ran on */
+} specrel_wave_clock;
+
 /* Traversal counters of one render (the instrumented variant), summed over all rays. */
 typedef struct specrel_stats {
     uint64_t rays;               /* primary rays */
@@ -170,6 +179,7 @@ static_assert(offsetof(specrel_frame, haze_origin) == 144, "frame layout");
 static_assert(sizeof(specrel_frame) == 800, "frame layout");
 static_assert(sizeof(specrel_scene_arrays) == 88 && offsetof(specrel_scene_arrays, light_pos) == 72, "arrays");
 static_assert(sizeof(specrel_stats) == 128, "stats layout");
+static_assert(sizeof(specrel_wave_clock) == 32, "wave clock layout");
 #else
 _Static_assert(sizeof(specrel_node) == 48, "GpuNode is 48 B");
 _Static_assert(sizeof(specrel_geom) == 80, "GpuGeom is 80 B");

@@ -90,6 +90,9 @@ struct specrel_ctx {
     DevBuf tile_cones;             // per tile: the star cone of the shading pass's culling
     DevBuf counters, mask, squeue;
     DevBuf touch;  // instrumented renders: bitmaps of the distinct nodes, primitives and hits touched
+    DevBuf wave_clock;          // SPECREL_OPT_WAVE_CLOCK: the last render's per-wave records
+    uint64_t wave_clock_n = 0;  // records the last render wrote
+    int clock_khz = 0;          // rate of the device clock the records use
     uint32_t touch_words[3] = {};  // words of each bitmap of the last instrumented render
     // Pinned staging of the frame uploads, a ring of slots so that renders can be enqueued ahead of
     // the device: a slot is refilled once its previous copy has completed.
@@ -128,6 +131,7 @@ struct specrel_ctx {
     bool timed_stages = true, stars_overlapped = false;
     specrel_timings timings{};
     specrel_stats stats_host{};
+    int opt_wave_clock = 0;
     int opt_stats = 0, opt_cull = 1, opt_stack = STACK_DEFAULT, opt_persistent = 0, opt_shadow_queue = 0;
     int opt_graph = 1;
     int opt_merged = 0;
@@ -815,6 +819,7 @@ struct Bufs {
     StarDir *dirs = nullptr;
     ShadowQuery *squeue = nullptr;
     unsigned long long *stats = nullptr;
+    specrel_wave_clock *wave_clock = nullptr;  // SPECREL_OPT_WAVE_CLOCK: one record per wave
     uint32_t start(int row, uint32_t view) const { return tab_host[(size_t)row * (n_views + 1) + view]; }
 };
 
@@ -845,6 +850,7 @@ Bufs ctx_bufs(specrel_ctx *ctx) {
     B.dirs = (StarDir *)ctx->dirs.p;
     B.squeue = (ShadowQuery *)ctx->squeue.p;
     B.stats = (unsigned long long *)ctx->stats.p;
+    B.wave_clock = ctx->opt_wave_clock ? (specrel_wave_clock *)ctx->wave_clock.p : nullptr;
     return B;
 }
 
@@ -987,6 +993,10 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ta.cost = order ? B.tile_cost : nullptr;
     ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
     ta.postpone = ctx->opt_postpone == 1 ? (uint32_t)(STACK - 2) : (uint32_t)ctx->opt_postpone;
+    // Per-wave clock records: the tiled kernel only (one record per scheduling slot).
+    const bool tiled = ctx->opt_stack > 0 && !ctx->opt_persistent;
+    ta.clock = tiled ? B.wave_clock : nullptr;
+    ctx->wave_clock_n = ta.clock ? 4ull * B.total_tiles : 0ull;
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(1);
@@ -1106,10 +1116,12 @@ void drop_graph(specrel_ctx *ctx) {
 int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, uint32_t n, bool want_hits) {
     auto t0 = std::chrono::steady_clock::now();
     const bool stats = ctx->opt_stats != 0;
-    const bool use_graph = !stats && (ctx->opt_graph == 2 || (ctx->opt_graph == 1 && n > 1));
+    const bool use_graph =
+        !stats && !ctx->opt_wave_clock && (ctx->opt_graph == 2 || (ctx->opt_graph == 1 && n > 1));
     const Plan pl = prepare(ctx, scene_idx, frames, n, want_hits, n == 1 && !use_graph);
     SceneSlot &sc = ctx->scenes[scene_idx];
     hipStream_t s = ctx->stream;
+    if (ctx->opt_wave_clock) dalloc(ctx->wave_clock, (size_t)ctx->total_tiles * 4 * sizeof(specrel_wave_clock));
     if (stats) {
         check(hipMemsetAsync(ctx->stats.p, 0, 16 * 8, s), "stats clear");
         ctx->touch_words[0] = (sc.num_nodes + 31) / 32;
@@ -1568,6 +1580,8 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         dalloc(ctx->stats, 16 * 8);
         dalloc(ctx->counters, COUNTERS * 4);
         ctx->cus = prop.multiProcessorCount;
+        if (hipDeviceGetAttribute(&ctx->clock_khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess)
+            ctx->clock_khz = 0;
         check(hipStreamSynchronize(ctx->stream), "create");
         return SPECREL_OK;
     });
@@ -1590,7 +1604,7 @@ void specrel_destroy(specrel_ctx *ctx) {
     }
     dfree(ctx->hdr), dfree(ctx->rgb), dfree(ctx->hit), dfree(ctx->lam), dfree(ctx->tau), dfree(ctx->prep);
     dfree(ctx->slot), dfree(ctx->dirs), dfree(ctx->super_lists), dfree(ctx->super_counts);
-    dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue), dfree(ctx->touch);
+    dfree(ctx->counters), dfree(ctx->mask), dfree(ctx->squeue), dfree(ctx->touch), dfree(ctx->wave_clock);
     dfree(ctx->tile_cost), dfree(ctx->tile_order), dfree(ctx->tile_cones);
     dfree(ctx->frames), dfree(ctx->tab), dfree(ctx->tables), dfree(ctx->stats);
     dfree(ctx->ex_tubes), dfree(ctx->ex_tris), dfree(ctx->ex_samples), dfree(ctx->ex_decided);
@@ -1622,6 +1636,7 @@ const char *specrel_device_name(const specrel_ctx *ctx) { return ctx ? ctx->name
 int specrel_set_option(specrel_ctx *ctx, int option, int value) {
     if (!ctx) return SPECREL_EINVAL;
     if (option == SPECREL_OPT_STATS) ctx->opt_stats = value != 0;
+    else if (option == SPECREL_OPT_WAVE_CLOCK) ctx->opt_wave_clock = value != 0;
     else if (option == SPECREL_OPT_STAR_CULL) ctx->opt_cull = value != 0;
     else if (option == SPECREL_OPT_STACK) {
         if (value != 8 && value != 10 && value != 12 && value != 16 && value != -4 && value != -8 && value != -12 &&
@@ -2250,6 +2265,22 @@ int specrel_last_stats(const specrel_ctx *ctx, specrel_stats *out) {
     if (!ctx || !out) return SPECREL_EINVAL;
     *out = ctx->stats_host;
     return SPECREL_OK;
+}
+
+int specrel_last_wave_clock(specrel_ctx *ctx, specrel_wave_clock *out, uint64_t cap, uint64_t *n_waves,
+                            uint32_t *clock_khz) {
+    if (!ctx || !n_waves || !clock_khz || (cap && !out)) return SPECREL_EINVAL;
+    return guarded(ctx, [&] {
+        check(hipSetDevice(ctx->device), "hipSetDevice");
+        check(hipStreamSynchronize(ctx->stream), "wave clock");
+        const uint64_t n = ctx->last_noalloc ? 0ull : ctx->wave_clock_n;
+        *n_waves = n;
+        *clock_khz = (uint32_t)ctx->clock_khz;
+        const uint64_t k = std::min(cap, n);
+        if (k) check(hipMemcpy(out, ctx->wave_clock.p, k * sizeof(specrel_wave_clock), hipMemcpyDeviceToHost),
+                     "wave clock copy");
+        return SPECREL_OK;
+    });
 }
 
 }  // extern "C"

@@ -206,6 +206,7 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
 template <bool STATS, int SK, bool M, bool SH, int W>
 __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
+    const uint64_t clock0 = a.clock ? wall_clock64() : 0ull;
     using SS = StackSel<SK, 64 * W>;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     // The wave's scheduling slot: blocks start in slot order, and the order table puts the tiles
@@ -266,6 +267,16 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) c = max(c, (uint32_t)__shfl_xor((int)c, off));
         if (lane == 0) a.cost[4 * (size_t)t + q] = c;
+    }
+    if (a.clock && lane == 0) {
+        specrel_wave_clock w;
+        w.start = clock0;
+        w.end = wall_clock64();
+        w.tile = t;
+        w.quarter = q;
+        w.hw_id = __builtin_amdgcn_s_getreg((31 << 11) | 4);  // HW_ID, all 32 bits
+        w.xcc = __builtin_amdgcn_s_getreg((31 << 11) | 20);  // XCC_ID
+        a.clock[slot_q] = w;
     }
     if (STATS) {
         __shared__ unsigned long long s_c[4];

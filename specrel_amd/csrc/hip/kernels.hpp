@@ -135,6 +135,7 @@ struct TraceArgs {
     uint32_t prio_slots;    // tiles in the first slots of the order run at a raised wave priority
     uint32_t postpone;      // 0, or uninstrumented closest-hit rays postpone their leaves (trace_pp) and
                             // are traced again by trace() past this many stack entries
+    specrel_wave_clock *clock = nullptr;  // or one record per wave (scheduling slot): start, end, where
 };
 
 struct OccludeArgs {

@@ -23,6 +23,7 @@ OPT_STATS, OPT_STAR_CULL, OPT_STACK, OPT_PERSISTENT, OPT_SHADOW_QUEUE, OPT_GRAPH
 OPT_SHADOW_IN_TRACE, OPT_TRACE_WAVES, OPT_STAR_OVERLAP, OPT_WAVE_SHAPE, OPT_TILE_ORDER = 8, 9, 10, 11, 12
 OPT_LEAF_POSTPONE = 13
 OPT_ADJOINT_MASK = 14
+OPT_WAVE_CLOCK = 15
 # specrel_timings.shadow_path: where the last render traced its shadow queries
 SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 
@@ -52,6 +53,15 @@ class AdjointOut(C.Structure):
 
 class Range(C.Structure):
     _fields_ = [("first", C.c_uint32), ("count", C.c_uint32)]
+
+
+class WaveClock(C.Structure):
+    _fields_ = [("start", C.c_uint64), ("end", C.c_uint64), ("tile", C.c_uint32), ("quarter", C.c_uint32),
+                ("hw_id", C.c_uint32), ("xcc", C.c_uint32)]
+
+
+WAVE_CLOCK_DTYPE = np.dtype([("start", "<u8"), ("end", "<u8"), ("tile", "<u4"), ("quarter", "<u4"),
+                             ("hw_id", "<u4"), ("xcc", "<u4")])
 
 
 class BvhVersion(C.Structure):
@@ -125,6 +135,8 @@ SIGNATURES = {
                                         C.POINTER(C.c_double)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
+    "specrel_last_wave_clock": (C.c_int, [_P, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
+                                          C.POINTER(C.c_uint32)]),
 }
 
 
@@ -426,6 +438,17 @@ class Context:
         s = F.Stats()
         self._check(self.lib.specrel_last_stats(self._h, C.byref(s)))
         return {n: getattr(s, n) for n, _ in F.Stats._fields_ if n != "reserved"}
+
+    def wave_clock(self):
+        """The last render's per-wave records (SPECREL_OPT_WAVE_CLOCK) as a structured array, and
+        the clock rate in kHz."""
+        n, khz = C.c_uint64(0), C.c_uint32(0)
+        self._check(self.lib.specrel_last_wave_clock(self._h, None, 0, C.byref(n), C.byref(khz)))
+        out = np.zeros(n.value, WAVE_CLOCK_DTYPE)
+        if n.value:
+            self._check(self.lib.specrel_last_wave_clock(self._h, out.ctypes.data, n.value, C.byref(n),
+                                                         C.byref(khz)))
+        return out, khz.value
 
 
 @dataclass

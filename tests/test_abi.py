@@ -67,6 +67,8 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_download_linear(None, 0, None, 0) == R.EINVAL
     assert lib.specrel_upload_scene_versioned(None, 0, None, None) == R.EINVAL
     assert lib.specrel_update_scene(None, 0, None, None, None, 0, None, 0, None) == R.EINVAL
+    assert lib.specrel_last_wave_clock(None, None, 0, None, None) == R.EINVAL
+    assert C.sizeof(R.WaveClock) == 32 and R.WAVE_CLOCK_DTYPE.itemsize == 32
 
 
 def test_create_reports_missing_device():

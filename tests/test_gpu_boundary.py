@@ -182,3 +182,30 @@ def test_distinct_records_touched_match_the_oracle(gpu_ctx, twin):
     for k in ("distinct_nodes", "distinct_prims", "distinct_hits", "node_pops", "prim_tests", "shadow_prim_tests"):
         assert st[k] == oc[k], (k, st[k], oc[k])
     assert st["distinct_nodes"] >= 30 and st["distinct_prims"] >= 30 and st["distinct_hits"] >= 30
+
+
+def test_wave_clock_records_every_wave_and_changes_nothing(gpu_ctx, twin):
+    """SPECREL_OPT_WAVE_CLOCK: one record per wave of the tiled primary-ray kernel (each quarter of
+    each tile once), ends after starts, and the image is the uninstrumented one."""
+    a, _ = frames(twin)
+    gpu_ctx.upload(0, twin)
+    plain = gpu_ctx.render_frame(0, a, want_hits=True)
+    gpu_ctx.set_option(R.OPT_WAVE_CLOCK, 1)
+    try:
+        timed = gpu_ctx.render_frame(0, a, want_hits=True)
+        rec, khz = gpu_ctx.wave_clock()
+    finally:
+        gpu_ctx.set_option(R.OPT_WAVE_CLOCK, 0)
+    for k in ("rgb", "hit", "lam", "tau"):
+        assert np.array_equal(plain[k].view(np.uint32), timed[k].view(np.uint32)), k
+    tiles = ((160 + 15) // 16) * ((100 + 15) // 16)
+    assert khz > 0 and len(rec) == 4 * tiles
+    assert np.all(rec["end"] >= rec["start"]) and rec["start"].min() > 0
+    seen = np.zeros((tiles, 4), bool)
+    seen[rec["tile"], rec["quarter"]] = True
+    assert seen.all()
+    span_us = (rec["end"].max() - rec["start"].min()) * 1000.0 / khz
+    assert 0 < span_us < 1e5
+    gpu_ctx.render_frame(0, a)  # without the option the next render records nothing
+    rec2, _ = gpu_ctx.wave_clock()
+    assert len(rec2) == 0
