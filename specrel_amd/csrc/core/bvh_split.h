@@ -69,39 +69,39 @@ SR_HD void bins_clear(Bins &B) {
 
 enum { SPLIT_SAH = 0, SPLIT_MIDPOINT = 1, SPLIT_LEAF = 2 };
 
-// The split of a node with centroid bounds cb and filled bins: SPLIT_SAH or SPLIT_MIDPOINT with
-// (axis, pos), or SPLIT_LEAF.
-SR_HD int choose_split(const Box4 &cb, const BinFrame &f, const Bins &bins, int &best_axis, float &best_pos) {
-    best_axis = 0;
-    best_pos = 0.0f;
-    float best_cost = __builtin_inff();
-    for (int a = 0; a < 4; ++a) {
-        if (!f.live[a]) continue;
-        Box4 left_box[BVH_BINS - 1];
-        uint32_t left_cnt[BVH_BINS - 1];
-        Box4 cur_b = box_empty();
-        uint32_t cur_c = 0;
-        for (int i = 0; i < BVH_BINS - 1; ++i) {
-            cur_b = box_union(cur_b, bins.b[a][i]);
-            cur_c += bins.n[a][i];
-            left_box[i] = cur_b;
-            left_cnt[i] = cur_c;
-        }
-        Box4 right_b = box_empty();
-        uint32_t right_c = 0;
-        for (int i = BVH_BINS - 2; i >= 0; --i) {
-            right_b = box_union(right_b, bins.b[a][i + 1]);
-            right_c += bins.n[a][i + 1];
-            if (left_cnt[i] > 0 && right_c > 0) {
-                float cost = (float)left_cnt[i] * box_measure(left_box[i]) + (float)right_c * box_measure(right_b);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_axis = a;
-                    best_pos = f.axis_min[a] + ((float)(i + 1) / (float)BVH_BINS) * f.ext[a];
-                }
+// One live axis a of the SAH sweep (bvh4d.rs:307-417) over that axis's bins b / n: every split
+// between bins, the cheapest so far kept (strictly cheaper replaces, axes in order 0..3).
+SR_HD void sah_axis(const Box4 (&b)[BVH_BINS], const uint32_t (&n)[BVH_BINS], const BinFrame &f, int a,
+                    float &best_cost, int &best_axis, float &best_pos) {
+    Box4 left_box[BVH_BINS - 1];
+    uint32_t left_cnt[BVH_BINS - 1];
+    Box4 cur_b = box_empty();
+    uint32_t cur_c = 0;
+    for (int i = 0; i < BVH_BINS - 1; ++i) {
+        cur_b = box_union(cur_b, b[i]);
+        cur_c += n[i];
+        left_box[i] = cur_b;
+        left_cnt[i] = cur_c;
+    }
+    Box4 right_b = box_empty();
+    uint32_t right_c = 0;
+    for (int i = BVH_BINS - 2; i >= 0; --i) {
+        right_b = box_union(right_b, b[i + 1]);
+        right_c += n[i + 1];
+        if (left_cnt[i] > 0 && right_c > 0) {
+            float cost = (float)left_cnt[i] * box_measure(left_box[i]) + (float)right_c * box_measure(right_b);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_axis = a;
+                best_pos = f.axis_min[a] + ((float)(i + 1) / (float)BVH_BINS) * f.ext[a];
             }
         }
     }
+}
+
+// After the sweeps (best_cost from them, +inf if none): SPLIT_SAH, else the midpoint of the
+// widest centroid axis (SPLIT_MIDPOINT), or SPLIT_LEAF when every axis is flat.
+SR_HD int split_after_sweeps(const Box4 &cb, const BinFrame &f, float best_cost, int &best_axis, float &best_pos) {
     if (best_cost != __builtin_inff()) return SPLIT_SAH;
     float max_e = -1.0f;
     for (int a = 0; a < 4; ++a) {
@@ -113,6 +113,17 @@ SR_HD int choose_split(const Box4 &cb, const BinFrame &f, const Bins &bins, int 
     if (max_e < 1e-5f) return SPLIT_LEAF;
     best_pos = axis((cb.mn + cb.mx) * 0.5f, best_axis);
     return SPLIT_MIDPOINT;
+}
+
+// The split of a node with centroid bounds cb and filled bins: SPLIT_SAH or SPLIT_MIDPOINT with
+// (axis, pos), or SPLIT_LEAF.
+SR_HD int choose_split(const Box4 &cb, const BinFrame &f, const Bins &bins, int &best_axis, float &best_pos) {
+    best_axis = 0;
+    best_pos = 0.0f;
+    float best_cost = __builtin_inff();
+    for (int a = 0; a < 4; ++a)
+        if (f.live[a]) sah_axis(bins.b[a], bins.n[a], f, a, best_cost, best_axis, best_pos);
+    return split_after_sweeps(cb, f, best_cost, best_axis, best_pos);
 }
 
 }  // namespace sr

@@ -454,12 +454,17 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
     const SmallTask task = tasks[t];
     specrel_node *out = scratch + 2 * (size_t)task.begin;
     const uint32_t cap = 2 * task.count;
+    // A pending range (first reference, count <= SMALL) and the node whose right child it becomes
+    // (a subtree-local index < 2 * SMALL, or NO_PARENT): 8 bytes.
     struct Ent {
-        uint32_t b, c, parent;
+        uint32_t b;
+        uint16_t c, parent;
     };
+    static_assert(SMALL < 0xFFFFu && 2 * SMALL < 0xFFFFu, "Ent fields");
+    constexpr uint16_t NO_PARENT = 0xFFFFu;
     Ent stack[SMALL + 2];  // pending right halves along a path: at most count + 1
     int sp = 0;
-    stack[sp++] = Ent{task.begin, task.count, NONE};
+    stack[sp++] = Ent{task.begin, (uint16_t)task.count, NO_PARENT};
     uint32_t nn = 0;
     while (sp > 0) {
         const Ent e = stack[--sp];
@@ -468,7 +473,7 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
             atomicOr(err, ERR_SMALL_NODES);
             return;
         }
-        if (e.parent != NONE) out[e.parent].right_or_first = me;
+        if (e.parent != NO_PARENT) out[e.parent].right_or_first = me;
         sr::Box4 bounds = sr::box_empty(), cb = sr::box_empty();
 #pragma unroll 4
         for (uint32_t i = e.b; i < e.b + e.c; ++i) {
@@ -483,10 +488,12 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
         float pos = 0.0f;
         if (e.c > 3) {
             const sr::BinFrame fr = sr::bin_frame(cb);
-            sr::Bins B;
-            sr::bins_clear(B);
             // One pass per pair of axes with those axes' 8 bins each in registers (unions and
-            // counts, so the order of the references changes nothing), stored into B once per pass.
+            // counts, so the order of the references changes nothing), each axis then swept for
+            // its cheapest split from the registers (choose_split's sweep, axes in the same order).
+            float best_cost = __builtin_inff();
+            axis = 0;
+#pragma unroll
             for (int a0 = 0; a0 < 4; a0 += 2) {
                 const bool live0 = fr.live[a0], live1 = fr.live[a0 + 1];
                 if (!live0 && !live1) continue;
@@ -524,14 +531,13 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     if (!(h ? live1 : live0)) continue;
+                    sr::Box4 b[8];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        B.b[a0 + h][q] = sr::Box4{v(bmn[h][q]), v(bmx[h][q])};
-                        B.n[a0 + h][q] = bn[h][q];
-                    }
+                    for (int q = 0; q < 8; ++q) b[q] = sr::Box4{v(bmn[h][q]), v(bmx[h][q])};
+                    sr::sah_axis(b, bn[h], fr, a0 + h, best_cost, axis, pos);
                 }
             }
-            kind = sr::choose_split(cb, fr, B, axis, pos);
+            kind = sr::split_after_sweeps(cb, fr, best_cost, axis, pos);
         }
         if (kind == K_LEAF) {
             nd.right_or_first = e.b;
@@ -586,8 +592,8 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
             atomicOr(err, ERR_SMALL_STACK);
             return;
         }
-        stack[sp++] = Ent{e.b + mid, e.c - mid, me};
-        stack[sp++] = Ent{e.b, mid, NONE};
+        stack[sp++] = Ent{e.b + mid, (uint16_t)(e.c - mid), (uint16_t)me};
+        stack[sp++] = Ent{e.b, (uint16_t)mid, NO_PARENT};
     }
     task_nodes[t] = nn;
 }

@@ -178,9 +178,10 @@ class Context:
         self._sizes = []  # (w, h) of every view of the last render, in view order
 
     def close(self):
-        if getattr(self, "_h", None) is not None and self._h.value:
-            self.lib.specrel_destroy(self._h)
-            self._h = C.c_void_p()
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self.lib.specrel_destroy(h)
+            h.value = None  # (not a new c_void_p: at interpreter exit the module may be gone)
 
     def __del__(self):
         self.close()
