@@ -242,6 +242,17 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
  * frame is copied before the call returns (to pinned staging memory, or into the launch's
  * arguments), so the caller may change it at once and enqueue further renders; each render's
  * outputs replace the previous one's. */
+/* GpuRenderer::render (gpu.rs:798-900) fed the reference's own per-view block: `u` is the
+ * GpuUniforms that GpuUniforms::build made (gpu.rs:146-212), passed unchanged, so a Rust caller
+ * hands over what it already builds. Its scene fields (num_nodes, num_stars, light_pos) must be
+ * the uploaded scene's, else SPECREL_EINVAL; `len` as for specrel_render. A flash arrives as
+ * colour x intensity and is rendered as that colour at intensity 1, which multiplies it as the
+ * shader does (render.wgsl:820); the CPU renderer scales the colour by intensity times the
+ * falloff instead (renderer.rs:685), so with flashes the flash term is rounded as the shader's,
+ * not bit for bit as the CPU renderer's. Without flashes it is specrel_render's image. */
+int specrel_render_uniforms(specrel_ctx *ctx, uint32_t scene_idx, const specrel_gpu_uniforms *u, uint32_t *out_rgb,
+                            uint64_t len);
+
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits);
 
 /* No counterpart in the reference. For callers that cannot allocate device memory -- code under

@@ -137,6 +137,29 @@ typedef struct specrel_frame {     /* per-view inputs; replaces GpuUniforms (gpu
     float flash_intensity[SPECREL_MAX_FLASHES];
 } specrel_frame;
 
+/* GpuUniforms (gpu.rs:120-144) byte for byte, 768 B: the reference's own per-view block as
+ * GpuUniforms::build makes it (gpu.rs:146-212), for specrel_render_uniforms. */
+typedef struct specrel_gpu_uniforms {
+    float cam_pos[4];
+    float e[4][4];
+    float light_pos[4];       /* the scene's (PackedSceneGpuData::light_pos) */
+    float half_wh[2];
+    uint32_t size[2];
+    uint32_t mode, shadow_mode, slice_mode;
+    uint32_t num_nodes;       /* the scene's */
+    float frontier_origin[4];
+    float frontier_view_offset;
+    uint32_t num_flashes;
+    float frontier_tau;
+    float prophecy_delta;
+    uint32_t num_haze;
+    uint32_t num_stars;       /* the scene's */
+    uint32_t pad[2];
+    float haze_origin[SPECREL_MAX_HAZE][4];
+    float flash_at[SPECREL_MAX_FLASHES][4];
+    float flash_col_width[SPECREL_MAX_FLASHES][4]; /* colour x intensity, width */
+} specrel_gpu_uniforms;
+
 /* One wave of a render's tiled primary-ray kernel (SPECREL_OPT_WAVE_CLOCK, specrel_last_wave_clock). */
 typedef struct specrel_wave_clock {
     uint64_t start, end;  /* device clock ticks at the wave's first and last instruction */
@@ -180,6 +203,11 @@ static_assert(sizeof(specrel_frame) == 800, "frame layout");
 static_assert(sizeof(specrel_scene_arrays) == 88 && offsetof(specrel_scene_arrays, light_pos) == 72, "arrays");
 static_assert(sizeof(specrel_stats) == 128, "stats layout");
 static_assert(sizeof(specrel_wave_clock) == 32, "wave clock layout");
+static_assert(sizeof(specrel_gpu_uniforms) == 768 && offsetof(specrel_gpu_uniforms, num_nodes) == 124 &&
+                  offsetof(specrel_gpu_uniforms, haze_origin) == 176 &&
+                  offsetof(specrel_gpu_uniforms, flash_at) == 256 &&
+                  offsetof(specrel_gpu_uniforms, flash_col_width) == 512,
+              "GpuUniforms layout");
 #else
 _Static_assert(sizeof(specrel_node) == 48, "GpuNode is 48 B");
 _Static_assert(sizeof(specrel_geom) == 80, "GpuGeom is 80 B");

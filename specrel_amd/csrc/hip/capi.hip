@@ -2029,6 +2029,46 @@ int specrel_render(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fr
     });
 }
 
+int specrel_render_uniforms(specrel_ctx *ctx, uint32_t scene_idx, const specrel_gpu_uniforms *u, uint32_t *out_rgb,
+                            uint64_t len) {
+    if (!ctx) return SPECREL_EINVAL;
+    specrel_frame f;
+    const int rc = guarded(ctx, [&] {
+        if (!u) throw Fail{SPECREL_EINVAL, "uniforms is NULL"};
+        if (scene_idx >= ctx->scenes.size() || !ctx->scenes[scene_idx].used)
+            throw Fail{SPECREL_EINVAL, "no scene uploaded at index " + std::to_string(scene_idx)};
+        const SceneSlot &sc = ctx->scenes[scene_idx];
+        const float4 l = sc.S.light;
+        const float light[4] = {l.x, l.y, l.z, l.w};
+        if (u->num_nodes != sc.num_nodes || u->num_stars != sc.num_stars ||
+            std::memcmp(light, u->light_pos, sizeof light) != 0)
+            throw Fail{SPECREL_EINVAL, "uniforms were built for another scene (num_nodes, num_stars or light_pos "
+                                       "differ from scene " + std::to_string(scene_idx) + "'s)"};
+        if (u->num_flashes > SPECREL_MAX_FLASHES) throw Fail{SPECREL_EINVAL, "num_flashes out of range"};
+        std::memset(&f, 0, sizeof f);
+        std::memcpy(f.cam_pos, u->cam_pos, sizeof f.cam_pos);
+        std::memcpy(f.e, u->e, sizeof f.e);
+        std::memcpy(f.half_wh, u->half_wh, sizeof f.half_wh);
+        std::memcpy(f.size, u->size, sizeof f.size);
+        f.mode = u->mode;
+        f.shadow_mode = u->shadow_mode;
+        f.slice_mode = u->slice_mode;
+        f.num_flashes = u->num_flashes;
+        std::memcpy(f.frontier_origin, u->frontier_origin, sizeof f.frontier_origin);
+        f.frontier_tau = u->frontier_tau;
+        f.frontier_view_offset = u->frontier_view_offset;
+        f.prophecy_delta = u->prophecy_delta;
+        f.num_haze = u->num_haze;
+        std::memcpy(f.haze_origin, u->haze_origin, sizeof f.haze_origin);
+        std::memcpy(f.flash_at, u->flash_at, sizeof f.flash_at);
+        std::memcpy(f.flash_color_width, u->flash_col_width, sizeof f.flash_color_width);
+        for (uint32_t i = 0; i < SPECREL_MAX_FLASHES; ++i) f.flash_intensity[i] = i < u->num_flashes ? 1.0f : 0.0f;
+        return SPECREL_OK;
+    });
+    if (rc != SPECREL_OK) return rc;
+    return specrel_render(ctx, scene_idx, &f, out_rgb, nullptr, nullptr, nullptr, len);
+}
+
 int specrel_render_resident(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frame, int want_hits) {
     if (!ctx) return SPECREL_EINVAL;
     return guarded(ctx, [&] {

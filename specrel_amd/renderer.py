@@ -64,6 +64,44 @@ WAVE_CLOCK_DTYPE = np.dtype([("start", "<u8"), ("end", "<u8"), ("tile", "<u4"), 
                              ("hw_id", "<u4"), ("xcc", "<u4")])
 
 
+class GpuUniforms(C.Structure):
+    """GpuUniforms (gpu.rs:120-144), 768 B: the reference's per-view block."""
+    _fields_ = [("cam_pos", C.c_float * 4), ("e", (C.c_float * 4) * 4), ("light_pos", C.c_float * 4),
+                ("half_wh", C.c_float * 2), ("size", C.c_uint32 * 2), ("mode", C.c_uint32),
+                ("shadow_mode", C.c_uint32), ("slice_mode", C.c_uint32), ("num_nodes", C.c_uint32),
+                ("frontier_origin", C.c_float * 4), ("frontier_view_offset", C.c_float),
+                ("num_flashes", C.c_uint32), ("frontier_tau", C.c_float), ("prophecy_delta", C.c_float),
+                ("num_haze", C.c_uint32), ("num_stars", C.c_uint32), ("pad", C.c_uint32 * 2),
+                ("haze_origin", (C.c_float * 4) * 5), ("flash_at", (C.c_float * 4) * 16),
+                ("flash_col_width", (C.c_float * 4) * 16)]
+
+
+def gpu_uniforms(frame, arrays) -> GpuUniforms:
+    """GpuUniforms::build's block (gpu.rs:146-212) for a frame of specrel_amd.scene.build_frame and
+    a scene's arrays (num_nodes, num_stars, light_pos): the same values, flash colours multiplied
+    by their intensities in f32 as the reference does."""
+    u = GpuUniforms()
+    C.memmove(u.cam_pos, frame.cam_pos, 16)
+    C.memmove(u.e, frame.e, 64)
+    for i in range(4):
+        u.light_pos[i] = arrays.light_pos[i]
+    C.memmove(u.half_wh, frame.half_wh, 8)
+    C.memmove(u.size, frame.size, 8)
+    u.mode, u.shadow_mode, u.slice_mode = frame.mode, frame.shadow_mode, frame.slice_mode
+    u.num_nodes, u.num_stars = arrays.num_nodes, arrays.num_stars
+    C.memmove(u.frontier_origin, frame.frontier_origin, 16)
+    u.frontier_view_offset, u.frontier_tau = frame.frontier_view_offset, frame.frontier_tau
+    u.prophecy_delta, u.num_haze, u.num_flashes = frame.prophecy_delta, frame.num_haze, frame.num_flashes
+    C.memmove(u.haze_origin, frame.haze_origin, 80)
+    C.memmove(u.flash_at, frame.flash_at, 256)
+    for i in range(frame.num_flashes):
+        k = np.float32(frame.flash_intensity[i])
+        for j in range(3):
+            u.flash_col_width[i][j] = float(np.float32(frame.flash_color_width[i][j]) * k)
+        u.flash_col_width[i][3] = frame.flash_color_width[i][3]
+    return u
+
+
 class BvhVersion(C.Structure):
     _fields_ = [("layout", C.c_uint64), ("edit", C.c_uint64)]
 
@@ -135,6 +173,7 @@ SIGNATURES = {
                                         C.POINTER(C.c_double)]),
     "specrel_last_timings": (C.c_int, [_P, C.POINTER(Timings)]),
     "specrel_last_stats": (C.c_int, [_P, C.POINTER(F.Stats)]),
+    "specrel_render_uniforms": (C.c_int, [_P, C.c_uint32, C.c_void_p, _U32P, C.c_uint64]),
     "specrel_last_wave_clock": (C.c_int, [_P, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64),
                                           C.POINTER(C.c_uint32)]),
 }
@@ -296,6 +335,14 @@ class Context:
                                             _ptr(lam, _FP), _ptr(tau, _FP), rgb.size))
         self._sizes = [(w, h)]
         return self._result(w, h, rgb, hit, lam, tau)
+
+    def render_uniforms(self, idx, u, rgb=None):
+        """specrel_render_uniforms: one view from the reference's GpuUniforms block; the RGB8 image."""
+        w, h = u.size[0], u.size[1]
+        rgb = np.zeros(w * h, np.uint32) if rgb is None else rgb
+        self._check(self.lib.specrel_render_uniforms(self._h, idx, C.byref(u), _ptr(rgb, _U32P), rgb.size))
+        self._sizes = [(w, h)]
+        return rgb.reshape(h, w)
 
     def render_resident(self, idx, frame, want_hits=False):
         self._check(self.lib.specrel_render_resident(self._h, idx, C.byref(frame), int(want_hits)))

@@ -69,6 +69,12 @@ def test_argument_errors_are_codes_not_aborts():
     assert lib.specrel_update_scene(None, 0, None, None, None, 0, None, 0, None) == R.EINVAL
     assert lib.specrel_last_wave_clock(None, None, 0, None, None) == R.EINVAL
     assert C.sizeof(R.WaveClock) == 32 and R.WAVE_CLOCK_DTYPE.itemsize == 32
+    assert lib.specrel_render_uniforms(None, 0, None, None, 0) == R.EINVAL
+    # GpuUniforms (gpu.rs:120-144): 768 B, fields where the reference's #[repr(C)] puts them
+    assert C.sizeof(R.GpuUniforms) == 768
+    for name, off in (("light_pos", 80), ("num_nodes", 124), ("frontier_origin", 128), ("num_stars", 164),
+                      ("haze_origin", 176), ("flash_at", 256), ("flash_col_width", 512)):
+        assert getattr(R.GpuUniforms, name).offset == off, name
 
 
 def test_create_reports_missing_device():

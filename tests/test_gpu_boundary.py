@@ -209,3 +209,44 @@ def test_wave_clock_records_every_wave_and_changes_nothing(gpu_ctx, twin):
     gpu_ctx.render_frame(0, a)  # without the option the next render records nothing
     rec2, _ = gpu_ctx.wave_clock()
     assert len(rec2) == 0
+
+
+def test_render_from_the_reference_uniform_block(gpu_ctx, twin):
+    """specrel_render_uniforms with GpuUniforms::build's 768-B block: without flashes the image is
+    specrel_render's (and the oracle's); with flashes it is the render of the same frame with each
+    flash colour pre-multiplied by its intensity at intensity 1 (the shader's product), checked
+    against the oracle on that frame; a block built for another scene is refused."""
+    gpu_ctx.upload(0, twin)
+    arrays = twin.arrays
+    a, b = frames(twin)
+    for fr in (a, b):
+        u = R.gpu_uniforms(fr, arrays)
+        got = gpu_ctx.render_uniforms(0, u)
+        assert np.array_equal(got, gpu_ctx.render_frame(0, fr)["rgb"])
+        assert np.array_equal(got, O.render(twin, fr)["rgb"])
+    cam = twin.default_camera()
+    p = cam.position
+    at = lambda r: (p[0] - 2.0 * r, p[1], p[2], p[3])
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent(at(3.0), 1.2, 2.0, (1.0, 0.3, 0.2)),
+                                   S.FlashEvent(at(6.0), 1.5, 3.0, (0.2, 0.6, 1.0))])
+    fr = S.build_frame(cam, S.RenderConfig(192, 120, S.RelativisticShaded, S.ShadowLorentzInvariant,
+                                           S.PastLightCone), dyn)
+    assert fr.num_flashes == 2
+    u = R.gpu_uniforms(fr, arrays)
+    got = gpu_ctx.render_uniforms(0, u)
+    pre = S.build_frame(cam, S.RenderConfig(192, 120, S.RelativisticShaded, S.ShadowLorentzInvariant,
+                                            S.PastLightCone), dyn)
+    for i in range(pre.num_flashes):
+        for j in range(3):
+            pre.flash_color_width[i][j] = u.flash_col_width[i][j]
+        pre.flash_intensity[i] = 1.0
+    assert np.array_equal(got, O.render(twin, pre)["rgb"])
+    assert np.array_equal(got, gpu_ctx.render_frame(0, pre)["rgb"])
+    bad = R.gpu_uniforms(a, arrays)
+    bad.num_nodes += 1
+    with pytest.raises(SpecrelError) as e:
+        gpu_ctx.render_uniforms(0, bad)
+    assert e.value.code == R.EINVAL and "another scene" in str(e.value)
+    short = np.zeros(160 * 100 - 1, np.uint32)
+    with pytest.raises(SpecrelError):
+        gpu_ctx.render_uniforms(0, R.gpu_uniforms(a, arrays), rgb=short)
