@@ -216,6 +216,34 @@ struct LdsStack {
         e = v.x;
         t = __uint_as_float(v.y);
     }
+    // Branch-free forms for loops that keep the lane's state in registers: put() writes (e, t)
+    // at position n without pushing it, grow(b) then pushes it when b; pop_if(b) pops when b and
+    // leaves e, t as they were otherwise. Every lane touches a position that exists (n < STACK
+    // for put, 0 for lanes that do not pop).
+    __device__ __forceinline__ void put(uint32_t e, float t) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(n >= K) == 0, 1) || n < K) {
+            lds_uint2 *p = lds + (uint32_t)n * stride;
+            p->x = e;
+            p->y = __float_as_uint(t);
+        } else {
+            deep[n - K] = make_uint2(e, __float_as_uint(t));
+        }
+    }
+    __device__ __forceinline__ void grow(bool b) { n += b ? 1 : 0; }
+    __device__ __forceinline__ void pop_if(bool b, uint32_t &e, float &t) {
+        const int i = b ? n - 1 : 0;
+        uint2 v;
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(i >= K) == 0, 1) || i < K) {
+            const volatile lds_uint2 *p = lds + (uint32_t)i * stride;
+            v.x = p->x;
+            v.y = p->y;
+        } else {
+            v = deep[i - K];
+        }
+        e = b ? v.x : e;
+        t = b ? __uint_as_float(v.y) : t;
+        n -= b ? 1 : 0;
+    }
 };
 
 template <int N>
@@ -251,6 +279,15 @@ struct RegStack {
             e_[N - 1] = v.x;
             t_[N - 1] = __uint_as_float(v.y);
         }
+    }
+    uint32_t put_e_;
+    float put_t_;
+    __device__ __forceinline__ void put(uint32_t e, float t) { put_e_ = e, put_t_ = t; }
+    __device__ __forceinline__ void grow(bool b) {
+        if (b) push(put_e_, put_t_);
+    }
+    __device__ __forceinline__ void pop_if(bool b, uint32_t &e, float &t) {
+        if (b) pop(e, t);
     }
 };
 
@@ -411,7 +448,16 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
 // whose drop rule depends on the exact depth), `redo` is set and the caller traces the ray with
 // trace(). Counters are not the reference's (more nodes are visited); instrumented renders use
 // trace().
+//
+// The lane's state lives in registers as values, not as flags: the entry at the cursor (e, or
+// NO_ENTRY), the pending leaf (pe, or 0: a leaf entry always has its top bit set) and the stack
+// size. Each step recomputes its conditions from them, pops and pushes by selects around one
+// unconditional stack access, and a lane past the stack limit ends by emptying its state. Only
+// the leaf round and the interior-node step are branches, so the wave's exec-mask bookkeeping
+// per step stays small.
 constexpr int POSTPONE_EIGHTHS = 3;  // leaf rounds start once 3/8 of a wave's tracing lanes hold a leaf
+constexpr uint32_t NO_ENTRY = 0x7FFFFFFFu;  // neither a leaf (top bit clear) nor a node (uploads keep
+                                            // node indices below 2^30)
 template <class ST>
 __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk,
                          int limit, bool &redo) {
@@ -421,30 +467,22 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float t_enter;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
-    uint32_t e = S.root_entry, pe = 0;
+    uint32_t e = S.root_entry, pe = 0u;
     float pt = 0.0f;
     stk.reset();
-    bool have = true, done = false, pending = false;
     while (true) {
-        if (!have && !done) {
-            if (stk.size() == 0) done = true;
-            else {
-                stk.pop(e, t_enter);
-                have = true;
-            }
-        }
-        if (done && !pending) break;
+        stk.pop_if(e == NO_ENTRY && stk.size() > 0, e, t_enter);
+        if (e == NO_ENTRY && pe == 0u) break;  // stack empty, nothing at the cursor or pending
         C.steps++;
-        if (have && (e & LEAF) && !pending) {  // the next leaf in order becomes the pending one
-            pe = e;
-            pt = t_enter;
-            pending = true;
-            have = false;
-        }
+        // The next leaf in order becomes the pending one.
+        const bool to_pending = (e & LEAF) != 0u && pe == 0u;
+        pe = to_pending ? e : pe;
+        pt = to_pending ? t_enter : pt;
+        e = to_pending ? NO_ENTRY : e;
         const int tracing = __popcll(__builtin_amdgcn_ballot_w64(true));
-        const int holding = __popcll(__builtin_amdgcn_ballot_w64(pending));
+        const int holding = __popcll(__builtin_amdgcn_ballot_w64(pe != 0u));
         if (8 * holding >= POSTPONE_EIGHTHS * tracing) {  // a leaf round: the lanes holding a leaf test it
-            if (pending && !(pt >= tmax)) {
+            if (pe != 0u && !(pt >= tmax)) {
                 uint32_t first, count;
                 leaf_range(S, pe, first, count);
                 const uint32_t end = first + count;
@@ -456,29 +494,35 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
                     leaf_tests<false>(x0, x1, x2, i, end, tmin, tmax, hit, C);
                 }
             }
-            pending = false;
-        } else if (have && !(e & LEAF)) {  // an interior node at the cursor
-            have = false;
-            if (!(t_enter >= tmax)) {
-                const int sz = stk.size();
-                if (sz + 2 > limit) {
-                    redo = true;
-                    return false;
-                }
+            pe = 0u;
+        } else {
+            // An interior node at the cursor, unless the running t_max prunes it: its two children.
+            const bool node = (e & LEAF) == 0u && e != NO_ENTRY;
+            if (node && !(t_enter >= tmax)) {
+                const bool over = stk.size() + 2 > limit;  // past the limit: traced again by trace()
                 const NodeData D = load_node(S, e);
                 float tl, tr;
-                const bool hl = box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl);
-                const bool hr = box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr);
+                uint32_t hits = (box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl) ? 1u : 0u) |
+                                (box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr) ? 2u : 0u);
+                // Both tests settled before the stack write's branch: a min/max operand defined
+                // in another block is canonicalized again there (one more instruction each).
+                asm volatile("" : "+v"(hits), "+v"(tl), "+v"(tr));
+                const bool hl = (hits & 1u) != 0u, hr = (hits & 2u) != 0u;
                 const bool both = hl && hr;
                 const bool near_left = tl <= tr;
-                if (both) stk.push(near_left ? D.info.y : D.info.x, near_left ? tr : tl);
+                // Both hit: the far one pushed, the near one (ties to the left) continued; one
+                // hit: continued.
+                stk.put(near_left ? D.info.y : D.info.x, near_left ? tr : tl);
+                stk.grow(both && !over);
                 const bool go_left = both ? near_left : (hl && !hr);
                 const bool go_right = both ? !near_left : (hr && !hl);
-                if (go_left || go_right) {
-                    e = go_left ? D.info.x : D.info.y;
-                    t_enter = go_left ? tl : tr;
-                    have = true;
-                }
+                e = (go_left || go_right) && !over ? (go_left ? D.info.x : D.info.y) : NO_ENTRY;
+                t_enter = go_left ? tl : tr;
+                redo = redo || over;
+                pe = over ? 0u : pe;
+                if (over) stk.reset();
+            } else {
+                e = node ? NO_ENTRY : e;
             }
         }
     }
@@ -492,24 +536,25 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float en;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, en)) return false;
+    // The lane's state as values (see trace_pp): the entry at the cursor or NO_ENTRY, and whether
+    // a primitive was hit, after which the lane ends.
     uint32_t e = S.root_entry;
+    bool occ = false;
     stk.reset();
-    bool have = true;
     while (true) {
-        if (!have) {
-            if (stk.size() == 0) break;
-            float unused;
-            stk.pop(e, unused);
-        }
-        have = false;
+        float unused = 0.0f;
+        stk.pop_if(e == NO_ENTRY && stk.size() > 0, e, unused);
+        if (e == NO_ENTRY) break;
         C.steps++;
         if (STATS) C.shadow_node_pops++;
-        if (e & LEAF) {
+        const uint32_t cur = e;
+        e = NO_ENTRY;
+        if (cur & LEAF) {
             uint32_t first, count;
-            leaf_range(S, e, first, count);
+            leaf_range(S, cur, first, count);
             // Up to three primitives per round of loads, tested in slot order (as in trace()).
             const uint32_t end = first + count;
-            for (uint32_t i = first; i < end; i += 3) {
+            for (uint32_t i = first; i < end && !occ; i += 3) {
                 const uint32_t last = end - 1, i1 = min(i + 1, last), i2 = min(i + 2, last);
                 const bool h0 = isect_accept(isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d), tmin, tmax);
                 const bool h1 = isect_accept(isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d), tmin, tmax);
@@ -517,28 +562,25 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
                 const bool hs[3] = {h0, h1, h2};
 #pragma unroll
                 for (uint32_t k = 0; k < 3; ++k) {
-                    if (i + k >= end) break;
+                    if (i + k >= end || occ) break;
                     if (STATS) C.shadow_prim_tests++;
                     if (STATS) touch(S.touch_prims, i + k);
-                    if (hs[k]) return true;
+                    occ = hs[k];
                 }
             }
+            if (occ) stk.reset();
         } else {
-            const float4 *bx = S.nbox + 4 * (size_t)e;
-            uint2 info = S.ninfo[e];
+            const NodeData D = load_node(S, cur);
             float t;
             if (STATS) C.shadow_box_tests += 2;
-            if (STATS) touch(S.touch_nodes, e);
-            bool hl = box_hit(bx[0], bx[1], o, inv, tmin, tmax, t);
-            bool hr = box_hit(bx[2], bx[3], o, inv, tmin, tmax, t);
-            if (hr && stk.size() < STACK) stk.push(info.y, 0.0f);
-            if (hl && stk.size() < STACK) {
-                e = info.x;
-                have = true;
-            }
+            if (STATS) touch(S.touch_nodes, cur);
+            const bool hl = box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, t);
+            const bool hr = box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, t);
+            if (hr && stk.size() < STACK) stk.push(D.info.y, 0.0f);
+            e = hl && stk.size() < STACK ? D.info.x : NO_ENTRY;
         }
     }
-    return false;
+    return occ;
 }
 
 // HitRecord4D via Simplex4D::interpolate_hit (mesh4d.rs:382-425)

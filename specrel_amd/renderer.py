@@ -180,7 +180,8 @@ SIGNATURES = {
 
 
 def hip_lib_path():
-    return os.path.join(F.LIB_DIR, "libspecrel_hip.so")
+    """The in-tree build, or the library SPECREL_HIP_LIB names (another build to compare against)."""
+    return os.environ.get("SPECREL_HIP_LIB") or os.path.join(F.LIB_DIR, "libspecrel_hip.so")
 
 
 def hip_lib():
