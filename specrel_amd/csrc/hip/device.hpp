@@ -221,7 +221,11 @@ struct LdsStack {
     // leaves e, t as they were otherwise. Every lane touches a position that exists (n < STACK
     // for put, 0 for lanes that do not pop).
     __device__ __forceinline__ void put(uint32_t e, float t) {
-        if (__builtin_expect(__builtin_amdgcn_ballot_w64(n >= K) == 0, 1) || n < K) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(n >= K) == 0, 1)) {  // every lane in LDS
+            lds_uint2 *p = lds + (uint32_t)n * stride;
+            p->x = e;
+            p->y = __float_as_uint(t);
+        } else if (n < K) {
             lds_uint2 *p = lds + (uint32_t)n * stride;
             p->x = e;
             p->y = __float_as_uint(t);
@@ -233,7 +237,11 @@ struct LdsStack {
     __device__ __forceinline__ void pop_if(bool b, uint32_t &e, float &t) {
         const int i = b ? n - 1 : 0;
         uint2 v;
-        if (__builtin_expect(__builtin_amdgcn_ballot_w64(i >= K) == 0, 1) || i < K) {
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(i >= K) == 0, 1)) {  // every lane in LDS
+            const volatile lds_uint2 *p = lds + (uint32_t)i * stride;
+            v.x = p->x;
+            v.y = p->y;
+        } else if (i < K) {
             const volatile lds_uint2 *p = lds + (uint32_t)i * stride;
             v.x = p->x;
             v.y = p->y;
@@ -458,6 +466,19 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
 constexpr int POSTPONE_EIGHTHS = 3;  // leaf rounds start once 3/8 of a wave's tracing lanes hold a leaf
 constexpr uint32_t NO_ENTRY = 0x7FFFFFFFu;  // neither a leaf (top bit clear) nor a node (uploads keep
                                             // node indices below 2^30)
+// One leaf's primitives i .. end-1 (up to three loaded), as leaf_tests but keeping only the winning
+// slot and the running t_max; the winner's event and barycentrics are recomputed once at the end.
+__device__ __forceinline__ void leaf_tests_slot(const Isect &x0, const Isect &x1, const Isect &x2, uint32_t i,
+                                                uint32_t end, float tmin, float &tmax, int32_t &slot) {
+    const Isect *xs[3] = {&x0, &x1, &x2};
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+        const bool acc = i + k < end && isect_accept(*xs[k], tmin, tmax);
+        tmax = acc ? xs[k]->lambda : tmax;
+        slot = acc ? (int32_t)(i + k) : slot;
+    }
+}
+
 template <class ST>
 __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk,
                          int limit, bool &redo) {
@@ -469,6 +490,7 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
     uint32_t e = S.root_entry, pe = 0u;
     float pt = 0.0f;
+    int32_t slot = -1;
     stk.reset();
     while (true) {
         stk.pop_if(e == NO_ENTRY && stk.size() > 0, e, t_enter);
@@ -491,42 +513,49 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
                     const Isect x0 = isect_values(load_geom(S.geoms + 5 * (size_t)i), o, d);
                     const Isect x1 = isect_values(load_geom(S.geoms + 5 * (size_t)i1), o, d);
                     const Isect x2 = isect_values(load_geom(S.geoms + 5 * (size_t)i2), o, d);
-                    leaf_tests<false>(x0, x1, x2, i, end, tmin, tmax, hit, C);
+                    leaf_tests_slot(x0, x1, x2, i, end, tmin, tmax, slot);
                 }
             }
             pe = 0u;
         } else {
             // An interior node at the cursor, unless the running t_max prunes it: its two children.
-            const bool node = (e & LEAF) == 0u && e != NO_ENTRY;
+            const uint32_t cur = e;
+            const bool node = (cur & LEAF) == 0u && cur != NO_ENTRY;
+            e = node ? NO_ENTRY : e;
             if (node && !(t_enter >= tmax)) {
                 const bool over = stk.size() + 2 > limit;  // past the limit: traced again by trace()
-                const NodeData D = load_node(S, e);
+                const NodeData D = load_node(S, cur);
                 float tl, tr;
                 uint32_t hits = (box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl) ? 1u : 0u) |
                                 (box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr) ? 2u : 0u);
-                // Both tests settled before the stack write's branch: a min/max operand defined
-                // in another block is canonicalized again there (one more instruction each).
+                // Both tests settled before the stack write's branch (a min/max operand defined in
+                // another block is canonicalized again there), and kept as an integer so that the
+                // choice below stays selects.
                 asm volatile("" : "+v"(hits), "+v"(tl), "+v"(tr));
-                const bool hl = (hits & 1u) != 0u, hr = (hits & 2u) != 0u;
-                const bool both = hl && hr;
-                const bool near_left = tl <= tr;
                 // Both hit: the far one pushed, the near one (ties to the left) continued; one
-                // hit: continued.
+                // hit: continued. go: 1 left, 2 right, 0 neither.
+                const bool near_left = tl <= tr;
+                const uint32_t go = hits == 3u ? (near_left ? 1u : 2u) : hits;
                 stk.put(near_left ? D.info.y : D.info.x, near_left ? tr : tl);
-                stk.grow(both && !over);
-                const bool go_left = both ? near_left : (hl && !hr);
-                const bool go_right = both ? !near_left : (hr && !hl);
-                e = (go_left || go_right) && !over ? (go_left ? D.info.x : D.info.y) : NO_ENTRY;
-                t_enter = go_left ? tl : tr;
-                redo = redo || over;
-                pe = over ? 0u : pe;
-                if (over) stk.reset();
-            } else {
-                e = node ? NO_ENTRY : e;
+                stk.grow(hits == 3u && !over);
+                e = go != 0u && !over ? (go == 1u ? D.info.x : D.info.y) : NO_ENTRY;
+                t_enter = go == 1u ? tl : tr;
+                if (over) {
+                    redo = true;
+                    pe = 0u;
+                    stk.reset();
+                }
             }
         }
     }
-    return hit.slot >= 0;
+    hit.slot = slot;
+    if (slot >= 0) {  // the winner's values, recomputed by the same operations
+        const Isect x = isect_values(load_geom(S.geoms + 5 * (size_t)slot), o, d);
+        hit.lambda = x.lambda;
+        hit.event = x.event;
+        isect_bary(x, hit.bary);
+    }
+    return slot >= 0;
 }
 
 // Bvh4D::is_occluded (bvh4d.rs:523-578): right pushed, then left pushed and popped at once.
