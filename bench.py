@@ -402,6 +402,10 @@ def main():
     ap.add_argument("--trace-waves", type=int, default=0, help="1, 2, 4: waves per primary-ray block (0: default)")
     ap.add_argument("--shadow-trace", type=int, default=-1,
                     help="1/0: shadow queries traced by the primary-ray kernel (-1: default)")
+    ap.add_argument("--node-width", type=int, default=0,
+                    help="2 or 4: BVH node fetch width of the primary trace (0: default)")
+    ap.add_argument("--lds-top", type=int, default=-1,
+                    help="0..8: top tree levels each primary-ray block reads from LDS (-1: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -417,7 +421,7 @@ def main():
     from specrel_amd import scene as S
     from specrel_amd.renderer import (OPT_PERSISTENT, OPT_MERGED_LOADS, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
                                       OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE,
-                                      SHADOWS_IN_TRACE,
+                                      OPT_NODE_WIDTH, OPT_LDS_TOP, SHADOWS_IN_TRACE,
                                       Context)
 
     w, h = args.width, args.height
@@ -447,6 +451,10 @@ def main():
         ctx.set_option(OPT_STAR_OVERLAP, args.star_overlap)
     if args.wave_shape >= 0:
         ctx.set_option(OPT_WAVE_SHAPE, args.wave_shape)
+    if args.node_width:
+        ctx.set_option(OPT_NODE_WIDTH, args.node_width)
+    if args.lds_top >= 0:
+        ctx.set_option(OPT_LDS_TOP, args.lds_top)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")

@@ -48,7 +48,17 @@ typedef struct specrel_timings {
     uint32_t shadow_path; /* where the last render traced its shadow queries: SPECREL_SHADOWS_* */
     float extrude_ms;     /* the last specrel_extrude_tubes' kernels */
     uint32_t trace_block_waves; /* waves per block the last render's primary-ray kernel ran with */
+    uint32_t trace_variant;     /* traversal the last render's primary-ray kernel ran: SPECREL_TRACE_* */
 } specrel_timings;
+
+/* Traversal variants of the primary-ray kernel (specrel_timings.trace_variant); every one visits
+ * the reference's sequence of nodes and leaves, so results do not depend on it. */
+enum {
+    SPECREL_TRACE_PLAIN = 0,  /* one round of loads per step */
+    SPECREL_TRACE_MERGED = 1, /* SPECREL_OPT_MERGED_LOADS */
+    SPECREL_TRACE_WIDE4 = 2,  /* SPECREL_OPT_NODE_WIDTH 4 */
+    SPECREL_TRACE_LDS_TOP = 3 /* SPECREL_OPT_LDS_TOP */
+};
 
 enum {
     SPECREL_SHADOWS_NONE = 0,     /* no shaded view with shadows on */
@@ -100,10 +110,19 @@ enum {
                                      every hit inside its tetrahedron whose neighbours are on the same
                                      object (silhouettes masked); 1 also no facet, texture or shadow
                                      edge among the neighbours (the set finite differences resolve) */
-    SPECREL_OPT_WAVE_CLOCK = 15   /* 1: each wave of the tiled primary-ray kernel records when it
+    SPECREL_OPT_WAVE_CLOCK = 15,  /* 1: each wave of the tiled primary-ray kernel records when it
                                      started and ended (the device's constant-rate clock) and where it
                                      ran; read with specrel_last_wave_clock. Renders do not replay
                                      graphs meanwhile. Results never depend on it */
+    SPECREL_OPT_NODE_WIDTH = 16,  /* BVH node fetch width of the postponed-leaf traversal (tiled primary
+                                     rays, LDS stack of 8 or 16): 2 (default) one child box pair per
+                                     step; 4: a step also fetches both children's box pairs, so a
+                                     continued child's boxes are in registers at its step. Same
+                                     visiting order. */
+    SPECREL_OPT_LDS_TOP = 17      /* 0 (default), or 1..8: each block of the tiled primary-ray kernel
+                                     (LDS stack of 8 or 16) copies the interior nodes of that many top
+                                     levels of the tree into LDS and reads them there (takes
+                                     precedence over SPECREL_OPT_NODE_WIDTH). Same visiting order. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the
@@ -130,7 +149,8 @@ typedef struct specrel_tuning {
     int32_t wave_shape;      /* SPECREL_OPT_WAVE_SHAPE */
     int32_t leaf_postpone;   /* SPECREL_OPT_LEAF_POSTPONE */
     int32_t trace_waves;     /* SPECREL_OPT_TRACE_WAVES */
-    uint32_t reserved[1];
+    int32_t node_width;      /* SPECREL_OPT_NODE_WIDTH */
+    int32_t lds_top;         /* SPECREL_OPT_LDS_TOP */
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
@@ -363,5 +383,7 @@ int specrel_last_wave_clock(specrel_ctx *ctx, specrel_wave_clock *out, uint64_t 
 
 #ifdef __cplusplus
 }
+static_assert(sizeof(specrel_timings) == 56 && offsetof(specrel_timings, trace_variant) == 52, "timings layout");
+static_assert(sizeof(specrel_tuning) == 52 && offsetof(specrel_tuning, lds_top) == 48, "tuning layout");
 #endif
 #endif

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../core/scene_file.hpp"
@@ -66,6 +67,10 @@ struct SceneSlot {
     // The BVH version (bvh4d.rs:47-61) the device arrays hold, when the caller gave one.
     bool versioned = false;
     uint64_t layout = 0, edit = 0;
+    // The tree's top levels for kernels that copy them into LDS (Scene::top_idx / top_info):
+    // top_count[l] = interior nodes of depth below l, in breadth-first order.
+    DevBuf top_idx, top_info;
+    uint32_t top_count[TOP_LEVELS_MAX + 1] = {};
 };
 
 struct Fail {
@@ -137,6 +142,8 @@ struct specrel_ctx {
     int opt_merged = 0;
     int opt_shadow_trace = 1;
     int opt_trace_waves = 1;
+    int opt_node_width = 2;  // SPECREL_OPT_NODE_WIDTH
+    int opt_lds_top = 0;     // SPECREL_OPT_LDS_TOP: top levels copied into LDS (0: none)
     int opt_wave_shape = 0;
     int opt_tile_order = 1;
     int opt_postpone = 1;
@@ -293,6 +300,49 @@ void check_stars(const specrel_scene_arrays *a) {
     }
 }
 
+// The interior nodes of the top TOP_LEVELS_MAX levels of the uploaded tree, breadth first, from
+// the entries as uploaded (ninfo_host, the root entry): their node indices, and their child
+// entries with those naming another top node rewritten as TOP_BIT | its position.
+void build_top(SceneSlot &slot) {
+    std::vector<uint32_t> idx;
+    std::vector<uint2> info;
+    const uint32_t root = slot.S.root_entry;
+    int l = 0;
+    if (!slot.S.empty && !(root & LEAF)) {
+        std::vector<uint32_t> level{root}, next;
+        for (; l < TOP_LEVELS_MAX && !level.empty(); ++l) {
+            slot.top_count[l] = (uint32_t)idx.size();
+            next.clear();
+            for (uint32_t j : level) {
+                idx.push_back(j);
+                const uint2 in = slot.ninfo_host[j];
+                if (!(in.x & LEAF)) next.push_back(in.x);
+                if (!(in.y & LEAF)) next.push_back(in.y);
+            }
+            level.swap(next);
+        }
+        std::unordered_map<uint32_t, uint32_t> pos;
+        for (uint32_t k = 0; k < idx.size(); ++k) pos[idx[k]] = k;
+        auto enc = [&](uint32_t e) {
+            if (e & LEAF) return e;
+            auto it = pos.find(e);
+            return it == pos.end() ? e : (TOP_BIT | it->second);
+        };
+        for (uint32_t j : idx) info.push_back(make_uint2(enc(slot.ninfo_host[j].x), enc(slot.ninfo_host[j].y)));
+    }
+    for (; l <= TOP_LEVELS_MAX; ++l) slot.top_count[l] = (uint32_t)idx.size();  // levels past the tree's depth
+    dalloc(slot.top_idx, std::max<size_t>(1, idx.size()) * 4);
+    dalloc(slot.top_info, std::max<size_t>(1, info.size()) * 8);
+    if (!idx.empty()) {
+        check(hipMemcpy(slot.top_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice), "top levels");
+        check(hipMemcpy(slot.top_info.p, info.data(), info.size() * 8, hipMemcpyHostToDevice), "top levels");
+    }
+    slot.S.top_idx = (const uint32_t *)slot.top_idx.p;
+    slot.S.top_info = (const uint2 *)slot.top_info.p;
+    slot.S.top_n = 0;
+    slot.S.top_root = TOP_BIT;
+}
+
 // Node j's two child boxes (left min, left max, right min, right max) and entries.
 void node_row(const specrel_scene_arrays *a, uint32_t j, float4 *row, uint2 &info, std::vector<uint2> &big) {
     const specrel_node &n = a->nodes[j];
@@ -369,6 +419,7 @@ void build_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_a
     slot.ninfo_host.swap(ninfo);
     slot.big_host.swap(big);
     slot.used = true;
+    build_top(slot);
 }
 
 // The encoded child entry of node j of `a` as it stands, compared with what was uploaded
@@ -605,6 +656,7 @@ bool update_device_scene(specrel_ctx *ctx, SceneSlot &slot, const specrel_scene_
     slot.num_materials = a->num_materials;
     slot.num_stars = a->num_stars;
     slot.num_star_words = a->num_star_words;
+    build_top(slot);
     return true;
 }
 
@@ -722,27 +774,42 @@ dim3 resident_grid(specrel_ctx *ctx, Kern kern) {
     return dim3((unsigned)(per_cu * ctx->cus));
 }
 
+// The traversal variant the primary-ray kernel runs (specrel_timings.trace_variant): merged load
+// rounds when asked for; else, for tiled rays with an LDS stack of 8 or 16 positions, the top
+// levels from LDS (SPECREL_OPT_LDS_TOP) or the width-4 fetch of postponed-leaf traversal
+// (SPECREL_OPT_NODE_WIDTH); plain rounds otherwise. Instrumented renders run plain or merged
+// rounds (the variants visit the same sequence).
+int trace_variant(const specrel_ctx *ctx, bool stats) {
+    if (ctx->opt_persistent || ctx->opt_stack <= 0) return TV_PLAIN;
+    if (ctx->opt_merged) return TV_MERGED;
+    if (stats || (ctx->opt_stack != 8 && ctx->opt_stack != 16)) return TV_PLAIN;
+    if (ctx->opt_lds_top > 0) return TV_LDSTOP;
+    if (ctx->opt_node_width == 4 && ctx->opt_postpone) return TV_WIDE4;
+    return TV_PLAIN;
+}
+
 // Primary rays with an LDS stack in blocks of 1, 2 or 4 waves (SPECREL_OPT_TRACE_WAVES), each wave
-// one quarter of a 16x16 tile; M: merged load rounds; SH: shadow queries traced here.
-template <int K, bool M, bool SH>
+// one quarter of a 16x16 tile; TV: traversal variant; SH: shadow queries traced here.
+template <int K, int TV, bool SH>
 void launch_lds_w(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
     const int w = ctx->opt_trace_waves;
     const dim3 g(tiles.x * 4 / w);
-    if (w == 1) {
-        if (stats) k_trace<true, K, M, SH, 1><<<g, 64, 0, s>>>(ta);
-        else k_trace<false, K, M, SH, 1><<<g, 64, 0, s>>>(ta);
-    } else if (w == 2) {
-        if (stats) k_trace<true, K, M, SH, 2><<<g, 128, 0, s>>>(ta);
-        else k_trace<false, K, M, SH, 2><<<g, 128, 0, s>>>(ta);
-    } else {
-        if (stats) k_trace<true, K, M, SH, 4><<<g, 256, 0, s>>>(ta);
-        else k_trace<false, K, M, SH, 4><<<g, 256, 0, s>>>(ta);
+    if constexpr (TV == TV_PLAIN || TV == TV_MERGED) {
+        if (stats) {
+            if (w == 1) k_trace<true, K, TV, SH, 1><<<g, 64, 0, s>>>(ta);
+            else if (w == 2) k_trace<true, K, TV, SH, 2><<<g, 128, 0, s>>>(ta);
+            else k_trace<true, K, TV, SH, 4><<<g, 256, 0, s>>>(ta);
+            return;
+        }
     }
+    if (w == 1) k_trace<false, K, TV, SH, 1><<<g, 64, 0, s>>>(ta);
+    else if (w == 2) k_trace<false, K, TV, SH, 2><<<g, 128, 0, s>>>(ta);
+    else k_trace<false, K, TV, SH, 4><<<g, 256, 0, s>>>(ta);
 }
-template <int K, bool M>
+template <int K, int TV>
 void launch_lds(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
-    if (ta.mask) launch_lds_w<K, M, true>(ctx, stats, tiles, s, ta);
-    else launch_lds_w<K, M, false>(ctx, stats, tiles, s, ta);
+    if (ta.mask) launch_lds_w<K, TV, true>(ctx, stats, tiles, s, ta);
+    else launch_lds_w<K, TV, false>(ctx, stats, tiles, s, ta);
 }
 
 template <int K>
@@ -751,11 +818,18 @@ void launch_k(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const Tra
         if (stats) k_trace_p<true, K><<<resident_grid(ctx, k_trace_p<true, K>), 256, 0, s>>>(ta);
         else k_trace_p<false, K><<<resident_grid(ctx, k_trace_p<false, K>), 256, 0, s>>>(ta);
     } else if constexpr (K > 0) {
-        if (ctx->opt_merged) launch_lds<K, true>(ctx, stats, tiles, s, ta);
-        else launch_lds<K, false>(ctx, stats, tiles, s, ta);
+        const int tv = trace_variant(ctx, stats);
+        if (tv == TV_MERGED) launch_lds<K, TV_MERGED>(ctx, stats, tiles, s, ta);
+        else if constexpr (K == 8 || K == 16) {
+            if (tv == TV_LDSTOP) launch_lds<K, TV_LDSTOP>(ctx, stats, tiles, s, ta);
+            else if (tv == TV_WIDE4) launch_lds<K, TV_WIDE4>(ctx, stats, tiles, s, ta);
+            else launch_lds<K, TV_PLAIN>(ctx, stats, tiles, s, ta);
+        } else {
+            launch_lds<K, TV_PLAIN>(ctx, stats, tiles, s, ta);
+        }
     } else {
-        if (stats) k_trace<true, K, false, false, 4><<<tiles, 256, 0, s>>>(ta);
-        else k_trace<false, K, false, false, 4><<<tiles, 256, 0, s>>>(ta);
+        if (stats) k_trace<true, K, TV_PLAIN, false, 4><<<tiles, 256, 0, s>>>(ta);
+        else k_trace<false, K, TV_PLAIN, false, 4><<<tiles, 256, 0, s>>>(ta);
     }
 }
 template <int K>
@@ -978,6 +1052,7 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
     TraceArgs ta;
     ta.S = S;
+    if (trace_variant(ctx, stats) == TV_LDSTOP) ta.S.top_n = sc.top_count[std::min(ctx->opt_lds_top, TOP_LEVELS_MAX)];
     ta.V = V;
     ta.num_tiles = B.total_tiles;
     ta.slot = B.slot;
@@ -1137,6 +1212,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)ctx->opt_merged,         (uint64_t)ctx->opt_trace_waves,
                                      (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
                                      (uint64_t)ctx->opt_tile_order,   (uint64_t)ctx->opt_postpone,
+                                     (uint64_t)ctx->opt_node_width,   (uint64_t)ctx->opt_lds_top,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -1174,6 +1250,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     ctx->timings.shadow_path = shadow_path(ctx, pl);
     ctx->timings.trace_block_waves = trace_block_waves(ctx);
+    ctx->timings.trace_variant = (uint32_t)trace_variant(ctx, ctx->opt_stats != 0);
     ctx->have_hits = pl.want_hits;
     ctx->last_noalloc = false;
     return SPECREL_OK;
@@ -1298,6 +1375,7 @@ void enqueue_noalloc(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
     ctx->timings.submit_ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
     ctx->timings.shadow_path = shadow_path(ctx, pl);
     ctx->timings.trace_block_waves = trace_block_waves(ctx);
+    ctx->timings.trace_variant = (uint32_t)trace_variant(ctx, ctx->opt_stats != 0);
     ctx->last_noalloc = true;
 }
 
@@ -1677,6 +1755,20 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
             return SPECREL_EINVAL;
         }
         ctx->opt_wave_shape = value;
+    }
+    else if (option == SPECREL_OPT_NODE_WIDTH) {
+        if (value != 2 && value != 4) {
+            ctx->err = "node width must be 2 (one box pair per step) or 4 (the children's pairs fetched too)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_node_width = value;
+    }
+    else if (option == SPECREL_OPT_LDS_TOP) {
+        if (value < 0 || value > TOP_LEVELS_MAX) {
+            ctx->err = "LDS top levels must be 0..8";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_lds_top = value;
     }
     else if (option == SPECREL_OPT_GRAPH) {
         if (value < 0 || value > 2) {
@@ -2180,10 +2272,11 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph, in_trace, shape, postpone, waves;
+            int stack, persistent, queue, graph, in_trace, shape, postpone, waves, width, top;
         };
         const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
-                         ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone,   ctx->opt_trace_waves};
+                         ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone,   ctx->opt_trace_waves,
+                         ctx->opt_node_width,   ctx->opt_lds_top};
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
@@ -2191,7 +2284,8 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                     for (int g : {0, 2})
                         for (int ws : {0, 1, 2})  // a wave's pixels: 8x8, 16x4, 4x16 (tiled primary rays only)
                             if (ws == 0 || !pe)
-                                cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone, saved.waves});
+                                cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone, saved.waves,
+                                                 saved.width, saved.top});
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
@@ -2201,6 +2295,8 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             ctx->opt_wave_shape = c.shape;
             ctx->opt_postpone = c.postpone;
             ctx->opt_trace_waves = c.waves;
+            ctx->opt_node_width = c.width;
+            ctx->opt_lds_top = c.top;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -2253,6 +2349,30 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                     }
                 }
             }
+            // Then the best one with the other BVH node fetch width, and with 4, 6 or 8 top levels
+            // of the tree in LDS (tiled primary rays with an LDS stack of 8 or 16).
+            if ((best.stack == 8 || best.stack == 16) && !best.persistent) {
+                const Cand base = best;
+                std::vector<Cand> more;
+                Cand c = base;
+                c.width = base.width == 4 ? 2 : 4;
+                c.top = 0;
+                more.push_back(c);
+                for (int lv : {0, 4, 6, 8}) {
+                    if (lv == base.top) continue;
+                    c = base;
+                    c.top = lv;
+                    more.push_back(c);
+                }
+                for (const Cand &m : more) {
+                    cands.push_back(m);
+                    float tm = time_cand(m);
+                    if (tm < best_ms) {
+                        best_ms = tm;
+                        best = m;
+                    }
+                }
+            }
         } catch (...) {
             apply(saved);
             throw;
@@ -2268,6 +2388,8 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->wave_shape = best.shape;
             out->leaf_postpone = best.postpone != 0;
             out->trace_waves = (best.stack > 0 && !best.persistent) ? best.waves : 4;
+            out->node_width = best.width;
+            out->lds_top = best.top;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

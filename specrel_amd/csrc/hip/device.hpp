@@ -27,6 +27,18 @@ constexpr uint32_t LEAF = 0x80000000u;
 constexpr uint32_t COUNT_SHIFT = 26;
 constexpr uint32_t FIRST_MASK = (1u << COUNT_SHIFT) - 1u;  // small leaf: first < 2^26
 constexpr uint32_t COUNT_MAX = 31u;                         // small leaf: count in 5 bits
+// An interior entry with TOP_BIT names a node of the block's LDS copy of the top levels by its
+// position there (uploads keep node indices below 2^30).
+constexpr uint32_t TOP_BIT = 0x40000000u;
+constexpr int TOP_LEVELS_MAX = 8;
+constexpr uint32_t TOP_MAX = (1u << TOP_LEVELS_MAX) - 1u;  // interior nodes of 8 levels
+// Traversal variants of the primary-ray kernel (the visited sequence is the same in all):
+//   TV_PLAIN  one round of loads per node step;
+//   TV_MERGED a step merges the loads of a wave's node lanes and leaf lanes (closest-hit trace());
+//   TV_WIDE4  fetch width 4: a node step also fetches both children's box pairs, so a continued
+//             child's boxes are in registers at the next step (postponed-leaf traversal);
+//   TV_LDSTOP the top levels of the tree are read from a per-block copy in LDS.
+constexpr int TV_PLAIN = 0, TV_MERGED = 1, TV_WIDE4 = 2, TV_LDSTOP = 3;
 // Big leaf: LEAF | index into big_leaves (count field 0).
 
 struct Scene {
@@ -46,6 +58,12 @@ struct Scene {
     // loaded, per primitive record tested and per primitive whose attributes a hit read, so the
     // distinct records a frame touches can be counted.
     uint32_t *touch_nodes, *touch_prims, *touch_hits;
+    // The interior nodes of the tree's top levels in breadth-first order (node index, and child
+    // entries naming top nodes as TOP_BIT | position), which a block of the primary-ray kernel
+    // may copy into LDS: the first top_n of them are used, entered from top_root.
+    const uint32_t *top_idx;
+    const uint2 *top_info;
+    uint32_t top_n, top_root;
 };
 
 __device__ __forceinline__ void touch(uint32_t *bits, uint32_t i) {
@@ -313,6 +331,49 @@ __device__ __forceinline__ NodeData load_node(const Scene &S, uint32_t e) {
     return NodeData{bx[0], bx[1], bx[2], bx[3], S.ninfo[e]};
 }
 
+// The block's LDS copy of the top levels (TV_LDSTOP kernels only: a kernel that never names
+// these arrays is given no LDS for them).
+typedef __attribute__((address_space(3))) float4 lds_float4;
+static __shared__ float4 g_top_box[4 * TOP_MAX];
+static __shared__ uint2 g_top_info[TOP_MAX];
+// Copies the first S.top_n top rows into LDS (every thread of the block takes part): the box
+// pairs from the node array as they stand, the child entries with those naming top nodes past
+// top_n turned back into node indices.
+__device__ __forceinline__ void top_fill(const Scene &S) {
+    const uint32_t n = S.top_n;
+    for (uint32_t i = threadIdx.x; i < 5 * n; i += blockDim.x) {
+        const uint32_t r = i / 5, part = i - 5 * r;
+        if (part < 4) {
+            g_top_box[4 * r + part] = S.nbox[4 * (size_t)S.top_idx[r] + part];
+        } else {
+            uint2 inf = S.top_info[r];
+            if ((inf.x & (LEAF | TOP_BIT)) == TOP_BIT && (inf.x & ~TOP_BIT) >= n) inf.x = S.top_idx[inf.x & ~TOP_BIT];
+            if ((inf.y & (LEAF | TOP_BIT)) == TOP_BIT && (inf.y & ~TOP_BIT) >= n) inf.y = S.top_idx[inf.y & ~TOP_BIT];
+            g_top_info[r] = inf;
+        }
+    }
+    __syncthreads();
+}
+template <int TV>
+__device__ __forceinline__ uint32_t root_of(const Scene &S) {
+    if constexpr (TV == TV_LDSTOP) return S.top_n ? S.top_root : S.root_entry;
+    else return S.root_entry;
+}
+// An interior node's row: from LDS for a top entry of a TV_LDSTOP kernel, else from the node array.
+template <int TV>
+__device__ __forceinline__ NodeData load_node_v(const Scene &S, uint32_t e) {
+    if constexpr (TV == TV_LDSTOP) {
+        if (e & TOP_BIT) {
+            const uint32_t i = e & ~TOP_BIT;
+            const lds_float4 *b = (const lds_float4 *)g_top_box + 4 * i;
+            const lds_uint2 *f = (const lds_uint2 *)g_top_info + i;
+            auto rd = [](const lds_float4 &v) { return make_float4(v.x, v.y, v.z, v.w); };
+            return NodeData{rd(b[0]), rd(b[1]), rd(b[2]), rd(b[3]), make_uint2(f->x, f->y)};
+        }
+    }
+    return load_node(S, e);
+}
+
 // Marks a loaded value as needed here, so that the loads of one step are all issued before any
 // of them is waited for (an empty asm statement: no instruction).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -353,14 +414,15 @@ __device__ __forceinline__ void leaf_tests(const Isect &x0, const Isect &x1, con
 // the node lanes each waiting for their own loads in turn (measured slower on the large scene:
 // the extra loads cost more than the round they save). The visiting order, and with it every
 // result and counter, is the same either way.
-template <bool STATS, class ST, bool M = false>
+template <bool STATS, class ST, bool M = false, int TV = TV_PLAIN>
 __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk) {
+    static_assert(!(STATS && TV == TV_LDSTOP), "instrumented traversals name nodes by their index");
     hit.slot = -1;
     if (S.empty) return false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float t_enter;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
-    uint32_t e = S.root_entry;
+    uint32_t e = root_of<TV>(S);
     stk.reset();
     bool have = true;
     while (true) {
@@ -415,7 +477,7 @@ __device__ bool trace(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &h
                 leaf_tests<STATS>(x0, x1, x2, i, end, tmin, tmax, hit, C);
             }
         } else {
-            if (!M) D = load_node(S, e);
+            if (!M) D = load_node_v<TV>(S, e);
             const uint2 info = D.info;
             float tl, tr;
             if (STATS) C.box_tests += 2;
@@ -479,7 +541,7 @@ __device__ __forceinline__ void leaf_tests_slot(const Isect &x0, const Isect &x1
     }
 }
 
-template <class ST>
+template <int TV, class ST>
 __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit &hit, Counters &C, ST &stk,
                          int limit, bool &redo) {
     hit.slot = -1;
@@ -488,9 +550,12 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float t_enter;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, t_enter)) return false;
-    uint32_t e = S.root_entry, pe = 0u;
+    uint32_t e = root_of<TV>(S), pe = 0u;
     float pt = 0.0f;
     int32_t slot = -1;
+    // TV_WIDE4: the row of the node at the cursor, when the previous step fetched it.
+    bool have_row = false;
+    NodeData row{};
     stk.reset();
     while (true) {
         stk.pop_if(e == NO_ENTRY && stk.size() > 0, e, t_enter);
@@ -522,9 +587,21 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
             const uint32_t cur = e;
             const bool node = (cur & LEAF) == 0u && cur != NO_ENTRY;
             e = node ? NO_ENTRY : e;
+            const bool had_row = have_row;
+            have_row = node ? false : have_row;
             if (node && !(t_enter >= tmax)) {
                 const bool over = stk.size() + 2 > limit;  // past the limit: traced again by trace()
-                const NodeData D = load_node(S, cur);
+                NodeData D, DL, DR;
+                if constexpr (TV == TV_WIDE4) {
+                    D = row;
+                    if (!had_row) D = load_node(S, cur);
+                    // Both children's box pairs (a leaf child's stands in with this node's own
+                    // row): their loads run under this step's box tests.
+                    DL = load_node(S, (D.info.x & LEAF) ? cur : D.info.x);
+                    DR = load_node(S, (D.info.y & LEAF) ? cur : D.info.y);
+                } else {
+                    D = load_node_v<TV>(S, cur);
+                }
                 float tl, tr;
                 uint32_t hits = (box_hit(D.lmn, D.lmx, o, inv, tmin, tmax, tl) ? 1u : 0u) |
                                 (box_hit(D.rmn, D.rmx, o, inv, tmin, tmax, tr) ? 2u : 0u);
@@ -540,6 +617,10 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
                 stk.grow(hits == 3u && !over);
                 e = go != 0u && !over ? (go == 1u ? D.info.x : D.info.y) : NO_ENTRY;
                 t_enter = go == 1u ? tl : tr;
+                if constexpr (TV == TV_WIDE4) {
+                    have_row = e != NO_ENTRY && (e & LEAF) == 0u;
+                    row = go == 1u ? DL : DR;
+                }
                 if (over) {
                     redo = true;
                     pe = 0u;
@@ -559,15 +640,16 @@ __device__ bool trace_pp(const Scene &S, V4 o, V4 d, float tmin, float tmax, Hit
 }
 
 // Bvh4D::is_occluded (bvh4d.rs:523-578): right pushed, then left pushed and popped at once.
-template <bool STATS, class ST>
+template <bool STATS, class ST, int TV = TV_PLAIN>
 __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Counters &C, ST &stk) {
+    static_assert(!(STATS && TV == TV_LDSTOP), "instrumented traversals name nodes by their index");
     if (S.empty) return false;
     V4 inv = V4{safe_inv(d.t), safe_inv(d.x), safe_inv(d.y), safe_inv(d.z)};
     float en;
     if (!box_hit(S.root_min, S.root_max, o, inv, tmin, tmax, en)) return false;
     // The lane's state as values (see trace_pp): the entry at the cursor or NO_ENTRY, and whether
     // a primitive was hit, after which the lane ends.
-    uint32_t e = S.root_entry;
+    uint32_t e = root_of<TV>(S);
     bool occ = false;
     stk.reset();
     while (true) {
@@ -599,7 +681,7 @@ __device__ bool occluded(const Scene &S, V4 o, V4 d, float tmin, float tmax, Cou
             }
             if (occ) stk.reset();
         } else {
-            const NodeData D = load_node(S, cur);
+            const NodeData D = load_node_v<TV>(S, cur);
             float t;
             if (STATS) C.shadow_box_tests += 2;
             if (STATS) touch(S.touch_nodes, cur);
@@ -1037,7 +1119,7 @@ struct EnqueueShadow {
 
 // Shadow queries traced where they are asked, their answers kept as bit `id` of `mask` for a
 // later shading pass that looks them up (LookupShadow).
-template <bool STATS, class ST>
+template <bool STATS, class ST, int TV = TV_PLAIN>
 struct RecordShadow {
     const Scene &S;
     Counters &C;
@@ -1045,7 +1127,7 @@ struct RecordShadow {
     uint32_t mask;
     __device__ bool query(uint32_t id, V4 o, V4 d, float tmin, float tmax) {
         if (STATS) C.shadow_rays++;
-        bool occ = occluded<STATS>(S, o, d, tmin, tmax, C, stk);
+        bool occ = occluded<STATS, ST, TV>(S, o, d, tmin, tmax, C, stk);
         if (occ) mask |= 1u << id;
         return occ;
     }

@@ -202,11 +202,15 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
 // Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT). With SH, a shaded
 // view's hit also traces its shadow queries at once (the ones compute_illumination and
 // compute_flash_illumination make, renderer.rs:585-688, from the same record the shading pass
-// rebuilds) and stores their answers for the shading pass (k_shade PASS 2).
-template <bool STATS, int SK, bool M, bool SH, int W>
+// rebuilds) and stores their answers for the shading pass (k_shade PASS 2). TV: the traversal
+// variant (TV_PLAIN, TV_MERGED, TV_WIDE4, TV_LDSTOP), each visiting the same sequence.
+template <bool STATS, int SK, int TV, bool SH, int W>
 __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
+    static_assert(!(STATS && (TV == TV_WIDE4 || TV == TV_LDSTOP)), "instrumented renders use TV_PLAIN");
+    constexpr bool M = TV == TV_MERGED;
     const uint64_t clock0 = a.clock ? wall_clock64() : 0ull;
+    if constexpr (TV == TV_LDSTOP) top_fill(a.S);
     using SS = StackSel<SK, 64 * W>;
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     // The wave's scheduling slot: blocks start in slot order, and the order table puts the tiles
@@ -242,8 +246,9 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
         if (!a.S.empty) {
             bool redo = true;
             if (!STATS && !M && a.postpone)
-                found = trace_pp(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk, (int)a.postpone, redo);
-            if (redo) found = trace<STATS, typename SS::type, M>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
+                found = trace_pp<TV>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk, (int)a.postpone, redo);
+            if (redo)
+                found = trace<STATS, typename SS::type, M, TV>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
         }
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
         a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
@@ -254,7 +259,7 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
                 interpolate(a.S.attrs + 12 * (size_t)hit.slot, hit, rec);
                 if (STATS) touch(a.S.touch_hits, (uint32_t)hit.slot);
                 const Mat m = load_mat(a.S, rec.material);
-                RecordShadow<STATS, typename SS::type> sh{a.S, C, stk, 0u};
+                RecordShadow<STATS, typename SS::type, TV> sh{a.S, C, stk, 0u};
                 illumination(a.S, f, dir, rec, m, sh);
                 if (f.num_flashes > 0) flashes(a.S, f, rec, sh);
                 bits = sh.mask;
@@ -704,14 +709,21 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     template __global__ void k_shade<true, K, 1>(ShadeArgs);     \
     template __global__ void k_shade<false, K, 2>(ShadeArgs);    \
     template __global__ void k_shade<true, K, 2>(ShadeArgs);
-#define SPECREL_TRACE(K, M, SH, W)                                    \
-    template __global__ void k_trace<false, K, M, SH, W>(TraceArgs); \
-    template __global__ void k_trace<true, K, M, SH, W>(TraceArgs);
-// LDS stacks: merged or separate load rounds, shadow queries traced here or not, blocks of 1, 2 or
-// 4 waves. Register stacks: separate rounds, blocks of 4 waves.
-#define SPECREL_TRACE_LDS_W(K, W) \
-    SPECREL_TRACE(K, false, false, W) SPECREL_TRACE(K, true, false, W) SPECREL_TRACE(K, false, true, W) SPECREL_TRACE(K, true, true, W)
+#define SPECREL_TRACE(K, TV, SH, W)                                    \
+    template __global__ void k_trace<false, K, TV, SH, W>(TraceArgs); \
+    template __global__ void k_trace<true, K, TV, SH, W>(TraceArgs);
+#define SPECREL_TRACE_FAST(K, TV, SH, W) template __global__ void k_trace<false, K, TV, SH, W>(TraceArgs);
+// LDS stacks: plain or merged load rounds (with instrumented twins), the fetch-width-4 and
+// LDS-top variants for stacks of 16 and 8 (uninstrumented only); shadow queries traced here or
+// not; blocks of 1, 2 or 4 waves. Register stacks: plain rounds, blocks of 4 waves.
+#define SPECREL_TRACE_LDS_W(K, W)                                                                    \
+    SPECREL_TRACE(K, TV_PLAIN, false, W) SPECREL_TRACE(K, TV_MERGED, false, W)                       \
+    SPECREL_TRACE(K, TV_PLAIN, true, W) SPECREL_TRACE(K, TV_MERGED, true, W)
+#define SPECREL_TRACE_LDS_X(K, W)                                                                    \
+    SPECREL_TRACE_FAST(K, TV_WIDE4, false, W) SPECREL_TRACE_FAST(K, TV_WIDE4, true, W)               \
+    SPECREL_TRACE_FAST(K, TV_LDSTOP, false, W) SPECREL_TRACE_FAST(K, TV_LDSTOP, true, W)
 #define SPECREL_TRACE_LDS(K) SPECREL_TRACE_LDS_W(K, 1) SPECREL_TRACE_LDS_W(K, 2) SPECREL_TRACE_LDS_W(K, 4)
+#define SPECREL_TRACE_LDS_XW(K) SPECREL_TRACE_LDS_X(K, 1) SPECREL_TRACE_LDS_X(K, 2) SPECREL_TRACE_LDS_X(K, 4)
 SPECREL_INSTANTIATE(8)
 SPECREL_INSTANTIATE(10)
 SPECREL_INSTANTIATE(12)
@@ -720,14 +732,16 @@ SPECREL_TRACE_LDS(8)
 SPECREL_TRACE_LDS(10)
 SPECREL_TRACE_LDS(12)
 SPECREL_TRACE_LDS(16)
+SPECREL_TRACE_LDS_XW(8)
+SPECREL_TRACE_LDS_XW(16)
 SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)
 SPECREL_INSTANTIATE(-16)
-SPECREL_TRACE(-4, false, false, 4)
-SPECREL_TRACE(-8, false, false, 4)
-SPECREL_TRACE(-12, false, false, 4)
-SPECREL_TRACE(-16, false, false, 4)
+SPECREL_TRACE(-4, TV_PLAIN, false, 4)
+SPECREL_TRACE(-8, TV_PLAIN, false, 4)
+SPECREL_TRACE(-12, TV_PLAIN, false, 4)
+SPECREL_TRACE(-16, TV_PLAIN, false, 4)
 
 // ---------------------------------------------------------------------------------------
 // Bloom (bloom.rs:22-125): the pyramid lives after the full-resolution HDR image.

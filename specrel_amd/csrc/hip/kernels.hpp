@@ -208,9 +208,10 @@ __global__ void k_adj_pixel(AdjArgs a);
 __global__ void k_adj_reduce(AdjArgs a);
 __global__ void k_adj_final(AdjArgs a);
 
-// M: merged load rounds for a wave's node and leaf lanes; SH: the hit's shadow queries traced
-// here; W: waves per block (each wave one quarter of a 16x16 tile), 1, 2 or 4.
-template <bool STATS, int SK, bool M, bool SH, int W = 4>
+// TV: traversal variant (TV_PLAIN, TV_MERGED: merged load rounds for a wave's node and leaf
+// lanes, TV_WIDE4: fetch width 4, TV_LDSTOP: top levels from LDS); SH: the hit's shadow queries
+// traced here; W: waves per block (each wave one quarter of a 16x16 tile), 1, 2 or 4.
+template <bool STATS, int SK, int TV, bool SH, int W = 4>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
 __global__ void k_trace_p(TraceArgs a);

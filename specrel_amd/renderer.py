@@ -24,6 +24,10 @@ OPT_SHADOW_IN_TRACE, OPT_TRACE_WAVES, OPT_STAR_OVERLAP, OPT_WAVE_SHAPE, OPT_TILE
 OPT_LEAF_POSTPONE = 13
 OPT_ADJOINT_MASK = 14
 OPT_WAVE_CLOCK = 15
+OPT_NODE_WIDTH = 16
+OPT_LDS_TOP = 17
+# specrel_timings.trace_variant: the primary-ray kernel's traversal
+TRACE_PLAIN, TRACE_MERGED, TRACE_WIDE4, TRACE_LDS_TOP = 0, 1, 2, 3
 # specrel_timings.shadow_path: where the last render traced its shadow queries
 SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 
@@ -31,7 +35,8 @@ SHADOWS_NONE, SHADOWS_IN_SHADE, SHADOWS_QUEUED, SHADOWS_IN_TRACE = 0, 1, 2, 3
 class Timings(C.Structure):
     _fields_ = [(n, C.c_float) for n in ("submit_ms", "wait_ms", "kernel_ms", "readback_ms", "star_prep_ms",
                                          "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "adjoint_ms")] + \
-        [("shadow_path", C.c_uint32), ("extrude_ms", C.c_float), ("trace_block_waves", C.c_uint32)]
+        [("shadow_path", C.c_uint32), ("extrude_ms", C.c_float), ("trace_block_waves", C.c_uint32),
+         ("trace_variant", C.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -110,7 +115,7 @@ class Tuning(C.Structure):
     _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
                 ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
                 ("shadow_in_trace", C.c_int32), ("wave_shape", C.c_int32), ("leaf_postpone", C.c_int32),
-                ("trace_waves", C.c_int32), ("reserved", C.c_uint32 * 1)]
+                ("trace_waves", C.c_int32), ("node_width", C.c_int32), ("lds_top", C.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

@@ -6,8 +6,9 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
-from specrel_amd.renderer import (OPT_GRAPH, OPT_LEAF_POSTPONE, OPT_PERSISTENT, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
-                                  OPT_STACK, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE)
+from specrel_amd.renderer import (OPT_GRAPH, OPT_LDS_TOP, OPT_LEAF_POSTPONE, OPT_NODE_WIDTH, OPT_PERSISTENT,
+                                  OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, OPT_TRACE_WAVES,
+                                  OPT_WAVE_SHAPE)
 from oracle import pyoracle as O
 from tests.helpers import assert_same
 
@@ -187,12 +188,16 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         assert t["wave_shape"] in (0, 1, 2) and not (t["persistent"] and t["wave_shape"])
         assert t["leaf_postpone"] in (0, 1)
         # 48 + the current options + the postponement flip, + the two other block sizes when the
-        # pick is tiled with an LDS stack
+        # pick is tiled with an LDS stack, + the other node fetch width and 4, 6 or 8 top levels
+        # in LDS
         tiled = t["stack"] > 0 and not t["persistent"]
         assert t["trace_waves"] in ((1, 2, 4) if tiled else (4,))
-        assert t["candidates"] == (52 if tiled else 50) and 0 < t["best_ms"] <= t["default_ms"]
+        assert t["node_width"] in (2, 4) and t["lds_top"] in (0, 4, 6, 8)
+        assert t["candidates"] == (56 if tiled else 50) and 0 < t["best_ms"] <= t["default_ms"]
         assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "autotuned")
     finally:
+        gpu_ctx.set_option(OPT_NODE_WIDTH, 2)
+        gpu_ctx.set_option(OPT_LDS_TOP, 0)
         gpu_ctx.set_option(OPT_STACK, 16)
         gpu_ctx.set_option(OPT_PERSISTENT, 0)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)

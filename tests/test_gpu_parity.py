@@ -275,6 +275,44 @@ def test_load_round_variants_are_exact(gpu_ctx, twin, merged):
         gpu_ctx.set_option(OPT_MERGED_LOADS, 0)
 
 
+@pytest.mark.parametrize("width,top,stack,waves,want", [
+    (4, 0, 16, 1, 2), (4, 0, 8, 4, 2), (4, 0, 16, 2, 2),
+    (2, 8, 16, 1, 3), (2, 4, 16, 4, 3), (2, 1, 8, 2, 3), (4, 6, 16, 1, 3)])
+def test_node_width_and_lds_top_levels_are_exact(gpu_ctx, twin, width, top, stack, waves, want):
+    # BVH node fetch width 4 (both children's box pairs fetched with the node's) and the tree's top
+    # levels read from a per-block LDS copy visit the reference's sequence: images equal the
+    # oracle's, also for rays traced again without postponement past a small stack limit (5), which
+    # then start from the LDS copy's root too, and for shadow rays and flashes.
+    from specrel_amd.renderer import OPT_LDS_TOP, OPT_LEAF_POSTPONE, OPT_NODE_WIDTH, OPT_STACK, OPT_TRACE_WAVES
+    p = S.gen_params("large", n_objects=12, slices=32, n_stars=300)
+    big = S.Scene.generate("large", p)
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.2, 0.1, 0.4))
+    q = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((q[0] - 6.0, q[1], q[2], q[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    cases = [(twin, S.build_frame(cam, S.RenderConfig(192, 120), dyn), "twin"),
+             (big, S.build_frame(S.default_camera("large", p), S.RenderConfig(256, 160)), "large 12x32")]
+    try:
+        gpu_ctx.set_option(OPT_NODE_WIDTH, width)
+        gpu_ctx.set_option(OPT_LDS_TOP, top)
+        gpu_ctx.set_option(OPT_STACK, stack)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+        for sc, fr, name in cases:
+            o = O.render(sc, fr)
+            gpu_ctx.upload(0, sc)
+            for post in (1, 5):
+                gpu_ctx.set_option(OPT_LEAF_POSTPONE, post)
+                assert_same(gpu_ctx.render_frame(0, fr, want_hits=True), o, f"{name}, width {width}, top {top}, "
+                                                                            f"postpone {post}")
+                assert gpu_ctx.timings()["trace_variant"] == want
+    finally:
+        gpu_ctx.set_option(OPT_NODE_WIDTH, 2)
+        gpu_ctx.set_option(OPT_LDS_TOP, 0)
+        gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
+        gpu_ctx.set_option(OPT_STACK, 16)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+
+
 @pytest.mark.parametrize("waves,stars", [(4, 100), (2, 100), (1, 100), (4, 0)])
 def test_tile_order_changes_nothing(gpu_ctx, waves, stars):
     # From the second render of the same views on, the primary-ray tiles start slowest first (the

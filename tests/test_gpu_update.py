@@ -61,6 +61,24 @@ def capacity(n):
 
 
 def test_edits_patch_in_place_or_upload_afresh(gpu_ctx, twin, frames):
+    run_edits(gpu_ctx, twin, frames)
+
+
+def test_edits_with_the_top_levels_in_lds(twin, frames):
+    # The same edits traced by kernels that copy the tree's top 8 levels into LDS: the copy's node
+    # list and child entries are rebuilt with each update, so every render stays the oracle's. A
+    # context of its own, so that its scene slots start with the sizes the edits give them.
+    from specrel_amd.renderer import OPT_LDS_TOP, TRACE_LDS_TOP, Context
+    ctx = Context(0)
+    try:
+        ctx.set_option(OPT_LDS_TOP, 8)
+        run_edits(ctx, twin, frames)
+        assert ctx.timings()["trace_variant"] == TRACE_LDS_TOP
+    finally:
+        ctx.close()
+
+
+def run_edits(gpu_ctx, twin, frames):
     parts, _ = H.cut(twin, 4)
     sizes = [tuple(map(len, H.subtree(twin, j))) for j in parts]
     big = [j for j, (n, p) in zip(parts, sizes) if p >= 1000]
