@@ -26,6 +26,8 @@ OPT_ADJOINT_MASK = 14
 OPT_WAVE_CLOCK = 15
 OPT_NODE_WIDTH = 16
 OPT_LDS_TOP = 17
+# the library's defaults for SPECREL_OPT_TRACE_WAVES and SPECREL_OPT_LDS_TOP
+DEFAULT_TRACE_WAVES, DEFAULT_LDS_TOP = 1, 0
 # specrel_timings.trace_variant: the primary-ray kernel's traversal
 TRACE_PLAIN, TRACE_MERGED, TRACE_WIDE4, TRACE_LDS_TOP = 0, 1, 2, 3
 # specrel_timings.shadow_path: where the last render traced its shadow queries

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
+from specrel_amd.renderer import DEFAULT_LDS_TOP, DEFAULT_TRACE_WAVES
 from specrel_amd.renderer import (OPT_GRAPH, OPT_LDS_TOP, OPT_LEAF_POSTPONE, OPT_NODE_WIDTH, OPT_PERSISTENT,
                                   OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, OPT_TRACE_WAVES,
                                   OPT_WAVE_SHAPE)
@@ -197,7 +198,7 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "autotuned")
     finally:
         gpu_ctx.set_option(OPT_NODE_WIDTH, 2)
-        gpu_ctx.set_option(OPT_LDS_TOP, 0)
+        gpu_ctx.set_option(OPT_LDS_TOP, DEFAULT_LDS_TOP)
         gpu_ctx.set_option(OPT_STACK, 16)
         gpu_ctx.set_option(OPT_PERSISTENT, 0)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)
@@ -205,7 +206,7 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         gpu_ctx.set_option(OPT_SHADOW_IN_TRACE, 1)
         gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
         gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
 
 
 def test_large_scene_random_observers(gpu_ctx):

@@ -144,7 +144,7 @@ def test_primary_ray_block_sizes_run_as_asked(gpu_ctx, twin, stack):
     finally:
         gpu_ctx.set_option(R.OPT_STACK, 16)
         gpu_ctx.set_option(R.OPT_SHADOW_IN_TRACE, 1)
-        gpu_ctx.set_option(R.OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(R.OPT_TRACE_WAVES, R.DEFAULT_TRACE_WAVES)
     assert ref
 
 
@@ -163,7 +163,7 @@ def test_block_size_of_other_kernels_is_reported(gpu_ctx, twin):
     finally:
         gpu_ctx.set_option(R.OPT_STACK, 16)
         gpu_ctx.set_option(R.OPT_PERSISTENT, 0)
-        gpu_ctx.set_option(R.OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(R.OPT_TRACE_WAVES, R.DEFAULT_TRACE_WAVES)
 
 
 def test_distinct_records_touched_match_the_oracle(gpu_ctx, twin):

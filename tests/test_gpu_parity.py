@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from specrel_amd import scene as S
+from specrel_amd.renderer import DEFAULT_LDS_TOP, DEFAULT_TRACE_WAVES
 from oracle import pyoracle as O
 from tests.helpers import DEMO_SCENES, HARNESS_CONFIGS, assert_same, frame_for, harness_frame, render_both
 
@@ -191,7 +192,7 @@ def test_primary_blocks_of_fewer_waves_are_exact(gpu_ctx, twin, waves, depth):
         st = gpu_ctx.stats()
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
         gpu_ctx.set_option(OPT_STACK, 16)
     assert_same(g, o, f"{waves} waves per block, stack {depth}")
     oc = O.render(twin, fr, counts=True)["counts"]
@@ -237,7 +238,7 @@ def test_wave_shapes_are_exact(gpu_ctx, twin, shape, waves):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
     assert_same(g, o, f"wave shape {shape}, {waves} waves per block")
     oc = O.render(twin, fr, counts=True)["counts"]
     for k in ("rays", "node_pops", "box_tests", "prim_tests", "hits", "shadow_rays", "shadow_node_pops"):
@@ -307,10 +308,10 @@ def test_node_width_and_lds_top_levels_are_exact(gpu_ctx, twin, width, top, stac
                 assert gpu_ctx.timings()["trace_variant"] == want
     finally:
         gpu_ctx.set_option(OPT_NODE_WIDTH, 2)
-        gpu_ctx.set_option(OPT_LDS_TOP, 0)
+        gpu_ctx.set_option(OPT_LDS_TOP, DEFAULT_LDS_TOP)
         gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
         gpu_ctx.set_option(OPT_STACK, 16)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
 
 
 @pytest.mark.parametrize("waves,stars", [(4, 100), (2, 100), (1, 100), (4, 0)])
@@ -340,7 +341,7 @@ def test_tile_order_changes_nothing(gpu_ctx, waves, stars):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_TILE_ORDER, 1)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
 
 
 def test_renders_enqueued_ahead(gpu_ctx, twin):
@@ -389,4 +390,4 @@ def test_postponed_leaves_are_exact(gpu_ctx, stack, waves):
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_LEAF_POSTPONE, 1)
         gpu_ctx.set_option(OPT_STACK, 16)
-        gpu_ctx.set_option(OPT_TRACE_WAVES, 1)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
