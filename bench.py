@@ -45,7 +45,7 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def work_bytes(st, w, h, n_stars, in_trace):
+def work_bytes(st, w, h, n_stars, in_trace, fused=False):
     """Bytes the traversal and shading request from the memory hierarchy per launch of the two
     per-pixel kernels, every visit priced with the reference's record sizes: 48 B per node pop
     (BvhNode4D / GpuNode), 32 B per child box tested, 80 B per hot record tested (SimplexGeom4D),
@@ -56,7 +56,9 @@ def work_bytes(st, w, h, n_stars, in_trace):
                and 4 B per pixel of occlusion mask written.
       k_shade: 4 B per pixel of hit slot read, per hit the 80 B record intersected again plus
                192 + 48 B, the shadow queries priced like the primary ones (or 4 B per pixel of
-               occlusion mask read), 64 B per star of the view, and 16 B per pixel of HDR written."""
+               occlusion mask read), 64 B per star of the view, and 16 B per pixel of HDR written.
+    With the shading fused into k_trace (fused), k_trace does both but reads neither the slot nor
+    the occlusion bits back."""
     shadow = 48 * st["shadow_node_pops"] + 32 * st["shadow_box_tests"] + 80 * st["shadow_prim_tests"]
     trace = 48 * st["node_pops"] + 32 * st["box_tests"] + 80 * st["prim_tests"] + 4 * w * h
     shade = 4 * w * h + (80 + 240) * st["hits"] + 64 * n_stars + 16 * w * h
@@ -65,10 +67,12 @@ def work_bytes(st, w, h, n_stars, in_trace):
         shade += 4 * w * h
     else:
         shade += shadow
+    if fused:
+        return {"k_trace": trace + shade - (8 if in_trace else 4) * w * h}
     return {"k_trace": trace, "k_shade": shade}
 
 
-def compulsory_bytes(st, w, h, n_stars, in_trace):
+def compulsory_bytes(st, w, h, n_stars, in_trace, fused=False):
     """Algorithmic HBM bytes per launch: every distinct record the frame touches, read once, and
     every per-pixel result written once -- what the kernels must move between HBM and the chip
     however well the caches serve repeated visits. Records in the device layout: 72 B per
@@ -78,8 +82,13 @@ def compulsory_bytes(st, w, h, n_stars, in_trace):
       k_trace: those records (primary and shadow rays), 4 B hit slot per pixel written, and with
                the shadow queries traced there 4 B of occlusion bits per pixel written.
       k_shade: 4 B slot (+ 4 B occlusion bits) per pixel read, per distinct hit primitive its
-               80 + 192 + 48 B, 64 B per star, 16 B of HDR per pixel written."""
+               80 + 192 + 48 B, 64 B per star, 16 B of HDR per pixel written.
+      fused:   k_trace with the shading in it: the records once (a hit primitive's record and
+               attributes are the traversal's), 64 B per star, slot, occlusion bits and HDR
+               written (24 B per pixel with shadows traced, else 20)."""
     records = 72 * st["distinct_nodes"] + 80 * st["distinct_prims"] + 240 * st["distinct_hits"]
+    if fused:
+        return {"k_trace": records + 64 * n_stars + ((8 if in_trace else 4) + 16) * w * h}
     trace = records + 4 * w * h + (4 * w * h if in_trace else 0)
     shade = (8 if in_trace else 4) * w * h + (80 + 240) * st["distinct_hits"] + 64 * n_stars + 16 * w * h
     return {"k_trace": trace, "k_shade": shade}
@@ -406,6 +415,8 @@ def main():
                     help="2 or 4: BVH node fetch width of the primary trace (0: default)")
     ap.add_argument("--lds-top", type=int, default=-1,
                     help="0..8: top tree levels each primary-ray block reads from LDS (-1: default)")
+    ap.add_argument("--fused-shade", type=int, default=-1,
+                    help="1/0: the primary-ray kernel shades its own pixels (-1: default)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -421,7 +432,7 @@ def main():
     from specrel_amd import scene as S
     from specrel_amd.renderer import (OPT_PERSISTENT, OPT_MERGED_LOADS, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
                                       OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE,
-                                      OPT_NODE_WIDTH, OPT_LDS_TOP, SHADOWS_IN_TRACE,
+                                      OPT_NODE_WIDTH, OPT_LDS_TOP, OPT_FUSED_SHADE, SHADOWS_IN_TRACE,
                                       Context)
 
     w, h = args.width, args.height
@@ -455,6 +466,8 @@ def main():
         ctx.set_option(OPT_NODE_WIDTH, args.node_width)
     if args.lds_top >= 0:
         ctx.set_option(OPT_LDS_TOP, args.lds_top)
+    if args.fused_shade >= 0:
+        ctx.set_option(OPT_FUSED_SHADE, args.fused_shade)
     t0 = time.perf_counter()
     ctx.upload(0, scene)
     log(f"uploaded to {ctx.device_name()} in {time.perf_counter() - t0:.2f} s")
@@ -496,9 +509,10 @@ def main():
             e2e.append((time.perf_counter() - t0) * 1e3)
     e2e_ms = float(np.median(e2e))
 
-    alg = compulsory_bytes(st, w, h, scene.num_stars, in_trace)
-    work = work_bytes(st, w, h, scene.num_stars, in_trace)
-    times = {"k_trace": parts["trace_ms"], "k_shade": parts["shade_ms"]}
+    fused = ctx.timings()["shade_ms"] == 0.0  # the primary-ray kernel shaded its own pixels
+    alg = compulsory_bytes(st, w, h, scene.num_stars, in_trace, fused)
+    work = work_bytes(st, w, h, scene.num_stars, in_trace, fused)
+    times = {"k_trace": parts["trace_ms"]} if fused else {"k_trace": parts["trace_ms"], "k_shade": parts["shade_ms"]}
     dom = max(times, key=times.get)
     achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
     traffic = None
@@ -581,6 +595,7 @@ def main():
                    "stars": scene.num_stars},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in parts.items()},
         "shadow_rays_traced_in": "k_trace" if in_trace else "k_shade",
+        "pixels_shaded_in": "k_trace" if fused else "k_shade",
         "e2e_ms_with_readback": round(e2e_ms, 3),
         "e2e_calls": len(e2e),
         "parity": parity,

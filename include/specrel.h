@@ -119,10 +119,16 @@ enum {
                                      step; 4: a step also fetches both children's box pairs, so a
                                      continued child's boxes are in registers at its step. Same
                                      visiting order. */
-    SPECREL_OPT_LDS_TOP = 17      /* 0 (default), or 1..8: each block of the tiled primary-ray kernel
+    SPECREL_OPT_LDS_TOP = 17,     /* 0 (default), or 1..8: each block of the tiled primary-ray kernel
                                      (LDS stack of 8 or 16) copies the interior nodes of that many top
                                      levels of the tree into LDS and reads them there (takes
                                      precedence over SPECREL_OPT_NODE_WIDTH). Same visiting order. */
+    SPECREL_OPT_FUSED_SHADE = 18  /* tiled primary rays with an LDS stack of 8 or 16 and their shadow
+                                     queries traced there shade their own pixels (the star pass runs
+                                     first): 1 (default) for renders of one view, 2 for every render,
+                                     0 never (a shading pass of its own after them). Instrumented
+                                     renders, queued or overlapped passes use the separate pass.
+                                     Results do not depend on it. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the
@@ -151,6 +157,7 @@ typedef struct specrel_tuning {
     int32_t trace_waves;     /* SPECREL_OPT_TRACE_WAVES */
     int32_t node_width;      /* SPECREL_OPT_NODE_WIDTH */
     int32_t lds_top;         /* SPECREL_OPT_LDS_TOP */
+    int32_t fused_shade;     /* SPECREL_OPT_FUSED_SHADE */
 } specrel_tuning;
 
 /* GpuRenderer::new (gpu.rs:472-620) without the scene upload. */
@@ -384,6 +391,6 @@ int specrel_last_wave_clock(specrel_ctx *ctx, specrel_wave_clock *out, uint64_t 
 #ifdef __cplusplus
 }
 static_assert(sizeof(specrel_timings) == 56 && offsetof(specrel_timings, trace_variant) == 52, "timings layout");
-static_assert(sizeof(specrel_tuning) == 52 && offsetof(specrel_tuning, lds_top) == 48, "tuning layout");
+static_assert(sizeof(specrel_tuning) == 56 && offsetof(specrel_tuning, fused_shade) == 52, "tuning layout");
 #endif
 #endif

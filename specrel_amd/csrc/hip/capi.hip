@@ -133,7 +133,7 @@ struct specrel_ctx {
     hipEvent_t fork = nullptr, join = nullptr;
     // The next frame's tile order runs on the side stream beside the shading and bloom passes.
     hipEvent_t order_fork = nullptr, order_join = nullptr;
-    bool timed_stages = true, stars_overlapped = false;
+    bool timed_stages = true, stars_overlapped = false, shade_fused = false;
     specrel_timings timings{};
     specrel_stats stats_host{};
     int opt_wave_clock = 0;
@@ -144,6 +144,7 @@ struct specrel_ctx {
     int opt_trace_waves = 1;
     int opt_node_width = 2;  // SPECREL_OPT_NODE_WIDTH
     int opt_lds_top = 0;     // SPECREL_OPT_LDS_TOP: top levels copied into LDS (0: none)
+    int opt_fuse_shade = 1;  // SPECREL_OPT_FUSED_SHADE
     int opt_wave_shape = 0;
     int opt_tile_order = 1;
     int opt_postpone = 1;
@@ -790,11 +791,11 @@ int trace_variant(const specrel_ctx *ctx, bool stats) {
 
 // Primary rays with an LDS stack in blocks of 1, 2 or 4 waves (SPECREL_OPT_TRACE_WAVES), each wave
 // one quarter of a 16x16 tile; TV: traversal variant; SH: shadow queries traced here.
-template <int K, int TV, bool SH>
+template <int K, int TV, int SH>
 void launch_lds_w(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
     const int w = ctx->opt_trace_waves;
     const dim3 g(tiles.x * 4 / w);
-    if constexpr (TV == TV_PLAIN || TV == TV_MERGED) {
+    if constexpr ((TV == TV_PLAIN || TV == TV_MERGED) && SH != 2) {
         if (stats) {
             if (w == 1) k_trace<true, K, TV, SH, 1><<<g, 64, 0, s>>>(ta);
             else if (w == 2) k_trace<true, K, TV, SH, 2><<<g, 128, 0, s>>>(ta);
@@ -808,8 +809,14 @@ void launch_lds_w(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const
 }
 template <int K, int TV>
 void launch_lds(specrel_ctx *ctx, bool stats, dim3 tiles, hipStream_t s, const TraceArgs &ta) {
-    if (ta.mask) launch_lds_w<K, TV, true>(ctx, stats, tiles, s, ta);
-    else launch_lds_w<K, TV, false>(ctx, stats, tiles, s, ta);
+    if (ta.shade_here) {
+        if constexpr ((K == 8 || K == 16) && TV != TV_MERGED) launch_lds_w<K, TV, 2>(ctx, stats, tiles, s, ta);
+        else throw Fail{SPECREL_EINVAL, "no fused shading for this traversal"};
+    } else if (ta.mask) {
+        launch_lds_w<K, TV, 1>(ctx, stats, tiles, s, ta);
+    } else {
+        launch_lds_w<K, TV, 0>(ctx, stats, tiles, s, ta);
+    }
 }
 
 template <int K>
@@ -1050,63 +1057,17 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const dim3 tiles(B.total_tiles);
     const uint32_t path = shadow_path(ctx, pl);
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
-    TraceArgs ta;
-    ta.S = S;
-    if (trace_variant(ctx, stats) == TV_LDSTOP) ta.S.top_n = sc.top_count[std::min(ctx->opt_lds_top, TOP_LEVELS_MAX)];
-    ta.V = V;
-    ta.num_tiles = B.total_tiles;
-    ta.slot = B.slot;
-    ta.mask = in_trace ? B.mask : nullptr;
-    ta.queue = counters + 0;
-    ta.stats = B.stats;
-    ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
-    // Slowest tiles first only when they cannot all start at once (more than four per CU): then
-    // the order decides when the slowest ones start; otherwise it only costs.
-    const bool order = B.tile_order && ctx->opt_tile_order && !ctx->opt_persistent &&
-                       B.total_tiles > 4u * (uint32_t)ctx->cus;
-    ta.order = order ? B.tile_order : nullptr;
-    ta.cost = order ? B.tile_cost : nullptr;
-    ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
-    ta.postpone = ctx->opt_postpone == 1 ? (uint32_t)(STACK - 2) : (uint32_t)ctx->opt_postpone;
-    // Per-wave clock records: the tiled kernel only (one record per scheduling slot).
     const bool tiled = ctx->opt_stack > 0 && !ctx->opt_persistent;
-    ta.clock = tiled ? B.wave_clock : nullptr;
-    ctx->wave_clock_n = ta.clock ? 4ull * B.total_tiles : 0ull;
-    launch_any(ctx, stats, tiles, s, ta);
-    check(hipGetLastError(), "trace launch");
-    mark(1);
-    if (overlap) {
-        check(hipStreamWaitEvent(ctx->side, ctx->fork, 0), "fork wait");
-        if (events) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
-        star_pass(ctx->side, 0u);
-        if (events) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
-        check(hipEventRecord(ctx->join, ctx->side), "join");
-    }
-    // The next frame's tile order, from this frame's costs: by the star binning pass when it runs
-    // after the primary rays on this stream and one block sorts the tiles, else on the side stream
-    // beside the later passes.
-    const bool order_in_bin =
-        order && pl.stars && !overlap && ctx->opt_cull && B.total_tiles <= TILE_ORDER_ONE_BLOCK;
-    const bool order_side = order && !order_in_bin;
-    if (order_side) {
-        check(hipEventRecord(ctx->order_fork, s), "fork");
-        check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
-        const uint32_t *cost = B.tile_cost, n = B.total_tiles;
-        uint32_t *torder = B.tile_order;
-        if (n <= TILE_ORDER_ONE_BLOCK) {
-            k_tile_order<<<1, 1024, 0, ctx->side>>>(cost, n, torder);
-        } else {
-            const uint32_t nb = (n + TILE_ORDER_CHUNK - 1) / TILE_ORDER_CHUNK;
-            k_tile_hist<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL);
-            k_tile_scatter<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL,
-                                                      counters + COUNTER_TILE_CURSOR, torder);
-        }
-        check(hipGetLastError(), "k_tile_order launch");
-        check(hipEventRecord(ctx->order_join, ctx->side), "join");
-    }
-    if (pl.stars && !overlap) star_pass(s, order_in_bin ? B.total_tiles : 0u);
-    mark(2);
-    if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
+    // The shading pass fused into the primary-ray kernel (for single views by default: their frame
+    // time is their heaviest tiles' traversal, under which the other tiles' shading hides; a batch
+    // of many light views is bound by throughput instead): tiled rays with an LDS stack of 8 or 16,
+    // shadow queries (if any) traced there, no instrumentation, the star pass not overlapped (it
+    // runs first instead, since the fused kernel shades with its lists).
+    const bool fuse_asked = ctx->opt_fuse_shade == 2 || (ctx->opt_fuse_shade == 1 && nv == 1);
+    const bool fuse = fuse_asked && !stats && tiled && (ctx->opt_stack == 8 || ctx->opt_stack == 16) &&
+                      trace_variant(ctx, stats) != TV_MERGED && !queued && !overlap &&
+                      (in_trace || path == SPECREL_SHADOWS_NONE);
+    ctx->shade_fused = fuse;
     ShadeArgs ra;
     ra.S = S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;
@@ -1130,7 +1091,72 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     ra.qcount = counters + 1;
     ra.qcap = 0;
     ra.stats = B.stats;
-    if (queued) {
+    TraceArgs ta;
+    ta.S = S;
+    if (trace_variant(ctx, stats) == TV_LDSTOP) ta.S.top_n = sc.top_count[std::min(ctx->opt_lds_top, TOP_LEVELS_MAX)];
+    ta.V = V;
+    ta.num_tiles = B.total_tiles;
+    ta.slot = B.slot;
+    ta.mask = in_trace || fuse ? B.mask : nullptr;
+    ta.queue = counters + 0;
+    ta.stats = B.stats;
+    ta.wave_shape = (uint32_t)ctx->opt_wave_shape;
+    // Slowest tiles first only when they cannot all start at once (more than four per CU): then
+    // the order decides when the slowest ones start; otherwise it only costs.
+    const bool order = B.tile_order && ctx->opt_tile_order && !ctx->opt_persistent &&
+                       B.total_tiles > 4u * (uint32_t)ctx->cus;
+    ta.order = order ? B.tile_order : nullptr;
+    ta.cost = order ? B.tile_cost : nullptr;
+    ta.prio_slots = order ? (uint32_t)ctx->cus : 0u;
+    ta.postpone = ctx->opt_postpone == 1 ? (uint32_t)(STACK - 2) : (uint32_t)ctx->opt_postpone;
+    // Per-wave clock records: the tiled kernel only (one record per scheduling slot).
+    ta.clock = tiled ? B.wave_clock : nullptr;
+    ctx->wave_clock_n = ta.clock ? 4ull * B.total_tiles : 0ull;
+    ta.shade = ra;
+    ta.shade_here = fuse;
+    // The tile order: sorted by a leading block of the star binning pass when that pass runs on
+    // this stream with one block's share of tiles (before the fused kernel: the previous render's
+    // costs, for this one; after the primary rays: this render's costs, for the next one), else
+    // on the side stream beside the later passes (this render's costs, for the next one).
+    const bool order_in_bin =
+        order && pl.stars && !overlap && ctx->opt_cull && B.total_tiles <= TILE_ORDER_ONE_BLOCK;
+    const bool order_side = order && !order_in_bin;
+    if (fuse) {
+        if (pl.stars) star_pass(s, order_in_bin ? B.total_tiles : 0u);
+        if (events) check(hipEventRecord(ctx->ev[10], s), "event");
+    }
+    launch_any(ctx, stats, tiles, s, ta);
+    check(hipGetLastError(), "trace launch");
+    mark(1);
+    if (overlap) {
+        check(hipStreamWaitEvent(ctx->side, ctx->fork, 0), "fork wait");
+        if (events) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
+        star_pass(ctx->side, 0u);
+        if (events) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
+        check(hipEventRecord(ctx->join, ctx->side), "join");
+    }
+    if (order_side) {
+        check(hipEventRecord(ctx->order_fork, s), "fork");
+        check(hipStreamWaitEvent(ctx->side, ctx->order_fork, 0), "fork wait");
+        const uint32_t *cost = B.tile_cost, n = B.total_tiles;
+        uint32_t *torder = B.tile_order;
+        if (n <= TILE_ORDER_ONE_BLOCK) {
+            k_tile_order<<<1, 1024, 0, ctx->side>>>(cost, n, torder);
+        } else {
+            const uint32_t nb = (n + TILE_ORDER_CHUNK - 1) / TILE_ORDER_CHUNK;
+            k_tile_hist<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL);
+            k_tile_scatter<<<nb, 256, 0, ctx->side>>>(cost, n, counters + COUNTER_TILE_TOTAL,
+                                                      counters + COUNTER_TILE_CURSOR, torder);
+        }
+        check(hipGetLastError(), "k_tile_order launch");
+        check(hipEventRecord(ctx->order_join, ctx->side), "join");
+    }
+    if (pl.stars && !overlap && !fuse) star_pass(s, order_in_bin ? B.total_tiles : 0u);
+    mark(2);
+    if (overlap) check(hipStreamWaitEvent(s, ctx->join, 0), "join wait");
+    if (fuse) {
+        // shaded by the primary-ray kernel
+    } else if (queued) {
         ra.queue = B.squeue;
         ra.qcap = (uint32_t)((uint64_t)B.total_pix * (1u + pl.max_flashes));
         launch_any(ctx, stats, tiles, s, ShadePass<1>{ra});
@@ -1213,6 +1239,7 @@ int enqueue_views(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *fra
                                      (uint64_t)ctx->opt_star_overlap,   (uint64_t)ctx->opt_wave_shape,
                                      (uint64_t)ctx->opt_tile_order,   (uint64_t)ctx->opt_postpone,
                                      (uint64_t)ctx->opt_node_width,   (uint64_t)ctx->opt_lds_top,
+                                     (uint64_t)ctx->opt_fuse_shade,
                                      pl.stars,         pl.queries,
                                      pl.max_flashes,    pl.want_hits};
         key.insert(key.end(), ctx->sizes.begin(), ctx->sizes.end());
@@ -1427,9 +1454,15 @@ void fetch(specrel_ctx *ctx, const std::vector<Fetch> &f, float *wait_ms, float 
 void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
     if (ctx->timed_stages) {
-        t.trace_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
-        t.star_prep_ms = ctx->stars_overlapped ? ev_ms(ctx->ev[9], ctx->ev[10]) : ev_ms(ctx->ev[1], ctx->ev[2]);
-        t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
+        if (ctx->shade_fused) {  // star pass, then primary rays with the shading (ev 0, 10, 1)
+            t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[10]);
+            t.trace_ms = ev_ms(ctx->ev[10], ctx->ev[1]);
+            t.shade_ms = 0.0f;
+        } else {
+            t.trace_ms = ev_ms(ctx->ev[0], ctx->ev[1]);
+            t.star_prep_ms = ctx->stars_overlapped ? ev_ms(ctx->ev[9], ctx->ev[10]) : ev_ms(ctx->ev[1], ctx->ev[2]);
+            t.shade_ms = ev_ms(ctx->ev[2], ctx->ev[3]);
+        }
         t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
         t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);
     }
@@ -1755,6 +1788,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
             return SPECREL_EINVAL;
         }
         ctx->opt_wave_shape = value;
+    }
+    else if (option == SPECREL_OPT_FUSED_SHADE) {
+        if (value < 0 || value > 2) {
+            ctx->err = "fused shading must be 0 (never), 1 (single views) or 2 (every render)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_fuse_shade = value;
     }
     else if (option == SPECREL_OPT_NODE_WIDTH) {
         if (value != 2 && value != 4) {
@@ -2272,11 +2312,11 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
         if (ctx->opt_stats) throw Fail{SPECREL_EINVAL, "autotune runs with SPECREL_OPT_STATS off"};
         if (iters == 0 || iters > 1000) throw Fail{SPECREL_EINVAL, "iters must be 1..1000"};
         struct Cand {
-            int stack, persistent, queue, graph, in_trace, shape, postpone, waves, width, top;
+            int stack, persistent, queue, graph, in_trace, shape, postpone, waves, width, top, fuse;
         };
         const Cand saved{ctx->opt_stack,        ctx->opt_persistent, ctx->opt_shadow_queue, ctx->opt_graph,
                          ctx->opt_shadow_trace, ctx->opt_wave_shape, ctx->opt_postpone,   ctx->opt_trace_waves,
-                         ctx->opt_node_width,   ctx->opt_lds_top};
+                         ctx->opt_node_width,   ctx->opt_lds_top,    ctx->opt_fuse_shade};
         std::vector<Cand> cands;
         for (int st : {16, 8})
             for (int pe : {0, 1})
@@ -2285,7 +2325,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                         for (int ws : {0, 1, 2})  // a wave's pixels: 8x8, 16x4, 4x16 (tiled primary rays only)
                             if (ws == 0 || !pe)
                                 cands.push_back({st, pe, sh == 1, g, sh == 2, ws, saved.postpone, saved.waves,
-                                                 saved.width, saved.top});
+                                                 saved.width, saved.top, saved.fuse});
         auto apply = [&](const Cand &c) {
             ctx->opt_stack = c.stack;
             ctx->opt_persistent = c.persistent;
@@ -2297,6 +2337,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             ctx->opt_trace_waves = c.waves;
             ctx->opt_node_width = c.width;
             ctx->opt_lds_top = c.top;
+            ctx->opt_fuse_shade = c.fuse;
         };
         // Median wall time per render of the batch, enqueue to completion.
         auto time_cand = [&](const Cand &c) {
@@ -2372,6 +2413,18 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
                         best = m;
                     }
                 }
+                // Then the best one with the shading fused into the primary rays for every render,
+                // or never.
+                Cand c2 = best;
+                c2.fuse = n_views > 1 ? 2 : 0;
+                if (c2.fuse != best.fuse) {
+                    cands.push_back(c2);
+                    float tf = time_cand(c2);
+                    if (tf < best_ms) {
+                        best_ms = tf;
+                        best = c2;
+                    }
+                }
             }
         } catch (...) {
             apply(saved);
@@ -2390,6 +2443,7 @@ int specrel_autotune(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *
             out->trace_waves = (best.stack > 0 && !best.persistent) ? best.waves : 4;
             out->node_width = best.width;
             out->lds_top = best.top;
+            out->fused_shade = best.fuse;
             out->candidates = (uint32_t)cands.size() + 1;
             out->best_ms = best_ms;
             out->default_ms = base_ms;

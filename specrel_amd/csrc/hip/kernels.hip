@@ -92,16 +92,16 @@ __device__ __forceinline__ bool star_in_cone(const StarDir &s, float cx, float c
     return s.x * cx + s.y * cy + s.z * cz >= cos_lim;
 }
 
-// Ordered compaction of candidates (in increasing index) into list: every thread of the block
-// calls it with the same `n`; keep(i) decides candidate i. Returns the total kept (which may
-// exceed cap, in which case the list holds only the first cap entries).
-template <class Keep>
+// Ordered compaction of candidates (in increasing index) into list by a block of W waves: every
+// thread of the block calls it with the same `n`; keep(i) decides candidate i. Returns the total
+// kept (which may exceed cap, in which case the list holds only the first cap entries).
+template <int W, class Keep>
 __device__ uint32_t ordered_compact(uint32_t n, Keep keep, uint32_t *list, uint32_t cap, uint32_t *s_wave,
                                     uint32_t *s_total) {
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid == 0) *s_total = 0;
     __syncthreads();
-    for (uint32_t base = 0; base < n; base += 256) {
+    for (uint32_t base = 0; base < n; base += 64 * W) {
         uint32_t idx = base + tid;
         uint32_t item = 0;
         bool k = idx < n && keep(idx, item);
@@ -113,10 +113,98 @@ __device__ uint32_t ordered_compact(uint32_t n, Keep keep, uint32_t *list, uint3
         for (uint32_t w = 0; w < wave; ++w) off += s_wave[w];
         if (k && off + before < cap) list[off + before] = item;
         __syncthreads();
-        if (tid == 0) *s_total += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        if (tid == 0)
+            for (int w = 0; w < W; ++w) *s_total += s_wave[w];
         __syncthreads();
     }
     return *s_total;
+}
+
+// The star list of a 16x16 tile (second culling level), in index order: culled from its super
+// tile's list (or from every star when that list overflowed) by the tile's cone; when the tile's
+// own list overflows, the super-tile list is used as is. Every thread of the block (W waves) calls
+// it. The culling keeps every star the reference's per-pixel test can accept.
+struct StarList {
+    const uint32_t *src;  // super-tile list (nullptr: every star)
+    uint32_t n_src, n_tile;
+    bool use_tile;  // the tile's own list (in LDS) is complete
+};
+template <int W>
+__device__ StarList tile_stars(const ShadeArgs &a, uint32_t vi, const ViewDerived &v, uint32_t t, uint32_t x0,
+                               uint32_t y0, uint32_t *s_list, uint32_t *s_wave, uint32_t *s_total) {
+    StarList L{nullptr, 0u, 0u, false};
+    if (!(v.draw_stars && a.cull)) return L;
+    const StarDir *dirs = a.dirs + (size_t)vi * a.num_stars;
+    const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
+    const uint32_t st = a.V.super_start[vi] + (y0 / SUPER) * nsx + x0 / SUPER;
+    const uint32_t sc = a.super_counts[st];
+    const bool all = sc > SUPER_CAP;
+    L.src = all ? nullptr : a.super_lists + (size_t)st * SUPER_CAP;
+    L.n_src = all ? a.num_stars : sc;
+    const float4 cone = a.tile_cones[t];  // tile_cone of this tile, from k_star_bin
+    const float cx = cone.x, cy = cone.y, cz = cone.z, cl = cone.w;
+    const uint32_t *src = L.src;
+    L.n_tile = ordered_compact<W>(
+        L.n_src,
+        [&](uint32_t k, uint32_t &item) {
+            item = src ? src[k] : k;
+            return star_in_cone(dirs[item], cx, cy, cz, cl);
+        },
+        s_list, TILE_CAP, s_wave, s_total);
+    L.use_tile = L.n_tile <= TILE_CAP;
+    return L;
+}
+
+// One pixel of the shading pass (not the queueing pass): re-intersect the winning slot (the same
+// operations as the traversal: the same lambda, event and barycentrics), interpolate, shade with
+// the shadow answers of `sh`, haze, the sky for a miss, the stars of the tile's list; writes the
+// linear HDR pixel and the requested hit index, lambda and tau.
+template <bool STATS, class SH>
+__device__ void shade_pixel(const ShadeArgs &a, uint32_t vi, const specrel_frame &f, const ViewDerived &v,
+                            uint32_t px, uint32_t py, uint32_t slot, SH &sh, const StarList &L,
+                            const uint32_t *s_list, Counters &C) {
+    const StarPrep *stars = a.stars + (size_t)vi * a.num_stars;
+    const size_t li = (size_t)py * v.w + px, pi = a.V.pix_start[vi] + li;
+    V4 dir = ray_dir(v, f, px, py);
+    const bool found = slot != SPECREL_NO_HIT;
+    V3 color = V3{0.0f, 0.0f, 0.0f};
+    Rec rec;
+    float lambda = 0.0f;
+    if (found) {
+        Hit hit;
+        hit.slot = (int32_t)slot;
+        intersect(a.S.geoms + 5 * (size_t)slot, v.origin, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
+        lambda = hit.lambda;
+        interpolate(a.S.attrs + 12 * (size_t)slot, hit, rec);
+        if (STATS) touch(a.S.touch_hits, slot);
+        color = shade_hit(a.S, a.T, f, dir, rec, sh);
+        if (f.num_haze > 0) color = haze(color, rec.event, f);
+    } else {
+        color = shade_sky(dir);
+    }
+    if (v.draw_stars) {
+        float sx = pix_sx(v, px), sy = pix_sy(v, py);
+        float inv_len = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
+        float spread = v.pitch * inv_len * sqrtf(inv_len);
+        float lmax = found ? lambda : __builtin_inff();
+        V4 n = dir - v.e0 * v.cone;
+        V3 total = V3{0.0f, 0.0f, 0.0f};
+        if (a.cull && L.use_tile) {
+            for (uint32_t k = 0; k < L.n_tile; ++k) star_pixel<STATS>(stars[s_list[k]], n, spread, lmax, total, C);
+        } else if (a.cull && L.src) {
+            for (uint32_t k = 0; k < L.n_src; ++k) star_pixel<STATS>(stars[L.src[k]], n, spread, lmax, total, C);
+        } else {
+            for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(stars[k], n, spread, lmax, total, C);
+        }
+        color += total;
+    }
+    a.hdr[a.V.pyr_start[vi] + li] = make_float4(color.x, color.y, color.z, 0.0f);
+    if (a.want_hits) {
+        a.hit[pi] = found ? a.S.order[slot] : SPECREL_NO_HIT;
+        a.lambda[pi] = found ? lambda : __builtin_inff();
+        a.tau[pi] = found ? rec.tau : 0.0f;
+    }
+    if (STATS && found) C.hits++;
 }
 
 // The stars of a cone in index order (ordered_compact for the star test, K candidates per thread
@@ -199,15 +287,22 @@ __global__ __launch_bounds__(256) void k_star_bin(Views V, const StarDir *dirs, 
     if (threadIdx.x == 0) counts[gst] = n;
 }
 
-// Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT). With SH, a shaded
+// Primary rays: closest hit per pixel (the leaf slot, or SPECREL_NO_HIT). With SH >= 1, a shaded
 // view's hit also traces its shadow queries at once (the ones compute_illumination and
 // compute_flash_illumination make, renderer.rs:585-688, from the same record the shading pass
-// rebuilds) and stores their answers for the shading pass (k_shade PASS 2). TV: the traversal
-// variant (TV_PLAIN, TV_MERGED, TV_WIDE4, TV_LDSTOP), each visiting the same sequence.
-template <bool STATS, int SK, int TV, bool SH, int W>
+// rebuilds) and stores their answers for the shading pass (k_shade PASS 2). With SH == 2 the
+// block then shades its own pixels (the shading pass fused in: the tile's star list culled into
+// the LDS the traversal stacks used, each pixel shaded with its occlusion bits in registers), so
+// the shading of the frame's light tiles runs while its heaviest tiles are still tracing; the
+// star pass must have run before. TV: the traversal variant (TV_PLAIN, TV_MERGED, TV_WIDE4,
+// TV_LDSTOP), each visiting the same sequence.
+template <bool STATS, int SK, int TV, int SH, int W>
 __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     static_assert(W == 1 || W == 2 || W == 4, "a block is 1, 2 or 4 quarters of one tile");
     static_assert(!(STATS && (TV == TV_WIDE4 || TV == TV_LDSTOP)), "instrumented renders use TV_PLAIN");
+    static_assert(!(STATS && SH == 2), "instrumented renders shade in their own pass");
+    static_assert(SK <= 0 || SH != 2 || StackSel<SK, 64 * W>::lds_entries * 8 >= TILE_CAP * 4,
+                  "the fused shading's star list fits in the stacks' LDS");
     constexpr bool M = TV == TV_MERGED;
     const uint64_t clock0 = a.clock ? wall_clock64() : 0ull;
     if constexpr (TV == TV_LDSTOP) top_fill(a.S);
@@ -239,6 +334,7 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
     typename SS::Private stk_priv;
     SS::init(stk, s_stack, tid, stk_priv);
     Counters C = {};
+    uint32_t out_slot = SPECREL_NO_HIT, out_bits = 0u;  // for the fused shading
     if (px < v.w && py < v.h) {
         V4 dir = ray_dir(v, f, px, py);
         Hit hit;
@@ -251,8 +347,9 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
                 found = trace<STATS, typename SS::type, M, TV>(a.S, v.origin, dir, 1e-3f, 180.0f, hit, C, stk);
         }
         const size_t pi = (size_t)a.V.pix_start[vi] + (size_t)py * v.w + px;
-        a.slot[pi] = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
-        if constexpr (SH) {
+        out_slot = found ? (uint32_t)hit.slot : SPECREL_NO_HIT;
+        a.slot[pi] = out_slot;
+        if constexpr (SH >= 1) {
             uint32_t bits = 0u;
             if (found && f.mode == SPECREL_MODE_SHADED && f.shadow_mode != SPECREL_SHADOW_OFF) {
                 Rec rec;
@@ -265,6 +362,19 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
                 bits = sh.mask;
             }
             a.mask[pi] = bits;
+            out_bits = bits;
+        }
+    }
+    if constexpr (SH == 2) {
+        // Every wave of the block is past its traversals: the stacks' LDS holds the tile's star list.
+        uint32_t *s_list = (uint32_t *)s_stack;
+        __shared__ uint32_t s_wave[W], s_total;
+        __syncthreads();
+        const ViewDerived vs = derive(f, a.shade.num_stars);
+        const StarList L = tile_stars<W>(a.shade, vi, vs, t, tx, ty, s_list, s_wave, &s_total);
+        if (px < v.w && py < v.h) {
+            LookupShadow sh{out_bits};
+            shade_pixel<false>(a.shade, vi, f, vs, px, py, out_slot, sh, L, s_list, C);
         }
     }
     if (a.cost) {  // the quarter's cost: its lanes' longest traversal (primary and shadow steps)
@@ -293,7 +403,7 @@ __global__ __launch_bounds__(64 * W) void k_trace(TraceArgs a) {
         atomicAdd(&s_c[3], (unsigned long long)C.prim_tests);
         __syncthreads();
         if (tid < 4) atomicAdd(&a.stats[tid], s_c[tid]);
-        if constexpr (SH) {
+        if constexpr (SH >= 1) {
             atomicAdd(&a.stats[5], (unsigned long long)C.shadow_rays);
             atomicAdd(&a.stats[6], (unsigned long long)C.shadow_node_pops);
             atomicAdd(&a.stats[7], (unsigned long long)C.shadow_box_tests);
@@ -585,8 +695,6 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     const uint32_t lt = t - a.V.level_start[0][vi], tiles_x = (v.w + TILE - 1) / TILE;
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const uint32_t x0 = (lt % tiles_x) * TILE, y0 = (lt / tiles_x) * TILE;
-    const StarPrep *stars = a.stars + (size_t)vi * a.num_stars;
-    const StarDir *dirs = a.dirs + (size_t)vi * a.num_stars;
     const uint32_t px = x0 + (wave & 1) * 8 + (lane & 7);
     const uint32_t py = y0 + (wave >> 1) * 8 + (lane >> 3);
     __shared__ uint2 s_stack[PASS == 0 ? StackSel<SK>::lds_entries : 1];
@@ -599,88 +707,33 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
     // The pixel's hit slot and occlusion bits, loaded before the star culling so that their
     // latency passes during it.
     const bool inside = px < v.w && py < v.h;
-    const size_t li = (size_t)py * v.w + px, pi = a.V.pix_start[vi] + li;
+    const size_t pi = a.V.pix_start[vi] + (size_t)py * v.w + px;
     const uint32_t slot0 = inside ? a.slot[pi] : SPECREL_NO_HIT;
     const uint32_t mask0 = PASS == 2 && inside ? a.mask[pi] : 0u;
-
-    // Star list of this tile: culled from its super tile's list (or from every star when that
-    // list overflowed); when this tile's own list overflows, the super-tile list is used as is.
-    const uint32_t *src = nullptr;
-    uint32_t n_src = 0, n_tile = 0;
-    bool use_tile_list = false;
-    if (PASS != 1 && v.draw_stars && a.cull) {
-        const uint32_t nsx = (v.w + SUPER - 1) / SUPER;
-        const uint32_t st = a.V.super_start[vi] + (y0 / SUPER) * nsx + x0 / SUPER;
-        uint32_t sc = a.super_counts[st];
-        bool all = sc > SUPER_CAP;
-        src = all ? nullptr : a.super_lists + (size_t)st * SUPER_CAP;
-        n_src = all ? a.num_stars : sc;
-        const float4 cone = a.tile_cones[t];  // tile_cone of this tile, from k_star_bin
-        const float cx = cone.x, cy = cone.y, cz = cone.z, cl = cone.w;
-        n_tile = ordered_compact(
-            n_src,
-            [&](uint32_t k, uint32_t &item) {
-                item = src ? src[k] : k;
-                return star_in_cone(dirs[item], cx, cy, cz, cl);
-            },
-            s_list, TILE_CAP, s_wave, &s_total);
-        use_tile_list = n_tile <= TILE_CAP;
-    }
-
-    if (inside) {
-        if (PASS == 1) a.mask[pi] = 0u;
-        V4 dir = ray_dir(v, f, px, py);
-        const uint32_t slot = slot0;
-        bool found = slot != SPECREL_NO_HIT;
-        V3 color = V3{0.0f, 0.0f, 0.0f};
-        Rec rec;
-        float lambda = 0.0f;
-        if (found) {
-            Hit hit;
-            hit.slot = (int32_t)slot;
-            intersect(a.S.geoms + 5 * (size_t)slot, v.origin, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
-            lambda = hit.lambda;
-            interpolate(a.S.attrs + 12 * (size_t)slot, hit, rec);
-            if (STATS) touch(a.S.touch_hits, slot);
-            if constexpr (PASS == 0) {
-                TraceShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk};
-                color = shade_hit(a.S, a.T, f, dir, rec, sh);
-            } else if constexpr (PASS == 1) {
+    if constexpr (PASS == 1) {  // queue the shadow queries of the hit pixels
+        if (inside) {
+            a.mask[pi] = 0u;
+            if (slot0 != SPECREL_NO_HIT) {
+                V4 dir = ray_dir(v, f, px, py);
+                Hit hit;
+                hit.slot = (int32_t)slot0;
+                intersect(a.S.geoms + 5 * (size_t)slot0, v.origin, dir, 1e-3f, 180.0f, hit.lambda, hit.event, hit.bary);
+                Rec rec;
+                interpolate(a.S.attrs + 12 * (size_t)slot0, hit, rec);
                 EnqueueShadow sh{a.queue, a.qcount, a.qcap, (uint32_t)pi};
                 shade_hit(a.S, a.T, f, dir, rec, sh);
+            }
+        }
+    } else {
+        const StarList L = tile_stars<4>(a, vi, v, t, x0, y0, s_list, s_wave, &s_total);
+        if (inside) {
+            if constexpr (PASS == 0) {
+                TraceShadow<STATS, typename StackSel<SK>::type> sh{a.S, C, stk};
+                shade_pixel<STATS>(a, vi, f, v, px, py, slot0, sh, L, s_list, C);
             } else {
                 LookupShadow sh{mask0};
-                color = shade_hit(a.S, a.T, f, dir, rec, sh);
+                shade_pixel<STATS>(a, vi, f, v, px, py, slot0, sh, L, s_list, C);
             }
-            if (PASS != 1 && f.num_haze > 0) color = haze(color, rec.event, f);
-        } else if (PASS != 1) {
-            color = shade_sky(dir);
-        }
-        if (PASS != 1) {
-            if (v.draw_stars) {
-                float sx = pix_sx(v, px), sy = pix_sy(v, py);
-                float inv_len = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
-                float spread = v.pitch * inv_len * sqrtf(inv_len);
-                float lmax = found ? lambda : __builtin_inff();
-                V4 n = dir - v.e0 * v.cone;
-                V3 total = V3{0.0f, 0.0f, 0.0f};
-                if (a.cull && use_tile_list) {
-                    for (uint32_t k = 0; k < n_tile; ++k)
-                        star_pixel<STATS>(stars[s_list[k]], n, spread, lmax, total, C);
-                } else if (a.cull && src) {
-                    for (uint32_t k = 0; k < n_src; ++k) star_pixel<STATS>(stars[src[k]], n, spread, lmax, total, C);
-                } else {
-                    for (uint32_t k = 0; k < a.num_stars; ++k) star_pixel<STATS>(stars[k], n, spread, lmax, total, C);
-                }
-                color += total;
-            }
-            a.hdr[a.V.pyr_start[vi] + li] = make_float4(color.x, color.y, color.z, 0.0f);
-            if (a.want_hits) {
-                a.hit[pi] = found ? a.S.order[slot] : SPECREL_NO_HIT;
-                a.lambda[pi] = found ? lambda : __builtin_inff();
-                a.tau[pi] = found ? rec.tau : 0.0f;
-            }
-            if (STATS && found) C.hits++;
         }
     }
     if (STATS && PASS != 1) {
@@ -715,13 +768,16 @@ __global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
 #define SPECREL_TRACE_FAST(K, TV, SH, W) template __global__ void k_trace<false, K, TV, SH, W>(TraceArgs);
 // LDS stacks: plain or merged load rounds (with instrumented twins), the fetch-width-4 and
 // LDS-top variants for stacks of 16 and 8 (uninstrumented only); shadow queries traced here or
-// not; blocks of 1, 2 or 4 waves. Register stacks: plain rounds, blocks of 4 waves.
+// not, and for stacks of 16 and 8 the fused shading; blocks of 1, 2 or 4 waves. Register stacks:
+// plain rounds, blocks of 4 waves.
 #define SPECREL_TRACE_LDS_W(K, W)                                                                    \
-    SPECREL_TRACE(K, TV_PLAIN, false, W) SPECREL_TRACE(K, TV_MERGED, false, W)                       \
-    SPECREL_TRACE(K, TV_PLAIN, true, W) SPECREL_TRACE(K, TV_MERGED, true, W)
+    SPECREL_TRACE(K, TV_PLAIN, 0, W) SPECREL_TRACE(K, TV_MERGED, 0, W)                               \
+    SPECREL_TRACE(K, TV_PLAIN, 1, W) SPECREL_TRACE(K, TV_MERGED, 1, W)
 #define SPECREL_TRACE_LDS_X(K, W)                                                                    \
-    SPECREL_TRACE_FAST(K, TV_WIDE4, false, W) SPECREL_TRACE_FAST(K, TV_WIDE4, true, W)               \
-    SPECREL_TRACE_FAST(K, TV_LDSTOP, false, W) SPECREL_TRACE_FAST(K, TV_LDSTOP, true, W)
+    SPECREL_TRACE_FAST(K, TV_WIDE4, 0, W) SPECREL_TRACE_FAST(K, TV_WIDE4, 1, W)                      \
+    SPECREL_TRACE_FAST(K, TV_LDSTOP, 0, W) SPECREL_TRACE_FAST(K, TV_LDSTOP, 1, W)                    \
+    SPECREL_TRACE_FAST(K, TV_PLAIN, 2, W) SPECREL_TRACE_FAST(K, TV_WIDE4, 2, W)                      \
+    SPECREL_TRACE_FAST(K, TV_LDSTOP, 2, W)
 #define SPECREL_TRACE_LDS(K) SPECREL_TRACE_LDS_W(K, 1) SPECREL_TRACE_LDS_W(K, 2) SPECREL_TRACE_LDS_W(K, 4)
 #define SPECREL_TRACE_LDS_XW(K) SPECREL_TRACE_LDS_X(K, 1) SPECREL_TRACE_LDS_X(K, 2) SPECREL_TRACE_LDS_X(K, 4)
 SPECREL_INSTANTIATE(8)
@@ -738,10 +794,10 @@ SPECREL_INSTANTIATE(-4)
 SPECREL_INSTANTIATE(-8)
 SPECREL_INSTANTIATE(-12)
 SPECREL_INSTANTIATE(-16)
-SPECREL_TRACE(-4, TV_PLAIN, false, 4)
-SPECREL_TRACE(-8, TV_PLAIN, false, 4)
-SPECREL_TRACE(-12, TV_PLAIN, false, 4)
-SPECREL_TRACE(-16, TV_PLAIN, false, 4)
+SPECREL_TRACE(-4, TV_PLAIN, 0, 4)
+SPECREL_TRACE(-8, TV_PLAIN, 0, 4)
+SPECREL_TRACE(-12, TV_PLAIN, 0, 4)
+SPECREL_TRACE(-16, TV_PLAIN, 0, 4)
 
 // ---------------------------------------------------------------------------------------
 // Bloom (bloom.rs:22-125): the pyramid lives after the full-resolution HDR image.

@@ -121,33 +121,6 @@ struct StackSel<SK, LANES, false> {
     __device__ static void init(type &s, uint2 *, uint32_t, Private &) { s.n = 0; }
 };
 
-struct TraceArgs {
-    Scene S;
-    Views V;
-    uint32_t num_tiles;
-    uint32_t *slot;
-    uint32_t *mask;   // with shadows traced here: per-pixel occlusion bits (bit 0 light, 1 + i flash i)
-    uint32_t *queue;  // persistent variant: ray-queue counter (zeroed per frame)
-    unsigned long long *stats;
-    uint32_t wave_shape;  // pixels of a tile one wave takes: 0 an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band
-    const uint32_t *order;  // tile of each scheduling slot (nullptr: slot = tile), heaviest first
-    uint32_t *cost;         // per tile quarter: the longest traversal of its lanes, for the next order
-    uint32_t prio_slots;    // tiles in the first slots of the order run at a raised wave priority
-    uint32_t postpone;      // 0, or uninstrumented closest-hit rays postpone their leaves (trace_pp) and
-                            // are traced again by trace() past this many stack entries
-    specrel_wave_clock *clock = nullptr;  // or one record per wave (scheduling slot): start, end, where
-};
-
-struct OccludeArgs {
-    Scene S;
-    const ShadowQuery *queue;
-    const uint32_t *count;  // queries enqueued
-    uint32_t *next;         // grab counter (zeroed per frame)
-    uint32_t cap;
-    uint32_t *mask;
-    unsigned long long *stats;
-};
-
 struct ShadeArgs {
     Scene S;
     Tables T;
@@ -171,6 +144,36 @@ struct ShadeArgs {
     uint32_t qcap;
     unsigned long long *stats;
 };
+
+struct TraceArgs {
+    Scene S;
+    Views V;
+    uint32_t num_tiles;
+    uint32_t *slot;
+    uint32_t *mask;   // with shadows traced here: per-pixel occlusion bits (bit 0 light, 1 + i flash i)
+    uint32_t *queue;  // persistent variant: ray-queue counter (zeroed per frame)
+    unsigned long long *stats;
+    uint32_t wave_shape;  // pixels of a tile one wave takes: 0 an 8x8 quarter, 1 a 16x4 band, 2 a 4x16 band
+    const uint32_t *order;  // tile of each scheduling slot (nullptr: slot = tile), heaviest first
+    uint32_t *cost;         // per tile quarter: the longest traversal of its lanes, for the next order
+    uint32_t prio_slots;    // tiles in the first slots of the order run at a raised wave priority
+    uint32_t postpone;      // 0, or uninstrumented closest-hit rays postpone their leaves (trace_pp) and
+                            // are traced again by trace() past this many stack entries
+    specrel_wave_clock *clock = nullptr;  // or one record per wave (scheduling slot): start, end, where
+    ShadeArgs shade;  // the shading pass's arguments, for the kernel that shades its own pixels
+    bool shade_here = false;  // (host side) launch the kernel that shades its own pixels
+};
+
+struct OccludeArgs {
+    Scene S;
+    const ShadowQuery *queue;
+    const uint32_t *count;  // queries enqueued
+    uint32_t *next;         // grab counter (zeroed per frame)
+    uint32_t cap;
+    uint32_t *mask;
+    unsigned long long *stats;
+};
+
 
 __global__ void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out, StarDir *dirs);
 __global__ void k_star_bin(Views V, const StarDir *dirs, uint32_t num_stars, uint32_t *lists, uint32_t *counts,
@@ -209,9 +212,10 @@ __global__ void k_adj_reduce(AdjArgs a);
 __global__ void k_adj_final(AdjArgs a);
 
 // TV: traversal variant (TV_PLAIN, TV_MERGED: merged load rounds for a wave's node and leaf
-// lanes, TV_WIDE4: fetch width 4, TV_LDSTOP: top levels from LDS); SH: the hit's shadow queries
-// traced here; W: waves per block (each wave one quarter of a 16x16 tile), 1, 2 or 4.
-template <bool STATS, int SK, int TV, bool SH, int W = 4>
+// lanes, TV_WIDE4: fetch width 4, TV_LDSTOP: top levels from LDS); SH: 1 the hit's shadow queries
+// traced here, 2 also the pixels shaded here; W: waves per block (each wave one quarter of a
+// 16x16 tile), 1, 2 or 4.
+template <bool STATS, int SK, int TV, int SH, int W = 4>
 __global__ void k_trace(TraceArgs a);
 template <bool STATS, int SK>
 __global__ void k_trace_p(TraceArgs a);

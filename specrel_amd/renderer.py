@@ -26,6 +26,7 @@ OPT_ADJOINT_MASK = 14
 OPT_WAVE_CLOCK = 15
 OPT_NODE_WIDTH = 16
 OPT_LDS_TOP = 17
+OPT_FUSED_SHADE = 18
 # the library's defaults for SPECREL_OPT_TRACE_WAVES and SPECREL_OPT_LDS_TOP
 DEFAULT_TRACE_WAVES, DEFAULT_LDS_TOP = 1, 0
 # specrel_timings.trace_variant: the primary-ray kernel's traversal
@@ -117,7 +118,8 @@ class Tuning(C.Structure):
     _fields_ = [("stack", C.c_int32), ("persistent", C.c_int32), ("shadow_queue", C.c_int32), ("graph", C.c_int32),
                 ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
                 ("shadow_in_trace", C.c_int32), ("wave_shape", C.c_int32), ("leaf_postpone", C.c_int32),
-                ("trace_waves", C.c_int32), ("node_width", C.c_int32), ("lds_top", C.c_int32)]
+                ("trace_waves", C.c_int32), ("node_width", C.c_int32), ("lds_top", C.c_int32),
+                ("fused_shade", C.c_int32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

@@ -46,7 +46,7 @@ def test_layouts_match_the_header():
     assert C.sizeof(R.Timings) == 56 and R.Timings.trace_block_waves.offset == 48
     assert R.Timings.trace_variant.offset == 52
     assert C.sizeof(R.ViewOut) == 40 and R.ViewOut.len.offset == 32
-    assert C.sizeof(R.Tuning) == 52 and R.Tuning.trace_waves.offset == 40 and R.Tuning.lds_top.offset == 48
+    assert C.sizeof(R.Tuning) == 56 and R.Tuning.trace_waves.offset == 40 and R.Tuning.fused_shade.offset == 52
     assert C.sizeof(R.AdjObject) == 24 and C.sizeof(R.AdjointOut) == 216 and C.sizeof(R.Range) == 8
     assert C.sizeof(F.Object) == 32
 

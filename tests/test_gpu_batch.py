@@ -7,7 +7,8 @@ import pytest
 
 from specrel_amd import scene as S
 from specrel_amd.renderer import DEFAULT_LDS_TOP, DEFAULT_TRACE_WAVES
-from specrel_amd.renderer import (OPT_GRAPH, OPT_LDS_TOP, OPT_LEAF_POSTPONE, OPT_NODE_WIDTH, OPT_PERSISTENT,
+from specrel_amd.renderer import (OPT_FUSED_SHADE, OPT_GRAPH, OPT_LDS_TOP, OPT_LEAF_POSTPONE, OPT_NODE_WIDTH,
+                                  OPT_PERSISTENT,
                                   OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE, OPT_STACK, OPT_STATS, OPT_TRACE_WAVES,
                                   OPT_WAVE_SHAPE)
 from oracle import pyoracle as O
@@ -92,8 +93,11 @@ def test_graph_replay_tracks_new_frames_and_shapes(gpu_ctx, twin):
         assert_views_equal([d], [gpu_ctx.render_frame(0, frames[i], want_hits=True)], f"download view {i}")
 
 
-@pytest.mark.parametrize("stack,persistent,queue", [(8, 1, 0), (16, 0, 1), (16, 1, 1), (-8, 0, 0)])
-def test_batch_launch_variants(gpu_ctx, twin, stack, persistent, queue):
+@pytest.mark.parametrize("stack,persistent,queue,fused", [(8, 1, 0, 1), (16, 0, 1, 1), (16, 1, 1, 1), (-8, 0, 0, 1),
+                                                           (16, 0, 0, 2), (8, 0, 0, 2), (16, 0, 0, 0)])
+def test_batch_launch_variants(gpu_ctx, twin, stack, persistent, queue, fused):
+    # (fused 2: every view of the batch shaded by the primary-ray kernel; 0: single views too by the
+    # shading pass)
     gpu_ctx.upload(0, twin)
     frames = twin_frames(twin, 21)
     ref = singles(gpu_ctx, frames)
@@ -101,11 +105,15 @@ def test_batch_launch_variants(gpu_ctx, twin, stack, persistent, queue):
         gpu_ctx.set_option(OPT_STACK, stack)
         gpu_ctx.set_option(OPT_PERSISTENT, persistent)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, queue)
-        assert_views_equal(gpu_ctx.render_batch(0, frames), ref, f"stack={stack} p={persistent} q={queue}")
+        gpu_ctx.set_option(OPT_FUSED_SHADE, fused)
+        what = f"stack={stack} p={persistent} q={queue} fused={fused}"
+        assert_views_equal(gpu_ctx.render_batch(0, frames), ref, what)
+        assert_views_equal(singles(gpu_ctx, frames), ref, what + ", single views")
     finally:
         gpu_ctx.set_option(OPT_STACK, 16)
         gpu_ctx.set_option(OPT_PERSISTENT, 0)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)
+        gpu_ctx.set_option(OPT_FUSED_SHADE, 1)
 
 
 def test_star_pass_on_a_second_stream_gives_the_same_views(gpu_ctx, twin):
@@ -145,10 +153,11 @@ def test_batch_stats_are_the_sum_of_single_views(gpu_ctx, twin):
         gpu_ctx.set_option(OPT_STATS, 0)
 
 
-def test_tile_order_over_many_tiles(gpu_ctx, twin):
+@pytest.mark.parametrize("fused", [1, 2])
+def test_tile_order_over_many_tiles(gpu_ctx, twin, fused):
     # 16,676 tiles: past one block's share (4,096 tiles), the next render's tile order is sorted
     # by blocks of 1,024 tiles in two launches. Renders in that order, and in row order, are the same views
-    # with the same counters.
+    # with the same counters; also with the views shaded by the primary-ray kernel (fused 2).
     from specrel_amd.renderer import OPT_TILE_ORDER
     gpu_ctx.upload(0, twin)
     rng = np.random.default_rng(51)
@@ -160,6 +169,7 @@ def test_tile_order_over_many_tiles(gpu_ctx, twin):
         cam.tetrad.boost_local(rng.uniform(-0.5, 0.5, 3).astype(np.float32))
         frames.append(S.build_frame(cam, S.RenderConfig(w, h)))
     try:
+        gpu_ctx.set_option(OPT_FUSED_SHADE, fused)
         gpu_ctx.set_option(OPT_TILE_ORDER, 0)
         ref = gpu_ctx.render_batch(0, frames)
         assert_views_equal([ref[0], ref[4]], [gpu_ctx.render_frame(0, frames[i], want_hits=True) for i in (0, 4)],
@@ -177,6 +187,7 @@ def test_tile_order_over_many_tiles(gpu_ctx, twin):
     finally:
         gpu_ctx.set_option(OPT_STATS, 0)
         gpu_ctx.set_option(OPT_TILE_ORDER, 1)
+        gpu_ctx.set_option(OPT_FUSED_SHADE, 1)
 
 
 def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
@@ -190,15 +201,16 @@ def test_autotune_picks_a_valid_configuration(gpu_ctx, twin):
         assert t["leaf_postpone"] in (0, 1)
         # 48 + the current options + the postponement flip, + the two other block sizes when the
         # pick is tiled with an LDS stack, + the other node fetch width and 4, 6 or 8 top levels
-        # in LDS
+        # in LDS, + the shading fused into the primary rays of every view
         tiled = t["stack"] > 0 and not t["persistent"]
         assert t["trace_waves"] in ((1, 2, 4) if tiled else (4,))
-        assert t["node_width"] in (2, 4) and t["lds_top"] in (0, 4, 6, 8)
-        assert t["candidates"] == (56 if tiled else 50) and 0 < t["best_ms"] <= t["default_ms"]
+        assert t["node_width"] in (2, 4) and t["lds_top"] in (0, 4, 6, 8) and t["fused_shade"] in (1, 2)
+        assert t["candidates"] == (57 if tiled else 50) and 0 < t["best_ms"] <= t["default_ms"]
         assert_views_equal(gpu_ctx.render_batch(0, frames), ref, "autotuned")
     finally:
         gpu_ctx.set_option(OPT_NODE_WIDTH, 2)
         gpu_ctx.set_option(OPT_LDS_TOP, DEFAULT_LDS_TOP)
+        gpu_ctx.set_option(OPT_FUSED_SHADE, 1)
         gpu_ctx.set_option(OPT_STACK, 16)
         gpu_ctx.set_option(OPT_PERSISTENT, 0)
         gpu_ctx.set_option(OPT_SHADOW_QUEUE, 0)

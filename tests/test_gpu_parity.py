@@ -314,6 +314,45 @@ def test_node_width_and_lds_top_levels_are_exact(gpu_ctx, twin, width, top, stac
         gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
 
 
+@pytest.mark.parametrize("waves,stack,shape,top", [(1, 16, 0, 0), (2, 16, 1, 0), (4, 16, 2, 0), (1, 8, 0, 0),
+                                                   (4, 8, 0, 0), (2, 16, 0, 6), (1, 16, 0, 5)])
+def test_fused_shading_equals_the_shading_pass(gpu_ctx, twin, waves, stack, shape, top):
+    # The primary-ray kernel shading its own pixels (its tile's star list culled into the LDS its
+    # stacks used) and the separate shading pass give the oracle's image: views with flashes and
+    # haze, decision-age views, and a scene with stars, for every block size, wave shape and stack.
+    from specrel_amd.renderer import (OPT_FUSED_SHADE, OPT_LDS_TOP, OPT_STACK, OPT_TRACE_WAVES, OPT_WAVE_SHAPE)
+    p = S.gen_params("large", n_objects=12, slices=32, n_stars=400)
+    big = S.Scene.generate("large", p)
+    cam = twin.default_camera()
+    cam.tetrad.boost_local((0.2, 0.1, 0.4))
+    q = cam.position
+    dyn = S.FrameDynamics(flashes=[S.FlashEvent((q[0] - 6.0, q[1], q[2], q[3]), 1.2, 2.0, (1.0, 0.3, 0.2))])
+    cases = [(twin, S.build_frame(cam, S.RenderConfig(200, 136), dyn), "twin flashes"),
+             (twin, S.build_frame(cam, S.RenderConfig(144, 90, S.DecisionAge, S.ShadowOff, S.PastLightCone)),
+              "twin decision age"),
+             (big, S.build_frame(S.default_camera("large", p), S.RenderConfig(300, 190)), "large 12x32, 400 stars")]
+    try:
+        gpu_ctx.set_option(OPT_STACK, stack)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, waves)
+        gpu_ctx.set_option(OPT_WAVE_SHAPE, shape)
+        gpu_ctx.set_option(OPT_LDS_TOP, top)
+        for sc, fr, name in cases:
+            o = O.render(sc, fr)
+            gpu_ctx.upload(0, sc)
+            for fused in (1, 0):
+                gpu_ctx.set_option(OPT_FUSED_SHADE, fused)
+                g = gpu_ctx.render_frame(0, fr, want_hits=True)
+                assert_same(g, o, f"{name}, fused {fused}, waves {waves}, stack {stack}, shape {shape}, top {top}")
+                if fused:  # no shading pass of its own ran
+                    assert gpu_ctx.timings()["shade_ms"] == 0.0, name
+    finally:
+        gpu_ctx.set_option(OPT_FUSED_SHADE, 1)
+        gpu_ctx.set_option(OPT_STACK, 16)
+        gpu_ctx.set_option(OPT_TRACE_WAVES, DEFAULT_TRACE_WAVES)
+        gpu_ctx.set_option(OPT_WAVE_SHAPE, 0)
+        gpu_ctx.set_option(OPT_LDS_TOP, DEFAULT_LDS_TOP)
+
+
 @pytest.mark.parametrize("waves,stars", [(4, 100), (2, 100), (1, 100), (4, 0)])
 def test_tile_order_changes_nothing(gpu_ctx, waves, stars):
     # From the second render of the same views on, the primary-ray tiles start slowest first (the
