@@ -1192,15 +1192,33 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         if (!small) break;
         --k0;
     }
-    for (int k = 1; k < k0; ++k)
-        k_bloom_down<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
+    // Down pass: levels below k0 two at a time (one if a single one is left), then the tail.
+    for (int k = 1; k < k0;) {
+        if (k + 1 < k0) {
+            k_bloom_down2<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
+            k += 2;
+        } else {
+            k_bloom_down<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
+            k += 1;
+        }
+    }
     k_bloom_tail<<<nv, 1024, 0, s>>>(V, k0, pyr);
-    for (int k = k0 - 1; k >= 1; --k)
-        k_bloom_up<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
+    // Up pass from level k0 - 1 down: an odd one out alone, then pairs, the last two levels (or
+    // the one) with the composite.
+    int top = k0 - 1;
+    const int nup = std::min(top, 2);
+    if ((top - nup) % 2 == 1) {
+        k_bloom_up<<<B.start(top, nv), 256, 0, s>>>(V, top, pyr);
+        --top;
+    }
+    for (; top > nup; top -= 2) k_bloom_up2<<<B.start(top - 1, nv), 256, 0, s>>>(V, top - 1, pyr);
     check(hipGetLastError(), "bloom launch");
     mark(4);
-    k_composite<<<tiles, 256, 0, s>>>(V, pyr, (const uint8_t *)ctx->tables.p, B.rgb);
-    check(hipGetLastError(), "k_composite launch");
+    const uint8_t *lut = (const uint8_t *)ctx->tables.p;
+    if (nup == 2) k_bloom_final<2><<<tiles, 256, 0, s>>>(V, pyr, lut, B.rgb);
+    else if (nup == 1) k_bloom_final<1><<<tiles, 256, 0, s>>>(V, pyr, lut, B.rgb);
+    else k_bloom_final<0><<<tiles, 256, 0, s>>>(V, pyr, lut, B.rgb);
+    check(hipGetLastError(), "k_bloom_final launch");
     if (order_side) check(hipStreamWaitEvent(s, ctx->order_join, 0), "join wait");
     mark(5);
 }

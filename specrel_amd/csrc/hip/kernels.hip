@@ -827,34 +827,45 @@ __device__ __forceinline__ LevelPix level_pix(const Views &V, int k) {
 }
 
 // bloom::downsample (bloom.rs:54-75): pixel (x, y) of a level from the finer level src.
-__device__ __forceinline__ void downsample_px(const float4 *src, uint32_t sw, uint32_t sh, float4 *dst, uint32_t dw,
-                                              uint32_t x, uint32_t y) {
+// The value of pixel (x, y) of a level from the finer level, whose pixel (sx, sy) src(sx, sy)
+// returns (taps clamped into the finer level).
+template <class F>
+__device__ __forceinline__ V3 down_at(F src, uint32_t sw, uint32_t sh, uint32_t x, uint32_t y) {
     const float W[4] = {0.125f, 0.375f, 0.375f, 0.125f};
     V3 acc = V3{0.0f, 0.0f, 0.0f};
     for (int j = 0; j < 4; ++j) {
         int64_t syy = 2 * (int64_t)y - 1 + j;
-        size_t sy = (size_t)(syy < 0 ? 0 : (syy > (int64_t)sh - 1 ? (int64_t)sh - 1 : syy));
+        uint32_t sy = (uint32_t)(syy < 0 ? 0 : (syy > (int64_t)sh - 1 ? (int64_t)sh - 1 : syy));
         V3 row = V3{0.0f, 0.0f, 0.0f};
         for (int i = 0; i < 4; ++i) {
             int64_t sxx = 2 * (int64_t)x - 1 + i;
-            size_t sx = (size_t)(sxx < 0 ? 0 : (sxx > (int64_t)sw - 1 ? (int64_t)sw - 1 : sxx));
-            row += ld3(src, sy * sw + sx) * W[i];
+            uint32_t sx = (uint32_t)(sxx < 0 ? 0 : (sxx > (int64_t)sw - 1 ? (int64_t)sw - 1 : sxx));
+            row += src(sx, sy) * W[i];
         }
         acc += row * W[j];
     }
+    return acc;
+}
+__device__ __forceinline__ void downsample_px(const float4 *src, uint32_t sw, uint32_t sh, float4 *dst, uint32_t dw,
+                                              uint32_t x, uint32_t y) {
+    const V3 acc = down_at([&](uint32_t sx, uint32_t sy) { return ld3(src, (size_t)sy * sw + sx); }, sw, sh, x, y);
     dst[(size_t)y * dw + x] = make_float4(acc.x, acc.y, acc.z, 0.0f);
 }
 
-// bloom::upsample_at (bloom.rs:80-90)
-__device__ __forceinline__ V3 upsample_at(const float4 *coarse, uint32_t cw, uint32_t ch, uint32_t x, uint32_t y) {
+// bloom::upsample_at (bloom.rs:80-90), the coarse level's pixel (cx, cy) given by c(cx, cy).
+template <class F>
+__device__ __forceinline__ V3 up_at(F c, uint32_t cw, uint32_t ch, uint32_t x, uint32_t y) {
     uint32_t mx = x / 2, my = y / 2;
     int64_t ox = (x % 2 == 0) ? (int64_t)mx - 1 : (int64_t)mx + 1;
     int64_t oy = (y % 2 == 0) ? (int64_t)my - 1 : (int64_t)my + 1;
     uint32_t x1 = (uint32_t)(ox < 0 ? 0 : (ox > (int64_t)cw - 1 ? (int64_t)cw - 1 : ox));
     uint32_t y1 = (uint32_t)(oy < 0 ? 0 : (oy > (int64_t)ch - 1 ? (int64_t)ch - 1 : oy));
-    V3 r0 = ld3(coarse, (size_t)my * cw + mx) * 0.75f + ld3(coarse, (size_t)my * cw + x1) * 0.25f;
-    V3 r1 = ld3(coarse, (size_t)y1 * cw + mx) * 0.75f + ld3(coarse, (size_t)y1 * cw + x1) * 0.25f;
+    V3 r0 = c(mx, my) * 0.75f + c(x1, my) * 0.25f;
+    V3 r1 = c(mx, y1) * 0.75f + c(x1, y1) * 0.25f;
     return r0 * 0.75f + r1 * 0.25f;
+}
+__device__ __forceinline__ V3 upsample_at(const float4 *coarse, uint32_t cw, uint32_t ch, uint32_t x, uint32_t y) {
+    return up_at([&](uint32_t cx, uint32_t cy) { return ld3(coarse, (size_t)cy * cw + cx); }, cw, ch, x, y);
 }
 
 // One step of the up pass (bloom.rs:110-114) at pixel (x, y): fine += upsample(coarse).
@@ -916,6 +927,19 @@ __global__ __launch_bounds__(1024) void k_bloom_tail(Views V, int k0, float4 *py
 }
 
 // Final mix (bloom.rs:116-124) and pack_rgb_with_lut (renderer.rs:1116-1129), every view.
+// bloom::apply's composite (bloom.rs:93-125) of pixel (x, y) from the linear image and level 1
+// (pixel (cx, cy) given by l1(cx, cy)), tone map and LUT pack (renderer.rs:1116-1129).
+template <class F>
+__device__ __forceinline__ uint32_t composite_at(const float4 *hdr, F l1, uint32_t w, uint32_t w1, uint32_t h1,
+                                                 uint32_t x, uint32_t y, const uint8_t *lut) {
+    size_t i = (size_t)y * w + x;
+    float halo = 0.04f / 6.0f;
+    V3 c = ld3(hdr, i) * (1.0f - 0.04f) + up_at(l1, w1, h1, x, y) * halo;
+    V3 m = clamp3(V3{c.x / (1.0f + c.x * 0.35f), c.y / (1.0f + c.y * 0.35f), c.z / (1.0f + c.z * 0.35f)}, 0.0f, 1.0f);
+    const float scale = 4095.0f;
+    uint32_t r = lut[f2u_sat(m.x * scale)], g = lut[f2u_sat(m.y * scale)], b = lut[f2u_sat(m.z * scale)];
+    return (r << 16) | (g << 8) | b;
+}
 __global__ __launch_bounds__(256) void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out) {
     const LevelPix p = level_pix(V, 0);
     const uint32_t w = p.w, h = p.h, x = p.x, y = p.y;
@@ -924,13 +948,173 @@ __global__ __launch_bounds__(256) void k_composite(Views V, const float4 *pyr, c
     level_size(w, h, 1, w1, h1);
     const float4 *hdr = pyr + V.pyr_start[p.vi];
     const float4 *l1 = hdr + level_offset(w, h, 1);
-    size_t i = (size_t)y * w + x;
-    float halo = 0.04f / 6.0f;
-    V3 c = ld3(hdr, i) * (1.0f - 0.04f) + upsample_at(l1, w1, h1, x, y) * halo;
-    V3 m = clamp3(V3{c.x / (1.0f + c.x * 0.35f), c.y / (1.0f + c.y * 0.35f), c.z / (1.0f + c.z * 0.35f)}, 0.0f, 1.0f);
-    const float scale = 4095.0f;
-    uint32_t r = lut[f2u_sat(m.x * scale)], g = lut[f2u_sat(m.y * scale)], b = lut[f2u_sat(m.z * scale)];
-    out[V.pix_start[p.vi] + i] = (r << 16) | (g << 8) | b;
+    out[V.pix_start[p.vi] + (size_t)y * w + x] = composite_at(
+        hdr, [&](uint32_t cx, uint32_t cy) { return ld3(l1, (size_t)cy * w1 + cx); }, w, w1, h1, x, y, lut);
 }
+
+// Fused bloom passes. A block takes a 16x16 tile of one level and keeps in LDS the rectangle of
+// a neighbouring level its pixels read (taps clamped into that level, so the rectangle spans at
+// most 18 x 18 pixels); each pixel is computed with the same operations as by the one-level
+// kernels, so the pyramid and the image are the same.
+struct LdsRect {
+    float *v;  // 3 floats per pixel, row-major
+    uint32_t x0, y0, nx;
+    __device__ V3 at(uint32_t x, uint32_t y) const {
+        const float *p = v + 3 * ((size_t)(y - y0) * nx + (x - x0));
+        return V3{p[0], p[1], p[2]};
+    }
+    __device__ void put(uint32_t i, V3 c) const {
+        v[3 * i] = c.x, v[3 * i + 1] = c.y, v[3 * i + 2] = c.z;
+    }
+};
+// The rectangle of a coarser level (cw x ch) that up_at reads for fine pixels x in [fx0, fx1].
+__device__ __forceinline__ void up_span(uint32_t fx0, uint32_t fx1, uint32_t cw, uint32_t &a, uint32_t &b) {
+    a = fx0 / 2 > 0 ? fx0 / 2 - 1 : 0;
+    b = min(fx1 / 2 + 1, cw - 1);
+}
+struct TileOf {
+    uint32_t vi, w0, h0, tx0, ty0;
+};
+__device__ __forceinline__ TileOf tile_of(const Views &V, int k) {
+    TileOf o;
+    const uint32_t t = blockIdx.x;
+    o.vi = find_view(V.level_start[k], V.n, t);
+    const specrel_frame &f = V.frames[o.vi];
+    o.w0 = f.size[0];
+    o.h0 = f.size[1];
+    uint32_t lw, lh;
+    level_size(o.w0, o.h0, k, lw, lh);
+    const uint32_t lt = t - V.level_start[k][o.vi], tiles_x = (lw + 15) / 16;
+    o.tx0 = (lt % tiles_x) * 16;
+    o.ty0 = (lt / tiles_x) * 16;
+    return o;
+}
+
+// Levels k and k + 1 of the down pass: the tile of level k (from level k - 1) and the level-(k+1)
+// pixels X, Y with 2X, 2Y in the tile (from the tile and its one-pixel rim, in LDS).
+__global__ __launch_bounds__(256) void k_bloom_down2(Views V, int k, float4 *pyr) {
+    __shared__ float s_lv[18 * 18 * 3];
+    const TileOf o = tile_of(V, k);
+    uint32_t sw, sh, dw, dh, ew, eh;
+    level_size(o.w0, o.h0, k - 1, sw, sh);
+    level_size(o.w0, o.h0, k, dw, dh);
+    level_size(o.w0, o.h0, k + 1, ew, eh);
+    float4 *base = pyr + V.pyr_start[o.vi];
+    const float4 *src = base + level_offset(o.w0, o.h0, k - 1);
+    float4 *lk = base + level_offset(o.w0, o.h0, k);
+    float4 *lk1 = base + level_offset(o.w0, o.h0, k + 1);
+    const uint32_t rx0 = o.tx0 > 0 ? o.tx0 - 1 : 0, ry0 = o.ty0 > 0 ? o.ty0 - 1 : 0;
+    const uint32_t rx1 = min(o.tx0 + 16, dw - 1), ry1 = min(o.ty0 + 16, dh - 1);
+    const LdsRect R{s_lv, rx0, ry0, rx1 - rx0 + 1};
+    const uint32_t n = R.nx * (ry1 - ry0 + 1);
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        const uint32_t x = rx0 + i % R.nx, y = ry0 + i / R.nx;
+        const V3 v = down_at([&](uint32_t a, uint32_t b) { return ld3(src, (size_t)b * sw + a); }, sw, sh, x, y);
+        R.put(i, v);
+        if (x >= o.tx0 && x < o.tx0 + 16 && y >= o.ty0 && y < o.ty0 + 16)
+            lk[(size_t)y * dw + x] = make_float4(v.x, v.y, v.z, 0.0f);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t X = o.tx0 / 2 + (threadIdx.x & 7), Y = o.ty0 / 2 + (threadIdx.x >> 3);
+        if (X < ew && Y < eh) {
+            const V3 v = down_at([&](uint32_t a, uint32_t b) { return R.at(a, b); }, dw, dh, X, Y);
+            lk1[(size_t)Y * ew + X] = make_float4(v.x, v.y, v.z, 0.0f);
+        }
+    }
+}
+
+// Levels k + 1 and k of the up pass: level k + 1 updated from level k + 2 (final) over the rectangle
+// the tile of level k reads (in LDS only: nothing later reads it), then the tile of level k
+// updated from it and written.
+__global__ __launch_bounds__(256) void k_bloom_up2(Views V, int k, float4 *pyr) {
+    __shared__ float s_lv[10 * 10 * 3];
+    const TileOf o = tile_of(V, k);
+    uint32_t fw, fh, cw, ch, gw, gh;
+    level_size(o.w0, o.h0, k, fw, fh);
+    level_size(o.w0, o.h0, k + 1, cw, ch);
+    level_size(o.w0, o.h0, k + 2, gw, gh);
+    float4 *base = pyr + V.pyr_start[o.vi];
+    float4 *lk = base + level_offset(o.w0, o.h0, k);
+    const float4 *lk1 = base + level_offset(o.w0, o.h0, k + 1);
+    const float4 *lk2 = base + level_offset(o.w0, o.h0, k + 2);
+    uint32_t rx0, rx1, ry0, ry1;
+    up_span(o.tx0, min(o.tx0 + 15, fw - 1), cw, rx0, rx1);
+    up_span(o.ty0, min(o.ty0 + 15, fh - 1), ch, ry0, ry1);
+    const LdsRect R{s_lv, rx0, ry0, rx1 - rx0 + 1};
+    const uint32_t n = R.nx * (ry1 - ry0 + 1);
+    for (uint32_t i = threadIdx.x; i < n; i += 256) {
+        const uint32_t X = rx0 + i % R.nx, Y = ry0 + i / R.nx;
+        R.put(i, ld3(lk1, (size_t)Y * cw + X) +
+                     up_at([&](uint32_t a, uint32_t b) { return ld3(lk2, (size_t)b * gw + a); }, gw, gh, X, Y));
+    }
+    __syncthreads();
+    const uint32_t x = o.tx0 + (threadIdx.x & 15), y = o.ty0 + (threadIdx.x >> 4);
+    if (x < fw && y < fh) {
+        const size_t i = (size_t)y * fw + x;
+        const V3 c = ld3(lk, i) + up_at([&](uint32_t a, uint32_t b) { return R.at(a, b); }, cw, ch, x, y);
+        lk[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
+// The last NUP levels of the up pass (2: levels 2 and 1; 1: level 1; 0: none) over the rectangles a
+// tile of the image reads, in LDS, then the composite of the tile.
+template <int NUP>
+__global__ __launch_bounds__(256) void k_bloom_final(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out) {
+    __shared__ float s_l1[10 * 10 * 3];
+    __shared__ float s_l2[(NUP == 2 ? 8 * 8 : 1) * 3];
+    const TileOf o = tile_of(V, 0);
+    const uint32_t w = o.w0, h = o.h0;
+    uint32_t w1, h1, w2, h2, w3, h3;
+    level_size(w, h, 1, w1, h1);
+    level_size(w, h, 2, w2, h2);
+    level_size(w, h, 3, w3, h3);
+    const float4 *hdr = pyr + V.pyr_start[o.vi];
+    const float4 *l1 = hdr + level_offset(w, h, 1);
+    const float4 *l2 = hdr + level_offset(w, h, 2);
+    const float4 *l3 = hdr + level_offset(w, h, 3);
+    uint32_t ax0, ax1, ay0, ay1;  // the level-1 rectangle the tile reads
+    up_span(o.tx0, min(o.tx0 + 15, w - 1), w1, ax0, ax1);
+    up_span(o.ty0, min(o.ty0 + 15, h - 1), h1, ay0, ay1);
+    const LdsRect R1{s_l1, ax0, ay0, ax1 - ax0 + 1};
+    if constexpr (NUP >= 1) {
+        const uint32_t n1 = R1.nx * (ay1 - ay0 + 1);
+        if constexpr (NUP == 2) {
+            uint32_t bx0, bx1, by0, by1;  // the level-2 rectangle those read
+            up_span(ax0, ax1, w2, bx0, bx1);
+            up_span(ay0, ay1, h2, by0, by1);
+            const LdsRect R2{s_l2, bx0, by0, bx1 - bx0 + 1};
+            const uint32_t n2 = R2.nx * (by1 - by0 + 1);
+            for (uint32_t i = threadIdx.x; i < n2; i += 256) {
+                const uint32_t X = bx0 + i % R2.nx, Y = by0 + i / R2.nx;
+                R2.put(i, ld3(l2, (size_t)Y * w2 + X) +
+                              up_at([&](uint32_t a, uint32_t b) { return ld3(l3, (size_t)b * w3 + a); }, w3, h3, X, Y));
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n1; i += 256) {
+                const uint32_t X = ax0 + i % R1.nx, Y = ay0 + i / R1.nx;
+                R1.put(i, ld3(l1, (size_t)Y * w1 + X) + up_at([&](uint32_t a, uint32_t b) { return R2.at(a, b); }, w2, h2, X, Y));
+            }
+        } else {
+            for (uint32_t i = threadIdx.x; i < n1; i += 256) {
+                const uint32_t X = ax0 + i % R1.nx, Y = ay0 + i / R1.nx;
+                R1.put(i, ld3(l1, (size_t)Y * w1 + X) +
+                              up_at([&](uint32_t a, uint32_t b) { return ld3(l2, (size_t)b * w2 + a); }, w2, h2, X, Y));
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t x = o.tx0 + (threadIdx.x & 15), y = o.ty0 + (threadIdx.x >> 4);
+    if (x >= w || y >= h) return;
+    uint32_t c;
+    if constexpr (NUP >= 1)
+        c = composite_at(hdr, [&](uint32_t a, uint32_t b) { return R1.at(a, b); }, w, w1, h1, x, y, lut);
+    else
+        c = composite_at(hdr, [&](uint32_t a, uint32_t b) { return ld3(l1, (size_t)b * w1 + a); }, w, w1, h1, x, y, lut);
+    out[V.pix_start[o.vi] + (size_t)y * w + x] = c;
+}
+template __global__ void k_bloom_final<0>(Views, const float4 *, const uint8_t *, uint32_t *);
+template __global__ void k_bloom_final<1>(Views, const float4 *, const uint8_t *, uint32_t *);
+template __global__ void k_bloom_final<2>(Views, const float4 *, const uint8_t *, uint32_t *);
 
 }  // namespace dev

@@ -254,6 +254,12 @@ __global__ void k_bloom_tail(Views V, int k0, float4 *pyr);
 // view's level holds at most this many pixels.
 constexpr uint32_t BLOOM_TAIL_PIXELS = 2560;
 __global__ void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);
+// Fused bloom passes (one block per 16x16 tile of the level named): levels k and k + 1 of the down
+// pass; levels k + 1 and k of the up pass; the last NUP (0..2) levels of the up pass and the composite.
+__global__ void k_bloom_down2(Views V, int k, float4 *pyr);
+__global__ void k_bloom_up2(Views V, int k, float4 *pyr);
+template <int NUP>
+__global__ void k_bloom_final(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);
 
 // Pyramid geometry of bloom.rs:28-49, shared by host and device.
 __host__ __device__ __forceinline__ void level_size(uint32_t w, uint32_t h, int k, uint32_t &ow, uint32_t &oh) {
