@@ -444,158 +444,265 @@ __global__ __launch_bounds__(256) void k_bb_permute(uint32_t begin, uint32_t cou
     move_ref(cur, begin + s, small ? fin : nxt, begin + k);
 }
 
-// A subtree of at most SMALL references, built sequentially by one thread exactly as the host
-// builds it (scene.cpp split / build_recursive), in place over fin, nodes in preorder to
-// scratch[2 * begin ...] with subtree-local child indices.
+// A subtree of at most SMALL references, built by one wave exactly as the host builds it
+// (scene.cpp split / build_recursive), its references staged in LDS:
+//  - a node's bounds and centroid bounds: 16 lanes, one component each, fold the references in
+//    order (the host's sequence of min / max operations per component);
+//  - the SAH bins: lane (axis, bin, min or max half) folds, in order, the references of its bin
+//    (the host's sequence per bin); every lane then sweeps the bins from LDS (the same sweep);
+//  - the host's in-place swap partition (bvh4d.rs:409-416): left-going references keep their
+//    order (prefix counts); right-going ones end in the order of a queue in which each one is
+//    appended as met and each left-going reference met after the first right-going one moves the
+//    queue's front to its back. After the first right-going reference f, step k appends stream
+//    entry k - f: reference k itself if it goes right, else the entry (number of right-going
+//    references before k) places earlier; the rights' final positions p hold entries p - f,
+//    resolved by pointer jumping;
+//  - the stable-sort fallback is run by lane 0 as the host runs it.
+// Nodes in preorder to scratch[2 * begin ...] with subtree-local child indices; the references
+// are written back in their final order. One wave per subtree (one block of 64 threads).
+__device__ __forceinline__ float comp_lds(const float4 *a, uint32_t i, int c) { return ((const float *)(a + i))[c]; }
+
 __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_t ntasks, Refs f,
                                                  specrel_node *scratch, uint32_t *task_nodes, uint32_t *err) {
-    const uint32_t t = blockIdx.x * 64 + threadIdx.x;
+    const uint32_t t = blockIdx.x, lane = threadIdx.x;
     if (t >= ntasks) return;
     const SmallTask task = tasks[t];
-    specrel_node *out = scratch + 2 * (size_t)task.begin;
-    const uint32_t cap = 2 * task.count;
-    // A pending range (first reference, count <= SMALL) and the node whose right child it becomes
-    // (a subtree-local index < 2 * SMALL, or NO_PARENT): 8 bytes.
+    const uint32_t n0 = task.count, g0 = task.begin;
+    if (n0 > SMALL) {
+        if (lane == 0) atomicOr(err, ERR_SMALL_STACK);
+        return;
+    }
+    specrel_node *out = scratch + 2 * (size_t)g0;
+    const uint32_t cap = 2 * n0;
+    // A pending range (first reference, local to the task; count <= SMALL) and the node whose right
+    // child it becomes (a subtree-local index < 2 * SMALL, or NO_PARENT).
     struct Ent {
         uint32_t b;
         uint16_t c, parent;
     };
     static_assert(SMALL < 0xFFFFu && 2 * SMALL < 0xFFFFu, "Ent fields");
+    static_assert(sr::BVH_BINS == 8, "one lane per axis, bin and half");
     constexpr uint16_t NO_PARENT = 0xFFFFu;
-    Ent stack[SMALL + 2];  // pending right halves along a path: at most count + 1
-    int sp = 0;
-    stack[sp++] = Ent{task.begin, (uint16_t)task.count, NO_PARENT};
+    constexpr uint32_t RES = 0x80000000u;  // a resolved stream entry: the reference it holds
+    __shared__ float4 s_mn[SMALL], s_mx[SMALL], s_c[SMALL];
+    __shared__ uint32_t s_idx[SMALL], s_src[SMALL], s_ptr[SMALL];
+    __shared__ Ent s_stack[SMALL + 2];
+    __shared__ float4 s_bb[4][8][2];
+    __shared__ uint32_t s_bn[4][8];
+    __shared__ float s_nb[16];
+    for (uint32_t i = lane; i < n0; i += 64) {
+        s_mn[i] = f.mn[g0 + i];
+        s_mx[i] = f.mx[g0 + i];
+        s_c[i] = f.c[g0 + i];
+        s_idx[i] = f.idx[g0 + i];
+    }
+    if (lane == 0) s_stack[0] = Ent{0u, (uint16_t)n0, NO_PARENT};
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const float inf = __builtin_inff();
+    int sp = 1;
     uint32_t nn = 0;
     while (sp > 0) {
-        const Ent e = stack[--sp];
+        const Ent e = s_stack[--sp];
         const uint32_t me = nn++;
         if (me >= cap) {
-            atomicOr(err, ERR_SMALL_NODES);
+            if (lane == 0) atomicOr(err, ERR_SMALL_NODES);
             return;
         }
-        if (e.parent != NO_PARENT) out[e.parent].right_or_first = me;
-        sr::Box4 bounds = sr::box_empty(), cb = sr::box_empty();
-#pragma unroll 4
-        for (uint32_t i = e.b; i < e.b + e.c; ++i) {
-            bounds = sr::box_union(bounds, sr::Box4{v(f.mn[i]), v(f.mx[i])});
-            sr::box_add_point(cb, v(f.c[i]));
+        if (lane == 0 && e.parent != NO_PARENT) out[e.parent].right_or_first = me;
+        const uint32_t eb = e.b, ec = e.c, ee = eb + ec;
+        // Bounds (components 0-7: the references' box union) and centroid bounds (8-15).
+        if (lane < 16) {
+            const int c = lane & 3;
+            const float4 *arr = lane < 4 ? s_mn : lane < 8 ? s_mx : s_c;
+            const bool is_min = lane < 4 || (lane >= 8 && lane < 12);
+            float acc = is_min ? inf : -inf;
+#pragma unroll 8
+            for (uint32_t i = eb; i < ee; ++i) {
+                const float x = comp_lds(arr, i, c);
+                acc = is_min ? sr::fmin_(acc, x) : sr::fmax_(acc, x);
+            }
+            s_nb[lane] = acc;
         }
+        __syncthreads();
+        const sr::Box4 bounds = sr::Box4{sr::v4(s_nb[0], s_nb[1], s_nb[2], s_nb[3]), sr::v4(s_nb[4], s_nb[5], s_nb[6], s_nb[7])};
+        const sr::Box4 cb = sr::Box4{sr::v4(s_nb[8], s_nb[9], s_nb[10], s_nb[11]), sr::v4(s_nb[12], s_nb[13], s_nb[14], s_nb[15])};
         specrel_node nd;
         sr::put4(nd.min, bounds.mn);
         sr::put4(nd.max, bounds.mx);
         nd.pad = 0;
         int kind = K_LEAF, axis = 0;
         float pos = 0.0f;
-        if (e.c > 3) {
+        if (ec > 3) {
             const sr::BinFrame fr = sr::bin_frame(cb);
-            // One pass per pair of axes with those axes' 8 bins each in registers (unions and
-            // counts, so the order of the references changes nothing), each axis then swept for
-            // its cheapest split from the registers (choose_split's sweep, axes in the same order).
-            float best_cost = __builtin_inff();
-            axis = 0;
-#pragma unroll
-            for (int a0 = 0; a0 < 4; a0 += 2) {
-                const bool live0 = fr.live[a0], live1 = fr.live[a0 + 1];
-                if (!live0 && !live1) continue;
-                const float inf = __builtin_inff();
-                float4 bmn[2][8], bmx[2][8];
-                uint32_t bn[2][8];
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        bmn[h][q] = make_float4(inf, inf, inf, inf);
-                        bmx[h][q] = make_float4(-inf, -inf, -inf, -inf);
-                        bn[h][q] = 0;
-                    }
-#pragma unroll 2
-                for (uint32_t i = e.b; i < e.b + e.c; ++i) {
-                    const float4 c = f.c[i], mn = f.mn[i], mx = f.mx[i];
-                    // an axis that is not live puts every reference in a bin no one reads
-                    const uint32_t k0 = live0 ? sr::bin_of(fr, a0, comp(c, a0)) : 0u;
-                    const uint32_t k1 = live1 ? sr::bin_of(fr, a0 + 1, comp(c, a0 + 1)) : 0u;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        if (k0 == (uint32_t)q) {
-                            bmn[0][q] = f4(sr::vmin(v(bmn[0][q]), v(mn)));
-                            bmx[0][q] = f4(sr::vmax(v(bmx[0][q]), v(mx)));
-                            bn[0][q] += 1;
-                        }
-                        if (k1 == (uint32_t)q) {
-                            bmn[1][q] = f4(sr::vmin(v(bmn[1][q]), v(mn)));
-                            bmx[1][q] = f4(sr::vmax(v(bmx[1][q]), v(mx)));
-                            bn[1][q] += 1;
+            {
+                const int a = (int)(lane >> 4), q = (int)((lane >> 1) & 7u), h = (int)(lane & 1u);
+                const float4 *arr = h ? s_mx : s_mn;
+                float4 acc = h ? make_float4(-inf, -inf, -inf, -inf) : make_float4(inf, inf, inf, inf);
+                uint32_t cnt = 0;
+                if (fr.live[a]) {
+#pragma unroll 4
+                    for (uint32_t i = eb; i < ee; ++i) {
+                        const uint32_t k = sr::bin_of(fr, a, comp_lds(s_c, i, a));
+                        const float4 x = arr[i];
+                        if (k == (uint32_t)q) {
+                            acc = h ? f4(sr::vmax(v(acc), v(x))) : f4(sr::vmin(v(acc), v(x)));
+                            cnt += 1;
                         }
                     }
                 }
+                s_bb[a][q][h] = acc;
+                if (!h) s_bn[a][q] = cnt;
+            }
+            __syncthreads();
+            float best_cost = inf;
+            for (int a = 0; a < 4; ++a) {
+                if (!fr.live[a]) continue;
+                sr::Box4 b[8];
+                uint32_t bn[8];
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    if (!(h ? live1 : live0)) continue;
-                    sr::Box4 b[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) b[q] = sr::Box4{v(bmn[h][q]), v(bmx[h][q])};
-                    sr::sah_axis(b, bn[h], fr, a0 + h, best_cost, axis, pos);
+                for (int q = 0; q < 8; ++q) {
+                    b[q] = sr::Box4{v(s_bb[a][q][0]), v(s_bb[a][q][1])};
+                    bn[q] = s_bn[a][q];
                 }
+                sr::sah_axis(b, bn, fr, a, best_cost, axis, pos);
             }
             kind = sr::split_after_sweeps(cb, fr, best_cost, axis, pos);
         }
         if (kind == K_LEAF) {
-            nd.right_or_first = e.b;
-            nd.count = e.c;
-            nd.left = 0;
-            out[me] = nd;
+            if (lane == 0) {
+                nd.right_or_first = g0 + eb;
+                nd.count = ec;
+                nd.left = 0;
+                out[me] = nd;
+            }
+            __syncthreads();
             continue;
         }
-        uint32_t mid = 0;
-        for (uint32_t i = e.b; i < e.b + e.c; ++i) {
-            if (comp(f.c[i], axis) < pos) {
-                const uint32_t j = e.b + mid;
-                const float4 a0 = f.mn[i], a1 = f.mx[i], a2 = f.c[i];
-                const uint32_t a3 = f.idx[i];
-                f.mn[i] = f.mn[j];
-                f.mx[i] = f.mx[j];
-                f.c[i] = f.c[j];
-                f.idx[i] = f.idx[j];
-                f.mn[j] = a0;
-                f.mx[j] = a1;
-                f.c[j] = a2;
-                f.idx[j] = a3;
-                mid += 1;
-            }
+        // Partition: flags, prefix counts of right-going references, the first right-going one.
+        bool goes_left[4];
+        uint32_t r_before[4];
+        uint32_t n_left = 0, n_right = 0, first_r = 0xFFFFFFFFu;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint32_t k = 64u * j + lane;
+            const bool in = k < ec;
+            goes_left[j] = in && comp_lds(s_c, eb + k, axis) < pos;
+            const uint64_t mr = __builtin_amdgcn_ballot_w64(in && !goes_left[j]);
+            const uint64_t ml = __builtin_amdgcn_ballot_w64(goes_left[j]);
+            r_before[j] = n_right + (uint32_t)__popcll(mr & below);
+            if (first_r == 0xFFFFFFFFu && mr) first_r = 64u * j + (uint32_t)__builtin_ctzll(mr);
+            n_right += (uint32_t)__popcll(mr);
+            n_left += (uint32_t)__popcll(ml);
         }
-        if (mid == 0 || mid == e.c) {
-            // stable insertion sort by the centroid on the axis
-            for (uint32_t i = e.b + 1; i < e.b + e.c; ++i) {
-                const float4 k0 = f.mn[i], k1 = f.mx[i], k2 = f.c[i];
-                const uint32_t k3 = f.idx[i];
-                const float key = comp(k2, axis);
-                uint32_t j = i;
-                while (j > e.b && key < comp(f.c[j - 1], axis)) {
-                    f.mn[j] = f.mn[j - 1];
-                    f.mx[j] = f.mx[j - 1];
-                    f.c[j] = f.c[j - 1];
-                    f.idx[j] = f.idx[j - 1];
-                    --j;
+        uint32_t mid = n_left;
+        if (mid == 0 || mid == ec) {
+            // stable insertion sort by the centroid on the axis, as the host does
+            if (lane == 0) {
+                for (uint32_t i = eb + 1; i < ee; ++i) {
+                    const float4 k0 = s_mn[i], k1 = s_mx[i], k2 = s_c[i];
+                    const uint32_t k3 = s_idx[i];
+                    const float key = comp(k2, axis);
+                    uint32_t j = i;
+                    while (j > eb && key < comp_lds(s_c, j - 1, axis)) {
+                        s_mn[j] = s_mn[j - 1];
+                        s_mx[j] = s_mx[j - 1];
+                        s_c[j] = s_c[j - 1];
+                        s_idx[j] = s_idx[j - 1];
+                        --j;
+                    }
+                    s_mn[j] = k0;
+                    s_mx[j] = k1;
+                    s_c[j] = k2;
+                    s_idx[j] = k3;
                 }
-                f.mn[j] = k0;
-                f.mx[j] = k1;
-                f.c[j] = k2;
-                f.idx[j] = k3;
             }
-            mid = e.c / 2;
+            mid = ec / 2;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t k = 64u * j + lane;
+                if (k >= ec) continue;
+                if (goes_left[j]) s_src[k - r_before[j]] = k;
+                if (k >= first_r) {
+                    const uint32_t st = k - first_r;
+                    s_ptr[st] = goes_left[j] ? st - r_before[j] : (RES | k);
+                }
+            }
+            __syncthreads();
+            const uint32_t ns = ec - first_r;
+            bool pending = true;
+            for (int round = 0; round < 10 && pending; ++round) {
+                bool mine = false;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t st = 64u * j + lane;
+                    if (st >= ns) continue;
+                    const uint32_t p = s_ptr[st];
+                    if (!(p & RES)) {
+                        const uint32_t q = s_ptr[p];
+                        s_ptr[st] = q;
+                        mine = mine || !(q & RES);
+                    }
+                }
+                __syncthreads();
+                pending = __builtin_amdgcn_ballot_w64(mine) != 0;
+            }
+            if (pending) {
+                if (lane == 0) atomicOr(err, ERR_SMALL_STACK);
+                return;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t p = 64u * j + lane;
+                if (p >= mid && p < ec) s_src[p] = s_ptr[p - first_r] & ~RES;
+            }
+            __syncthreads();
+            float4 ga[4], gb[4], gc[4];
+            uint32_t gi[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t p = 64u * j + lane;
+                if (p >= ec) continue;
+                const uint32_t k = eb + s_src[p];
+                ga[j] = s_mn[k];
+                gb[j] = s_mx[k];
+                gc[j] = s_c[k];
+                gi[j] = s_idx[k];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t p = 64u * j + lane;
+                if (p >= ec) continue;
+                s_mn[eb + p] = ga[j];
+                s_mx[eb + p] = gb[j];
+                s_c[eb + p] = gc[j];
+                s_idx[eb + p] = gi[j];
+            }
         }
-        nd.right_or_first = 0;
-        nd.count = 0;
-        nd.left = me + 1;
-        out[me] = nd;
         if (sp + 2 > (int)SMALL + 2) {
-            atomicOr(err, ERR_SMALL_STACK);
+            if (lane == 0) atomicOr(err, ERR_SMALL_STACK);
             return;
         }
-        stack[sp++] = Ent{e.b + mid, (uint16_t)(e.c - mid), (uint16_t)me};
-        stack[sp++] = Ent{e.b, (uint16_t)mid, NO_PARENT};
+        if (lane == 0) {
+            nd.right_or_first = 0;
+            nd.count = 0;
+            nd.left = me + 1;
+            out[me] = nd;
+            s_stack[sp] = Ent{eb + mid, (uint16_t)(ec - mid), (uint16_t)me};
+            s_stack[sp + 1] = Ent{eb, (uint16_t)mid, NO_PARENT};
+        }
+        sp += 2;
+        __syncthreads();
     }
-    task_nodes[t] = nn;
+    for (uint32_t i = lane; i < n0; i += 64) {
+        f.mn[g0 + i] = s_mn[i];
+        f.mx[g0 + i] = s_mx[i];
+        f.c[g0 + i] = s_c[i];
+        f.idx[g0 + i] = s_idx[i];
+    }
+    if (lane == 0) task_nodes[t] = nn;
 }
 
 __global__ __launch_bounds__(256) void k_bb_place(const Place *places, uint32_t nplaces, const specrel_node *scratch,
@@ -820,7 +927,7 @@ std::string build_bvh(const float4 *d_boxes, uint32_t n, specrel_node *d_nodes, 
         uint32_t *d_task_nodes = m.get<uint32_t>(nt);
         put(d_tasks, small, s);
         if (nt) {  // every segment may have ended as a leaf of the level phase
-            k_bb_small<<<(nt + 63) / 64, 64, 0, s>>>(d_tasks, nt, Fn, scratch, d_task_nodes, d_err);
+            k_bb_small<<<nt, 64, 0, s>>>(d_tasks, nt, Fn, scratch, d_task_nodes, d_err);
             ok(hipGetLastError(), "k_bb_small");
         }
         std::vector<uint32_t> tn(nt);
