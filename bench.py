@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s at 1280x800 (15.7M tets) vs CPU ref; batched views/s; fwd+adjoint ms"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 CUS = 256  # MI355X compute units
+SIMDS_PER_CU = 4
 ENGINE_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 
 
@@ -517,6 +518,7 @@ def main():
     achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
     traffic = None
     l1 = None
+    valu = None
     tf = os.path.join(ROOT, "profiles", f"{dom}_large_1280x800_traffic.txt")
     if os.path.exists(tf) and (w, h) == (1280, 800):
         with open(tf) as f:
@@ -529,6 +531,13 @@ def main():
                   "hit_rate": prof.get("l1_hit_rate"),
                   "note": "vector L1 tag lookups per CU per cycle at the 2.4 GHz peak engine clock "
                           "(TCP_TOTAL_CACHE_ACCESSES of the profile over this run's kernel time)"}
+        if prof.get("valu_insts_per_launch"):
+            # a wave64 VALU instruction occupies its SIMD (16 lanes wide) for 4 cycles
+            simd_cycles = times[dom] * 1e-3 * CUS * SIMDS_PER_CU * ENGINE_CLOCK_HZ
+            valu = {"insts_per_launch": prof["valu_insts_per_launch"],
+                    "issue_frac": round(prof["valu_insts_per_launch"] * 4 / simd_cycles, 4),
+                    "note": "SIMD cycles spent issuing VALU instructions (SQ_INSTS_VALU of the profile, 4 cycles "
+                            "each) over the SIMD cycles of this run's kernel time at the 2.4 GHz peak clock"}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": dom, "kernel_ms": round(times[dom], 4),
@@ -541,6 +550,7 @@ def main():
                          "note": "every node / record visit priced with the reference's record sizes; served "
                                  "mostly by the L1, so not bounded by HBM"},
                 "l1": l1,
+                "valu": valu,
                 "other_kernels": {k: {"ms": round(times[k], 4), "algorithmic_bytes": int(alg[k]),
                                       "achieved_GBs": round(alg[k] / (times[k] * 1e-3) / 1e9, 2)}
                                   for k in times if k != dom}}
