@@ -461,6 +461,20 @@ __global__ __launch_bounds__(256) void k_bb_permute(uint32_t begin, uint32_t cou
 // Nodes in preorder to scratch[2 * begin ...] with subtree-local child indices; the references
 // are written back in their final order. One wave per subtree (one block of 64 threads).
 __device__ __forceinline__ float comp_lds(const float4 *a, uint32_t i, int c) { return ((const float *)(a + i))[c]; }
+// arr[eb + p] = arr[eb + src[p]] for p < ec <= 256 (one wave: every lane gathers four positions,
+// then writes them back).
+template <class T>
+__device__ __forceinline__ void permute4(T *arr, const uint32_t *src, uint32_t eb, uint32_t ec, uint32_t lane) {
+    const uint32_t p0 = lane, p1 = lane + 64u, p2 = lane + 128u, p3 = lane + 192u;
+    const T v0 = arr[eb + src[p0 < ec ? p0 : 0u]], v1 = arr[eb + src[p1 < ec ? p1 : 0u]];
+    const T v2 = arr[eb + src[p2 < ec ? p2 : 0u]], v3 = arr[eb + src[p3 < ec ? p3 : 0u]];
+    __syncthreads();
+    if (p0 < ec) arr[eb + p0] = v0;
+    if (p1 < ec) arr[eb + p1] = v1;
+    if (p2 < ec) arr[eb + p2] = v2;
+    if (p3 < ec) arr[eb + p3] = v3;
+    __syncthreads();
+}
 
 __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_t ntasks, Refs f,
                                                  specrel_node *scratch, uint32_t *task_nodes, uint32_t *err) {
@@ -540,10 +554,16 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
                 const float4 *arr = h ? s_mx : s_mn;
                 float4 acc = h ? make_float4(-inf, -inf, -inf, -inf) : make_float4(inf, inf, inf, inf);
                 uint32_t cnt = 0;
-                if (fr.live[a]) {
+                // the lane's axis of the frame by selects (no dynamically indexed copy in private
+                // memory); the bin is sr::bin_of's expression
+                const bool live = a == 0 ? fr.live[0] : a == 1 ? fr.live[1] : a == 2 ? fr.live[2] : fr.live[3];
+                const float am = a == 0 ? fr.axis_min[0] : a == 1 ? fr.axis_min[1] : a == 2 ? fr.axis_min[2] : fr.axis_min[3];
+                const float ie = a == 0 ? fr.inv_ext[0] : a == 1 ? fr.inv_ext[1] : a == 2 ? fr.inv_ext[2] : fr.inv_ext[3];
+                if (live) {
 #pragma unroll 4
                     for (uint32_t i = eb; i < ee; ++i) {
-                        const uint32_t k = sr::bin_of(fr, a, comp_lds(s_c, i, a));
+                        const uint64_t k64 = sr::f2u_sat((comp_lds(s_c, i, a) - am) * ie);
+                        const uint32_t k = k64 > (uint64_t)(sr::BVH_BINS - 1) ? (uint32_t)(sr::BVH_BINS - 1) : (uint32_t)k64;
                         const float4 x = arr[i];
                         if (k == (uint32_t)q) {
                             acc = h ? f4(sr::vmax(v(acc), v(x))) : f4(sr::vmin(v(acc), v(x)));
@@ -555,17 +575,55 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
                 if (!h) s_bn[a][q] = cnt;
             }
             __syncthreads();
-            float best_cost = inf;
-            for (int a = 0; a < 4; ++a) {
-                if (!fr.live[a]) continue;
-                sr::Box4 b[8];
-                uint32_t bn[8];
+            // The SAH sweep (sr::sah_axis over the live axes in order), one lane per candidate:
+            // lane 7a + j is axis a's split after bin i = 6 - j, the host's visiting order. Each
+            // lane forms its left box from bins 0..i and its right box from bins 7..i+1 in the
+            // host's order of unions, and its cost with the host's expression; the host keeps the
+            // first candidate strictly cheaper than every one before it, which is the first in
+            // that order with the least cost (candidates without both sides, or with a cost that
+            // is not below +inf, never win).
+            float cost = inf, cpos = 0.0f;
+            bool elig = false;
+            if (lane < 28) {
+                const int a = (int)(lane / 7u), i = 6 - (int)(lane % 7u);
+                const bool live = a == 0 ? fr.live[0] : a == 1 ? fr.live[1] : a == 2 ? fr.live[2] : fr.live[3];
+                if (live) {
+                    sr::Box4 lb = sr::box_empty(), rb = sr::box_empty();
+                    uint32_t lc = 0, rc = 0;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    b[q] = sr::Box4{v(s_bb[a][q][0]), v(s_bb[a][q][1])};
-                    bn[q] = s_bn[a][q];
+                    for (int q = 0; q < 8; ++q) {
+                        if (q <= i) {
+                            lb = sr::box_union(lb, sr::Box4{v(s_bb[a][q][0]), v(s_bb[a][q][1])});
+                            lc += s_bn[a][q];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 7; q >= 0; --q) {
+                        if (q > i) {
+                            rb = sr::box_union(rb, sr::Box4{v(s_bb[a][q][0]), v(s_bb[a][q][1])});
+                            rc += s_bn[a][q];
+                        }
+                    }
+                    if (lc > 0 && rc > 0) {
+                        cost = (float)lc * sr::box_measure(lb) + (float)rc * sr::box_measure(rb);
+                        elig = cost < inf;
+                        const float am = a == 0 ? fr.axis_min[0] : a == 1 ? fr.axis_min[1] : a == 2 ? fr.axis_min[2] : fr.axis_min[3];
+                        const float ex = a == 0 ? fr.ext[0] : a == 1 ? fr.ext[1] : a == 2 ? fr.ext[2] : fr.ext[3];
+                        cpos = am + ((float)(i + 1) / (float)sr::BVH_BINS) * ex;
+                    }
                 }
-                sr::sah_axis(b, bn, fr, a, best_cost, axis, pos);
+            }
+            float m = elig ? cost : inf;
+#pragma unroll
+            for (int off = 16; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off, 64));
+            m = __shfl(m, 0, 64);  // lanes 0..31 hold every candidate
+            const uint64_t win = __builtin_amdgcn_ballot_w64(elig && cost == m);
+            float best_cost = inf;
+            if (win) {
+                const int w = __builtin_ctzll(win);
+                best_cost = m;
+                axis = w / 7;
+                pos = __shfl(cpos, w, 64);
             }
             kind = sr::split_after_sweeps(cb, fr, best_cost, axis, pos);
         }
@@ -658,28 +716,10 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
                 if (p >= mid && p < ec) s_src[p] = s_ptr[p - first_r] & ~RES;
             }
             __syncthreads();
-            float4 ga[4], gb[4], gc[4];
-            uint32_t gi[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t p = 64u * j + lane;
-                if (p >= ec) continue;
-                const uint32_t k = eb + s_src[p];
-                ga[j] = s_mn[k];
-                gb[j] = s_mx[k];
-                gc[j] = s_c[k];
-                gi[j] = s_idx[k];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const uint32_t p = 64u * j + lane;
-                if (p >= ec) continue;
-                s_mn[eb + p] = ga[j];
-                s_mx[eb + p] = gb[j];
-                s_c[eb + p] = gc[j];
-                s_idx[eb + p] = gi[j];
-            }
+            permute4(s_mn, s_src, eb, ec, lane);
+            permute4(s_mx, s_src, eb, ec, lane);
+            permute4(s_c, s_src, eb, ec, lane);
+            permute4(s_idx, s_src, eb, ec, lane);
         }
         if (sp + 2 > (int)SMALL + 2) {
             if (lane == 0) atomicOr(err, ERR_SMALL_STACK);
