@@ -69,6 +69,15 @@ SR_HD void bins_clear(Bins &B) {
 
 enum { SPLIT_SAH = 0, SPLIT_MIDPOINT = 1, SPLIT_LEAF = 2 };
 
+// The SAH cost of a split into boxes l and r holding nl and nr references, and the position of the
+// split after bin i of an axis starting at axis_min with extent ext (bvh4d.rs:307-417).
+SR_HD float sah_cost(const Box4 &l, uint32_t nl, const Box4 &r, uint32_t nr) {
+    return (float)nl * box_measure(l) + (float)nr * box_measure(r);
+}
+SR_HD float sah_split_pos(float axis_min, float ext, int i) {
+    return axis_min + ((float)(i + 1) / (float)BVH_BINS) * ext;
+}
+
 // One live axis a of the SAH sweep (bvh4d.rs:307-417) over that axis's bins b / n: every split
 // between bins, the cheapest so far kept (strictly cheaper replaces, axes in order 0..3).
 SR_HD void sah_axis(const Box4 (&b)[BVH_BINS], const uint32_t (&n)[BVH_BINS], const BinFrame &f, int a,
@@ -89,11 +98,11 @@ SR_HD void sah_axis(const Box4 (&b)[BVH_BINS], const uint32_t (&n)[BVH_BINS], co
         right_b = box_union(right_b, b[i + 1]);
         right_c += n[i + 1];
         if (left_cnt[i] > 0 && right_c > 0) {
-            float cost = (float)left_cnt[i] * box_measure(left_box[i]) + (float)right_c * box_measure(right_b);
+            float cost = sah_cost(left_box[i], left_cnt[i], right_b, right_c);
             if (cost < best_cost) {
                 best_cost = cost;
                 best_axis = a;
-                best_pos = f.axis_min[a] + ((float)(i + 1) / (float)BVH_BINS) * f.ext[a];
+                best_pos = sah_split_pos(f.axis_min[a], f.ext[a], i);
             }
         }
     }

@@ -10,9 +10,9 @@
 // after the first right-going one moves the queue's front to its back. The final queue is the last
 // F entries of the stream of appends, and each re-append copies an earlier stream entry, so the
 // stream is filled by pointer jumping (a few passes) after a scan of the left flags. Segments at
-// or below SMALL references are then built one per thread, sequentially, exactly as the host
-// does. Node indices are the host's preorder: subtree sizes are summed bottom-up over the large
-// nodes (few) on the host, the small subtrees are copied into place with their offsets.
+// or below SMALL references are then built one wave each from LDS, every decision the host's
+// (k_bb_small). Node indices are the host's preorder: subtree sizes are summed bottom-up over the
+// large nodes (few) on the host, the small subtrees are copied into place with their offsets.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,7 +30,7 @@ namespace dev {
 namespace {
 
 constexpr uint32_t CHUNK = 4096;  // references per chunk of a large segment
-constexpr uint32_t SMALL = 256;   // subtrees at or below this size are built by one thread
+constexpr uint32_t SMALL = 256;   // subtrees at or below this size are built by one wave each
 constexpr uint32_t MERGE = 8;     // segments of more chunks have their bins merged by a block first
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 enum { K_SAH = sr::SPLIT_SAH, K_MID = sr::SPLIT_MIDPOINT, K_LEAF = sr::SPLIT_LEAF, K_FALLBACK = 3 };
@@ -605,11 +605,11 @@ __global__ __launch_bounds__(64) void k_bb_small(const SmallTask *tasks, uint32_
                         }
                     }
                     if (lc > 0 && rc > 0) {
-                        cost = (float)lc * sr::box_measure(lb) + (float)rc * sr::box_measure(rb);
+                        cost = sr::sah_cost(lb, lc, rb, rc);
                         elig = cost < inf;
                         const float am = a == 0 ? fr.axis_min[0] : a == 1 ? fr.axis_min[1] : a == 2 ? fr.axis_min[2] : fr.axis_min[3];
                         const float ex = a == 0 ? fr.ext[0] : a == 1 ? fr.ext[1] : a == 2 ? fr.ext[2] : fr.ext[3];
-                        cpos = am + ((float)(i + 1) / (float)sr::BVH_BINS) * ex;
+                        cpos = sr::sah_split_pos(am, ex, i);
                     }
                 }
             }
