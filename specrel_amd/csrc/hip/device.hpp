@@ -763,14 +763,41 @@ __device__ V3 sky_gradient(V4 dir) {  // renderer.rs:1056-1062
     return V3{0.12f, 0.16f, 0.25f} * (1.0f - horizon) + V3{0.04f, 0.06f, 0.13f} * horizon;
 }
 
+// Whether rem_euclid(x, 1) < 0.025 is certain for every x within m of xa: 1 certainly below, 0
+// certainly not, -1 undecided (within m of 0, 0.025 or 1, or NaN).
+__device__ __forceinline__ int grid_side(float xa, float m) {
+    const float f = xa - floorf(xa);  // exact: in [0, 1)
+    if (f >= 0.025f + m && f <= 1.0f - m) return 0;
+    if (f >= m && f <= 0.025f - m) return 1;
+    return -1;
+}
+
+// shade_sky's grid lines depend on acos and atan2 only through those comparisons, so the f32
+// library functions decide them first: the f32 acosf / atan2f are within a few ulp of the true
+// angle and acosf_ / atan2f_ within half an ulp, so the reference's gu and gv (the same operations
+// on their results) lie within 1e-5 of the values computed from the fast angles, far inside the
+// margin 2e-3 below which a comparison is left undecided. Only undecided directions (about 1% of
+// the sky) evaluate acosf_ and atan2f_; the line, and with it the colour, is the reference's.
 __device__ V3 shade_sky(V4 dir) {  // renderer.rs:1064-1079
     V3 s = normalized(spatial(dir));
     V3 sky = sky_gradient(dir);
-    float theta = acosf_(clamp_(s.y, -1.0f, 1.0f));
-    float phi = atan2f_(s.z, s.x);
-    float gu = rem_euclid_(phi / PI_F * 12.0f, 1.0f);
-    float gv = rem_euclid_(theta / PI_F * 6.0f, 1.0f);
-    if (gu < 0.025f || gv < 0.025f) sky += V3{0.06f, 0.09f, 0.14f};
+    const float cy = clamp_(s.y, -1.0f, 1.0f);
+    constexpr float M = 2e-3f;
+    const int su = grid_side(atan2f(s.z, s.x) / PI_F * 12.0f, M);
+    const int sv = grid_side(acosf(cy) / PI_F * 6.0f, M);
+    bool line;
+    if (su == 1 || sv == 1) {
+        line = true;
+    } else if (su == 0 && sv == 0) {
+        line = false;
+    } else {
+        float theta = acosf_(cy);
+        float phi = atan2f_(s.z, s.x);
+        float gu = rem_euclid_(phi / PI_F * 12.0f, 1.0f);
+        float gv = rem_euclid_(theta / PI_F * 6.0f, 1.0f);
+        line = gu < 0.025f || gv < 0.025f;
+    }
+    if (line) sky += V3{0.06f, 0.09f, 0.14f};
     return sky;
 }
 
