@@ -92,3 +92,57 @@ def test_acos(mathcheck):
     n = compare(mathcheck, "acos", sweep(-1.0, 1.0))
     n += compare(mathcheck, "acos", uniform(-1.0, 1.0, 6))
     print(f"acos: {n} points identical")
+
+
+def sky_directions():
+    """Uniform directions, and directions whose polar / azimuthal angle puts the sky grid's
+    fractional parts (theta / pi * 6, phi / pi * 12) just below, at and just above 0, 0.025 and 1,
+    where the device's f32 decision of the grid lines hands over to the f64 routines."""
+    rng = np.random.default_rng(12)
+    v = rng.normal(size=(1 << 17, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    dirs = [v]
+    offs = np.concatenate([-np.logspace(-7, -2, 40), [0.0], np.logspace(-7, -2, 40)])
+    for base in (0.0, 0.025, 1.0):
+        fr = base + offs
+        # azimuth near the thresholds, polar angle spread
+        k = rng.integers(-12, 12, fr.size * 40)
+        phi = (np.repeat(fr, 40) + k) * np.pi / 12.0
+        theta = rng.uniform(0.05, np.pi - 0.05, phi.size)
+        dirs.append(np.stack([np.sin(theta) * np.cos(phi), np.cos(theta), np.sin(theta) * np.sin(phi)], 1))
+        # polar angle near the thresholds, azimuth spread
+        k = rng.integers(0, 6, fr.size * 40)
+        theta = np.clip((np.repeat(fr, 40) + k) * np.pi / 6.0, 0.0, np.pi)
+        phi = rng.uniform(-np.pi, np.pi, theta.size)
+        dirs.append(np.stack([np.sin(theta) * np.cos(phi), np.cos(theta), np.sin(theta) * np.sin(phi)], 1))
+    s = np.concatenate(dirs).astype(np.float32)
+    d = np.empty((s.shape[0], 4), np.float32)
+    d[:, 0] = -1.0  # past light cone: the time component does not enter the sky
+    d[:, 1:] = s
+    return d
+
+
+def test_sky_grid_lines(mathcheck):
+    """shade_sky on the device (grid lines decided by the f32 acosf / atan2f away from the
+    thresholds, by the f64-rounded routines near them) against the oracle's shade_sky, bit for bit."""
+    lib = mathcheck
+    lib.specrel_skycheck.restype = C.c_int
+    lib.specrel_skycheck.argtypes = [FP, FP, C.c_uint64]
+    d = np.ascontiguousarray(sky_directions())
+    got = np.empty((d.shape[0], 3), np.float32)
+    assert lib.specrel_skycheck(d.ctypes.data_as(FP), got.ctypes.data_as(FP), d.shape[0]) == 0
+    ref = O.lib()
+    want = np.empty_like(got)
+    out = np.empty(3, np.float32)
+    for i in range(d.shape[0]):
+        ref.oracle_shade_sky(d[i].ctypes.data_as(FP), out.ctypes.data_as(FP))
+        want[i] = out
+    bad = np.flatnonzero((got.view(np.uint32) != want.view(np.uint32)).any(axis=1))
+    assert bad.size == 0, f"{bad.size} of {d.shape[0]} sky colours differ, e.g. direction {d[bad[0]]}"
+    s = d[:, 1:].astype(np.float64)
+    s /= np.linalg.norm(s, axis=1, keepdims=True)
+    fu = np.mod(np.arctan2(s[:, 2], s[:, 0]) / np.pi * 12.0, 1.0)
+    fv = np.mod(np.arccos(np.clip(s[:, 1], -1.0, 1.0)) / np.pi * 6.0, 1.0)
+    near = lambda f: (np.minimum(f, 1.0 - f) < 2e-3) | (np.abs(f - 0.025) < 2e-3)
+    print(f"sky: {d.shape[0]} directions identical, {int((near(fu) | near(fv)).sum())} of them within 2e-3 "
+          f"of a grid threshold")
