@@ -1195,7 +1195,9 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     // Down pass: levels below k0 two at a time (one if a single one is left), then the tail.
     for (int k = 1; k < k0;) {
         if (k + 1 < k0) {
-            k_bloom_down2<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);
+            const uint32_t tiles = B.start(k, nv);
+            if (tiles > BLOOM_STAGE_TILES) k_bloom_down2<true><<<tiles, 256, 0, s>>>(V, k, pyr);
+            else k_bloom_down2<false><<<tiles, 256, 0, s>>>(V, k, pyr);
             k += 2;
         } else {
             k_bloom_down<<<B.start(k, nv), 256, 0, s>>>(V, k, pyr);

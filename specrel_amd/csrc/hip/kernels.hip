@@ -992,8 +992,13 @@ __device__ __forceinline__ TileOf tile_of(const Views &V, int k) {
 
 // Levels k and k + 1 of the down pass: the tile of level k (from level k - 1) and the level-(k+1)
 // pixels X, Y with 2X, 2Y in the tile (from the tile and its one-pixel rim, in LDS).
+// STAGE: the finer level's pixels the tile's taps read are first copied into LDS (one coalesced
+// load each instead of 16 loads per pixel from the caches): faster on grids of many tiles (batches),
+// slower on a single view's 1,000 tiles, where the extra phase lengthens every block.
+template <bool STAGE>
 __global__ __launch_bounds__(256) void k_bloom_down2(Views V, int k, float4 *pyr) {
     __shared__ float s_lv[18 * 18 * 3];
+    __shared__ float s_src[STAGE ? 38 * 38 * 3 : 1];  // the finer level's pixels the rim tile's taps read
     const TileOf o = tile_of(V, k);
     uint32_t sw, sh, dw, dh, ew, eh;
     level_size(o.w0, o.h0, k - 1, sw, sh);
@@ -1007,9 +1012,21 @@ __global__ __launch_bounds__(256) void k_bloom_down2(Views V, int k, float4 *pyr
     const uint32_t rx1 = min(o.tx0 + 16, dw - 1), ry1 = min(o.ty0 + 16, dh - 1);
     const LdsRect R{s_lv, rx0, ry0, rx1 - rx0 + 1};
     const uint32_t n = R.nx * (ry1 - ry0 + 1);
+    // The finer level's rectangle the taps of the rim tile read (clamped into that level, at most
+    // 38 x 38 pixels).
+    const uint32_t sx0 = 2 * rx0 > 0 ? 2 * rx0 - 1 : 0, sy0 = 2 * ry0 > 0 ? 2 * ry0 - 1 : 0;
+    const uint32_t sx1 = min(2 * rx1 + 2, sw - 1), sy1 = min(2 * ry1 + 2, sh - 1);
+    const LdsRect S{s_src, sx0, sy0, sx1 - sx0 + 1};
+    if constexpr (STAGE) {
+        const uint32_t ns = S.nx * (sy1 - sy0 + 1);
+        for (uint32_t i = threadIdx.x; i < ns; i += 256)
+            S.put(i, ld3(src, (size_t)(sy0 + i / S.nx) * sw + sx0 + i % S.nx));
+        __syncthreads();
+    }
     for (uint32_t i = threadIdx.x; i < n; i += 256) {
         const uint32_t x = rx0 + i % R.nx, y = ry0 + i / R.nx;
-        const V3 v = down_at([&](uint32_t a, uint32_t b) { return ld3(src, (size_t)b * sw + a); }, sw, sh, x, y);
+        const V3 v = STAGE ? down_at([&](uint32_t a, uint32_t b) { return S.at(a, b); }, sw, sh, x, y)
+                           : down_at([&](uint32_t a, uint32_t b) { return ld3(src, (size_t)b * sw + a); }, sw, sh, x, y);
         R.put(i, v);
         if (x >= o.tx0 && x < o.tx0 + 16 && y >= o.ty0 && y < o.ty0 + 16)
             lk[(size_t)y * dw + x] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -1113,6 +1130,8 @@ __global__ __launch_bounds__(256) void k_bloom_final(Views V, const float4 *pyr,
         c = composite_at(hdr, [&](uint32_t a, uint32_t b) { return ld3(l1, (size_t)b * w1 + a); }, w, w1, h1, x, y, lut);
     out[V.pix_start[o.vi] + (size_t)y * w + x] = c;
 }
+template __global__ void k_bloom_down2<false>(Views, int, float4 *);
+template __global__ void k_bloom_down2<true>(Views, int, float4 *);
 template __global__ void k_bloom_final<0>(Views, const float4 *, const uint8_t *, uint32_t *);
 template __global__ void k_bloom_final<1>(Views, const float4 *, const uint8_t *, uint32_t *);
 template __global__ void k_bloom_final<2>(Views, const float4 *, const uint8_t *, uint32_t *);

@@ -256,7 +256,9 @@ constexpr uint32_t BLOOM_TAIL_PIXELS = 2560;
 __global__ void k_composite(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);
 // Fused bloom passes (one block per 16x16 tile of the level named): levels k and k + 1 of the down
 // pass; levels k + 1 and k of the up pass; the last NUP (0..2) levels of the up pass and the composite.
+template <bool STAGE>
 __global__ void k_bloom_down2(Views V, int k, float4 *pyr);
+constexpr uint32_t BLOOM_STAGE_TILES = 4096;  // k_bloom_down2 stages its taps in LDS on grids above this
 __global__ void k_bloom_up2(Views V, int k, float4 *pyr);
 template <int NUP>
 __global__ void k_bloom_final(Views V, const float4 *pyr, const uint8_t *lut, uint32_t *out);
