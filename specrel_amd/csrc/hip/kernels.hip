@@ -688,7 +688,7 @@ __global__ void k_reset_view(uint32_t *counters, uint32_t n, FrameWords f, uint3
 //   PASS 1: shadow queries queued (and the pixel's occlusion mask cleared); nothing else written.
 //   PASS 2: shadow queries answered from the occlusion masks.
 template <bool STATS, int SK, int PASS>
-__global__ __launch_bounds__(256) void k_shade(ShadeArgs a) {
+__global__ __launch_bounds__(256, (PASS == 2 && !STATS) ? 4 : 1) void k_shade(ShadeArgs a) {
     const uint32_t t = blockIdx.x, vi = find_view(a.V.level_start[0], a.V.n, t);
     const specrel_frame &f = a.V.frames[vi];
     const ViewDerived v = derive(f, a.num_stars);
