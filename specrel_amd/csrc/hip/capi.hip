@@ -1029,31 +1029,10 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
         S.touch_prims = t + ctx->touch_words[0];
         S.touch_hits = t + ctx->touch_words[0] + ctx->touch_words[1];
     }
-    if (B.table)  // one view without staging: frame and view table from the kernel arguments
-        k_reset_view<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames, *B.table, B.table_dst);
-    else if (pl.frame_in_reset)
-        k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames);
-    else
-        k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
-    check(hipGetLastError(), "k_reset launch");
-    mark(0);
     // The primary rays come first (they need only the frame); the star pass follows them, or runs
     // beside them on the side stream when overlapped. The shading pass waits for both.
     const bool overlap = pl.stars && ctx->opt_star_overlap;
     ctx->stars_overlapped = overlap;
-    if (overlap) check(hipEventRecord(ctx->fork, s), "fork");
-    // order_tiles: k_star_bin also sorts the next render's tile order (one block's share of tiles).
-    auto star_pass = [&](hipStream_t ss, uint32_t order_tiles) {
-        k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
-            (const float4 *)sc.stars.p, sc.num_stars, V, B.prep, B.dirs);
-        check(hipGetLastError(), "k_star_prep launch");
-        if (ctx->opt_cull) {
-            k_star_bin<<<B.total_supers + (order_tiles ? 1u : 0u), 256, 0, ss>>>(
-                V, B.dirs, sc.num_stars, B.super_lists, B.super_counts, B.tile_cones, B.tile_cost, order_tiles,
-                B.tile_order);
-            check(hipGetLastError(), "k_star_bin launch");
-        }
-    };
     const dim3 tiles(B.total_tiles);
     const uint32_t path = shadow_path(ctx, pl);
     const bool queued = path == SPECREL_SHADOWS_QUEUED, in_trace = path == SPECREL_SHADOWS_IN_TRACE;
@@ -1068,6 +1047,35 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
                       trace_variant(ctx, stats) != TV_MERGED && !queued && !overlap &&
                       (in_trace || path == SPECREL_SHADOWS_NONE);
     ctx->shade_fused = fuse;
+    // One view launched directly whose star pass comes first (the fused kernel shades with its
+    // lists): the star pass's first kernel also clears the counters and writes the frame.
+    const bool reset_in_prep = pl.frame_in_reset && !B.table && fuse && pl.stars;
+    if (B.table) {  // one view without staging: frame and view table from the kernel arguments
+        k_reset_view<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames, *B.table, B.table_dst);
+    } else if (!reset_in_prep) {
+        if (pl.frame_in_reset) k_reset_frame<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames);
+        else k_reset<<<1, 256, 0, s>>>(counters, COUNTERS);
+    }
+    check(hipGetLastError(), "k_reset launch");
+    mark(0);
+    if (overlap) check(hipEventRecord(ctx->fork, s), "fork");
+    // order_tiles: k_star_bin also sorts the next render's tile order (one block's share of tiles).
+    auto star_pass = [&](hipStream_t ss, uint32_t order_tiles) {
+        if (reset_in_prep)
+            k_star_prep_first<<<(sc.num_stars + 255) / 256, 256, 0, ss>>>((const float4 *)sc.stars.p, sc.num_stars,
+                                                                           B.prep, B.dirs, counters, COUNTERS,
+                                                                           pl.frame, B.frames);
+        else
+            k_star_prep<<<dim3((sc.num_stars + 255) / 256, nv), 256, 0, ss>>>(
+                (const float4 *)sc.stars.p, sc.num_stars, V, B.prep, B.dirs);
+        check(hipGetLastError(), "k_star_prep launch");
+        if (ctx->opt_cull) {
+            k_star_bin<<<B.total_supers + (order_tiles ? 1u : 0u), 256, 0, ss>>>(
+                V, B.dirs, sc.num_stars, B.super_lists, B.super_counts, B.tile_cones, B.tile_cost, order_tiles,
+                B.tile_order);
+            check(hipGetLastError(), "k_star_bin launch");
+        }
+    };
     ShadeArgs ra;
     ra.S = S;
     ra.T.lut = (const uint8_t *)ctx->tables.p;

@@ -6,15 +6,13 @@ namespace dev {
 
 // Star pass: for every star, its cone event and the view-only parts of star_light
 // (star.rs:251-283), plus its unit direction in the camera rest frame for the tile culling.
-// Grid: (star blocks, views).
-__global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out,
-                                                   StarDir *dirs) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const specrel_frame &f = V.frames[blockIdx.y];
+// Star i of view `view`, whose frame is f.
+__device__ __forceinline__ void star_prep_one(const float4 *words, uint32_t num_stars, const specrel_frame &f,
+                                              uint32_t view, uint32_t i, StarPrep *out, StarDir *dirs) {
     if (i >= num_stars || f.mode != SPECREL_MODE_SHADED) return;
     ViewDerived v = derive(f, num_stars);
-    out += (size_t)blockIdx.y * num_stars;
-    dirs += (size_t)blockIdx.y * num_stars;
+    out += (size_t)view * num_stars;
+    dirs += (size_t)view * num_stars;
     StarPrep p;
     p.flags = 0;
     p.r = make_float4(0, 0, 0, 0);
@@ -50,6 +48,27 @@ __global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t
     }
     out[i] = p;
     dirs[i] = sd;
+}
+// Grid: (star blocks, views).
+__global__ __launch_bounds__(256) void k_star_prep(const float4 *words, uint32_t num_stars, Views V, StarPrep *out,
+                                                   StarDir *dirs) {
+    star_prep_one(words, num_stars, V.frames[blockIdx.y], blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x, out,
+                  dirs);
+}
+// The same for the one view of a render launched directly whose star pass comes first: its block 0
+// also does k_reset_frame's work (the counters cleared, the frame, passed by value, written to the
+// frame table for the later kernels), and every block reads the frame from the arguments. One
+// launch instead of two.
+__global__ __launch_bounds__(256) void k_star_prep_first(const float4 *words, uint32_t num_stars, StarPrep *out,
+                                                         StarDir *dirs, uint32_t *counters, uint32_t n, FrameWords fw,
+                                                         uint32_t *dst) {
+    if (blockIdx.x == 0) {
+        const uint32_t i = threadIdx.x;
+        if (i < n) counters[i] = 0u;
+        if (i < FrameWords::N) dst[i] = fw.w[i];
+    }
+    star_prep_one(words, num_stars, *reinterpret_cast<const specrel_frame *>(fw.w), 0u,
+                  blockIdx.x * blockDim.x + threadIdx.x, out, dirs);
 }
 
 // Bounding cone of the pixel rays (pixel centres) of the image rectangle [xa,xb] x [ya,yb], widened

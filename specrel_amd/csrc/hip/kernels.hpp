@@ -233,6 +233,8 @@ struct FrameWords {
 };
 static_assert(sizeof(specrel_frame) % 4 == 0 && FrameWords::N <= 256, "one frame per k_reset_frame thread block");
 __global__ void k_reset_frame(uint32_t *counters, uint32_t n, FrameWords f, uint32_t *dst);
+__global__ void k_star_prep_first(const float4 *words, uint32_t num_stars, StarPrep *out, StarDir *dirs,
+                                  uint32_t *counters, uint32_t n, FrameWords fw, uint32_t *dst);
 // The view table of one view (level tile prefix sums, super tiles, pixels, pyramid floats: each
 // row's two entries), passed by value to k_reset_view.
 constexpr int TABLE_ROWS = SPECREL_BLOOM_LEVELS + 4;
