@@ -418,6 +418,12 @@ def main():
                     help="0..8: top tree levels each primary-ray block reads from LDS (-1: default)")
     ap.add_argument("--fused-shade", type=int, default=-1,
                     help="1/0: the primary-ray kernel shades its own pixels (-1: default)")
+    ap.add_argument("--stage-events", type=int, default=1,
+                    help="device events per timed frame: 1 only around the primary-ray kernel (its "
+                         "duration for the roofline), 2 at every stage boundary, 0 none; the stage "
+                         "breakdown comes from --breakdown-steps further frames with every event")
+    ap.add_argument("--breakdown-steps", type=int, default=20,
+                    help="frames rendered after the timed ones with an event at every stage boundary")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a plain 1 GiB device copy (the HBM-counter calibration) and exit")
     args = ap.parse_args()
@@ -433,7 +439,8 @@ def main():
     from specrel_amd import scene as S
     from specrel_amd.renderer import (OPT_PERSISTENT, OPT_MERGED_LOADS, OPT_SHADOW_IN_TRACE, OPT_SHADOW_QUEUE,
                                       OPT_STACK, OPT_STAR_OVERLAP, OPT_STATS, OPT_TRACE_WAVES, OPT_WAVE_SHAPE,
-                                      OPT_NODE_WIDTH, OPT_LDS_TOP, OPT_FUSED_SHADE, SHADOWS_IN_TRACE,
+                                      OPT_NODE_WIDTH, OPT_LDS_TOP, OPT_FUSED_SHADE, OPT_STAGE_EVENTS,
+                                      SHADOWS_IN_TRACE,
                                       Context)
 
     w, h = args.width, args.height
@@ -482,22 +489,42 @@ def main():
     ctx.set_option(OPT_STATS, 0)
     log(f"counters: {st}")
 
+    # Timed frames: device events only where the roofline needs them (around the primary-ray
+    # kernel), since each event record idles the device between the kernels it separates.
+    ctx.set_option(OPT_STAGE_EVENTS, args.stage_events)
     for _ in range(args.warmup):
         ctx.render_resident(0, frame)
         ctx.sync()
-    parts = {k: 0.0 for k in ("star_prep_ms", "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "kernel_ms")}
+    stage_keys = ("star_prep_ms", "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "kernel_ms")
+    trace_timed = 0.0
     t0 = time.perf_counter()
     for i in range(args.steps):
+        ctx.render_resident(0, frame)
+        ctx.sync()
+        trace_timed += ctx.timings()["trace_ms"]
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = w * h * args.steps / wall / 1e6
+    trace_timed /= args.steps
+    log(f"{args.steps} steps in {wall:.3f} s: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s; "
+        f"primary-ray kernel {trace_timed:.4f} ms")
+    # Stage breakdown (not timed): the same frames with an event at every stage boundary.
+    ctx.set_option(OPT_STAGE_EVENTS, 2)
+    parts = {k: 0.0 for k in stage_keys}
+    n_break = max(args.breakdown_steps, 1)
+    t0 = time.perf_counter()
+    for i in range(n_break):
         ctx.render_resident(0, frame)
         ctx.sync()
         t = ctx.timings()
         for k in parts:
             parts[k] += t[k]
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    value = w * h * args.steps / wall / 1e6
-    parts = {k: v / args.steps for k, v in parts.items()}
-    log(f"{args.steps} steps in {wall:.3f} s: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s; kernels {parts}")
+    ms_break = (time.perf_counter() - t0) * 1e3 / n_break
+    parts = {k: v / n_break for k, v in parts.items()}
+    log(f"stage breakdown over {n_break} frames ({ms_break:.3f} ms/frame with every stage event): {parts}")
+    if args.stage_events == 0:
+        trace_timed = parts["trace_ms"]
+    ctx.set_option(OPT_STAGE_EVENTS, args.stage_events)
 
     # End to end through the host boundary, as GpuRenderer::render (the reference's render()
     # includes the readback): frame upload, render, RGB8 into the caller's framebuffer. Median.
@@ -509,11 +536,12 @@ def main():
         if i >= 3:
             e2e.append((time.perf_counter() - t0) * 1e3)
     e2e_ms = float(np.median(e2e))
+    ctx.set_option(OPT_STAGE_EVENTS, 2)  # the other legs read every stage's time
 
-    fused = ctx.timings()["shade_ms"] == 0.0  # the primary-ray kernel shaded its own pixels
+    fused = parts["shade_ms"] == 0.0  # the primary-ray kernel shaded its own pixels (breakdown frames)
     alg = compulsory_bytes(st, w, h, scene.num_stars, in_trace, fused)
     work = work_bytes(st, w, h, scene.num_stars, in_trace, fused)
-    times = {"k_trace": parts["trace_ms"]} if fused else {"k_trace": parts["trace_ms"], "k_shade": parts["shade_ms"]}
+    times = {"k_trace": trace_timed} if fused else {"k_trace": trace_timed, "k_shade": parts["shade_ms"]}
     dom = max(times, key=times.get)
     achieved = alg[dom] / (times[dom] * 1e-3) / 1e9
     traffic = None
@@ -604,6 +632,11 @@ def main():
                    "width": w, "height": h, "tets": scene.num_prims, "nodes": scene.num_nodes,
                    "stars": scene.num_stars},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in parts.items()},
+        "stage_events": {"timed_frames": args.stage_events, "breakdown_frames": n_break,
+                         "breakdown_ms_per_step": round(ms_break, 4),
+                         "note": "value and roofline.kernel_ms come from the timed frames (events only around "
+                                 "the primary-ray kernel at 1); kernel_ms_per_step from the breakdown frames "
+                                 "after them, with an event at every stage boundary"},
         "shadow_rays_traced_in": "k_trace" if in_trace else "k_shade",
         "pixels_shaded_in": "k_trace" if fused else "k_shade",
         "e2e_ms_with_readback": round(e2e_ms, 3),

@@ -37,7 +37,7 @@ typedef struct specrel_ctx specrel_ctx;
 typedef struct specrel_timings {
     float submit_ms;      /* host time to enqueue everything */
     float wait_ms;        /* host wait for the device */
-    float kernel_ms;      /* first kernel start to last kernel end, device clock */
+    float kernel_ms;      /* first kernel start to last kernel end, device clock (SPECREL_OPT_STAGE_EVENTS 2) */
     float readback_ms;    /* device->host copies */
     float star_prep_ms;   /* per-view star pass + star binning */
     float trace_ms;       /* primary rays: closest hit per pixel */
@@ -123,12 +123,17 @@ enum {
                                      (LDS stack of 8 or 16) copies the interior nodes of that many top
                                      levels of the tree into LDS and reads them there (takes
                                      precedence over SPECREL_OPT_NODE_WIDTH). Same visiting order. */
-    SPECREL_OPT_FUSED_SHADE = 18  /* tiled primary rays with an LDS stack of 8 or 16 and their shadow
+    SPECREL_OPT_FUSED_SHADE = 18, /* tiled primary rays with an LDS stack of 8 or 16 and their shadow
                                      queries traced there shade their own pixels (the star pass runs
                                      first): 1 (default) for renders of one view, 2 for every render,
                                      0 never (a shading pass of its own after them). Instrumented
                                      renders, queued or overlapped passes use the separate pass.
                                      Results do not depend on it. */
+    SPECREL_OPT_STAGE_EVENTS = 19 /* device events of a directly launched render (specrel_timings):
+                                     2 (default) at every stage boundary (every *_ms field), 1 only
+                                     around the primary-ray kernel (trace_ms; the other device
+                                     fields read 0), 0 none. Each event idles the device for a few
+                                     microseconds between the kernels it separates. */
 };
 
 /* Per-view outputs of specrel_render_batch; every pointer may be NULL (not copied). len is the

@@ -134,6 +134,7 @@ struct specrel_ctx {
     // The next frame's tile order runs on the side stream beside the shading and bloom passes.
     hipEvent_t order_fork = nullptr, order_join = nullptr;
     bool timed_stages = true, stars_overlapped = false, shade_fused = false;
+    int stage_level = 2;  // SPECREL_OPT_STAGE_EVENTS of the last directly launched render
     specrel_timings timings{};
     specrel_stats stats_host{};
     int opt_wave_clock = 0;
@@ -145,6 +146,7 @@ struct specrel_ctx {
     int opt_node_width = 2;  // SPECREL_OPT_NODE_WIDTH
     int opt_lds_top = 0;     // SPECREL_OPT_LDS_TOP: top levels copied into LDS (0: none)
     int opt_fuse_shade = 1;  // SPECREL_OPT_FUSED_SHADE
+    int opt_stage_events = 2;  // SPECREL_OPT_STAGE_EVENTS
     int opt_wave_shape = 0;
     int opt_tile_order = 1;
     int opt_postpone = 1;
@@ -1016,9 +1018,11 @@ Plan prepare(specrel_ctx *ctx, uint32_t scene_idx, const specrel_frame *frames, 
 void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool events, const Bufs &B) {
     hipStream_t s = ctx->stream;
     const bool stats = ctx->opt_stats != 0;
-    auto mark = [&](int i) {
-        if (events) check(hipEventRecord(ctx->ev[i], s), "event");
-    };
+    // Stage events (SPECREL_OPT_STAGE_EVENTS): 2 every stage boundary, 1 only the pair around the
+    // primary-ray kernel, 0 none. Each record leaves the device idle for a few microseconds
+    // between the kernels it separates.
+    const int level = events ? ctx->opt_stage_events : 0;
+    ctx->stage_level = level;
     const Views V = B.V;
     const uint32_t nv = B.n_views;
     uint32_t *counters = B.counters;
@@ -1050,6 +1054,12 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     // One view launched directly whose star pass comes first (the fused kernel shades with its
     // lists): the star pass's first kernel also clears the counters and writes the frame.
     const bool reset_in_prep = pl.frame_in_reset && !B.table && fuse && pl.stars;
+    // ev 0 and 1 bracket the primary rays when they come first (ev 10 and 1 when the fused kernel
+    // follows the star pass); the others mark the later stages.
+    auto mark = [&](int i) {
+        if (level == 2 || (level == 1 && (i == 1 || (i == 0 && !fuse))))
+            check(hipEventRecord(ctx->ev[i], s), "event");
+    };
     if (B.table) {  // one view without staging: frame and view table from the kernel arguments
         k_reset_view<<<1, 256, 0, s>>>(counters, COUNTERS, pl.frame, B.frames, *B.table, B.table_dst);
     } else if (!reset_in_prep) {
@@ -1131,16 +1141,16 @@ void record_launches(specrel_ctx *ctx, SceneSlot &sc, const Plan &pl, bool event
     const bool order_side = order && !order_in_bin;
     if (fuse) {
         if (pl.stars) star_pass(s, order_in_bin ? B.total_tiles : 0u);
-        if (events) check(hipEventRecord(ctx->ev[10], s), "event");
+        if (level >= 1) check(hipEventRecord(ctx->ev[10], s), "event");
     }
     launch_any(ctx, stats, tiles, s, ta);
     check(hipGetLastError(), "trace launch");
     mark(1);
     if (overlap) {
         check(hipStreamWaitEvent(ctx->side, ctx->fork, 0), "fork wait");
-        if (events) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
+        if (level == 2) check(hipEventRecord(ctx->ev[9], ctx->side), "event");
         star_pass(ctx->side, 0u);
-        if (events) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
+        if (level == 2) check(hipEventRecord(ctx->ev[10], ctx->side), "event");
         check(hipEventRecord(ctx->join, ctx->side), "join");
     }
     if (order_side) {
@@ -1481,7 +1491,10 @@ void fetch(specrel_ctx *ctx, const std::vector<Fetch> &f, float *wait_ms, float 
 
 void collect(specrel_ctx *ctx) {
     specrel_timings &t = ctx->timings;
-    if (ctx->timed_stages) {
+    const int level = ctx->timed_stages ? ctx->stage_level : 2;  // a graph replay: ev 0 and 5 around it
+    if (ctx->timed_stages && level == 1) {
+        t.trace_ms = ctx->shade_fused ? ev_ms(ctx->ev[10], ctx->ev[1]) : ev_ms(ctx->ev[0], ctx->ev[1]);
+    } else if (ctx->timed_stages && level == 2) {
         if (ctx->shade_fused) {  // star pass, then primary rays with the shading (ev 0, 10, 1)
             t.star_prep_ms = ev_ms(ctx->ev[0], ctx->ev[10]);
             t.trace_ms = ev_ms(ctx->ev[10], ctx->ev[1]);
@@ -1494,7 +1507,7 @@ void collect(specrel_ctx *ctx) {
         t.bloom_ms = ev_ms(ctx->ev[3], ctx->ev[4]);
         t.composite_ms = ev_ms(ctx->ev[4], ctx->ev[5]);
     }
-    t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[5]);
+    if (level == 2) t.kernel_ms = ev_ms(ctx->ev[0], ctx->ev[5]);
     if (ctx->opt_stats) {
         unsigned long long *dst = (unsigned long long *)ctx->stats.p;
         const uint32_t *bits = (const uint32_t *)ctx->touch.p;
@@ -1704,7 +1717,9 @@ int specrel_create(int device_ordinal, specrel_ctx **out) {
         check(hipSetDevice(device_ordinal), "hipSetDevice");
         ctx->scratch = scratch_address(device_ordinal);
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
-        for (auto &e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
+        // The stage events only time (the host waits on the stream, not on them): no system-scope
+        // cache writeback and invalidation at each record.
+        for (auto &e : ctx->ev) check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreate");
         check(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming), "hipEventCreate");
         check(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming), "hipEventCreate");
@@ -1816,6 +1831,13 @@ int specrel_set_option(specrel_ctx *ctx, int option, int value) {
             return SPECREL_EINVAL;
         }
         ctx->opt_wave_shape = value;
+    }
+    else if (option == SPECREL_OPT_STAGE_EVENTS) {
+        if (value < 0 || value > 2) {
+            ctx->err = "stage events must be 0 (none), 1 (the primary-ray kernel) or 2 (every stage)";
+            return SPECREL_EINVAL;
+        }
+        ctx->opt_stage_events = value;
     }
     else if (option == SPECREL_OPT_FUSED_SHADE) {
         if (value < 0 || value > 2) {

@@ -27,6 +27,7 @@ OPT_WAVE_CLOCK = 15
 OPT_NODE_WIDTH = 16
 OPT_LDS_TOP = 17
 OPT_FUSED_SHADE = 18
+OPT_STAGE_EVENTS = 19
 # the library's defaults for SPECREL_OPT_TRACE_WAVES and SPECREL_OPT_LDS_TOP
 DEFAULT_TRACE_WAVES, DEFAULT_LDS_TOP = 1, 0
 # specrel_timings.trace_variant: the primary-ray kernel's traversal

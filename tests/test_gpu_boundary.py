@@ -211,6 +211,35 @@ def test_wave_clock_records_every_wave_and_changes_nothing(gpu_ctx, twin):
     assert len(rec2) == 0
 
 
+@pytest.mark.parametrize("fused", [1, 0])
+def test_stage_events_change_only_the_timings(gpu_ctx, twin, fused):
+    """SPECREL_OPT_STAGE_EVENTS: 1 times only the primary-ray kernel, 0 nothing, and the outputs are
+    those of a render with every stage event, with the shading fused into the primary-ray kernel
+    or in a pass of its own."""
+    a, _ = frames(twin)
+    gpu_ctx.upload(0, twin)
+    gpu_ctx.set_option(R.OPT_FUSED_SHADE, fused)
+    out = {}
+    try:
+        full = gpu_ctx.render_frame(0, a, want_hits=True)
+        t_full = gpu_ctx.timings()
+        for level in (1, 0):
+            gpu_ctx.set_option(R.OPT_STAGE_EVENTS, level)
+            out[level] = (gpu_ctx.render_frame(0, a, want_hits=True), gpu_ctx.timings())
+        with pytest.raises(SpecrelError):
+            gpu_ctx.set_option(R.OPT_STAGE_EVENTS, 3)
+    finally:
+        gpu_ctx.set_option(R.OPT_STAGE_EVENTS, 2)
+        gpu_ctx.set_option(R.OPT_FUSED_SHADE, 1)
+    assert t_full["kernel_ms"] > 0.0 and t_full["trace_ms"] > 0.0 and t_full["bloom_ms"] > 0.0
+    assert (t_full["shade_ms"] == 0.0) == (fused == 1)
+    for level, (img, t) in out.items():
+        for k in ("rgb", "hit", "lam", "tau"):
+            assert np.array_equal(full[k].view(np.uint32), img[k].view(np.uint32)), (level, k)
+        assert t["kernel_ms"] == 0.0 and t["bloom_ms"] == 0.0 and t["star_prep_ms"] == 0.0, (level, t)
+        assert (t["trace_ms"] > 0.0) == (level == 1), (level, t)
+
+
 def test_render_from_the_reference_uniform_block(gpu_ctx, twin):
     """specrel_render_uniforms with GpuUniforms::build's 768-B block: without flashes the image is
     specrel_render's (and the oracle's); with flashes it is the render of the same frame with each
