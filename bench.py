@@ -422,6 +422,9 @@ def main():
                     help="device events per timed frame: 1 only around the primary-ray kernel (its "
                          "duration for the roofline), 2 at every stage boundary, 0 none; the stage "
                          "breakdown comes from --breakdown-steps further frames with every event")
+    ap.add_argument("--trace-events-every", type=int, default=8,
+                    help="with --stage-events 1, the timed frames that record the pair of events around "
+                         "the primary-ray kernel: every Nth (the others record none)")
     ap.add_argument("--breakdown-steps", type=int, default=20,
                     help="frames rendered after the timed ones with an event at every stage boundary")
     ap.add_argument("--copy-probe", action="store_true",
@@ -490,24 +493,32 @@ def main():
     log(f"counters: {st}")
 
     # Timed frames: device events only where the roofline needs them (around the primary-ray
-    # kernel), since each event record idles the device between the kernels it separates.
+    # kernel, on every Nth frame), since each event record idles the device for a few
+    # microseconds between the kernels it separates.
+    every = max(args.trace_events_every, 1)
     ctx.set_option(OPT_STAGE_EVENTS, args.stage_events)
     for _ in range(args.warmup):
         ctx.render_resident(0, frame)
         ctx.sync()
     stage_keys = ("star_prep_ms", "trace_ms", "shade_ms", "bloom_ms", "composite_ms", "kernel_ms")
-    trace_timed = 0.0
+    trace_sum, trace_n, level = 0.0, 0, args.stage_events
     t0 = time.perf_counter()
     for i in range(args.steps):
+        want = args.stage_events if args.stage_events != 1 or i % every == 0 else 0
+        if want != level:
+            ctx.set_option(OPT_STAGE_EVENTS, want)
+            level = want
         ctx.render_resident(0, frame)
         ctx.sync()
-        trace_timed += ctx.timings()["trace_ms"]
+        if level >= 1:
+            trace_sum += ctx.timings()["trace_ms"]
+            trace_n += 1
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     value = w * h * args.steps / wall / 1e6
-    trace_timed /= args.steps
+    trace_timed = trace_sum / max(trace_n, 1)
     log(f"{args.steps} steps in {wall:.3f} s: {ms_per_step:.3f} ms/frame, {value:.1f} Mrays/s; "
-        f"primary-ray kernel {trace_timed:.4f} ms")
+        f"primary-ray kernel {trace_timed:.4f} ms over {trace_n} timed launches")
     # Stage breakdown (not timed): the same frames with an event at every stage boundary.
     ctx.set_option(OPT_STAGE_EVENTS, 2)
     parts = {k: 0.0 for k in stage_keys}
@@ -632,11 +643,12 @@ def main():
                    "width": w, "height": h, "tets": scene.num_prims, "nodes": scene.num_nodes,
                    "stars": scene.num_stars},
         "kernel_ms_per_step": {k: round(v, 4) for k, v in parts.items()},
-        "stage_events": {"timed_frames": args.stage_events, "breakdown_frames": n_break,
-                         "breakdown_ms_per_step": round(ms_break, 4),
-                         "note": "value and roofline.kernel_ms come from the timed frames (events only around "
-                                 "the primary-ray kernel at 1); kernel_ms_per_step from the breakdown frames "
-                                 "after them, with an event at every stage boundary"},
+        "stage_events": {"timed_frames": args.stage_events, "trace_timed_launches": trace_n,
+                         "breakdown_frames": n_break, "breakdown_ms_per_step": round(ms_break, 4),
+                         "note": "value and roofline.kernel_ms come from the timed frames (at 1: a pair of events "
+                                 "around the primary-ray kernel on every Nth frame, none on the others); "
+                                 "kernel_ms_per_step from the breakdown frames after them, with an event at "
+                                 "every stage boundary"},
         "shadow_rays_traced_in": "k_trace" if in_trace else "k_shade",
         "pixels_shaded_in": "k_trace" if fused else "k_shade",
         "e2e_ms_with_readback": round(e2e_ms, 3),
